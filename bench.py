@@ -1,0 +1,135 @@
+#!/usr/bin/env python3
+"""Throughput of the explicit Moelans phase-field step on one MI355X.
+
+Workload: 512^3 periodic grid, 2000 seeded spheres with R ~ U[6, 14] (overlaps
+merged by max etaS), etaL = 1 - etaS, c = 0.1 + 0.8*etaS, the reference's
+constants (phasefield.c:76-97).  A step is one sweep of the whole grid
+(phasefield.c:186-215).  The state is created on the device (inputs resident
+in HBM before timing starts); K steps are timed between two device
+synchronisations.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--size N]
+                    [--cpu-steps S] [--no-cpu]
+
+Prints one JSON line.  value = FP64 GLUPS = N^3 * K / seconds / 1e9.
+roofline: algorithmic bytes per launch (48 B/site in FP64: read etaS, etaL, c
+and write them) over the kernel's average launch duration, from HIP events
+recorded on the stream the kernels run on.  cpu_baseline: the CPU oracle (a
+restatement of the reference algorithm, bit-exact against it) timed on this
+host's cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "GLUPS & %HBM peak at 512^3 FP64 vs FP32; surrogate speedup @≤2% R̄ error"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_SITE = {"f64": 48, "f32": 24}
+
+
+def traffic_from_profiles(kernel, workload):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by a copy-kernel calibration), if committed."""
+    p = os.path.join(HERE, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        return d.get(f"{kernel}@{workload}", {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def run_gpu(S, shape, precision, steps, warmup, seed=2, nsph=2000, rmin=6.0, rmax=14.0):
+    cen, rad = S.spheres(seed, nsph, shape, rmin, rmax)
+    sv = S.Solver(shape, precision=precision)
+    sv.init_spheres(cen, rad)
+    st0 = sv.reduce()
+    sv.step(warmup)
+    sv.sync()
+    t0 = time.perf_counter()
+    sv.timer_start()
+    sv.step(steps)
+    ev_ms = sv.timer_stop()
+    sv.sync()
+    wall = time.perf_counter() - t0
+    st1 = sv.reduce()
+    kname = sv.kernel_name
+    sv.close()
+    return {"wall_s": wall, "event_ms": ev_ms, "kernel": kname, "stats0": st0, "stats1": st1}
+
+
+def cpu_baseline(S, shape, nsteps):
+    """The oracle on host cores: nsteps sweeps of the same 512^3 workload."""
+    from oracle import pyoracle as O
+    cen, rad = S.spheres(2, 2000, shape, 6.0, 14.0)
+    with S.Solver(shape) as sv:
+        sv.init_spheres(cen, rad)
+        s, l, c = sv.download()
+    sec, threads = O.time_steps(s, l, c, nsteps)
+    n = shape[0] * shape[1] * shape[2]
+    return {"value": n * nsteps / sec / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} sweeps of the {shape[0]}^3 workload ({sec:.1f} s), oracle "
+                      f"(C restatement of phasefield.c:301-403, bit-exact vs the reference), "
+                      f"OpenMP {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f32", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(multi-GPU sharding is design only)", file=sys.stderr)
+        sys.exit(2)
+
+    from phasefield_coarsening_amd import solver as S
+    n = a.size
+    shape = (n, n, n)
+    sites = n ** 3
+    r = run_gpu(S, shape, S.F64, a.steps, a.warmup)
+    glups = sites * a.steps / r["wall_s"] / 1e9
+    avg_launch_s = r["event_ms"] * 1e-3 / a.steps
+    achieved = BYTES_PER_SITE["f64"] * sites / avg_launch_s / 1e9
+    workload = f"{n}^3 periodic, 2000 spheres R~U[6,14], FP64"
+    out = {
+        "metric": METRIC,
+        "value": round(glups, 3),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(r["wall_s"] * 1e3 / a.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded sphere initial condition generated on the device)",
+        "config": {"workload": workload, "grid": [n, n, n], "bc": "periodic", "spheres": 2000,
+                   "radius": [6, 14], "kernel": r["kernel"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(r["kernel"], f"{n}^3-f64"),
+                     "bytes_per_site": BYTES_PER_SITE["f64"],
+                     "avg_launch_us": round(avg_launch_s * 1e6, 2)},
+        "solute_drift_rel": abs(r["stats1"]["sum_c"] - r["stats0"]["sum_c"]) /
+        abs(r["stats0"]["sum_c"]),
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(S, shape, a.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+/*
+ * ppf.h -- C ABI of libppf.so, the MI355X (gfx950) explicit Moelans
+ * phase-field stepper.
+ *
+ * The reference program (jijn/phasefield_coarsening, phasefield.c) steps
+ * nine caller-owned, ghost-shelled static arrays on the CPU:
+ *
+ *   for each step:                                   phasefield.c:178-258
+ *       ghost fill (periodic or no-flux)             phasefield.c:186-212
+ *       sweep_fields(etaS, etaL, c, detaS, ...)      phasefield.c:215, 301-403
+ *         (sweep_fields_c once i_iter >= n_iter_init phasefield.c:217, 406-476)
+ *       at saves: sums of dc^2, detaS^2, detaL^2     phasefield.c:228-248
+ *
+ * Here the three fields live in HBM for the life of a context, the ghost
+ * fill becomes index wrap (or clamp) inside the kernel, and one launch does
+ * one whole sweep.  All arrays crossing this ABI are dense C order
+ * [nz][ny][nx] (x fastest, the reference's k index; z is its i index,
+ * phasefield.c:53-56) without ghost cells, float64, caller-owned host memory.
+ *
+ * Every call returns 0 or a negative PPF_E* status and never prints or
+ * exits; ppf_last_error(ctx) explains the last failure on that context
+ * (ppf_last_error(NULL) the last failure of ppf_create).  One host thread
+ * per context; calls are ordered on the context's own HIP stream, and the
+ * calls that hand data back to the host synchronise it.
+ */
+#ifndef PPF_H
+#define PPF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PPF_OK = 0,
+    PPF_EARG = -1,     /* bad argument (null pointer, size, enum value) */
+    PPF_ENOMEM = -2,   /* device or host allocation failed */
+    PPF_EHIP = -3,     /* a HIP runtime call failed */
+    PPF_ENODEV = -4,   /* no usable gfx950 device */
+    PPF_ESTATE = -5    /* call not valid in the context's state */
+};
+
+enum { PPF_BC_PERIODIC = 0, PPF_BC_NOFLUX = 1 };  /* USE_BC_TYPE, phasefield.c:15 */
+
+enum { PPF_F64 = 0, PPF_F32 = 1 };
+
+/* Kernel selection for PPF_F64.  PPF_KERNEL_AUTO picks the fastest exact
+ * kernel; PPF_KERNEL_NAIVE is the one-thread-per-site global-memory form
+ * kept as a second, independent implementation for cross-checks. */
+enum { PPF_KERNEL_AUTO = 0, PPF_KERNEL_NAIVE = 1, PPF_KERNEL_TILED = 2 };
+
+/* The physical arguments of sweep_fields (phasefield.c:309-314), in that
+ * order.  ppf_params_default fills the reference's values
+ * (phasefield.c:76-97). */
+typedef struct ppf_params {
+    double MS, ML, AS, AL, cL0, cS0, CS, CL, L, dt, h2, m, gamma, kappa;
+} ppf_params;
+
+typedef struct ppf_desc {
+    int nz, ny, nx;      /* interior sizes (NI, NJ, NK of phasefield.c:53-56) */
+    int bc;              /* PPF_BC_* */
+    int precision;       /* PPF_F64 or PPF_F32 */
+    int kernel;          /* PPF_KERNEL_* */
+    int device;          /* HIP device ordinal */
+    int reserved;
+    ppf_params params;
+} ppf_desc;
+
+/* Field statistics computed on the device (ppf_reduce). */
+typedef struct ppf_stats {
+    double sum_c;        /* total solute, compensated FP64 sum */
+    double sum_h;        /* sum of hS = s^2/(s^2+l^2): solid phase fraction * N */
+    double n_solid;      /* number of sites with etaS > 0.5 */
+    double sum_s;        /* sum of etaS */
+    double min_c, max_c; /* range of c */
+    double n_sites;      /* nz*ny*nx */
+    double n_nonfinite;  /* sites where any field is NaN or Inf */
+} ppf_stats;
+
+/* Sums of the last step's squared increments over the interior
+ * (phasefield.c:228-238): err = sqrt(sum_dc2) is the reference's stopping
+ * error (phasefield.c:247).  Summed in a fixed tree order, so they agree
+ * with the reference's serial sums to rounding (~1e-15 relative). */
+typedef struct ppf_incsums {
+    double sum_dc2, sum_ds2, sum_dl2;
+} ppf_incsums;
+
+typedef struct ppf_ctx ppf_ctx;
+
+/* Reference constants, phasefield.c:76-97. */
+int ppf_params_default(ppf_params *p);
+
+/* Allocate a context: two ping-pong copies of (etaS, etaL, c) in HBM.
+ * Replaces the static arrays of phasefield.c:67-70 and their init loop
+ * (phasefield.c:156-167). */
+int ppf_create(ppf_ctx **out, const ppf_desc *desc);
+int ppf_destroy(ppf_ctx *ctx);
+
+/* Host -> device state.  Replaces read_hdf5's landing of the fields in the
+ * arrays (phasefield.c:1011-1041).  l may be NULL: etaL is then 1.0
+ * everywhere, the state the reference's reader leaves (phasefield.c:1032). */
+int ppf_upload(ppf_ctx *ctx, const double *s, const double *l, const double *c);
+
+/* Device -> host state (synchronises).  Any pointer may be NULL. */
+int ppf_download(ppf_ctx *ctx, double *s, double *l, double *c);
+
+/* nsteps sweeps, each = ghost fill + sweep_fields (phasefield.c:186-215).
+ * Asynchronous on the context stream. */
+int ppf_step(ppf_ctx *ctx, int64_t nsteps);
+
+/* nsteps c-only sweeps = ghost fill + sweep_fields_c (phasefield.c:217,
+ * 406-476): etaS, etaL frozen. */
+int ppf_step_c_only(ppf_ctx *ctx, int64_t nsteps);
+
+/* One step (full or c-only) that also returns the sums of its squared
+ * increments, the values the reference prints at a save
+ * (phasefield.c:228-248).  Synchronises. */
+int ppf_step_report(ppf_ctx *ctx, int c_only, ppf_incsums *out);
+
+/* Device reductions over the current state (synchronises). */
+int ppf_reduce(ppf_ctx *ctx, ppf_stats *out);
+
+/* Seeded sphere initial condition on the device: etaS = max over spheres of
+ * 0.5*(1 - tanh((r - R)/2)) with periodic minimum-image distance r, etaL =
+ * 1 - etaS, c = c_liq + c_span*etaS.  centers_zyx holds n (z, y, x) triples
+ * in cell units, radii n values.  No counterpart in the reference, which
+ * takes its initial state from a MATLAB-written file (README.md:26-34). */
+int ppf_init_spheres(ppf_ctx *ctx, int n, const double *centers_zyx, const double *radii,
+                     double c_liq, double c_span);
+
+/* Host-side seeded sphere list (SplitMix64): centers uniform in the box,
+ * radii uniform in [rmin, rmax).  No GPU needed. */
+int ppf_spheres_generate(uint64_t seed, int n, int nz, int ny, int nx, double rmin, double rmax,
+                         double *centers_zyx, double *radii);
+
+/* Timing on the context stream: ppf_timer_start records a HIP event,
+ * ppf_timer_stop records a second one, synchronises and returns the elapsed
+ * milliseconds between them. */
+int ppf_timer_start(ppf_ctx *ctx);
+int ppf_timer_stop(ppf_ctx *ctx, double *ms);
+
+/* Wait for all work queued on the context. */
+int ppf_sync(ppf_ctx *ctx);
+
+/* Steps taken since the last upload. */
+int ppf_steps_taken(ppf_ctx *ctx, int64_t *out);
+
+/* Name of the kernel ppf_step launches (static string). */
+const char *ppf_kernel_name(ppf_ctx *ctx);
+
+const char *ppf_last_error(ppf_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

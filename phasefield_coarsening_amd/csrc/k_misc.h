@@ -1,0 +1,186 @@
+// k_misc.h -- state statistics, the sphere initial condition and small
+// helpers.  None of these is on the per-step path.
+#pragma once
+
+#include "ppf_physics.h"
+#include "ppf_reduce.h"
+
+// Per-workgroup partial statistics of the state.  Record per block:
+//   [0,1] sum c as double-double, [2] sum h, [3] count etaS > 0.5,
+//   [4] sum etaS, [5] min c, [6] max c, [7] non-finite sites.
+constexpr int STAT_W = 8;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_stats(const T *__restrict__ s, const T *__restrict__ l,
+                                               const T *__restrict__ c, size_t n,
+                                               double *__restrict__ partials)
+{
+    DD sc = {0.0, 0.0};
+    double sh = 0.0, ns = 0.0, ss = 0.0, mn = INFINITY, mx = -INFINITY, nf = 0.0;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        const double sv = (double)s[q], lv = (double)l[q], cv = (double)c[q];
+        if (!isfinite(sv) || !isfinite(lv) || !isfinite(cv)) {
+            nf += 1.0;
+            continue;
+        }
+        sc = dd_add1(sc, cv);
+        sh += sv * sv / (sv * sv + lv * lv);
+        ns += sv > 0.5 ? 1.0 : 0.0;
+        ss += sv;
+        mn = fmin(mn, cv);
+        mx = fmax(mx, cv);
+    }
+    sc = wave_dd_sum(sc);
+    sh = wave_sum(sh);
+    ns = wave_sum(ns);
+    ss = wave_sum(ss);
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    nf = wave_sum(nf);
+    __shared__ double part[STAT_W][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        part[0][w] = sc.hi;
+        part[1][w] = sc.lo;
+        part[2][w] = sh;
+        part[3][w] = ns;
+        part[4][w] = ss;
+        part[5][w] = mn;
+        part[6][w] = mx;
+        part[7][w] = nf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD t = {part[0][0], part[1][0]};
+        double a = part[2][0], b = part[3][0], d = part[4][0], e = part[5][0], f = part[6][0],
+               g = part[7][0];
+        for (int j = 1; j < 4; j++) {
+            t = dd_add(t, DD{part[0][j], part[1][j]});
+            a += part[2][j];
+            b += part[3][j];
+            d += part[4][j];
+            e = fmin(e, part[5][j]);
+            f = fmax(f, part[6][j]);
+            g += part[7][j];
+        }
+        double *o = partials + (size_t)blockIdx.x * STAT_W;
+        o[0] = t.hi;
+        o[1] = t.lo;
+        o[2] = a;
+        o[3] = b;
+        o[4] = d;
+        o[5] = e;
+        o[6] = f;
+        o[7] = g;
+    }
+}
+
+// Combine nb stat records (one workgroup of 256, fixed order) into out[0..7].
+__global__ __launch_bounds__(256) void k_stats_final(const double *__restrict__ p, int nb,
+                                                     double *__restrict__ out)
+{
+    DD sc = {0.0, 0.0};
+    double sh = 0.0, ns = 0.0, ss = 0.0, mn = INFINITY, mx = -INFINITY, nf = 0.0;
+    for (int j = threadIdx.x; j < nb; j += 256) {
+        const double *r = p + (size_t)j * STAT_W;
+        sc = dd_add(sc, DD{r[0], r[1]});
+        sh += r[2];
+        ns += r[3];
+        ss += r[4];
+        mn = fmin(mn, r[5]);
+        mx = fmax(mx, r[6]);
+        nf += r[7];
+    }
+    sc = wave_dd_sum(sc);
+    sh = wave_sum(sh);
+    ns = wave_sum(ns);
+    ss = wave_sum(ss);
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    nf = wave_sum(nf);
+    __shared__ double part[STAT_W][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        part[0][w] = sc.hi;
+        part[1][w] = sc.lo;
+        part[2][w] = sh;
+        part[3][w] = ns;
+        part[4][w] = ss;
+        part[5][w] = mn;
+        part[6][w] = mx;
+        part[7][w] = nf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD t = {part[0][0], part[1][0]};
+        double a = part[2][0], b = part[3][0], d = part[4][0], e = part[5][0], f = part[6][0],
+               g = part[7][0];
+        for (int j = 1; j < 4; j++) {
+            t = dd_add(t, DD{part[0][j], part[1][j]});
+            a += part[2][j];
+            b += part[3][j];
+            d += part[4][j];
+            e = fmin(e, part[5][j]);
+            f = fmax(f, part[6][j]);
+            g += part[7][j];
+        }
+        out[0] = t.hi + t.lo;
+        out[1] = a;
+        out[2] = b;
+        out[3] = d;
+        out[4] = e;
+        out[5] = f;
+        out[6] = g;
+    }
+}
+
+// Sum nb triples (one workgroup of 256, fixed order) into out[0..2].
+__global__ __launch_bounds__(256) void k_sum3_final(const double *__restrict__ p, int nb,
+                                                    double *__restrict__ out)
+{
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int j = threadIdx.x; j < nb; j += 256)
+        for (int i = 0; i < 3; i++)
+            a[i] += p[3 * (size_t)j + i];
+    block_sum3<256>(a, out);
+}
+
+// Sphere initial condition.  sph holds n records (cz, cy, cx, R).  The
+// profile 0.5*(1 - tanh(d/2)) falls with d = r - R, so the max over spheres
+// is the profile at the smallest d.
+template <typename T>
+__global__ __launch_bounds__(256) void k_init_spheres(T *__restrict__ s, T *__restrict__ l,
+                                                      T *__restrict__ c, int nx, int ny, int nz,
+                                                      const double *__restrict__ sph, int n,
+                                                      double c_liq, double c_span)
+{
+    const size_t total = (size_t)nx * ny * nz;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < total;
+         q += (size_t)gridDim.x * 256) {
+        const int x = (int)(q % nx);
+        const int y = (int)((q / nx) % ny);
+        const int z = (int)(q / ((size_t)nx * ny));
+        double dmin = 1e30;
+        for (int i = 0; i < n; i++) {
+            double dz = fabs(z - sph[4 * i + 0]);
+            double dy = fabs(y - sph[4 * i + 1]);
+            double dx = fabs(x - sph[4 * i + 2]);
+            dz = fmin(dz, nz - dz);
+            dy = fmin(dy, ny - dy);
+            dx = fmin(dx, nx - dx);
+            const double d = sqrt(dz * dz + dy * dy + dx * dx) - sph[4 * i + 3];
+            dmin = fmin(dmin, d);
+        }
+        const double sv = n > 0 ? 0.5 * (1.0 - tanh(dmin / 2.0)) : 0.0;
+        s[q] = (T)sv;
+        l[q] = (T)(1.0 - sv);
+        c[q] = (T)(c_liq + c_span * sv);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T *__restrict__ p, size_t n, T v)
+{
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        p[q] = v;
+}

@@ -1,0 +1,227 @@
+// k_step_f64.h -- FP64 step kernels (exact: bitwise equal to the reference).
+//
+// One launch = one sweep of phasefield.c:301-403 over the whole grid, with
+// the ghost fill of phasefield.c:1112-1189 replaced by the index map
+// ghost<BC>.  Input and output are separate buffers (ping-pong); the
+// reference updates in place after computing every increment from the old
+// state (pass C, phasefield.c:393-401), which is the same thing.
+#pragma once
+
+#include "ppf_physics.h"
+#include "ppf_reduce.h"
+
+// Any coordinate (also beyond one cell outside the box: threads of a
+// partial tile) to a valid source index.
+template <int BC>
+__device__ __forceinline__ int ghost_any(int i, int n)
+{
+    if (BC == 0) {
+        int r = i % n;
+        return r < 0 ? r + n : r;
+    }
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+// ---------------------------------------------------------------------------
+// Naive form: one thread per site, every neighbour read from global memory and
+// its h, mu, M recomputed (7x the pointwise work).  Kept as an independent
+// second implementation for cross-checks.
+// ---------------------------------------------------------------------------
+template <int BC, bool EQA, bool CONLY, bool REPORT>
+__global__ __launch_bounds__(256) void k_naive_f64(
+    const double *__restrict__ s_in, const double *__restrict__ l_in,
+    const double *__restrict__ c_in, double *__restrict__ s_out, double *__restrict__ l_out,
+    double *__restrict__ c_out, int nx, int ny, int nz, KConst k, double *__restrict__ partials)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (x < nx && y < ny) {
+        const size_t plane = (size_t)nx * ny;
+        const size_t q = (size_t)z * plane + (size_t)y * nx + x;
+        // neighbour flat indices in reference order z+1, z-1, y+1, y-1, x+1, x-1
+        size_t nb[6];
+        nb[0] = (size_t)ghost<BC>(z + 1, nz) * plane + (size_t)y * nx + x;
+        nb[1] = (size_t)ghost<BC>(z - 1, nz) * plane + (size_t)y * nx + x;
+        nb[2] = (size_t)z * plane + (size_t)ghost<BC>(y + 1, ny) * nx + x;
+        nb[3] = (size_t)z * plane + (size_t)ghost<BC>(y - 1, ny) * nx + x;
+        nb[4] = (size_t)z * plane + (size_t)y * nx + ghost<BC>(x + 1, nx);
+        nb[5] = (size_t)z * plane + (size_t)y * nx + ghost<BC>(x - 1, nx);
+        const double s = s_in[q], l = l_in[q], c = c_in[q];
+        double h, mu, M;
+        pointwise<EQA>(s, l, c, k, h, mu, M);
+        double sn[6], ln[6], mun[6], Mn[6];
+        for (int a = 0; a < 6; a++) {
+            sn[a] = s_in[nb[a]];
+            ln[a] = l_in[nb[a]];
+            double hh;
+            pointwise<EQA>(sn[a], ln[a], c_in[nb[a]], k, hh, mun[a], Mn[a]);
+        }
+        if (CONLY) {
+            const double dc = dc_only(mu, M, mun, Mn, k);
+            c_out[q] = c + dc;
+            acc[0] = dc * dc;
+        } else {
+            double so, lo, co, ds, dl, dc;
+            site_update<EQA>(s, l, c, h, mu, M, sn, ln, mun, Mn, k, so, lo, co, ds, dl, dc);
+            s_out[q] = so;
+            l_out[q] = lo;
+            c_out[q] = co;
+            acc[0] = dc * dc;
+            acc[1] = ds * ds;
+            acc[2] = dl * dl;
+        }
+    }
+    if (REPORT) {
+        const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        block_sum3<256>(acc, partials + 3 * (size_t)bid);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled form.  A workgroup owns a TX x TY column tile of the (x, y) plane and
+// marches it through tz planes of z.  Per plane:
+//   * each thread loads (s, l, c) of its column one plane ahead and computes
+//     h, mu, M there once (z neighbours then come from registers);
+//   * the current plane's s, l, mu, M of the tile and of its one-cell ring
+//     (faces only; corners are never read) go to LDS, double buffered so
+//     one barrier per plane suffices;
+//   * every thread updates its site from LDS (x, y neighbours) and registers
+//     (z neighbours) and writes the new (s, l, c).
+// Compulsory HBM traffic is one read and one write of each field per site;
+// ring reads and the two extra planes per tz chunk come mostly from L2.
+// ---------------------------------------------------------------------------
+template <int TX, int TY, int BC, bool EQA, bool CONLY, bool REPORT>
+__global__ __launch_bounds__(TX *TY) void k_tiled_f64(
+    const double *__restrict__ s_in, const double *__restrict__ l_in,
+    const double *__restrict__ c_in, double *__restrict__ s_out, double *__restrict__ l_out,
+    double *__restrict__ c_out, int nx, int ny, int nz, int tz, int ntx, int nty, KConst k,
+    double *__restrict__ partials)
+{
+    constexpr int NT = TX * TY;
+    constexpr int PX = TX + 2;
+    constexpr int PY = TY + 2;
+    constexpr int NRING = 2 * TX + 2 * TY;
+    static_assert(NRING <= NT, "ring larger than the workgroup");
+    __shared__ double sh[2][4][PY][PX];
+
+    const int tid = threadIdx.x;
+    const int tx = tid % TX, ty = tid / TX;
+    const int b = blockIdx.x;
+    const int tile = b % (ntx * nty);
+    const int zc = b / (ntx * nty);
+    const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY, z0 = zc * tz;
+    const int zend = min(z0 + tz, nz);
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = x < nx && y < ny;
+    const size_t plane = (size_t)nx * ny;
+    const size_t col = (size_t)ghost_any<BC>(y, ny) * nx + ghost_any<BC>(x, nx);
+
+    // This thread's ring site, if any: left, right, bottom, top faces.
+    int rx = 0, ry = 0;
+    if (tid < TY) {
+        rx = 0;
+        ry = tid + 1;
+    } else if (tid < 2 * TY) {
+        rx = TX + 1;
+        ry = tid - TY + 1;
+    } else if (tid < 2 * TY + TX) {
+        rx = tid - 2 * TY + 1;
+        ry = 0;
+    } else if (tid < NRING) {
+        rx = tid - 2 * TY - TX + 1;
+        ry = TY + 1;
+    }
+    const bool ring = tid < NRING;
+    const size_t rcol =
+        (size_t)ghost_any<BC>(y0 + ry - 1, ny) * nx + ghost_any<BC>(x0 + rx - 1, nx);
+
+    double acc[3] = {0.0, 0.0, 0.0};
+
+    // z-1 and z planes of this column
+    double s_p, l_p, mu_p, M_p;
+    double s_c, l_c, c_c, h_c, mu_c, M_c;
+    {
+        const size_t qp = (size_t)ghost<BC>(z0 - 1, nz) * plane + col;
+        double hp;
+        s_p = s_in[qp];
+        l_p = l_in[qp];
+        pointwise<EQA>(s_p, l_p, c_in[qp], k, hp, mu_p, M_p);
+        const size_t qc = (size_t)z0 * plane + col;
+        s_c = s_in[qc];
+        l_c = l_in[qc];
+        c_c = c_in[qc];
+        pointwise<EQA>(s_c, l_c, c_c, k, h_c, mu_c, M_c);
+    }
+
+    for (int z = z0; z < zend; ++z) {
+        const int buf = (z - z0) & 1;
+        const size_t qn = (size_t)ghost<BC>(z + 1, nz) * plane + col;
+        const double s_n = s_in[qn], l_n = l_in[qn], c_n = c_in[qn];
+        double h_n, mu_n, M_n;
+        pointwise<EQA>(s_n, l_n, c_n, k, h_n, mu_n, M_n);
+
+        if (!CONLY) {
+            sh[buf][0][ty + 1][tx + 1] = s_c;
+            sh[buf][1][ty + 1][tx + 1] = l_c;
+        }
+        sh[buf][2][ty + 1][tx + 1] = mu_c;
+        sh[buf][3][ty + 1][tx + 1] = M_c;
+        if (ring) {
+            const size_t qr = (size_t)z * plane + rcol;
+            const double sr = s_in[qr], lr = l_in[qr];
+            double hr, mur, Mr;
+            pointwise<EQA>(sr, lr, c_in[qr], k, hr, mur, Mr);
+            if (!CONLY) {
+                sh[buf][0][ry][rx] = sr;
+                sh[buf][1][ry][rx] = lr;
+            }
+            sh[buf][2][ry][rx] = mur;
+            sh[buf][3][ry][rx] = Mr;
+        }
+        __syncthreads();
+
+        if (active) {
+            const size_t q = (size_t)z * plane + (size_t)y * nx + x;
+            const double mun[6] = {mu_n, mu_p, sh[buf][2][ty + 2][tx + 1], sh[buf][2][ty][tx + 1],
+                                   sh[buf][2][ty + 1][tx + 2], sh[buf][2][ty + 1][tx]};
+            const double Mn[6] = {M_n, M_p, sh[buf][3][ty + 2][tx + 1], sh[buf][3][ty][tx + 1],
+                                  sh[buf][3][ty + 1][tx + 2], sh[buf][3][ty + 1][tx]};
+            if (CONLY) {
+                const double dc = dc_only(mu_c, M_c, mun, Mn, k);
+                c_out[q] = c_c + dc;
+                if (REPORT)
+                    acc[0] += dc * dc;
+            } else {
+                const double sn[6] = {s_n, s_p, sh[buf][0][ty + 2][tx + 1], sh[buf][0][ty][tx + 1],
+                                      sh[buf][0][ty + 1][tx + 2], sh[buf][0][ty + 1][tx]};
+                const double ln[6] = {l_n, l_p, sh[buf][1][ty + 2][tx + 1], sh[buf][1][ty][tx + 1],
+                                      sh[buf][1][ty + 1][tx + 2], sh[buf][1][ty + 1][tx]};
+                double so, lo, co, ds, dl, dc;
+                site_update<EQA>(s_c, l_c, c_c, h_c, mu_c, M_c, sn, ln, mun, Mn, k, so, lo, co, ds,
+                                 dl, dc);
+                s_out[q] = so;
+                l_out[q] = lo;
+                c_out[q] = co;
+                if (REPORT) {
+                    acc[0] += dc * dc;
+                    acc[1] += ds * ds;
+                    acc[2] += dl * dl;
+                }
+            }
+        }
+        s_p = s_c;
+        l_p = l_c;
+        mu_p = mu_c;
+        M_p = M_c;
+        s_c = s_n;
+        l_c = l_n;
+        c_c = c_n;
+        h_c = h_n;
+        mu_c = mu_n;
+        M_c = M_n;
+    }
+    if (REPORT)
+        block_sum3<NT>(acc, partials + 3 * (size_t)blockIdx.x);
+}

@@ -1,0 +1,677 @@
+// ppf.hip -- libppf.so: context, launches and the C ABI of include/ppf.h.
+//
+// The fields live in HBM for the life of a context as two ping-pong copies
+// of (etaS, etaL, c), dense [nz][ny][nx] with x fastest.  One step is one
+// kernel launch on the context's stream; the ghost fill of the reference
+// (phasefield.c:1112-1189) is the kernel's index map.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ppf.h"
+#include "k_step_f64.h"
+#include "k_misc.h"
+
+namespace {
+
+constexpr int MAX_DIM = 1 << 16;
+
+struct TileCfg {
+    int tx, ty;
+};
+
+}  // namespace
+
+struct ppf_ctx {
+    ppf_desc d;
+    KConst k;
+    bool eqa;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    void *buf[2][3];  // [copy][field]: etaS, etaL, c
+    int cur[3];       // copy holding the current value of each field
+    size_t n;
+    size_t esz;
+    double *partials;
+    size_t partials_cap;  // in doubles
+    double *dres;         // 16 doubles of results
+    int64_t steps;
+    int kernel;           // resolved PPF_KERNEL_*
+    TileCfg tile;
+    int tz;
+    const char *kname;
+    char err[512];
+};
+
+static char g_create_err[512];
+
+static int set_err(ppf_ctx *ctx, int code, const char *fmt, ...)
+{
+    char *dst = ctx ? ctx->err : g_create_err;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(dst, 512, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_err((ctx), PPF_EHIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+static KConst make_kconst(const ppf_params &p)
+{
+    KConst k;
+    k.AS = p.AS;
+    k.AL = p.AL;
+    k.dA = p.AL - p.AS;
+    k.tmp5 = p.AS * p.AL;
+    k.tmp6 = p.cL0 - p.cS0;
+    k.cL0 = p.cL0;
+    k.cS0 = p.cS0;
+    k.ML = p.ML;
+    k.dM = p.MS - p.ML;
+    k.tmp1 = -1.0 * p.L * p.dt;
+    k.tmp2 = (p.AS * p.cS0 - p.AL * p.cL0);
+    k.tmp3 = p.dt * 0.5 * p.h2;
+    k.hAS = 0.5 * p.AS;
+    k.hAL = 0.5 * p.AL;
+    k.CS = p.CS;
+    k.CL = p.CL;
+    k.m = p.m;
+    k.g2 = 2.0 * p.gamma;
+    k.kh2 = p.kappa * p.h2;
+    k.K5 = k.tmp5 / p.AS;
+    k.invA = 1.0 / p.AS;
+    return k;
+}
+
+extern "C" int ppf_params_default(ppf_params *p)
+{
+    if (!p)
+        return PPF_EARG;
+    // phasefield.c:76-97, same literals and expression order.
+    const double AS = 1e+09;
+    const double AL = 1e+09;
+    const double CS = -1e+06;
+    const double CL = 1.e+07;
+    const double cS0 = 0.455;
+    const double cL0 = 0.978;
+    const double cSeq = 0.999;
+    const double cLeq = 0.476;
+    const double DL = 1.0e-14, DS = 0.0;
+    const double ML = DL / AL, MS = DS / AS;
+    const double sigma = 2.0;
+    const double gamma = 1.5;
+    const double dx = 1;
+    const double h2 = 1.0 / (dx * dx);
+    const double dt = 0.125 * dx * dx / DL;
+    const double l = 7.0 * dx;
+    const double m = 6.0 * sigma / l;
+    const double kappa = 3 * 0.25 * sigma * l;
+    const double L = 4.0 / 3.0 * m / kappa * (0.5 * (MS + ML) / ((cSeq - cLeq) * (cSeq - cLeq)));
+    p->MS = MS;
+    p->ML = ML;
+    p->AS = AS;
+    p->AL = AL;
+    p->cL0 = cL0;
+    p->cS0 = cS0;
+    p->CS = CS;
+    p->CL = CL;
+    p->L = L;
+    p->dt = dt;
+    p->h2 = h2;
+    p->m = m;
+    p->gamma = gamma;
+    p->kappa = kappa;
+    return PPF_OK;
+}
+
+// ----------------------------------------------------------------------------
+// FP64 launch table
+// ----------------------------------------------------------------------------
+typedef void (*tiled_fn)(const double *, const double *, const double *, double *, double *,
+                         double *, int, int, int, int, int, int, KConst, double *);
+typedef void (*naive_fn)(const double *, const double *, const double *, double *, double *,
+                         double *, int, int, int, KConst, double *);
+
+template <int TX, int TY>
+static tiled_fn pick_tiled(int bc, bool eqa, bool conly, bool report)
+{
+#define PPF_T(BC, E, C, R) \
+    if (bc == BC && eqa == E && conly == C && report == R) return k_tiled_f64<TX, TY, BC, E, C, R>;
+#define PPF_T4(BC, E) PPF_T(BC, E, false, false) PPF_T(BC, E, false, true) \
+                      PPF_T(BC, E, true, false) PPF_T(BC, E, true, true)
+    PPF_T4(0, true) PPF_T4(0, false) PPF_T4(1, true) PPF_T4(1, false)
+#undef PPF_T4
+#undef PPF_T
+    return nullptr;
+}
+
+static naive_fn pick_naive(int bc, bool eqa, bool conly, bool report)
+{
+#define PPF_N(BC, E, C, R) \
+    if (bc == BC && eqa == E && conly == C && report == R) return k_naive_f64<BC, E, C, R>;
+#define PPF_N4(BC, E) PPF_N(BC, E, false, false) PPF_N(BC, E, false, true) \
+                      PPF_N(BC, E, true, false) PPF_N(BC, E, true, true)
+    PPF_N4(0, true) PPF_N4(0, false) PPF_N4(1, true) PPF_N4(1, false)
+#undef PPF_N4
+#undef PPF_N
+    return nullptr;
+}
+
+static tiled_fn pick_tiled_shape(TileCfg t, int bc, bool eqa, bool conly, bool report)
+{
+    if (t.tx == 64 && t.ty == 8)
+        return pick_tiled<64, 8>(bc, eqa, conly, report);
+    if (t.tx == 32 && t.ty == 8)
+        return pick_tiled<32, 8>(bc, eqa, conly, report);
+    if (t.tx == 16 && t.ty == 16)
+        return pick_tiled<16, 16>(bc, eqa, conly, report);
+    return nullptr;
+}
+
+static void tiled_grid(const ppf_ctx *ctx, int &ntx, int &nty, int &nzc)
+{
+    ntx = (ctx->d.nx + ctx->tile.tx - 1) / ctx->tile.tx;
+    nty = (ctx->d.ny + ctx->tile.ty - 1) / ctx->tile.ty;
+    nzc = (ctx->d.nz + ctx->tz - 1) / ctx->tz;
+}
+
+static size_t report_blocks(const ppf_ctx *ctx)
+{
+    if (ctx->kernel == PPF_KERNEL_NAIVE)
+        return (size_t)((ctx->d.nx + 63) / 64) * ((ctx->d.ny + 3) / 4) * ctx->d.nz;
+    int ntx, nty, nzc;
+    tiled_grid(ctx, ntx, nty, nzc);
+    return (size_t)ntx * nty * nzc;
+}
+
+static int ensure_partials(ppf_ctx *ctx, size_t ndoubles)
+{
+    if (ndoubles <= ctx->partials_cap)
+        return PPF_OK;
+    if (ctx->partials)
+        HIPCHK(ctx, hipFree(ctx->partials));
+    ctx->partials = nullptr;
+    ctx->partials_cap = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->partials, ndoubles * sizeof(double)));
+    ctx->partials_cap = ndoubles;
+    return PPF_OK;
+}
+
+// Launch one sweep from copy cur to copy 1-cur; swaps the moved fields.
+static int launch_step(ppf_ctx *ctx, bool conly, bool report)
+{
+    const int nx = ctx->d.nx, ny = ctx->d.ny, nz = ctx->d.nz;
+    const int bc = ctx->d.bc;
+    double *part = report ? ctx->partials : nullptr;
+    const int cs = ctx->cur[0], cl = ctx->cur[1], cc = ctx->cur[2];
+    {
+        const double *si = (const double *)ctx->buf[cs][0];
+        const double *li = (const double *)ctx->buf[cl][1];
+        const double *ci = (const double *)ctx->buf[cc][2];
+        double *so = (double *)ctx->buf[1 - cs][0];
+        double *lo = (double *)ctx->buf[1 - cl][1];
+        double *co = (double *)ctx->buf[1 - cc][2];
+        if (ctx->kernel == PPF_KERNEL_NAIVE) {
+            naive_fn f = pick_naive(bc, ctx->eqa, conly, report);
+            dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
+            hipLaunchKernelGGL(f, grid, dim3(256), 0, ctx->stream, si, li, ci, so, lo, co, nx, ny,
+                               nz, ctx->k, part);
+        } else {
+            int ntx, nty, nzc;
+            tiled_grid(ctx, ntx, nty, nzc);
+            tiled_fn f = pick_tiled_shape(ctx->tile, bc, ctx->eqa, conly, report);
+            if (!f)
+                return set_err(ctx, PPF_EARG, "no kernel for tile %dx%d", ctx->tile.tx,
+                               ctx->tile.ty);
+            hipLaunchKernelGGL(f, dim3((unsigned)((size_t)ntx * nty * nzc)),
+                               dim3(ctx->tile.tx * ctx->tile.ty), 0, ctx->stream, si, li, ci, so,
+                               lo, co, nx, ny, nz, ctx->tz, ntx, nty, ctx->k, part);
+        }
+        HIPCHK(ctx, hipGetLastError());
+    }
+    if (!conly) {
+        ctx->cur[0] ^= 1;
+        ctx->cur[1] ^= 1;
+    }
+    ctx->cur[2] ^= 1;
+    ctx->steps++;
+    return PPF_OK;
+}
+
+static void choose_tiles(ppf_ctx *ctx)
+{
+    const int nx = ctx->d.nx;
+    if (nx >= 64)
+        ctx->tile = {64, 8};
+    else if (nx >= 32)
+        ctx->tile = {32, 8};
+    else
+        ctx->tile = {16, 16};
+    int ntx = (ctx->d.nx + ctx->tile.tx - 1) / ctx->tile.tx;
+    int nty = (ctx->d.ny + ctx->tile.ty - 1) / ctx->tile.ty;
+    // Enough workgroups to fill 256 CUs several times over, but chunks long
+    // enough that the two extra planes per chunk stay a few percent.
+    const long tiles = (long)ntx * nty;
+    long chunks = (8192 + tiles - 1) / tiles;
+    long maxchunks = (ctx->d.nz + 15) / 16;
+    if (chunks > maxchunks)
+        chunks = maxchunks;
+    if (chunks < 1)
+        chunks = 1;
+    ctx->tz = (int)((ctx->d.nz + chunks - 1) / chunks);
+}
+
+extern "C" int ppf_create(ppf_ctx **out, const ppf_desc *desc)
+{
+    if (!out || !desc)
+        return set_err(nullptr, PPF_EARG, "null argument");
+    *out = nullptr;
+    const ppf_desc &d = *desc;
+    if (d.nz < 1 || d.ny < 1 || d.nx < 1 || d.nz > MAX_DIM || d.ny > MAX_DIM || d.nx > MAX_DIM)
+        return set_err(nullptr, PPF_EARG, "bad grid %d x %d x %d", d.nz, d.ny, d.nx);
+    if (d.bc != PPF_BC_PERIODIC && d.bc != PPF_BC_NOFLUX)
+        return set_err(nullptr, PPF_EARG, "bad bc %d", d.bc);
+    if (d.precision != PPF_F64)
+        return set_err(nullptr, PPF_EARG, "bad precision %d", d.precision);
+    if (d.kernel < PPF_KERNEL_AUTO || d.kernel > PPF_KERNEL_TILED)
+        return set_err(nullptr, PPF_EARG, "bad kernel %d", d.kernel);
+    const size_t n = (size_t)d.nz * d.ny * d.nx;
+    if ((n >> 36) != 0)
+        return set_err(nullptr, PPF_EARG, "grid too large");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_err(nullptr, PPF_ENODEV, "no HIP device");
+    if (d.device < 0 || d.device >= ndev)
+        return set_err(nullptr, PPF_ENODEV, "device %d not present (%d devices)", d.device, ndev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d.device) != hipSuccess)
+        return set_err(nullptr, PPF_ENODEV, "cannot query device %d", d.device);
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_err(nullptr, PPF_ENODEV, "device %d is %s, this library is built for gfx950",
+                       d.device, prop.gcnArchName);
+    if (hipSetDevice(d.device) != hipSuccess)
+        return set_err(nullptr, PPF_EHIP, "hipSetDevice(%d) failed", d.device);
+
+    ppf_ctx *ctx = (ppf_ctx *)calloc(1, sizeof(ppf_ctx));
+    if (!ctx)
+        return set_err(nullptr, PPF_ENOMEM, "host allocation failed");
+    ctx->d = d;
+    ctx->n = n;
+    ctx->esz = d.precision == PPF_F64 ? sizeof(double) : sizeof(float);
+    ctx->k = make_kconst(d.params);
+    ctx->eqa = (d.params.AL == d.params.AS);
+    ctx->kernel = d.kernel == PPF_KERNEL_AUTO ? PPF_KERNEL_TILED : d.kernel;
+    choose_tiles(ctx);
+    ctx->kname = d.precision == PPF_F32 ? "k_step_f32"
+                 : ctx->kernel == PPF_KERNEL_NAIVE ? "k_naive_f64" : "k_tiled_f64";
+    int rc = PPF_OK;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipEventCreate(&ctx->ev0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&ctx->ev1);
+    if (e != hipSuccess) {
+        set_err(nullptr, PPF_EHIP, "stream/event creation failed: %s", hipGetErrorString(e));
+        ppf_destroy(ctx);
+        return PPF_EHIP;
+    }
+    for (int c = 0; c < 2 && rc == PPF_OK; c++)
+        for (int f = 0; f < 3 && rc == PPF_OK; f++) {
+            e = hipMalloc(&ctx->buf[c][f], n * ctx->esz);
+            if (e != hipSuccess)
+                rc = set_err(nullptr, PPF_ENOMEM, "hipMalloc of %zu bytes failed: %s",
+                             n * ctx->esz, hipGetErrorString(e));
+        }
+    if (rc == PPF_OK) {
+        e = hipMalloc(&ctx->dres, 16 * sizeof(double));
+        if (e != hipSuccess)
+            rc = set_err(nullptr, PPF_ENOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
+    }
+    if (rc != PPF_OK) {
+        ppf_destroy(ctx);
+        return rc;
+    }
+    // phasefield.c:156-167 starts every field at 1.0.
+    for (int f = 0; f < 3; f++) {
+        if (ctx->esz == 8)
+            hipLaunchKernelGGL(k_fill<double>, dim3(1024), dim3(256), 0, ctx->stream,
+                               (double *)ctx->buf[0][f], n, 1.0);
+        else
+            hipLaunchKernelGGL(k_fill<float>, dim3(1024), dim3(256), 0, ctx->stream,
+                               (float *)ctx->buf[0][f], n, 1.0f);
+    }
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        set_err(nullptr, PPF_EHIP, "init fill failed: %s", hipGetErrorString(e));
+        ppf_destroy(ctx);
+        return PPF_EHIP;
+    }
+    *out = ctx;
+    return PPF_OK;
+}
+
+extern "C" int ppf_destroy(ppf_ctx *ctx)
+{
+    if (!ctx)
+        return PPF_OK;
+    hipSetDevice(ctx->d.device);
+    if (ctx->stream)
+        hipStreamSynchronize(ctx->stream);
+    for (int c = 0; c < 2; c++)
+        for (int f = 0; f < 3; f++)
+            if (ctx->buf[c][f])
+                hipFree(ctx->buf[c][f]);
+    if (ctx->partials)
+        hipFree(ctx->partials);
+    if (ctx->dres)
+        hipFree(ctx->dres);
+    if (ctx->ev0)
+        hipEventDestroy(ctx->ev0);
+    if (ctx->ev1)
+        hipEventDestroy(ctx->ev1);
+    if (ctx->stream)
+        hipStreamDestroy(ctx->stream);
+    free(ctx);
+    return PPF_OK;
+}
+
+// Host <-> device for one field; FP32 contexts convert on the host.
+static int put_field(ppf_ctx *ctx, int f, const double *h)
+{
+    void *dst = ctx->buf[ctx->cur[f]][f];
+    if (ctx->esz == 8) {
+        HIPCHK(ctx, hipMemcpyAsync(dst, h, ctx->n * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return PPF_OK;
+    }
+    float *tmp = (float *)malloc(ctx->n * sizeof(float));
+    if (!tmp)
+        return set_err(ctx, PPF_ENOMEM, "host staging allocation failed");
+    for (size_t q = 0; q < ctx->n; q++)
+        tmp[q] = (float)h[q];
+    hipError_t e = hipMemcpyAsync(dst, tmp, ctx->n * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    free(tmp);
+    if (e != hipSuccess)
+        return set_err(ctx, PPF_EHIP, "upload failed: %s", hipGetErrorString(e));
+    return PPF_OK;
+}
+
+static int get_field(ppf_ctx *ctx, int f, double *h)
+{
+    const void *src = ctx->buf[ctx->cur[f]][f];
+    if (ctx->esz == 8) {
+        HIPCHK(ctx, hipMemcpyAsync(h, src, ctx->n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return PPF_OK;
+    }
+    float *tmp = (float *)malloc(ctx->n * sizeof(float));
+    if (!tmp)
+        return set_err(ctx, PPF_ENOMEM, "host staging allocation failed");
+    hipError_t e = hipMemcpyAsync(tmp, src, ctx->n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess)
+        for (size_t q = 0; q < ctx->n; q++)
+            h[q] = (double)tmp[q];
+    free(tmp);
+    if (e != hipSuccess)
+        return set_err(ctx, PPF_EHIP, "download failed: %s", hipGetErrorString(e));
+    return PPF_OK;
+}
+
+extern "C" int ppf_upload(ppf_ctx *ctx, const double *s, const double *l, const double *c)
+{
+    if (!ctx || !s || !c)
+        return ctx ? set_err(ctx, PPF_EARG, "null field pointer") : PPF_EARG;
+    HIPCHK(ctx, hipSetDevice(ctx->d.device));
+    int rc = put_field(ctx, 0, s);
+    if (rc)
+        return rc;
+    if (l) {
+        rc = put_field(ctx, 1, l);
+    } else {
+        if (ctx->esz == 8)
+            hipLaunchKernelGGL(k_fill<double>, dim3(1024), dim3(256), 0, ctx->stream,
+                               (double *)ctx->buf[ctx->cur[1]][1], ctx->n, 1.0);
+        else
+            hipLaunchKernelGGL(k_fill<float>, dim3(1024), dim3(256), 0, ctx->stream,
+                               (float *)ctx->buf[ctx->cur[1]][1], ctx->n, 1.0f);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    if (rc)
+        return rc;
+    rc = put_field(ctx, 2, c);
+    if (rc)
+        return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->steps = 0;
+    return PPF_OK;
+}
+
+extern "C" int ppf_download(ppf_ctx *ctx, double *s, double *l, double *c)
+{
+    if (!ctx)
+        return PPF_EARG;
+    HIPCHK(ctx, hipSetDevice(ctx->d.device));
+    int rc = PPF_OK;
+    if (s && (rc = get_field(ctx, 0, s)))
+        return rc;
+    if (l && (rc = get_field(ctx, 1, l)))
+        return rc;
+    if (c && (rc = get_field(ctx, 2, c)))
+        return rc;
+    return PPF_OK;
+}
+
+static int do_steps(ppf_ctx *ctx, int64_t nsteps, bool conly)
+{
+    if (!ctx)
+        return PPF_EARG;
+    if (nsteps < 0)
+        return set_err(ctx, PPF_EARG, "negative step count");
+    HIPCHK(ctx, hipSetDevice(ctx->d.device));
+    for (int64_t i = 0; i < nsteps; i++) {
+        int rc = launch_step(ctx, conly, false);
+        if (rc)
+            return rc;
+    }
+    return PPF_OK;
+}
+
+extern "C" int ppf_step(ppf_ctx *ctx, int64_t nsteps) { return do_steps(ctx, nsteps, false); }
+
+extern "C" int ppf_step_c_only(ppf_ctx *ctx, int64_t nsteps)
+{
+    return do_steps(ctx, nsteps, true);
+}
+
+extern "C" int ppf_step_report(ppf_ctx *ctx, int c_only, ppf_incsums *out)
+{
+    if (!ctx || !out)
+        return ctx ? set_err(ctx, PPF_EARG, "null argument") : PPF_EARG;
+    HIPCHK(ctx, hipSetDevice(ctx->d.device));
+    const size_t nb = report_blocks(ctx);
+    if (nb > (size_t)INT32_MAX)
+        return set_err(ctx, PPF_EARG, "too many workgroups");
+    int rc = ensure_partials(ctx, 3 * nb);
+    if (rc)
+        return rc;
+    rc = launch_step(ctx, c_only != 0, true);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(256), 0, ctx->stream, ctx->partials, (int)nb,
+                       ctx->dres);
+    HIPCHK(ctx, hipGetLastError());
+    double r[3];
+    HIPCHK(ctx, hipMemcpyAsync(r, ctx->dres, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    out->sum_dc2 = r[0];
+    // c-only sweeps leave detaS/detaL as they were (phasefield.c:406-476):
+    // report zero here; the caller keeps the earlier values if it needs them.
+    out->sum_ds2 = c_only ? 0.0 : r[1];
+    out->sum_dl2 = c_only ? 0.0 : r[2];
+    return PPF_OK;
+}
+
+extern "C" int ppf_reduce(ppf_ctx *ctx, ppf_stats *out)
+{
+    if (!ctx || !out)
+        return ctx ? set_err(ctx, PPF_EARG, "null argument") : PPF_EARG;
+    HIPCHK(ctx, hipSetDevice(ctx->d.device));
+    const int nb = 1024;
+    int rc = ensure_partials(ctx, (size_t)STAT_W * nb);
+    if (rc)
+        return rc;
+    if (ctx->esz == 8)
+        hipLaunchKernelGGL(k_stats<double>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const double *)ctx->buf[ctx->cur[0]][0],
+                           (const double *)ctx->buf[ctx->cur[1]][1],
+                           (const double *)ctx->buf[ctx->cur[2]][2], ctx->n, ctx->partials);
+    else
+        hipLaunchKernelGGL(k_stats<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const float *)ctx->buf[ctx->cur[0]][0],
+                           (const float *)ctx->buf[ctx->cur[1]][1],
+                           (const float *)ctx->buf[ctx->cur[2]][2], ctx->n, ctx->partials);
+    HIPCHK(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, ctx->stream, ctx->partials, nb,
+                       ctx->dres);
+    HIPCHK(ctx, hipGetLastError());
+    double r[8];
+    HIPCHK(ctx, hipMemcpyAsync(r, ctx->dres, 7 * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    out->sum_c = r[0];
+    out->sum_h = r[1];
+    out->n_solid = r[2];
+    out->sum_s = r[3];
+    out->min_c = r[4];
+    out->max_c = r[5];
+    out->n_nonfinite = r[6];
+    out->n_sites = (double)ctx->n;
+    return PPF_OK;
+}
+
+extern "C" int ppf_init_spheres(ppf_ctx *ctx, int n, const double *centers_zyx,
+                                const double *radii, double c_liq, double c_span)
+{
+    if (!ctx || n < 0 || (n > 0 && (!centers_zyx || !radii)))
+        return ctx ? set_err(ctx, PPF_EARG, "bad sphere arguments") : PPF_EARG;
+    HIPCHK(ctx, hipSetDevice(ctx->d.device));
+    double *host = (double *)malloc(sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
+    if (!host)
+        return set_err(ctx, PPF_ENOMEM, "host allocation failed");
+    for (int i = 0; i < n; i++) {
+        host[4 * i + 0] = centers_zyx[3 * i + 0];
+        host[4 * i + 1] = centers_zyx[3 * i + 1];
+        host[4 * i + 2] = centers_zyx[3 * i + 2];
+        host[4 * i + 3] = radii[i];
+    }
+    double *dev = nullptr;
+    hipError_t e = hipMalloc(&dev, sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
+    if (e == hipSuccess)
+        e = hipMemcpy(dev, host, sizeof(double) * 4 * (size_t)n, hipMemcpyHostToDevice);
+    free(host);
+    if (e != hipSuccess) {
+        if (dev)
+            hipFree(dev);
+        return set_err(ctx, PPF_EHIP, "sphere upload failed: %s", hipGetErrorString(e));
+    }
+    const unsigned nb = (unsigned)((ctx->n + 255) / 256 < 65536 ? (ctx->n + 255) / 256 : 65536);
+    if (ctx->esz == 8)
+        hipLaunchKernelGGL(k_init_spheres<double>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (double *)ctx->buf[ctx->cur[0]][0], (double *)ctx->buf[ctx->cur[1]][1],
+                           (double *)ctx->buf[ctx->cur[2]][2], ctx->d.nx, ctx->d.ny, ctx->d.nz,
+                           dev, n, c_liq, c_span);
+    else
+        hipLaunchKernelGGL(k_init_spheres<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (float *)ctx->buf[ctx->cur[0]][0], (float *)ctx->buf[ctx->cur[1]][1],
+                           (float *)ctx->buf[ctx->cur[2]][2], ctx->d.nx, ctx->d.ny, ctx->d.nz,
+                           dev, n, c_liq, c_span);
+    e = hipGetLastError();
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    hipFree(dev);
+    if (e != hipSuccess || e2 != hipSuccess)
+        return set_err(ctx, PPF_EHIP, "sphere kernel failed: %s",
+                       hipGetErrorString(e != hipSuccess ? e : e2));
+    ctx->steps = 0;
+    return PPF_OK;
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014) -> uniform double in [0, 1).
+static inline uint64_t splitmix64(uint64_t &x)
+{
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static inline double unit(uint64_t &x) { return (double)(splitmix64(x) >> 11) * 0x1.0p-53; }
+
+extern "C" int ppf_spheres_generate(uint64_t seed, int n, int nz, int ny, int nx, double rmin,
+                                    double rmax, double *centers_zyx, double *radii)
+{
+    if (n < 0 || (n > 0 && (!centers_zyx || !radii)) || nz < 1 || ny < 1 || nx < 1 ||
+        !(rmin <= rmax))
+        return PPF_EARG;
+    uint64_t st = seed;
+    for (int i = 0; i < n; i++) {
+        centers_zyx[3 * i + 0] = unit(st) * nz;
+        centers_zyx[3 * i + 1] = unit(st) * ny;
+        centers_zyx[3 * i + 2] = unit(st) * nx;
+        radii[i] = rmin + (rmax - rmin) * unit(st);
+    }
+    return PPF_OK;
+}
+
+extern "C" int ppf_timer_start(ppf_ctx *ctx)
+{
+    if (!ctx)
+        return PPF_EARG;
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    return PPF_OK;
+}
+
+extern "C" int ppf_timer_stop(ppf_ctx *ctx, double *ms)
+{
+    if (!ctx || !ms)
+        return PPF_EARG;
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
+    float f = 0.0f;
+    HIPCHK(ctx, hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+    *ms = (double)f;
+    return PPF_OK;
+}
+
+extern "C" int ppf_sync(ppf_ctx *ctx)
+{
+    if (!ctx)
+        return PPF_EARG;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return PPF_OK;
+}
+
+extern "C" int ppf_steps_taken(ppf_ctx *ctx, int64_t *out)
+{
+    if (!ctx || !out)
+        return PPF_EARG;
+    *out = ctx->steps;
+    return PPF_OK;
+}
+
+extern "C" const char *ppf_kernel_name(ppf_ctx *ctx) { return ctx ? ctx->kname : ""; }
+
+extern "C" const char *ppf_last_error(ppf_ctx *ctx) { return ctx ? ctx->err : g_create_err; }

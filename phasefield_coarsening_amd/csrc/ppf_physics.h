@@ -1,0 +1,115 @@
+// ppf_physics.h -- the site arithmetic of one explicit Moelans step in FP64,
+// shared by every FP64 kernel.
+//
+// Operand order follows phasefield.c:324-384 term by term (C evaluates left
+// to right), and the library is compiled with -ffp-contract=off, so each
+// product and sum is rounded on its own exactly as the reference's x86-64
+// build rounds it.  FP64 division on gfx950 is the correctly rounded IEEE
+// sequence, so the results are bitwise equal to the reference's.
+//
+// Neighbour order everywhere is the reference's: i+1, i-1, j+1, j-1, k+1,
+// k-1 = z+1, z-1, y+1, y-1, x+1, x-1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+// Constants hoisted out of the loops, each computed on the host with the
+// expression the reference uses (phasefield.c:322-323, 337-339, 351-358).
+struct KConst {
+    double AS, AL;
+    double dA;     // AL - AS
+    double tmp5;   // AS*AL
+    double tmp6;   // cL0 - cS0
+    double cL0, cS0;
+    double ML;
+    double dM;     // MS - ML
+    double tmp1;   // -1.0*L*dt
+    double tmp2;   // AS*cS0 - AL*cL0
+    double tmp3;   // dt*0.5*h2
+    double hAS;    // 0.5*AS
+    double hAL;    // 0.5*AL
+    double CS, CL;
+    double m;
+    double g2;     // 2.0*gamma
+    double kh2;    // kappa*h2
+    // With AL == AS the denominator AS + (AL-AS)*h is AS + (+-0) = AS for
+    // every finite h, so tmp5/(AS+(AL-AS)*h) and 1.0/(AS+(AL-AS)*h) are the
+    // constants below, bit for bit (a NaN h still propagates through the
+    // other factor).  The kernels use them when AL == AS (EQA = true).
+    double K5;     // tmp5/AS
+    double invA;   // 1.0/AS
+};
+
+// Pass A at one site (phasefield.c:327-333): hS, mu, M.
+template <bool EQA>
+__device__ __forceinline__ void pointwise(double s, double l, double c, const KConst &k,
+                                          double &h, double &mu, double &M)
+{
+    h = s * s / (s * s + l * l);
+    if (EQA)
+        mu = k.K5 * (c - k.cL0 + k.tmp6 * h);
+    else
+        mu = k.tmp5 / (k.AS + k.dA * h) * (c - k.cL0 + k.tmp6 * h);
+    M = k.ML + k.dM * h;
+}
+
+// Pass B + C at one site (phasefield.c:351-384, 396-398).
+//   sn, ln   : etaS, etaL at the six neighbours, reference order
+//   mun, Mn  : mu, M at the six neighbours, reference order
+// Returns the new (s, l, c) and the increments.
+template <bool EQA>
+__device__ __forceinline__ void site_update(double s, double l, double c, double h, double mu,
+                                            double M, const double sn[6], const double ln[6],
+                                            const double mun[6], const double Mn[6],
+                                            const KConst &k, double &s_new, double &l_new,
+                                            double &c_new, double &ds, double &dl, double &dc)
+{
+    const double tmp4 = EQA ? k.invA : 1.0 / (k.AS + k.dA * h);
+    const double cS = tmp4 * (k.AL * c + (1.0 - h) * k.tmp2);
+    const double cL = tmp4 * (k.AS * c - h * k.tmp2);
+    const double fS = k.hAS * (cS - k.cS0) * (cS - k.cS0) + k.CS;
+    const double fL = k.hAL * (cL - k.cL0) * (cL - k.cL0) + k.CL;
+    const double dmu = 2.0 * (s * l) / ((s * s + l * l) * (s * s + l * l)) *
+                       (fS - fL - mu * (cS - cL));
+    ds = k.tmp1 * (k.m * s * (s * s - 1.0 + k.g2 * l * l)
+                   - k.kh2 * (sn[0] + sn[1] + sn[2] + sn[3] + sn[4] + sn[5] - 6.0 * s)
+                   + l * dmu);
+    dl = k.tmp1 * (k.m * l * (l * l - 1.0 + k.g2 * s * s)
+                   - k.kh2 * (ln[0] + ln[1] + ln[2] + ln[3] + ln[4] + ln[5] - 6.0 * l)
+                   - s * dmu);
+    dc = k.tmp3 * (Mn[0] * (mun[0] - mu) -
+                   Mn[1] * (mu - mun[1]) +
+                   Mn[2] * (mun[2] - mu) -
+                   Mn[3] * (mu - mun[3]) +
+                   Mn[4] * (mun[4] - mu) -
+                   Mn[5] * (mu - mun[5]) +
+                   M * (mun[0] + mun[1] + mun[2] + mun[3] + mun[4] + mun[5] - 6.0 * mu));
+    s_new = s + ds;
+    l_new = l + dl;
+    c_new = c + dc;
+}
+
+// c-only update (phasefield.c:451-461, 470): eta frozen.
+__device__ __forceinline__ double dc_only(double mu, double M, const double mun[6],
+                                          const double Mn[6], const KConst &k)
+{
+    return k.tmp3 * (Mn[0] * (mun[0] - mu) -
+                     Mn[1] * (mu - mun[1]) +
+                     Mn[2] * (mun[2] - mu) -
+                     Mn[3] * (mu - mun[3]) +
+                     Mn[4] * (mun[4] - mu) -
+                     Mn[5] * (mu - mun[5]) +
+                     M * (mun[0] + mun[1] + mun[2] + mun[3] + mun[4] + mun[5] - 6.0 * mu));
+}
+
+// Ghost-cell index map of one axis.  Periodic (phasefield.c:1117-1146):
+// ghost 0 <- N, N+1 <- 1, i.e. -1 -> n-1 and n -> 0.  No-flux
+// (phasefield.c:1158-1187): ghost 0 <- 1, N+1 <- N, i.e. -1 -> 0 and
+// n -> n-1.  Only a one-cell excursion is ever needed.
+template <int BC>
+__device__ __forceinline__ int ghost(int i, int n)
+{
+    if (BC == 0)
+        return i < 0 ? i + n : (i >= n ? i - n : i);
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}

@@ -1,0 +1,227 @@
+"""Python front end of ``libppf.so`` (the C ABI of include/ppf.h).
+
+``Solver`` holds one device context: the fields (etaS, etaL, c) resident in
+HBM, stepped by the gfx950 kernels.  It mirrors the reference's stepping
+loop (phasefield.c:178-258): ``step`` is ghost fill + ``sweep_fields``
+(phasefield.c:186-215), ``step_c_only`` is ghost fill + ``sweep_fields_c``
+(phasefield.c:217), and ``step_report`` also returns the three save-time sums
+of phasefield.c:228-238.
+
+Every call goes through the native library; there is no CPU fallback.  A
+failure raises ``PpfError`` with the library's message.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+BC_PERIODIC = 0
+BC_NOFLUX = 1
+F64 = 0
+F32 = 1
+KERNEL_AUTO = 0
+KERNEL_NAIVE = 1
+KERNEL_TILED = 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class Params(ctypes.Structure):
+    """Arguments of sweep_fields (phasefield.c:309-314), in that order."""
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("MS", "ML", "AS", "AL", "cL0", "cS0", "CS", "CL", "L", "dt", "h2", "m", "gamma",
+                 "kappa")]
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("nz", ctypes.c_int), ("ny", ctypes.c_int), ("nx", ctypes.c_int),
+                ("bc", ctypes.c_int), ("precision", ctypes.c_int), ("kernel", ctypes.c_int),
+                ("device", ctypes.c_int), ("reserved", ctypes.c_int), ("params", Params)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("sum_c", "sum_h", "n_solid", "sum_s", "min_c", "max_c", "n_sites",
+                 "n_nonfinite")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class IncSums(ctypes.Structure):
+    _fields_ = [("sum_dc2", ctypes.c_double), ("sum_ds2", ctypes.c_double),
+                ("sum_dl2", ctypes.c_double)]
+
+
+class PpfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libppf error {code}: {msg}")
+        self.code = code
+
+
+# Every symbol include/ppf.h declares, with its ctypes signature.
+SIGNATURES = {
+    "ppf_params_default": (ctypes.c_int, [ctypes.POINTER(Params)]),
+    "ppf_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(Desc)]),
+    "ppf_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "ppf_upload": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "ppf_download": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "ppf_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "ppf_step_c_only": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "ppf_step_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(IncSums)]),
+    "ppf_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
+    "ppf_init_spheres": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                        ctypes.c_double]),
+    "ppf_spheres_generate": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_double, _dp, _dp]),
+    "ppf_timer_start": (ctypes.c_int, [ctypes.c_void_p]),
+    "ppf_timer_stop": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "ppf_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "ppf_steps_taken": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "ppf_kernel_name": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "ppf_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+}
+
+
+def lib():
+    L = _lib.load("libppf.so")
+    if not getattr(L, "_ppf_typed", False):
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        L._ppf_typed = True
+    return L
+
+
+def default_params():
+    p = Params()
+    rc = lib().ppf_params_default(ctypes.byref(p))
+    if rc:
+        raise PpfError(rc, "ppf_params_default")
+    return p
+
+
+def spheres(seed, n, shape, rmin, rmax):
+    """Seeded sphere list: (centers [n,3] as (z, y, x), radii [n]).  Host only."""
+    nz, ny, nx = shape
+    cen = np.zeros((max(n, 1), 3))
+    rad = np.zeros(max(n, 1))
+    rc = lib().ppf_spheres_generate(int(seed), int(n), nz, ny, nx, float(rmin), float(rmax),
+                                    cen.ctypes.data_as(_dp), rad.ctypes.data_as(_dp))
+    if rc:
+        raise PpfError(rc, "ppf_spheres_generate: bad arguments")
+    return cen[:n], rad[:n]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Solver:
+    """One device-resident grid.  shape = (nz, ny, nx), x fastest."""
+
+    def __init__(self, shape, bc=BC_PERIODIC, precision=F64, kernel=KERNEL_AUTO, device=0,
+                 params=None):
+        L = lib()
+        d = Desc()
+        d.nz, d.ny, d.nx = (int(v) for v in shape)
+        d.bc, d.precision, d.kernel, d.device = int(bc), int(precision), int(kernel), int(device)
+        d.params = params if params is not None else default_params()
+        h = ctypes.c_void_p()
+        rc = L.ppf_create(ctypes.byref(h), ctypes.byref(d))
+        if rc:
+            raise PpfError(rc, L.ppf_last_error(None).decode(errors="replace"))
+        self._h = h
+        self.shape = (d.nz, d.ny, d.nx)
+        self.bc = bc
+        self.precision = precision
+
+    def _check(self, rc):
+        if rc:
+            raise PpfError(rc, lib().ppf_last_error(self._h).decode(errors="replace"))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ppf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def kernel_name(self):
+        return lib().ppf_kernel_name(self._h).decode()
+
+    def upload(self, s, l, c):
+        """Set the state.  l=None leaves etaL = 1.0 everywhere (the state the
+        reference's reader produces, phasefield.c:1032)."""
+        arrs = []
+        for a in (s, l, c):
+            if a is None:
+                arrs.append(None)
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.shape != self.shape:
+                raise ValueError(f"field shape {a.shape} != grid {self.shape}")
+            arrs.append(a)
+        self._check(lib().ppf_upload(self._h, _ptr(arrs[0]),
+                                     _ptr(arrs[1]) if arrs[1] is not None else None,
+                                     _ptr(arrs[2])))
+
+    def download(self):
+        out = [np.empty(self.shape, np.float64) for _ in range(3)]
+        self._check(lib().ppf_download(self._h, *(_ptr(a) for a in out)))
+        return tuple(out)
+
+    def init_spheres(self, centers, radii, c_liq=0.1, c_span=0.8):
+        centers = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        radii = np.ascontiguousarray(radii, dtype=np.float64).reshape(-1)
+        if len(centers) != len(radii):
+            raise ValueError("centers and radii differ in length")
+        self._check(lib().ppf_init_spheres(self._h, len(radii), _ptr(centers), _ptr(radii),
+                                           float(c_liq), float(c_span)))
+
+    def step(self, nsteps=1):
+        self._check(lib().ppf_step(self._h, int(nsteps)))
+
+    def step_c_only(self, nsteps=1):
+        self._check(lib().ppf_step_c_only(self._h, int(nsteps)))
+
+    def step_report(self, c_only=False):
+        r = IncSums()
+        self._check(lib().ppf_step_report(self._h, int(bool(c_only)), ctypes.byref(r)))
+        return r.sum_dc2, r.sum_ds2, r.sum_dl2
+
+    def reduce(self):
+        st = Stats()
+        self._check(lib().ppf_reduce(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def sync(self):
+        self._check(lib().ppf_sync(self._h))
+
+    def timer_start(self):
+        self._check(lib().ppf_timer_start(self._h))
+
+    def timer_stop(self):
+        ms = ctypes.c_double()
+        self._check(lib().ppf_timer_stop(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    @property
+    def steps_taken(self):
+        v = ctypes.c_int64()
+        self._check(lib().ppf_steps_taken(self._h, ctypes.byref(v)))
+        return v.value

@@ -1,0 +1,178 @@
+"""The CPU oracle against the reference's own outputs and against what the
+mathematics fixes.
+
+Golden fixtures (tests/golden, made by tests/golden/make_golden.py from the
+reference program phasefield.c compiled in its serial configuration) pin the
+oracle bit for bit: states after 2..1000 sweeps, periodic and no-flux
+boundaries, etaL read or left at 1.0 (phasefield.c:1032), a non-cubic grid,
+and the c-only sweep (phasefield.c:406-476).  The closed-form tests pin the
+arithmetic independently of the reference's rounding.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+
+def _bc(m):
+    return O.BC_PERIODIC if m["bc"] == "periodic" else O.BC_NOFLUX
+
+
+@pytest.mark.parametrize("case", ["p16_s2", "p16_s10", "p16_s100", "p16_s1000", "pnc_s10",
+                                  "pnc_s100", "n16_s10", "nnc_s100", "p16compat_s10",
+                                  "nnc_compat_s10", "pnc_conly"])
+def test_oracle_bitwise_vs_reference(golden, case):
+    man, arrays = golden
+    m, z = man["cases"][case], arrays[case]
+    l0 = z["l0"] if m["etaL"] == "read" else None
+    for k, nsw in enumerate(m["sweeps_before_output"], start=1):
+        s, l, c, sums = O.run(z["s0"], l0, z["c0"], nsw, bc=_bc(m), n_iter_init=m["n_iter_init"])
+        for got, key in ((s, "s"), (l, "l"), (c, "c")):
+            np.testing.assert_array_equal(got, z[f"{key}{k}"], err_msg=f"{case} output {k} {key}")
+        # the reference's save-time line, phasefield.c:240 and :248
+        line = [ln for ln in m["stderr"] if ln.startswith(f"{nsw - 1}: ")][0]
+        want = "%ld: c=%10.4e; S=%10.4e; L=%10.4e\titer: %ld; err: %10.4e" % (
+            nsw - 1, sums[0], sums[1], sums[2], nsw - 1, np.sqrt(sums[0]))
+        assert line == want
+
+
+def test_reference_output_file_numbering(golden):
+    """Output k holds the state after k*n_save + 1 sweeps (phasefield.c:221,257)."""
+    man, _ = golden
+    for name, m in man["cases"].items():
+        if m["n_iter_init"] >= m["n_iter"]:
+            assert m["outputs"] == ["in_1.h5"]
+            assert m["sweeps_before_output"] == [m["n_save"] + 1]
+            assert m["stdout"] == ["in_1.h5"]
+            assert m["stderr"][0] == "The job name is in_"
+            assert m["stderr"][-1].startswith(f"Total steps: {m['n_iter']}; error: ")
+
+
+def test_planted_mistake_is_caught(golden):
+    """Mistakes planted in the oracle must break bitwise parity with the
+    reference: a dropped gradient term (kappa = 0), a one-ulp change of cS0,
+    and a mirror boundary where the periodic one belongs.  (One ulp of kappa
+    alone is invisible: the gradient term is scaled by -L*dt ~ -5e-11.)"""
+    man, arrays = golden
+    z = arrays["p16_s10"]
+    p = O.default_params()
+    p.kappa = 0.0
+    s, l, c, _ = O.run(z["s0"], z["l0"], z["c0"], 10, params=p)
+    assert not np.array_equal(s, z["s1"])
+    p = O.default_params()
+    p.cS0 = np.nextafter(p.cS0, 1.0)
+    s, l, c, _ = O.run(z["s0"], z["l0"], z["c0"], 10, params=p)
+    assert not np.array_equal(c, z["c1"])
+    s, l, c, _ = O.run(z["s0"], z["l0"], z["c0"], 10, bc=O.BC_NOFLUX)
+    assert not np.array_equal(c, z["c1"])
+
+
+def _uniform(shape, s, l, c):
+    return (np.full(shape, s), np.full(shape, l), np.full(shape, c))
+
+
+def test_pure_solid_and_pure_liquid_uniform_are_fixed_points():
+    shape = (6, 5, 4)
+    for bc in (O.BC_PERIODIC, O.BC_NOFLUX):
+        for s0, l0 in ((1.0, 0.0), (0.0, 1.0)):
+            s, l, c = _uniform(shape, s0, l0, 0.37)
+            s1, l1, c1, sums = O.run(s, l, c, 5, bc=bc)
+            np.testing.assert_array_equal(s1, s)
+            np.testing.assert_array_equal(l1, l)
+            np.testing.assert_array_equal(c1, c)
+            assert sums.tolist() == [0.0, 0.0, 0.0]
+
+
+def test_uniform_state_matches_single_site_formula():
+    """Uniform fields: the Laplacians vanish, so the increments are the
+    double-well and chemical terms of phasefield.c:351-373 at one site."""
+    p = O.default_params()
+    s0, l0, c0 = 0.7, 0.4, 0.6
+    s, l, c = _uniform((4, 4, 4), s0, l0, c0)
+    ds, dl, dc = O.increments(s, l, c)
+    A = p.AS
+    h = s0 * s0 / (s0 * s0 + l0 * l0)
+    mu = A * A / A * (c0 - p.cL0 + (p.cL0 - p.cS0) * h)
+    t2 = A * p.cS0 - A * p.cL0
+    cS = (A * c0 + (1 - h) * t2) / A
+    cL = (A * c0 - h * t2) / A
+    fS = 0.5 * A * (cS - p.cS0) ** 2 + p.CS
+    fL = 0.5 * A * (cL - p.cL0) ** 2 + p.CL
+    D = 2 * s0 * l0 / (s0 * s0 + l0 * l0) ** 2 * (fS - fL - mu * (cS - cL))
+    t1 = -p.L * p.dt
+    want_s = t1 * (p.m * s0 * (s0 * s0 - 1 + 2 * p.gamma * l0 * l0) + l0 * D)
+    want_l = t1 * (p.m * l0 * (l0 * l0 - 1 + 2 * p.gamma * s0 * s0) - s0 * D)
+    np.testing.assert_allclose(ds, want_s, rtol=1e-12)
+    np.testing.assert_allclose(dl, want_l, rtol=1e-12)
+    assert np.abs(dc).max() < 1e-20
+    # the collapsed chemistry for AS == AL: cS - cL = cS0 - cL0 and fS - fL = CS - CL
+    assert abs((cS - cL) - (p.cS0 - p.cL0)) < 1e-15
+    assert abs((fS - fL) - (p.CS - p.CL)) < 1e-6
+
+
+@pytest.mark.parametrize("axis,n,k", [(2, 16, 1), (2, 16, 3), (1, 12, 2), (0, 10, 1)])
+def test_pure_liquid_cosine_mode_decays_by_closed_form(axis, n, k):
+    """Pure liquid (s=0, l=1): M = ML, mu = AS*(c - cL0), and the flux form
+    reduces to dc = dt*ML*AS*h2 * Lap(c) = 0.125 * Lap(c).  A Fourier mode
+    of wavenumber k decays by 1 - 0.25*(1 - cos(2 pi k / n)) per step."""
+    shape = [4, 4, 4]
+    shape[axis] = n
+    shape = tuple(shape)
+    coord = np.indices(shape)[axis]
+    a, c0 = 0.01, 0.5
+    c = c0 + a * np.cos(2 * np.pi * k * coord / n)
+    s, l = np.zeros(shape), np.ones(shape)
+    steps = 7
+    s1, l1, c1, _ = O.run(s, l, c, steps)
+    g = 1 - 0.25 * (1 - np.cos(2 * np.pi * k / n))
+    np.testing.assert_allclose(c1, c0 + a * g ** steps * np.cos(2 * np.pi * k * coord / n),
+                               rtol=0, atol=1e-13)
+    np.testing.assert_array_equal(s1, s)
+    np.testing.assert_array_equal(l1, l)
+
+
+def test_periodic_translation_equivariance_bitwise(golden):
+    _, arrays = golden
+    z = arrays["pnc_s10"]
+    shift = (3, -5, 7)
+    rolled = [np.roll(z[k], shift, axis=(0, 1, 2)) for k in ("s0", "l0", "c0")]
+    a = O.run(z["s0"], z["l0"], z["c0"], 6)
+    b = O.run(*rolled, 6)
+    for x, y in zip(a[:3], b[:3]):
+        np.testing.assert_array_equal(np.roll(x, shift, axis=(0, 1, 2)), y)
+
+
+def test_axis_permutation_cubic():
+    """Swapping x and y on a cubic grid changes only the summation order."""
+    rng = np.random.default_rng(5)
+    s = rng.uniform(0.2, 0.8, (8, 8, 8))
+    l = 1 - s + rng.uniform(-0.05, 0.05, s.shape)
+    c = 0.1 + 0.8 * s
+    a = O.run(s, l, c, 3)
+    t = lambda x: np.ascontiguousarray(np.swapaxes(x, 1, 2))  # noqa: E731
+    b = O.run(t(s), t(l), t(c), 3)
+    for x, y in zip(a[:3], b[:3]):
+        np.testing.assert_allclose(t(x), y, rtol=0, atol=1e-13)
+
+
+def test_noflux_mirror_symmetry():
+    """No-flux ghosts mirror the face (phasefield.c:1158-1187), so a field
+    symmetric under x -> n-1-x stays symmetric."""
+    rng = np.random.default_rng(7)
+    half = rng.uniform(0.2, 0.8, (6, 5, 4))
+    s = np.concatenate([half, half[:, :, ::-1]], axis=2)
+    l = 1 - s
+    c = 0.1 + 0.8 * s
+    s1, l1, c1, _ = O.run(s, l, c, 4, bc=O.BC_NOFLUX)
+    for f in (s1, l1, c1):
+        np.testing.assert_allclose(f, f[:, :, ::-1], rtol=0, atol=1e-14)
+
+
+def test_solute_conserved_periodic(golden):
+    """The flux form (phasefield.c:374-384) telescopes: sum(dc) = 0 exactly in
+    real arithmetic, so sum(c) moves only by rounding."""
+    _, arrays = golden
+    z = arrays["pnc_s10"]
+    c0 = z["c0"].sum()
+    _, _, c, _ = O.run(z["s0"], z["l0"], z["c0"], 300)
+    assert abs(c.sum() - c0) / c0 < 1e-12
