@@ -43,6 +43,9 @@ enum {
 
 enum { PPF_BC_PERIODIC = 0, PPF_BC_NOFLUX = 1 };  /* USE_BC_TYPE, phasefield.c:15 */
 
+/* PPF_F64 reproduces the reference bit for bit.  PPF_F32 stores and steps
+ * the fields in single precision with the dimensionless form of the same
+ * scheme (requires AS == AL and ML > 0), halving the bytes per site. */
 enum { PPF_F64 = 0, PPF_F32 = 1 };
 
 /* Kernel selection for PPF_F64.  PPF_KERNEL_AUTO picks the fastest exact
@@ -133,6 +136,27 @@ int ppf_init_spheres(ppf_ctx *ctx, int n, const double *centers_zyx, const doubl
  * radii uniform in [rmin, rmax).  No GPU needed. */
 int ppf_spheres_generate(uint64_t seed, int n, int nz, int ny, int nx, double rmin, double rmax,
                          double *centers_zyx, double *radii);
+
+/* Column tiling of the tiled FP64 kernel: a workgroup owns a tx x ty tile
+ * of the (x, y) plane and marches tz planes of z.  Compiled shapes (tx, ty):
+ * (64,8) (64,16) (64,4) (32,16) (32,8) (16,16); any tz >= 1.  Results do
+ * not depend on the tiling (each site's arithmetic is fixed); only speed
+ * does.  No counterpart in the reference, whose cache blocking is compiled
+ * out (phasefield.c:14, 58-62). */
+int ppf_set_tiling(ppf_ctx *ctx, int tx, int ty, int tz);
+int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz);
+
+/* FP32 solute bookkeeping.  An FP32 context keeps an FP64 target for the
+ * total solute sum(c): the compensated sum of the FP64 input at
+ * ppf_upload, or of the stored field after ppf_init_spheres.  Every `every`
+ * steps (default 50; 0 switches it off) it sums c in FP64 on the device and
+ * shifts c uniformly back onto the target, rounding each site
+ * stochastically so that a shift far below one FP32 ulp still restores the
+ * total.  ppf_mass_target reports the target and how many shifts ran.  No
+ * counterpart in the reference, which is FP64 throughout and conserves
+ * sum(c) by its flux form (phasefield.c:374-384). */
+int ppf_set_mass_correction(ppf_ctx *ctx, int every);
+int ppf_mass_target(ppf_ctx *ctx, double *target, int64_t *n_shifts);
 
 /* Timing on the context stream: ppf_timer_start records a HIP event,
  * ppf_timer_stop records a second one, synchronises and returns the elapsed

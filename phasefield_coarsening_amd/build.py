@@ -1,6 +1,7 @@
 """Build recipes for the native parts, all in-tree.
 
     libppf.so      HIP kernels for gfx950 + the C ABI of include/ppf.h (hipcc)
+    libppfcal.so   copy kernel of known traffic for measurement calibration
     libppfio.so    HDF5 file contract of include/ppf_io.h (gcc + libhdf5)
     ppf            the drop-in stepping program (gcc, links both)
     oracle/liboracle.so  the CPU oracle (test infrastructure, gcc)
@@ -44,6 +45,13 @@ def build_libppf():
     return out
 
 
+def build_libppfcal():
+    out = os.path.join(PKG, "libppfcal.so")
+    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "calib.hip"), "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def have_hdf5():
     return os.path.exists(os.path.join(HDF5_DIR, "include", "hdf5.h")) and \
         os.path.exists(os.path.join(HDF5_DIR, "lib", "libhdf5.so"))
@@ -71,7 +79,7 @@ def build_oracle():
 
 
 def build_all(verbose=True):
-    built = [build_oracle(), build_libppf()]
+    built = [build_oracle(), build_libppf(), build_libppfcal()]
     if have_hdf5():
         built.append(build_libppfio())
         if os.path.exists(os.path.join(CSRC, "ppf_main.c")):

@@ -184,3 +184,102 @@ __global__ __launch_bounds__(256) void k_fill(T *__restrict__ p, size_t n, T v)
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
         p[q] = v;
 }
+
+// Double-double sum of a field: one (hi, lo) pair per workgroup.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sum_dd(const T *__restrict__ x, size_t n,
+                                                double *__restrict__ partials)
+{
+    DD acc = {0.0, 0.0};
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        acc = dd_add1(acc, (double)x[q]);
+    acc = wave_dd_sum(acc);
+    __shared__ double part[2][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        part[0][w] = acc.hi;
+        part[1][w] = acc.lo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD t = {part[0][0], part[1][0]};
+        for (int j = 1; j < 4; j++)
+            t = dd_add(t, DD{part[0][j], part[1][j]});
+        partials[2 * (size_t)blockIdx.x] = t.hi;
+        partials[2 * (size_t)blockIdx.x + 1] = t.lo;
+    }
+}
+
+// Combine nb (hi, lo) pairs; out[1] = the sum, out[0] = (target - sum)/n,
+// the per-site shift that puts sum(c) back on the FP64 target.
+__global__ __launch_bounds__(256) void k_mass_delta(const double *__restrict__ p, int nb,
+                                                    double target, double n,
+                                                    double *__restrict__ out)
+{
+    DD acc = {0.0, 0.0};
+    for (int j = threadIdx.x; j < nb; j += 256)
+        acc = dd_add(acc, DD{p[2 * (size_t)j], p[2 * (size_t)j + 1]});
+    acc = wave_dd_sum(acc);
+    __shared__ double part[2][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        part[0][w] = acc.hi;
+        part[1][w] = acc.lo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD t = {part[0][0], part[1][0]};
+        for (int j = 1; j < 4; j++)
+            t = dd_add(t, DD{part[0][j], part[1][j]});
+        const double sum = t.hi + t.lo;
+        // target - sum in double-double, then divided by n
+        double d, e;
+        two_sum(target, -t.hi, d, e);
+        out[0] = ((d + (e - t.lo))) / n;
+        out[1] = sum;
+    }
+}
+
+// 64-bit mix (SplitMix64 finaliser) for counter-based random numbers.
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// c += delta[0] with stochastic rounding to FP32: the exact value t = c +
+// delta lies between two adjacent floats lo <= t <= hi; pick hi with
+// probability (t - lo)/(hi - lo) from a hash of (site, seed).  The expected
+// new sum is exactly sum(c) + n*delta, so a shift far below one ulp per site
+// still moves the total (round-to-nearest would drop it).
+__global__ __launch_bounds__(256) void k_mass_shift(float *__restrict__ c, size_t n,
+                                                    const double *__restrict__ delta,
+                                                    uint64_t seed)
+{
+    const double d = delta[0];
+    if (d == 0.0 || !isfinite(d))
+        return;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        const double t = (double)c[q] + d;
+        const float f = (float)t;
+        const double fd = (double)f;
+        if (fd == t) {
+            c[q] = f;
+            continue;
+        }
+        float lo, hi;
+        if (fd < t) {
+            lo = f;
+            hi = nextafterf(f, INFINITY);
+        } else {
+            hi = f;
+            lo = nextafterf(f, -INFINITY);
+        }
+        const double p = (t - (double)lo) / ((double)hi - (double)lo);
+        const double r =
+            (double)(mix64(q * 0x9E3779B97F4A7C15ull + seed * 0xD1B54A32D192ED03ull + 1) >> 11) *
+            0x1.0p-53;
+        c[q] = r < p ? hi : lo;
+    }
+}

@@ -1,4 +1,5 @@
-// k_step_f64.h -- FP64 step kernels (exact: bitwise equal to the reference).
+// k_step.h -- step kernels: the naive FP64 form and the tiled form for any
+// physics policy (PhysF64: bitwise equal to the reference; PhysF32).
 //
 // One launch = one sweep of phasefield.c:301-403 over the whole grid, with
 // the ghost fill of phasefield.c:1112-1189 replaced by the index map
@@ -82,29 +83,34 @@ __global__ __launch_bounds__(256) void k_naive_f64(
 // ---------------------------------------------------------------------------
 // Tiled form.  A workgroup owns a TX x TY column tile of the (x, y) plane and
 // marches it through tz planes of z.  Per plane:
-//   * each thread loads (s, l, c) of its column one plane ahead and computes
-//     h, mu, M there once (z neighbours then come from registers);
+//   * each thread holds its column's s, l, mu, M for planes z-1, z and z+1
+//     in registers (z neighbours), computing h, mu, M once per site;
 //   * the current plane's s, l, mu, M of the tile and of its one-cell ring
-//     (faces only; corners are never read) go to LDS, double buffered so
-//     one barrier per plane suffices;
+//     (faces only; corners are never read) go to LDS, double buffered so one
+//     barrier per plane suffices;
 //   * every thread updates its site from LDS (x, y neighbours) and registers
 //     (z neighbours) and writes the new (s, l, c).
-// Compulsory HBM traffic is one read and one write of each field per site;
-// ring reads and the two extra planes per tz chunk come mostly from L2.
+// The march is software pipelined: the raw (s, l, c) of the column two
+// planes ahead and of the ring one plane ahead are loaded an iteration
+// before they are used, so each wave keeps two planes of loads in flight
+// while it computes.  Compulsory HBM traffic is one read and one write of
+// each field per site; ring reads and the extra planes of each tz chunk
+// come mostly from L2.
 // ---------------------------------------------------------------------------
-template <int TX, int TY, int BC, bool EQA, bool CONLY, bool REPORT>
-__global__ __launch_bounds__(TX *TY) void k_tiled_f64(
-    const double *__restrict__ s_in, const double *__restrict__ l_in,
-    const double *__restrict__ c_in, double *__restrict__ s_out, double *__restrict__ l_out,
-    double *__restrict__ c_out, int nx, int ny, int nz, int tz, int ntx, int nty, KConst k,
-    double *__restrict__ partials)
+template <class P, int TX, int TY, int BC, bool CONLY, bool REPORT>
+__global__ __launch_bounds__(TX *TY) void k_tiled(
+    const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
+    const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
+    typename P::T *__restrict__ l_out, typename P::T *__restrict__ c_out, int nx, int ny, int nz,
+    int tz, int ntx, int nty, typename P::K k, double *__restrict__ partials)
 {
     constexpr int NT = TX * TY;
     constexpr int PX = TX + 2;
     constexpr int PY = TY + 2;
     constexpr int NRING = 2 * TX + 2 * TY;
     static_assert(NRING <= NT, "ring larger than the workgroup");
-    __shared__ double sh[2][4][PY][PX];
+    using T = typename P::T;
+    __shared__ T sh[2][4][PY][PX];
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
@@ -139,28 +145,53 @@ __global__ __launch_bounds__(TX *TY) void k_tiled_f64(
 
     double acc[3] = {0.0, 0.0, 0.0};
 
-    // z-1 and z planes of this column
-    double s_p, l_p, mu_p, M_p;
-    double s_c, l_c, c_c, h_c, mu_c, M_c;
+    // Prologue: planes z0-1 and z0 computed, plane z0+1 and the ring of z0
+    // loaded raw.
+    T s_p, l_p, mu_p, M_p;
+    T s_c, l_c, c_c, h_c, mu_c, M_c;
+    T rs_n, rl_n, rc_n;                // raw column values one plane ahead
+    T rs_r = 0, rl_r = 0, rc_r = 0;    // raw ring values of the current plane
     {
         const size_t qp = (size_t)ghost<BC>(z0 - 1, nz) * plane + col;
-        double hp;
+        const size_t qc = (size_t)z0 * plane + col;
+        const size_t qn = (size_t)ghost_any<BC>(z0 + 1, nz) * plane + col;
         s_p = s_in[qp];
         l_p = l_in[qp];
-        pointwise<EQA>(s_p, l_p, c_in[qp], k, hp, mu_p, M_p);
-        const size_t qc = (size_t)z0 * plane + col;
+        const T c_p = c_in[qp];
         s_c = s_in[qc];
         l_c = l_in[qc];
         c_c = c_in[qc];
-        pointwise<EQA>(s_c, l_c, c_c, k, h_c, mu_c, M_c);
+        rs_n = s_in[qn];
+        rl_n = l_in[qn];
+        rc_n = c_in[qn];
+        if (ring) {
+            const size_t qr = (size_t)z0 * plane + rcol;
+            rs_r = s_in[qr];
+            rl_r = l_in[qr];
+            rc_r = c_in[qr];
+        }
+        T hp;
+        P::point(s_p, l_p, c_p, k, hp, mu_p, M_p);
+        P::point(s_c, l_c, c_c, k, h_c, mu_c, M_c);
     }
 
     for (int z = z0; z < zend; ++z) {
         const int buf = (z - z0) & 1;
-        const size_t qn = (size_t)ghost<BC>(z + 1, nz) * plane + col;
-        const double s_n = s_in[qn], l_n = l_in[qn], c_n = c_in[qn];
-        double h_n, mu_n, M_n;
-        pointwise<EQA>(s_n, l_n, c_n, k, h_n, mu_n, M_n);
+        const bool more = z + 1 < zend;
+        // prefetch: column two planes ahead, ring one plane ahead
+        T ps = 0, pl = 0, pc = 0, prs = 0, prl = 0, prc = 0;
+        if (more) {
+            const size_t qf = (size_t)ghost_any<BC>(z + 2, nz) * plane + col;
+            ps = s_in[qf];
+            pl = l_in[qf];
+            pc = c_in[qf];
+            if (ring) {
+                const size_t qr = (size_t)(z + 1) * plane + rcol;
+                prs = s_in[qr];
+                prl = l_in[qr];
+                prc = c_in[qr];
+            }
+        }
 
         if (!CONLY) {
             sh[buf][0][ty + 1][tx + 1] = s_c;
@@ -169,38 +200,39 @@ __global__ __launch_bounds__(TX *TY) void k_tiled_f64(
         sh[buf][2][ty + 1][tx + 1] = mu_c;
         sh[buf][3][ty + 1][tx + 1] = M_c;
         if (ring) {
-            const size_t qr = (size_t)z * plane + rcol;
-            const double sr = s_in[qr], lr = l_in[qr];
-            double hr, mur, Mr;
-            pointwise<EQA>(sr, lr, c_in[qr], k, hr, mur, Mr);
+            T hr, mur, Mr;
+            P::point(rs_r, rl_r, rc_r, k, hr, mur, Mr);
             if (!CONLY) {
-                sh[buf][0][ry][rx] = sr;
-                sh[buf][1][ry][rx] = lr;
+                sh[buf][0][ry][rx] = rs_r;
+                sh[buf][1][ry][rx] = rl_r;
             }
             sh[buf][2][ry][rx] = mur;
             sh[buf][3][ry][rx] = Mr;
         }
+        T h_n, mu_n, M_n;
+        P::point(rs_n, rl_n, rc_n, k, h_n, mu_n, M_n);
         __syncthreads();
 
         if (active) {
             const size_t q = (size_t)z * plane + (size_t)y * nx + x;
-            const double mun[6] = {mu_n, mu_p, sh[buf][2][ty + 2][tx + 1], sh[buf][2][ty][tx + 1],
+            const T mun[6] = {mu_n, mu_p, sh[buf][2][ty + 2][tx + 1], sh[buf][2][ty][tx + 1],
                                    sh[buf][2][ty + 1][tx + 2], sh[buf][2][ty + 1][tx]};
-            const double Mn[6] = {M_n, M_p, sh[buf][3][ty + 2][tx + 1], sh[buf][3][ty][tx + 1],
+            const T Mn[6] = {M_n, M_p, sh[buf][3][ty + 2][tx + 1], sh[buf][3][ty][tx + 1],
                                   sh[buf][3][ty + 1][tx + 2], sh[buf][3][ty + 1][tx]};
             if (CONLY) {
-                const double dc = dc_only(mu_c, M_c, mun, Mn, k);
+                const T dc = P::conly(mu_c, M_c, mun, Mn, k);
                 c_out[q] = c_c + dc;
                 if (REPORT)
-                    acc[0] += dc * dc;
+                    acc[0] += (double)dc * (double)dc;
             } else {
-                const double sn[6] = {s_n, s_p, sh[buf][0][ty + 2][tx + 1], sh[buf][0][ty][tx + 1],
+                const T sn[6] = {rs_n, s_p, sh[buf][0][ty + 2][tx + 1], sh[buf][0][ty][tx + 1],
                                       sh[buf][0][ty + 1][tx + 2], sh[buf][0][ty + 1][tx]};
-                const double ln[6] = {l_n, l_p, sh[buf][1][ty + 2][tx + 1], sh[buf][1][ty][tx + 1],
+                const T ln[6] = {rl_n, l_p, sh[buf][1][ty + 2][tx + 1], sh[buf][1][ty][tx + 1],
                                       sh[buf][1][ty + 1][tx + 2], sh[buf][1][ty + 1][tx]};
-                double so, lo, co, ds, dl, dc;
-                site_update<EQA>(s_c, l_c, c_c, h_c, mu_c, M_c, sn, ln, mun, Mn, k, so, lo, co, ds,
-                                 dl, dc);
+                T so, lo, co;
+                double ds, dl, dc;
+                P::update(s_c, l_c, c_c, h_c, mu_c, M_c, sn, ln, mun, Mn, k, so, lo, co, ds, dl,
+                          dc);
                 s_out[q] = so;
                 l_out[q] = lo;
                 c_out[q] = co;
@@ -215,12 +247,18 @@ __global__ __launch_bounds__(TX *TY) void k_tiled_f64(
         l_p = l_c;
         mu_p = mu_c;
         M_p = M_c;
-        s_c = s_n;
-        l_c = l_n;
-        c_c = c_n;
+        s_c = rs_n;
+        l_c = rl_n;
+        c_c = rc_n;
         h_c = h_n;
         mu_c = mu_n;
         M_c = M_n;
+        rs_n = ps;
+        rl_n = pl;
+        rc_n = pc;
+        rs_r = prs;
+        rl_r = prl;
+        rc_r = prc;
     }
     if (REPORT)
         block_sum3<NT>(acc, partials + 3 * (size_t)blockIdx.x);

@@ -13,7 +13,7 @@
 #include <string.h>
 
 #include "ppf.h"
-#include "k_step_f64.h"
+#include "k_step.h"
 #include "k_misc.h"
 
 namespace {
@@ -29,7 +29,14 @@ struct TileCfg {
 struct ppf_ctx {
     ppf_desc d;
     KConst k;
+    KConst32 k32;
     bool eqa;
+    // FP32 solute bookkeeping: FP64 target of sum(c), and a mean shift of c
+    // back onto it every shift_every steps (0: never).
+    double c_target;
+    int shift_every;
+    int since_shift;
+    int64_t n_shifts;
     hipStream_t stream;
     hipEvent_t ev0, ev1;
     void *buf[2][3];  // [copy][field]: etaS, etaL, c
@@ -134,23 +141,42 @@ extern "C" int ppf_params_default(ppf_params *p)
     return PPF_OK;
 }
 
+static KConst32 make_kconst32(const ppf_params &p)
+{
+    // the dimensionless FP32 constants (ppf_physics.h, PhysF32); valid for
+    // AS == AL, which ppf_create checks
+    const double tmp1 = -1.0 * p.L * p.dt;
+    const double tmp3 = p.dt * 0.5 * p.h2;
+    KConst32 k;
+    k.cL0 = (float)p.cL0;
+    k.tmp6 = (float)(p.cL0 - p.cS0);
+    k.rM = (float)((p.MS - p.ML) / p.ML);
+    k.a1 = (float)(tmp1 * p.m);
+    k.a2 = (float)(-tmp1 * p.kappa * p.h2);
+    k.g2 = (float)(2.0 * p.gamma);
+    k.b0 = (float)(tmp1 * (p.CS - p.CL));
+    k.b1 = (float)(tmp1 * p.AL * (p.cL0 - p.cS0));
+    k.d0 = (float)(tmp3 * p.ML * p.AL);
+    return k;
+}
+
 // ----------------------------------------------------------------------------
-// FP64 launch table
+// Launch tables
 // ----------------------------------------------------------------------------
-typedef void (*tiled_fn)(const double *, const double *, const double *, double *, double *,
-                         double *, int, int, int, int, int, int, KConst, double *);
+template <class P>
+using tiled_fn = void (*)(const typename P::T *, const typename P::T *, const typename P::T *,
+                          typename P::T *, typename P::T *, typename P::T *, int, int, int, int,
+                          int, int, typename P::K, double *);
 typedef void (*naive_fn)(const double *, const double *, const double *, double *, double *,
                          double *, int, int, int, KConst, double *);
 
-template <int TX, int TY>
-static tiled_fn pick_tiled(int bc, bool eqa, bool conly, bool report)
+template <class P, int TX, int TY>
+static tiled_fn<P> pick_tiled(int bc, bool conly, bool report)
 {
-#define PPF_T(BC, E, C, R) \
-    if (bc == BC && eqa == E && conly == C && report == R) return k_tiled_f64<TX, TY, BC, E, C, R>;
-#define PPF_T4(BC, E) PPF_T(BC, E, false, false) PPF_T(BC, E, false, true) \
-                      PPF_T(BC, E, true, false) PPF_T(BC, E, true, true)
-    PPF_T4(0, true) PPF_T4(0, false) PPF_T4(1, true) PPF_T4(1, false)
-#undef PPF_T4
+#define PPF_T(BC, C, R) \
+    if (bc == BC && conly == C && report == R) return k_tiled<P, TX, TY, BC, C, R>;
+    PPF_T(0, false, false) PPF_T(0, false, true) PPF_T(0, true, false) PPF_T(0, true, true)
+    PPF_T(1, false, false) PPF_T(1, false, true) PPF_T(1, true, false) PPF_T(1, true, true)
 #undef PPF_T
     return nullptr;
 }
@@ -167,14 +193,24 @@ static naive_fn pick_naive(int bc, bool eqa, bool conly, bool report)
     return nullptr;
 }
 
-static tiled_fn pick_tiled_shape(TileCfg t, int bc, bool eqa, bool conly, bool report)
+// Column-tile shapes compiled for the tiled kernel (TX x TY).
+static const TileCfg k_tiles[] = {{64, 8}, {64, 16}, {64, 4}, {32, 16}, {32, 8}, {16, 16}};
+
+template <class P>
+static tiled_fn<P> pick_tiled_shape(TileCfg t, int bc, bool conly, bool report)
 {
     if (t.tx == 64 && t.ty == 8)
-        return pick_tiled<64, 8>(bc, eqa, conly, report);
+        return pick_tiled<P, 64, 8>(bc, conly, report);
+    if (t.tx == 64 && t.ty == 16)
+        return pick_tiled<P, 64, 16>(bc, conly, report);
+    if (t.tx == 64 && t.ty == 4)
+        return pick_tiled<P, 64, 4>(bc, conly, report);
+    if (t.tx == 32 && t.ty == 16)
+        return pick_tiled<P, 32, 16>(bc, conly, report);
     if (t.tx == 32 && t.ty == 8)
-        return pick_tiled<32, 8>(bc, eqa, conly, report);
+        return pick_tiled<P, 32, 8>(bc, conly, report);
     if (t.tx == 16 && t.ty == 16)
-        return pick_tiled<16, 16>(bc, eqa, conly, report);
+        return pick_tiled<P, 16, 16>(bc, conly, report);
     return nullptr;
 }
 
@@ -207,6 +243,25 @@ static int ensure_partials(ppf_ctx *ctx, size_t ndoubles)
     return PPF_OK;
 }
 
+template <class P>
+static int launch_tiled(ppf_ctx *ctx, const typename P::K &k, bool conly, double *part)
+{
+    using T = typename P::T;
+    const int cs = ctx->cur[0], cl = ctx->cur[1], cc = ctx->cur[2];
+    int ntx, nty, nzc;
+    tiled_grid(ctx, ntx, nty, nzc);
+    tiled_fn<P> f = pick_tiled_shape<P>(ctx->tile, ctx->d.bc, conly, part != nullptr);
+    if (!f)
+        return set_err(ctx, PPF_EARG, "no kernel for tile %dx%d", ctx->tile.tx, ctx->tile.ty);
+    hipLaunchKernelGGL(f, dim3((unsigned)((size_t)ntx * nty * nzc)),
+                       dim3(ctx->tile.tx * ctx->tile.ty), 0, ctx->stream,
+                       (const T *)ctx->buf[cs][0], (const T *)ctx->buf[cl][1],
+                       (const T *)ctx->buf[cc][2], (T *)ctx->buf[1 - cs][0],
+                       (T *)ctx->buf[1 - cl][1], (T *)ctx->buf[1 - cc][2], ctx->d.nx, ctx->d.ny,
+                       ctx->d.nz, ctx->tz, ntx, nty, k, part);
+    return PPF_OK;
+}
+
 // Launch one sweep from copy cur to copy 1-cur; swaps the moved fields.
 static int launch_step(ppf_ctx *ctx, bool conly, bool report)
 {
@@ -214,37 +269,79 @@ static int launch_step(ppf_ctx *ctx, bool conly, bool report)
     const int bc = ctx->d.bc;
     double *part = report ? ctx->partials : nullptr;
     const int cs = ctx->cur[0], cl = ctx->cur[1], cc = ctx->cur[2];
-    {
-        const double *si = (const double *)ctx->buf[cs][0];
-        const double *li = (const double *)ctx->buf[cl][1];
-        const double *ci = (const double *)ctx->buf[cc][2];
-        double *so = (double *)ctx->buf[1 - cs][0];
-        double *lo = (double *)ctx->buf[1 - cl][1];
-        double *co = (double *)ctx->buf[1 - cc][2];
-        if (ctx->kernel == PPF_KERNEL_NAIVE) {
-            naive_fn f = pick_naive(bc, ctx->eqa, conly, report);
-            dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
-            hipLaunchKernelGGL(f, grid, dim3(256), 0, ctx->stream, si, li, ci, so, lo, co, nx, ny,
-                               nz, ctx->k, part);
-        } else {
-            int ntx, nty, nzc;
-            tiled_grid(ctx, ntx, nty, nzc);
-            tiled_fn f = pick_tiled_shape(ctx->tile, bc, ctx->eqa, conly, report);
-            if (!f)
-                return set_err(ctx, PPF_EARG, "no kernel for tile %dx%d", ctx->tile.tx,
-                               ctx->tile.ty);
-            hipLaunchKernelGGL(f, dim3((unsigned)((size_t)ntx * nty * nzc)),
-                               dim3(ctx->tile.tx * ctx->tile.ty), 0, ctx->stream, si, li, ci, so,
-                               lo, co, nx, ny, nz, ctx->tz, ntx, nty, ctx->k, part);
-        }
-        HIPCHK(ctx, hipGetLastError());
+    int rc = PPF_OK;
+    if (ctx->d.precision == PPF_F32) {
+        rc = launch_tiled<PhysF32>(ctx, ctx->k32, conly, part);
+    } else if (ctx->kernel == PPF_KERNEL_NAIVE) {
+        naive_fn f = pick_naive(bc, ctx->eqa, conly, report);
+        dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
+        hipLaunchKernelGGL(f, grid, dim3(256), 0, ctx->stream, (const double *)ctx->buf[cs][0],
+                           (const double *)ctx->buf[cl][1], (const double *)ctx->buf[cc][2],
+                           (double *)ctx->buf[1 - cs][0], (double *)ctx->buf[1 - cl][1],
+                           (double *)ctx->buf[1 - cc][2], nx, ny, nz, ctx->k, part);
+    } else if (ctx->eqa) {
+        rc = launch_tiled<PhysF64<true>>(ctx, ctx->k, conly, part);
+    } else {
+        rc = launch_tiled<PhysF64<false>>(ctx, ctx->k, conly, part);
     }
+    if (rc)
+        return rc;
+    HIPCHK(ctx, hipGetLastError());
     if (!conly) {
         ctx->cur[0] ^= 1;
         ctx->cur[1] ^= 1;
     }
     ctx->cur[2] ^= 1;
     ctx->steps++;
+    return PPF_OK;
+}
+
+// FP32 solute bookkeeping (no-op for FP64): after a step, every shift_every
+// steps, sum c in FP64 on the device and shift c back onto the FP64 target
+// with stochastic rounding (k_mass_shift), all on the context stream.
+static int after_step(ppf_ctx *ctx)
+{
+    if (ctx->d.precision != PPF_F32 || ctx->shift_every <= 0)
+        return PPF_OK;
+    if (++ctx->since_shift < ctx->shift_every)
+        return PPF_OK;
+    ctx->since_shift = 0;
+    const int nb = 1024;
+    int rc = ensure_partials(ctx, (size_t)2 * nb);
+    if (rc)
+        return rc;
+    float *c = (float *)ctx->buf[ctx->cur[2]][2];
+    hipLaunchKernelGGL(k_sum_dd<float>, dim3(nb), dim3(256), 0, ctx->stream, (const float *)c,
+                       ctx->n, ctx->partials);
+    hipLaunchKernelGGL(k_mass_delta, dim3(1), dim3(256), 0, ctx->stream, ctx->partials, nb,
+                       ctx->c_target, (double)ctx->n, ctx->dres + 8);
+    const unsigned g = (unsigned)((ctx->n + 255) / 256 < 65536 ? (ctx->n + 255) / 256 : 65536);
+    hipLaunchKernelGGL(k_mass_shift, dim3(g), dim3(256), 0, ctx->stream, c, ctx->n,
+                       (const double *)(ctx->dres + 8), (uint64_t)ctx->steps);
+    HIPCHK(ctx, hipGetLastError());
+    ctx->n_shifts++;
+    return PPF_OK;
+}
+
+// FP64 target of sum(c) from the device field (FP32 contexts).
+static int set_target_from_device(ppf_ctx *ctx)
+{
+    if (ctx->d.precision != PPF_F32)
+        return PPF_OK;
+    const int nb = 1024;
+    int rc = ensure_partials(ctx, (size_t)2 * nb);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_sum_dd<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                       (const float *)ctx->buf[ctx->cur[2]][2], ctx->n, ctx->partials);
+    hipLaunchKernelGGL(k_mass_delta, dim3(1), dim3(256), 0, ctx->stream, ctx->partials, nb, 0.0,
+                       1.0, ctx->dres + 8);
+    HIPCHK(ctx, hipGetLastError());
+    double r[2];
+    HIPCHK(ctx, hipMemcpyAsync(r, ctx->dres + 8, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->c_target = r[1];
+    ctx->since_shift = 0;
     return PPF_OK;
 }
 
@@ -281,8 +378,14 @@ extern "C" int ppf_create(ppf_ctx **out, const ppf_desc *desc)
         return set_err(nullptr, PPF_EARG, "bad grid %d x %d x %d", d.nz, d.ny, d.nx);
     if (d.bc != PPF_BC_PERIODIC && d.bc != PPF_BC_NOFLUX)
         return set_err(nullptr, PPF_EARG, "bad bc %d", d.bc);
-    if (d.precision != PPF_F64)
+    if (d.precision != PPF_F64 && d.precision != PPF_F32)
         return set_err(nullptr, PPF_EARG, "bad precision %d", d.precision);
+    if (d.precision == PPF_F32 && d.params.AS != d.params.AL)
+        return set_err(nullptr, PPF_EARG, "the FP32 form needs AS == AL");
+    if (d.precision == PPF_F32 && !(d.params.ML > 0.0))
+        return set_err(nullptr, PPF_EARG, "the FP32 form needs ML > 0");
+    if (d.precision == PPF_F32 && d.kernel == PPF_KERNEL_NAIVE)
+        return set_err(nullptr, PPF_EARG, "the naive kernel is FP64 only");
     if (d.kernel < PPF_KERNEL_AUTO || d.kernel > PPF_KERNEL_TILED)
         return set_err(nullptr, PPF_EARG, "bad kernel %d", d.kernel);
     const size_t n = (size_t)d.nz * d.ny * d.nx;
@@ -309,11 +412,13 @@ extern "C" int ppf_create(ppf_ctx **out, const ppf_desc *desc)
     ctx->n = n;
     ctx->esz = d.precision == PPF_F64 ? sizeof(double) : sizeof(float);
     ctx->k = make_kconst(d.params);
+    ctx->k32 = d.precision == PPF_F32 ? make_kconst32(d.params) : KConst32{};
+    ctx->shift_every = 50;
     ctx->eqa = (d.params.AL == d.params.AS);
     ctx->kernel = d.kernel == PPF_KERNEL_AUTO ? PPF_KERNEL_TILED : d.kernel;
     choose_tiles(ctx);
-    ctx->kname = d.precision == PPF_F32 ? "k_step_f32"
-                 : ctx->kernel == PPF_KERNEL_NAIVE ? "k_naive_f64" : "k_tiled_f64";
+    ctx->kname = d.precision == PPF_F32 ? "k_tiled<PhysF32>"
+                 : ctx->kernel == PPF_KERNEL_NAIVE ? "k_naive_f64" : "k_tiled<PhysF64>";
     int rc = PPF_OK;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess)
@@ -457,6 +562,18 @@ extern "C" int ppf_upload(ppf_ctx *ctx, const double *s, const double *l, const 
         return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->steps = 0;
+    if (ctx->d.precision == PPF_F32) {
+        // the FP64 solute target is the sum of the FP64 input, not of its
+        // FP32 rounding (Neumaier summation on the host)
+        double sum = 0.0, comp = 0.0;
+        for (size_t q = 0; q < ctx->n; q++) {
+            const double t = sum + c[q];
+            comp += fabs(sum) >= fabs(c[q]) ? (sum - t) + c[q] : (c[q] - t) + sum;
+            sum = t;
+        }
+        ctx->c_target = sum + comp;
+        ctx->since_shift = 0;
+    }
     return PPF_OK;
 }
 
@@ -484,6 +601,8 @@ static int do_steps(ppf_ctx *ctx, int64_t nsteps, bool conly)
     HIPCHK(ctx, hipSetDevice(ctx->d.device));
     for (int64_t i = 0; i < nsteps; i++) {
         int rc = launch_step(ctx, conly, false);
+        if (rc == PPF_OK)
+            rc = after_step(ctx);
         if (rc)
             return rc;
     }
@@ -522,7 +641,7 @@ extern "C" int ppf_step_report(ppf_ctx *ctx, int c_only, ppf_incsums *out)
     // report zero here; the caller keeps the earlier values if it needs them.
     out->sum_ds2 = c_only ? 0.0 : r[1];
     out->sum_dl2 = c_only ? 0.0 : r[2];
-    return PPF_OK;
+    return after_step(ctx);
 }
 
 extern "C" int ppf_reduce(ppf_ctx *ctx, ppf_stats *out)
@@ -606,7 +725,7 @@ extern "C" int ppf_init_spheres(ppf_ctx *ctx, int n, const double *centers_zyx,
         return set_err(ctx, PPF_EHIP, "sphere kernel failed: %s",
                        hipGetErrorString(e != hipSuccess ? e : e2));
     ctx->steps = 0;
-    return PPF_OK;
+    return set_target_from_device(ctx);
 }
 
 // SplitMix64 (Steele, Lea, Flood 2014) -> uniform double in [0, 1).
@@ -633,6 +752,56 @@ extern "C" int ppf_spheres_generate(uint64_t seed, int n, int nz, int ny, int nx
         centers_zyx[3 * i + 2] = unit(st) * nx;
         radii[i] = rmin + (rmax - rmin) * unit(st);
     }
+    return PPF_OK;
+}
+
+extern "C" int ppf_set_tiling(ppf_ctx *ctx, int tx, int ty, int tz)
+{
+    if (!ctx)
+        return PPF_EARG;
+    if (ctx->kernel != PPF_KERNEL_TILED)
+        return set_err(ctx, PPF_ESTATE, "tiling applies to the tiled kernels only");
+    bool known = false;
+    for (const TileCfg &c : k_tiles)
+        known = known || (c.tx == tx && c.ty == ty);
+    if (!known)
+        return set_err(ctx, PPF_EARG, "tile %dx%d is not compiled", tx, ty);
+    if (tz < 1)
+        return set_err(ctx, PPF_EARG, "tz must be >= 1");
+    ctx->tile = {tx, ty};
+    ctx->tz = tz > ctx->d.nz ? ctx->d.nz : tz;
+    return PPF_OK;
+}
+
+extern "C" int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz)
+{
+    if (!ctx || !tx || !ty || !tz)
+        return PPF_EARG;
+    *tx = ctx->tile.tx;
+    *ty = ctx->tile.ty;
+    *tz = ctx->tz;
+    return PPF_OK;
+}
+
+extern "C" int ppf_set_mass_correction(ppf_ctx *ctx, int every)
+{
+    if (!ctx || every < 0)
+        return ctx ? set_err(ctx, PPF_EARG, "bad interval %d", every) : PPF_EARG;
+    if (ctx->d.precision != PPF_F32)
+        return set_err(ctx, PPF_ESTATE, "mass correction applies to FP32 contexts only");
+    ctx->shift_every = every;
+    ctx->since_shift = 0;
+    return PPF_OK;
+}
+
+extern "C" int ppf_mass_target(ppf_ctx *ctx, double *target, int64_t *n_shifts)
+{
+    if (!ctx || !target || !n_shifts)
+        return PPF_EARG;
+    if (ctx->d.precision != PPF_F32)
+        return set_err(ctx, PPF_ESTATE, "mass bookkeeping applies to FP32 contexts only");
+    *target = ctx->c_target;
+    *n_shifts = ctx->n_shifts;
     return PPF_OK;
 }
 

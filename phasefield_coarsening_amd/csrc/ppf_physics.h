@@ -1,5 +1,5 @@
-// ppf_physics.h -- the site arithmetic of one explicit Moelans step in FP64,
-// shared by every FP64 kernel.
+// ppf_physics.h -- the site arithmetic of one explicit Moelans step: the
+// FP64 form that reproduces the reference bit for bit, and an FP32 form.
 //
 // Operand order follows phasefield.c:324-384 term by term (C evaluates left
 // to right), and the library is compiled with -ffp-contract=off, so each
@@ -113,3 +113,90 @@ __device__ __forceinline__ int ghost(int i, int n)
         return i < 0 ? i + n : (i >= n ? i - n : i);
     return i < 0 ? 0 : (i >= n ? n - 1 : i);
 }
+
+// ---------------------------------------------------------------------------
+// Physics policies for the tiled kernel.  Each provides the storage type T,
+// the per-site pass A (point), the stencil update (update) and the c-only
+// update (conly).
+// ---------------------------------------------------------------------------
+
+// FP64, bitwise equal to the reference (the functions above).
+template <bool EQA>
+struct PhysF64 {
+    using T = double;
+    using K = KConst;
+    static __device__ __forceinline__ void point(T s, T l, T c, const K &k, T &h, T &mu, T &M)
+    {
+        pointwise<EQA>(s, l, c, k, h, mu, M);
+    }
+    static __device__ __forceinline__ void update(T s, T l, T c, T h, T mu, T M, const T sn[6],
+                                                  const T ln[6], const T mun[6], const T Mn[6],
+                                                  const K &k, T &so, T &lo, T &co, double &ds,
+                                                  double &dl, double &dc)
+    {
+        site_update<EQA>(s, l, c, h, mu, M, sn, ln, mun, Mn, k, so, lo, co, ds, dl, dc);
+    }
+    static __device__ __forceinline__ T conly(T mu, T M, const T mun[6], const T Mn[6],
+                                              const K &k)
+    {
+        return dc_only(mu, M, mun, Mn, k);
+    }
+};
+
+// FP32 form for AS == AL.  With equal free-energy curvatures the chemistry
+// collapses (phasefield.c:351-359): cS - cL = cS0 - cL0, fS - fL = CS - CL
+// and mu = A*u with the dimensionless u = c - cL0 + (cL0 - cS0)*h.  Writing
+// M = ML*mm with mm = 1 + ((MS - ML)/ML)*h, one step is
+//   D  = 2 s l / (s^2 + l^2)^2 * (b0 + b1*u)
+//   ds = a1*s*(s^2 - 1 + 2 gamma l^2) + a2*Lap6(s) + l*D
+//   dl = a1*l*(l^2 - 1 + 2 gamma s^2) + a2*Lap6(l) - s*D
+//   dc = d0 * sum_nb (mm_nb + mm)*(u_nb - u)            (flux form)
+// with a1 = tmp1*m, a2 = -tmp1*kappa*h2, b0 = tmp1*(CS - CL),
+// b1 = tmp1*A*(cL0 - cS0), d0 = tmp3*ML*A, tmp1 = -L*dt, tmp3 = dt*h2/2.
+// Every quantity is O(1) or a small increment: no 1e8-sized free energies
+// are subtracted, which single precision could not resolve.  The flux
+// (mm_i + mm_j)*(u_j - u_i) is computed identically, up to sign, at both
+// ends of a face, so the grid sum of dc differs from zero only by the
+// rounding of each site's six-term sum and of the final products.
+struct KConst32 {
+    float cL0, tmp6, rM, a1, a2, g2, b0, b1, d0;
+};
+
+struct PhysF32 {
+    using T = float;
+    using K = KConst32;
+    static __device__ __forceinline__ void point(T s, T l, T c, const K &k, T &h, T &u, T &mm)
+    {
+        const float ss = s * s;
+        h = ss / (ss + l * l);
+        u = (c - k.cL0) + k.tmp6 * h;
+        mm = 1.0f + k.rM * h;
+    }
+    static __device__ __forceinline__ void update(T s, T l, T c, T h, T u, T mm, const T sn[6],
+                                                  const T ln[6], const T un[6], const T mn[6],
+                                                  const K &k, T &so, T &lo, T &co, double &ds_out,
+                                                  double &dl_out, double &dc_out)
+    {
+        (void)h;
+        const float ss = s * s, ll = l * l, den = ss + ll;
+        const float D = (2.0f * s * l) / (den * den) * (k.b0 + k.b1 * u);
+        const float lap_s = ((sn[0] + sn[1]) + (sn[2] + sn[3])) + (sn[4] + sn[5]) - 6.0f * s;
+        const float lap_l = ((ln[0] + ln[1]) + (ln[2] + ln[3])) + (ln[4] + ln[5]) - 6.0f * l;
+        const float ds = k.a1 * s * (ss - 1.0f + k.g2 * ll) + k.a2 * lap_s + l * D;
+        const float dl = k.a1 * l * (ll - 1.0f + k.g2 * ss) + k.a2 * lap_l - s * D;
+        const float dc = conly(u, mm, un, mn, k);
+        so = s + ds;
+        lo = l + dl;
+        co = c + dc;
+        ds_out = ds;
+        dl_out = dl;
+        dc_out = dc;
+    }
+    static __device__ __forceinline__ T conly(T u, T mm, const T un[6], const T mn[6], const K &k)
+    {
+        const float f = ((mn[0] + mm) * (un[0] - u) + (mn[1] + mm) * (un[1] - u)) +
+                        ((mn[2] + mm) * (un[2] - u) + (mn[3] + mm) * (un[3] - u)) +
+                        ((mn[4] + mm) * (un[4] - u) + (mn[5] + mm) * (un[5] - u));
+        return k.d0 * f;
+    }
+};

@@ -76,6 +76,10 @@ SIGNATURES = {
     "ppf_spheres_generate": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                             ctypes.c_double, _dp, _dp]),
+    "ppf_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "ppf_get_tiling": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3),
+    "ppf_set_mass_correction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ppf_mass_target": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "ppf_timer_start": (ctypes.c_int, [ctypes.c_void_p]),
     "ppf_timer_stop": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "ppf_sync": (ctypes.c_int, [ctypes.c_void_p]),
@@ -211,6 +215,25 @@ class Solver:
 
     def sync(self):
         self._check(lib().ppf_sync(self._h))
+
+    def set_tiling(self, tx, ty, tz):
+        self._check(lib().ppf_set_tiling(self._h, int(tx), int(ty), int(tz)))
+
+    def get_tiling(self):
+        v = [ctypes.c_int() for _ in range(3)]
+        self._check(lib().ppf_get_tiling(self._h, *(ctypes.byref(a) for a in v)))
+        return tuple(a.value for a in v)
+
+    def set_mass_correction(self, every):
+        """FP32 only: shift c back onto the FP64 solute target every `every`
+        steps (0 = never)."""
+        self._check(lib().ppf_set_mass_correction(self._h, int(every)))
+
+    def mass_target(self):
+        """FP32 only: (FP64 target of sum(c), number of shifts applied)."""
+        t, n = ctypes.c_double(), ctypes.c_int64()
+        self._check(lib().ppf_mass_target(self._h, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
 
     def timer_start(self):
         self._check(lib().ppf_timer_start(self._h))

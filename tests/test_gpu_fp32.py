@@ -1,0 +1,82 @@
+"""The FP32 form: bit-exact against its own float32 restatement for one
+step, and within stated tolerances of the FP64 path (which is bitwise the
+reference) over many steps, with FP64 solute bookkeeping."""
+import numpy as np
+import pytest
+
+from phasefield_coarsening_amd import solver as S
+from tests import fp32_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(shape, n, rmin, rmax, seed):
+    cen, rad = S.spheres(seed, n, shape, rmin, rmax)
+    with S.Solver(shape) as sv:
+        sv.init_spheres(cen, rad)
+        return sv.download()
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (20, 16, 12), (33, 17, 65)])
+def test_fp32_one_step_bitwise_vs_float32_restatement(shape):
+    s, l, c = (a.astype(np.float32) for a in _state(shape, 4, 2.5, 5.0, 3))
+    k = fp32_ref.constants(S.default_params())
+    want = fp32_ref.step(s, l, c, k)
+    with S.Solver(shape, precision=S.F32) as sv:
+        sv.set_mass_correction(0)
+        sv.upload(s.astype(np.float64), l.astype(np.float64), c.astype(np.float64))
+        sv.step(1)
+        got = sv.download()
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g.astype(np.float32), w)
+
+
+def test_fp32_vs_fp64_64cubed():
+    """200 steps at 64^3 from the 40-sphere state: FP32 fields within 2e-4 of
+    FP64 (max abs), solid fraction within 1e-4, total solute on its FP64
+    target to 1e-9 relative."""
+    shape = (64, 64, 64)
+    s0, l0, c0 = _state(shape, 40, 4.0, 8.0, 1)
+    with S.Solver(shape) as a:
+        a.upload(s0, l0, c0)
+        a.step(200)
+        ref = a.download()
+        st64 = a.reduce()
+    with S.Solver(shape, precision=S.F32) as b:
+        b.upload(s0, l0, c0)
+        b.step(200)
+        got = b.download()
+        st32 = b.reduce()
+        target, nshift = b.mass_target()
+    assert nshift == 4
+    for g, r in zip(got, ref):
+        assert np.abs(g - r).max() < 2e-4
+    assert abs(st32["sum_h"] - st64["sum_h"]) / st64["n_sites"] < 1e-4
+    assert abs(st32["sum_c"] - target) / target < 1e-9
+    assert abs(target - c0.sum()) / c0.sum() < 1e-14
+
+
+def test_fp32_mass_shift_restores_target():
+    """With the shift after every step the FP32 sum(c) sits on the FP64
+    target to about sqrt(n) ulps; with it off it drifts."""
+    shape = (48, 48, 48)
+    s0, l0, c0 = _state(shape, 20, 3.0, 6.0, 4)
+    with S.Solver(shape, precision=S.F32) as b:
+        b.set_mass_correction(1)
+        b.upload(s0, l0, c0)
+        b.step(100)
+        st = b.reduce()
+        target, nshift = b.mass_target()
+    assert nshift == 100
+    assert abs(st["sum_c"] - target) / target < 1e-10
+
+
+def test_fp32_report_sums_close_to_fp64():
+    shape = (32, 32, 32)
+    s0, l0, c0 = _state(shape, 8, 3.0, 6.0, 5)
+    with S.Solver(shape) as a, S.Solver(shape, precision=S.F32) as b:
+        a.upload(s0, l0, c0)
+        b.upload(s0, l0, c0)
+        ra = a.step_report()
+        rb = b.step_report()
+    np.testing.assert_allclose(rb, ra, rtol=1e-3)

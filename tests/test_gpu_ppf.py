@@ -1,0 +1,69 @@
+"""The ppf program end to end: same invocation, log lines, file numbering
+and file contents as the reference program (phasefield.c:73-269), checked
+against the reference's own runs stored in tests/golden (inputs, outputs,
+stdout and stderr)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from phasefield_coarsening_amd import _lib, h5
+
+pytestmark = pytest.mark.gpu
+PPF = os.path.join(_lib.REPO_DIR, "ppf")
+CASES = ["p16_s2", "p16_s10", "p16_s100", "pnc_s10", "n16_s10", "nnc_s100", "p16compat_s10",
+         "nnc_compat_s10", "pnc_conly"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_ppf_matches_reference_run(golden, case, tmp_path):
+    man, arrays = golden
+    m, z = man["cases"][case], arrays[case]
+    print(f"case={case}", flush=True)
+    h5.write(tmp_path / "in.h5", z["s0"], z["l0"], z["c0"])
+    args = [PPF, f"--bc={m['bc']}", f"--n-iter={m['n_iter']}", f"--n-save={m['n_save']}",
+            f"--n-iter-init={m['n_iter_init']}"]
+    if m["etaL"] != "read":
+        args.append("--etal-compat")
+    r = subprocess.run(args + ["in.h5"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.splitlines() == m["stdout"]
+    assert r.stderr.splitlines() == m["stderr"]
+    for k, name in enumerate(m["outputs"], start=1):
+        s, l, c = h5.read(tmp_path / name)
+        np.testing.assert_array_equal(s, z[f"s{k}"])
+        np.testing.assert_array_equal(l, z[f"l{k}"])
+        np.testing.assert_array_equal(c, z[f"c{k}"])
+
+
+def test_ppf_output_is_a_restart_input(golden, tmp_path):
+    """An output file is a valid input; 4 + 6 sweeps through a restart equal
+    10 sweeps in one run (periodic, /etaL read)."""
+    _, arrays = golden
+    z = arrays["p16_s10"]
+    h5.write(tmp_path / "a.h5", z["s0"], z["l0"], z["c0"])
+    base = [PPF, "--bc=periodic"]
+    subprocess.run(base + ["--n-iter=4", "--n-save=3", "a.h5"], cwd=tmp_path, check=True,
+                   capture_output=True, timeout=120)
+    os.rename(tmp_path / "a_1.h5", tmp_path / "b.h5")
+    subprocess.run(base + ["--n-iter=6", "--n-save=5", "b.h5"], cwd=tmp_path, check=True,
+                   capture_output=True, timeout=120)
+    s, l, c = h5.read(tmp_path / "b_1.h5")
+    np.testing.assert_array_equal(s, z["s1"])
+    np.testing.assert_array_equal(c, z["c1"])
+
+
+def test_ppf_stops_on_threshold(tmp_path):
+    """A uniform pure-liquid state has dc = 0, so err = 0 < 1e-12 at the
+    first save and the loop ends there (phasefield.c:179, 247)."""
+    shape = (4, 5, 6)
+    h5.write(tmp_path / "u.h5", np.zeros(shape), np.ones(shape), np.full(shape, 0.3))
+    r = subprocess.run([PPF, "--n-save=5", "u.h5"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0
+    lines = r.stderr.splitlines()
+    assert lines[1] == "5: c=0.0000e+00; S=0.0000e+00; L=0.0000e+00\titer: 5; err: 0.0000e+00"
+    assert lines[-1] == "Total steps: 6; error: 0.000000e+00"
+    assert r.stdout.splitlines() == ["u_1.h5"]
