@@ -89,6 +89,15 @@ typedef struct ppf_incsums {
     double sum_dc2, sum_ds2, sum_dl2;
 } ppf_incsums;
 
+/* Particles of the solid phase (ppf_particles). */
+typedef struct ppf_particle_stats {
+    int64_t n_particles;  /* connected components of {etaS > threshold} */
+    double solid_sites;   /* sites in them */
+    double mean_radius;   /* mean over particles of (3V/(4 pi))^(1/3), V in sites */
+    double max_radius;
+    double mean_volume;
+} ppf_particle_stats;
+
 typedef struct ppf_ctx ppf_ctx;
 
 /* Reference constants, phasefield.c:76-97. */
@@ -123,6 +132,15 @@ int ppf_step_report(ppf_ctx *ctx, int c_only, ppf_incsums *out);
 
 /* Device reductions over the current state (synchronises). */
 int ppf_reduce(ppf_ctx *ctx, ppf_stats *out);
+
+/* Particle analysis of the current state: label the connected components
+ * of {etaS > threshold} (6-neighbour connectivity, wrapping around a
+ * periodic box) on the device, then report their count, total volume and
+ * mean equivalent radius.  sizes (may be NULL with cap 0) receives up to cap
+ * component sizes in sites, largest first.  Synchronises.  No counterpart
+ * in the reference, which only reports the increment sums. */
+int ppf_particles(ppf_ctx *ctx, double threshold, ppf_particle_stats *out, int64_t *sizes,
+                  int64_t cap);
 
 /* Seeded sphere initial condition on the device: etaS = max over spheres of
  * 0.5*(1 - tanh((r - R)/2)) with periodic minimum-image distance r, etaL =

@@ -15,6 +15,10 @@
 #include "ppf.h"
 #include "k_step.h"
 #include "k_misc.h"
+#include "k_ccl.h"
+
+#include <algorithm>
+#include <vector>
 
 namespace {
 
@@ -37,6 +41,8 @@ struct ppf_ctx {
     int shift_every;
     int since_shift;
     int64_t n_shifts;
+    int *ccl_labels;  // per-site labels and counts for ppf_particles, on first use
+    int *ccl_counts;
     hipStream_t stream;
     hipEvent_t ev0, ev1;
     void *buf[2][3];  // [copy][field]: etaS, etaL, c
@@ -478,6 +484,10 @@ extern "C" int ppf_destroy(ppf_ctx *ctx)
                 hipFree(ctx->buf[c][f]);
     if (ctx->partials)
         hipFree(ctx->partials);
+    if (ctx->ccl_labels)
+        hipFree(ctx->ccl_labels);
+    if (ctx->ccl_counts)
+        hipFree(ctx->ccl_counts);
     if (ctx->dres)
         hipFree(ctx->dres);
     if (ctx->ev0)
@@ -780,6 +790,87 @@ extern "C" int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz)
     *tx = ctx->tile.tx;
     *ty = ctx->tile.ty;
     *tz = ctx->tz;
+    return PPF_OK;
+}
+
+extern "C" int ppf_particles(ppf_ctx *ctx, double threshold, ppf_particle_stats *out,
+                             int64_t *sizes, int64_t cap)
+{
+    if (!ctx || !out || cap < 0 || (cap > 0 && !sizes))
+        return ctx ? set_err(ctx, PPF_EARG, "bad particle arguments") : PPF_EARG;
+    if (ctx->n >= (size_t)INT32_MAX)
+        return set_err(ctx, PPF_EARG, "grid too large for 32-bit labels");
+    HIPCHK(ctx, hipSetDevice(ctx->d.device));
+    if (!ctx->ccl_labels) {
+        HIPCHK(ctx, hipMalloc(&ctx->ccl_labels, ctx->n * sizeof(int)));
+        hipError_t e = hipMalloc(&ctx->ccl_counts, ctx->n * sizeof(int));
+        if (e != hipSuccess) {
+            hipFree(ctx->ccl_labels);
+            ctx->ccl_labels = nullptr;
+            ctx->ccl_counts = nullptr;
+            return set_err(ctx, PPF_ENOMEM, "label allocation failed: %s", hipGetErrorString(e));
+        }
+    }
+    const unsigned g = (unsigned)((ctx->n + 255) / 256 < 65536 ? (ctx->n + 255) / 256 : 65536);
+    if (ctx->esz == 8)
+        hipLaunchKernelGGL(k_ccl_init<double>, dim3(g), dim3(256), 0, ctx->stream,
+                           (const double *)ctx->buf[ctx->cur[0]][0], ctx->ccl_labels,
+                           ctx->ccl_counts, ctx->n, threshold);
+    else
+        hipLaunchKernelGGL(k_ccl_init<float>, dim3(g), dim3(256), 0, ctx->stream,
+                           (const float *)ctx->buf[ctx->cur[0]][0], ctx->ccl_labels,
+                           ctx->ccl_counts, ctx->n, threshold);
+    if (ctx->d.bc == PPF_BC_PERIODIC)
+        hipLaunchKernelGGL(k_ccl_merge<0>, dim3(g), dim3(256), 0, ctx->stream, ctx->ccl_labels,
+                           ctx->d.nx, ctx->d.ny, ctx->d.nz);
+    else
+        hipLaunchKernelGGL(k_ccl_merge<1>, dim3(g), dim3(256), 0, ctx->stream, ctx->ccl_labels,
+                           ctx->d.nx, ctx->d.ny, ctx->d.nz);
+    hipLaunchKernelGGL(k_ccl_flatten, dim3(g), dim3(256), 0, ctx->stream, ctx->ccl_labels,
+                       ctx->ccl_counts, ctx->n);
+    unsigned long long *dn = (unsigned long long *)(ctx->dres + 12);
+    HIPCHK(ctx, hipMemsetAsync(dn, 0, sizeof *dn, ctx->stream));
+    hipLaunchKernelGGL(k_ccl_collect, dim3(g), dim3(256), 0, ctx->stream, ctx->ccl_labels,
+                       ctx->ccl_counts, ctx->n, dn, (int *)nullptr, 0ull);
+    HIPCHK(ctx, hipGetLastError());
+    unsigned long long nroots = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&nroots, dn, sizeof nroots, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<int> host(nroots);
+    if (nroots > 0) {
+        int *list = nullptr;
+        HIPCHK(ctx, hipMalloc(&list, nroots * sizeof(int)));
+        hipError_t e = hipMemsetAsync(dn, 0, sizeof *dn, ctx->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_ccl_collect, dim3(g), dim3(256), 0, ctx->stream, ctx->ccl_labels,
+                               ctx->ccl_counts, ctx->n, dn, list, nroots);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(host.data(), list, nroots * sizeof(int), hipMemcpyDeviceToHost,
+                               ctx->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(ctx->stream);
+        hipFree(list);
+        if (e != hipSuccess)
+            return set_err(ctx, PPF_EHIP, "particle collection failed: %s", hipGetErrorString(e));
+    }
+    // sizes in descending order: the order of the device list is arbitrary
+    std::sort(host.begin(), host.end(), [](int a, int b) { return a > b; });
+    double vol = 0.0, rsum = 0.0, rmax = 0.0;
+    for (int v : host) {
+        const double r = cbrt(3.0 * (double)v / (4.0 * M_PI));
+        vol += v;
+        rsum += r;
+        rmax = r > rmax ? r : rmax;
+    }
+    out->n_particles = (int64_t)nroots;
+    out->solid_sites = vol;
+    out->mean_radius = nroots ? rsum / (double)nroots : 0.0;
+    out->max_radius = rmax;
+    out->mean_volume = nroots ? vol / (double)nroots : 0.0;
+    for (int64_t i = 0; i < cap && i < (int64_t)nroots; i++)
+        sizes[i] = host[(size_t)i];
     return PPF_OK;
 }
 

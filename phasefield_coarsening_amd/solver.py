@@ -49,6 +49,15 @@ class Stats(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class ParticleStats(ctypes.Structure):
+    _fields_ = [("n_particles", ctypes.c_int64), ("solid_sites", ctypes.c_double),
+                ("mean_radius", ctypes.c_double), ("max_radius", ctypes.c_double),
+                ("mean_volume", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class IncSums(ctypes.Structure):
     _fields_ = [("sum_dc2", ctypes.c_double), ("sum_ds2", ctypes.c_double),
                 ("sum_dl2", ctypes.c_double)]
@@ -78,6 +87,9 @@ SIGNATURES = {
                                             ctypes.c_double, _dp, _dp]),
     "ppf_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "ppf_get_tiling": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3),
+    "ppf_particles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
+                                     ctypes.POINTER(ParticleStats),
+                                     ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]),
     "ppf_set_mass_correction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ppf_mass_target": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "ppf_timer_start": (ctypes.c_int, [ctypes.c_void_p]),
@@ -223,6 +235,17 @@ class Solver:
         v = [ctypes.c_int() for _ in range(3)]
         self._check(lib().ppf_get_tiling(self._h, *(ctypes.byref(a) for a in v)))
         return tuple(a.value for a in v)
+
+    def particles(self, threshold=0.5, max_sizes=1 << 20):
+        """Connected solid particles {etaS > threshold}: (stats dict, sizes in
+        sites, largest first)."""
+        st = ParticleStats()
+        sizes = np.zeros(max(int(max_sizes), 1), np.int64)
+        self._check(lib().ppf_particles(self._h, float(threshold), ctypes.byref(st),
+                                        sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                        int(max_sizes)))
+        d = st.as_dict()
+        return d, sizes[:min(d["n_particles"], int(max_sizes))]
 
     def set_mass_correction(self, every):
         """FP32 only: shift c back onto the FP64 solute target every `every`

@@ -173,3 +173,21 @@ def test_full_size_512_properties():
         gr = r.download()
     for x, y in zip(ga, gr):
         np.testing.assert_array_equal(np.roll(x, shift, axis=(0, 1, 2)), y)
+
+
+@pytest.mark.parametrize("bc", [S.BC_PERIODIC, S.BC_NOFLUX])
+@pytest.mark.parametrize("shape", [(40, 36, 44), (17, 64, 9)])
+def test_particles_match_cpu_labelling(shape, bc):
+    from tests.ccl_ref import component_sizes, mean_radius
+    print(f"particles shape={shape} bc={bc}", flush=True)
+    cen, rad = S.spheres(11, 25, shape, 1.5, 4.5)
+    with S.Solver(shape, bc=bc) as sv:
+        sv.init_spheres(cen, rad)
+        sv.step(30)
+        s, _, _ = sv.download()
+        st, sizes = sv.particles(0.5)
+    want = component_sizes(s > 0.5, periodic=(bc == S.BC_PERIODIC))
+    assert st["n_particles"] == len(want)
+    np.testing.assert_array_equal(sizes, want)
+    assert st["solid_sites"] == (s > 0.5).sum()
+    assert abs(st["mean_radius"] - mean_radius(want)) < 1e-12
