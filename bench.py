@@ -45,6 +45,22 @@ def traffic_from_profiles(kernel, workload):
         return None
 
 
+def copy_ceiling(nbytes=3 << 30, iters=20):
+    """Measured device copy rate (read + write of nbytes) with the plain
+    streaming kernel of libppfcal.so: the achievable HBM ceiling."""
+    import ctypes
+
+    from phasefield_coarsening_amd import _lib
+    cal = _lib.load("libppfcal.so")
+    cal.ppfcal_copy.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_double)]
+    ms = ctypes.c_double()
+    rc = cal.ppfcal_copy(0, nbytes, iters, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"ppfcal_copy failed ({rc})")
+    return 2 * nbytes / (ms.value * 1e-3) / 1e9
+
+
 def run_gpu(S, shape, precision, steps, warmup, seed=2, nsph=2000, rmin=6.0, rmax=14.0):
     cen, rad = S.spheres(seed, nsph, shape, rmin, rmax)
     sv = S.Solver(shape, precision=precision)
@@ -60,8 +76,11 @@ def run_gpu(S, shape, precision, steps, warmup, seed=2, nsph=2000, rmin=6.0, rma
     wall = time.perf_counter() - t0
     st1 = sv.reduce()
     kname = sv.kernel_name
+    tiling = sv.get_tiling()
+    target = sv.mass_target()[0] if precision == S.F32 else None
     sv.close()
-    return {"wall_s": wall, "event_ms": ev_ms, "kernel": kname, "stats0": st0, "stats1": st1}
+    return {"wall_s": wall, "event_ms": ev_ms, "kernel": kname, "stats0": st0, "stats1": st1,
+            "tiling": tiling, "c_target": target}
 
 
 def cpu_baseline(S, shape, nsteps):
@@ -85,9 +104,10 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32", action="store_true")
+    ap.add_argument("--no-copy", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -126,6 +146,22 @@ def main():
         "solute_drift_rel": abs(r["stats1"]["sum_c"] - r["stats0"]["sum_c"]) /
         abs(r["stats0"]["sum_c"]),
     }
+    out["config"]["tiling"] = {"tx": r["tiling"][0], "ty": r["tiling"][1], "tz": r["tiling"][2],
+                               "order": r["tiling"][3]}
+    if not a.no_f32:
+        r32 = run_gpu(S, shape, S.F32, a.steps, a.warmup)
+        g32 = sites * a.steps / r32["wall_s"] / 1e9
+        ach32 = BYTES_PER_SITE["f32"] * sites / (r32["event_ms"] * 1e-3 / a.steps) / 1e9
+        out["f32"] = {"GLUPS": round(g32, 3), "ms_per_step": round(r32["wall_s"] * 1e3 / a.steps, 4),
+                      "vs_f64": round(g32 / glups, 3), "kernel": r32["kernel"],
+                      "roofline_achieved_GBs": round(ach32, 1),
+                      "roofline_frac": round(ach32 / HBM_PEAK_GBS, 4),
+                      "solute_vs_fp64_target_rel":
+                          abs(r32["stats1"]["sum_c"] - r32["c_target"]) / abs(r32["c_target"])}
+    if not a.no_copy:
+        cc = copy_ceiling()
+        out["roofline"]["copy_ceiling_GBs"] = round(cc, 1)
+        out["roofline"]["frac_of_copy_ceiling"] = round(achieved / cc, 4)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(S, shape, a.cpu_steps)
     print(json.dumps(out), flush=True)

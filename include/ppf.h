@@ -155,14 +155,26 @@ int ppf_init_spheres(ppf_ctx *ctx, int n, const double *centers_zyx, const doubl
 int ppf_spheres_generate(uint64_t seed, int n, int nz, int ny, int nx, double rmin, double rmax,
                          double *centers_zyx, double *radii);
 
-/* Column tiling of the tiled FP64 kernel: a workgroup owns a tx x ty tile
- * of the (x, y) plane and marches tz planes of z.  Compiled shapes (tx, ty):
- * (64,8) (64,16) (64,4) (32,16) (32,8) (16,16); any tz >= 1.  Results do
+/* Column tiling of the tiled kernels: a workgroup owns a tx x ty tile of
+ * the (x, y) plane and marches tz planes of z.  Compiled shapes (tx, ty):
+ * (64,8) (64,16) (64,4) (32,16) (32,8) (16,16); any tz >= 1.  order picks
+ * the workgroup order: bit 0 set = the z chunks of one tile are adjacent
+ * (else the tiles of one z chunk are); bit 1 set = each XCD gets one
+ * contiguous run of that order instead of every 8th workgroup; bit 2 set =
+ * the unpipelined z march (fewer registers, loads not issued a plane
+ * ahead).  Results do
  * not depend on the tiling (each site's arithmetic is fixed); only speed
  * does.  No counterpart in the reference, whose cache blocking is compiled
  * out (phasefield.c:14, 58-62). */
-int ppf_set_tiling(ppf_ctx *ctx, int tx, int ty, int tz);
-int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz);
+int ppf_set_tiling(ppf_ctx *ctx, int tx, int ty, int tz, int order);
+int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz, int *order);
+
+/* Time every compiled tiling (tile shape x tz in {16,32,64} x order in
+ * {0,1,2,4,5,6}) on the current state, `reps` sweeps each, and keep the
+ * fastest (ppf_get_tiling reads it back; *best_ms gets its time per
+ * sweep).  The state is not advanced: the trial sweeps write the spare
+ * copy only.  Synchronises. */
+int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms);
 
 /* FP32 solute bookkeeping.  An FP32 context keeps an FP64 target for the
  * total solute sum(c): the compensated sum of the FP64 input at

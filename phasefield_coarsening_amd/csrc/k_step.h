@@ -23,6 +23,18 @@ __device__ __forceinline__ int ghost_any(int i, int n)
     return i < 0 ? 0 : (i >= n ? n - 1 : i);
 }
 
+// A pointer every lane holds the same value of, moved to scalar registers
+// (readfirstlane) so that loads can use the scalar-base + 32-bit lane
+// offset addressing form.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *uniform_ptr(T *p)
+{
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (__attribute__((address_space(1))) T *)(((unsigned long long)hi << 32) | lo);
+}
+
 // ---------------------------------------------------------------------------
 // Naive form: one thread per site, every neighbour read from global memory and
 // its h, mu, M recomputed (7x the pointwise work).  Kept as an independent
@@ -90,19 +102,19 @@ __global__ __launch_bounds__(256) void k_naive_f64(
 //     barrier per plane suffices;
 //   * every thread updates its site from LDS (x, y neighbours) and registers
 //     (z neighbours) and writes the new (s, l, c).
-// The march is software pipelined: the raw (s, l, c) of the column two
-// planes ahead and of the ring one plane ahead are loaded an iteration
-// before they are used, so each wave keeps two planes of loads in flight
-// while it computes.  Compulsory HBM traffic is one read and one write of
+// With PIPE the march is software pipelined: the raw (s, l, c) of the
+// column two planes ahead and of the ring one plane ahead are loaded an
+// iteration before they are used, so each wave keeps two planes of loads in
+// flight while it computes (more registers, fewer waves per SIMD).  Compulsory HBM traffic is one read and one write of
 // each field per site; ring reads and the extra planes of each tz chunk
 // come mostly from L2.
 // ---------------------------------------------------------------------------
-template <class P, int TX, int TY, int BC, bool CONLY, bool REPORT>
+template <class P, int TX, int TY, int BC, bool CONLY, bool REPORT, bool PIPE>
 __global__ __launch_bounds__(TX *TY) void k_tiled(
     const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
     const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
     typename P::T *__restrict__ l_out, typename P::T *__restrict__ c_out, int nx, int ny, int nz,
-    int tz, int ntx, int nty, typename P::K k, double *__restrict__ partials)
+    int tz, int ntx, int nty, int order, typename P::K k, double *__restrict__ partials)
 {
     constexpr int NT = TX * TY;
     constexpr int PX = TX + 2;
@@ -114,15 +126,29 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
-    const int b = blockIdx.x;
-    const int tile = b % (ntx * nty);
-    const int zc = b / (ntx * nty);
+    const int ntile = ntx * nty;
+    const int nblk = gridDim.x;
+    int b = blockIdx.x;
+    // Workgroups are dealt round-robin to the 8 XCDs (b % 8); with the XCD
+    // bit of `order` set, each XCD instead gets one contiguous run of the
+    // logical order, so tiles that share ring and halo planes share an L2.
+    if ((order & 2) && (nblk & 7) == 0)
+        b = (b & 7) * (nblk >> 3) + (b >> 3);
+    // order bit 0: z chunk fastest (all chunks of one tile adjacent);
+    // otherwise tiles fastest (one z chunk of every tile adjacent).
+    const int tile = (order & 1) ? b / ((nblk + ntile - 1) / ntile) : b % ntile;
+    const int zc = (order & 1) ? b % ((nblk + ntile - 1) / ntile) : b / ntile;
     const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY, z0 = zc * tz;
     const int zend = min(z0 + tz, nz);
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x < nx && y < ny;
     const size_t plane = (size_t)nx * ny;
-    const size_t col = (size_t)ghost_any<BC>(y, ny) * nx + ghost_any<BC>(x, nx);
+    // Addressing: the plane base (z is uniform over the workgroup) lives in
+    // scalar registers; each lane keeps only 32-bit offsets within a plane
+    // (nx*ny < 2^31 is checked at context creation).
+#define PLANE(p, zz) uniform_ptr((p) + (size_t)(zz) * plane)
+    const unsigned col = (unsigned)ghost_any<BC>(y, ny) * (unsigned)nx + (unsigned)ghost_any<BC>(x, nx);
+    const unsigned ocol = (unsigned)y * (unsigned)nx + (unsigned)x;
 
     // This thread's ring site, if any: left, right, bottom, top faces.
     int rx = 0, ry = 0;
@@ -140,8 +166,8 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
         ry = TY + 1;
     }
     const bool ring = tid < NRING;
-    const size_t rcol =
-        (size_t)ghost_any<BC>(y0 + ry - 1, ny) * nx + ghost_any<BC>(x0 + rx - 1, nx);
+    const unsigned rcol = (unsigned)ghost_any<BC>(y0 + ry - 1, ny) * (unsigned)nx +
+                          (unsigned)ghost_any<BC>(x0 + rx - 1, nx);
 
     double acc[3] = {0.0, 0.0, 0.0};
 
@@ -152,23 +178,20 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
     T rs_n, rl_n, rc_n;                // raw column values one plane ahead
     T rs_r = 0, rl_r = 0, rc_r = 0;    // raw ring values of the current plane
     {
-        const size_t qp = (size_t)ghost<BC>(z0 - 1, nz) * plane + col;
-        const size_t qc = (size_t)z0 * plane + col;
-        const size_t qn = (size_t)ghost_any<BC>(z0 + 1, nz) * plane + col;
-        s_p = s_in[qp];
-        l_p = l_in[qp];
-        const T c_p = c_in[qp];
-        s_c = s_in[qc];
-        l_c = l_in[qc];
-        c_c = c_in[qc];
-        rs_n = s_in[qn];
-        rl_n = l_in[qn];
-        rc_n = c_in[qn];
+        const int zp = ghost<BC>(z0 - 1, nz), zn = ghost_any<BC>(z0 + 1, nz);
+        s_p = PLANE(s_in, zp)[col];
+        l_p = PLANE(l_in, zp)[col];
+        const T c_p = PLANE(c_in, zp)[col];
+        s_c = PLANE(s_in, z0)[col];
+        l_c = PLANE(l_in, z0)[col];
+        c_c = PLANE(c_in, z0)[col];
+        rs_n = PLANE(s_in, zn)[col];
+        rl_n = PLANE(l_in, zn)[col];
+        rc_n = PLANE(c_in, zn)[col];
         if (ring) {
-            const size_t qr = (size_t)z0 * plane + rcol;
-            rs_r = s_in[qr];
-            rl_r = l_in[qr];
-            rc_r = c_in[qr];
+            rs_r = PLANE(s_in, z0)[rcol];
+            rl_r = PLANE(l_in, z0)[rcol];
+            rc_r = PLANE(c_in, z0)[rcol];
         }
         T hp;
         P::point(s_p, l_p, c_p, k, hp, mu_p, M_p);
@@ -178,18 +201,29 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
     for (int z = z0; z < zend; ++z) {
         const int buf = (z - z0) & 1;
         const bool more = z + 1 < zend;
+        if (!PIPE && z > z0) {
+            // unpipelined form: this plane's ring and the next plane now
+            const int zn = ghost_any<BC>(z + 1, nz);
+            rs_n = PLANE(s_in, zn)[col];
+            rl_n = PLANE(l_in, zn)[col];
+            rc_n = PLANE(c_in, zn)[col];
+            if (ring) {
+                rs_r = PLANE(s_in, z)[rcol];
+                rl_r = PLANE(l_in, z)[rcol];
+                rc_r = PLANE(c_in, z)[rcol];
+            }
+        }
         // prefetch: column two planes ahead, ring one plane ahead
         T ps = 0, pl = 0, pc = 0, prs = 0, prl = 0, prc = 0;
-        if (more) {
-            const size_t qf = (size_t)ghost_any<BC>(z + 2, nz) * plane + col;
-            ps = s_in[qf];
-            pl = l_in[qf];
-            pc = c_in[qf];
+        if (PIPE && more) {
+            const int zf = ghost_any<BC>(z + 2, nz);
+            ps = PLANE(s_in, zf)[col];
+            pl = PLANE(l_in, zf)[col];
+            pc = PLANE(c_in, zf)[col];
             if (ring) {
-                const size_t qr = (size_t)(z + 1) * plane + rcol;
-                prs = s_in[qr];
-                prl = l_in[qr];
-                prc = c_in[qr];
+                prs = PLANE(s_in, z + 1)[rcol];
+                prl = PLANE(l_in, z + 1)[rcol];
+                prc = PLANE(c_in, z + 1)[rcol];
             }
         }
 
@@ -214,14 +248,13 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
         __syncthreads();
 
         if (active) {
-            const size_t q = (size_t)z * plane + (size_t)y * nx + x;
             const T mun[6] = {mu_n, mu_p, sh[buf][2][ty + 2][tx + 1], sh[buf][2][ty][tx + 1],
                                    sh[buf][2][ty + 1][tx + 2], sh[buf][2][ty + 1][tx]};
             const T Mn[6] = {M_n, M_p, sh[buf][3][ty + 2][tx + 1], sh[buf][3][ty][tx + 1],
                                   sh[buf][3][ty + 1][tx + 2], sh[buf][3][ty + 1][tx]};
             if (CONLY) {
                 const T dc = P::conly(mu_c, M_c, mun, Mn, k);
-                c_out[q] = c_c + dc;
+                __builtin_nontemporal_store((T)(c_c + dc), PLANE(c_out, z) + ocol);
                 if (REPORT)
                     acc[0] += (double)dc * (double)dc;
             } else {
@@ -233,9 +266,10 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
                 double ds, dl, dc;
                 P::update(s_c, l_c, c_c, h_c, mu_c, M_c, sn, ln, mun, Mn, k, so, lo, co, ds, dl,
                           dc);
-                s_out[q] = so;
-                l_out[q] = lo;
-                c_out[q] = co;
+                // streaming stores: the new state is not read again this step
+                __builtin_nontemporal_store(so, PLANE(s_out, z) + ocol);
+                __builtin_nontemporal_store(lo, PLANE(l_out, z) + ocol);
+                __builtin_nontemporal_store(co, PLANE(c_out, z) + ocol);
                 if (REPORT) {
                     acc[0] += dc * dc;
                     acc[1] += ds * ds;
@@ -253,13 +287,16 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
         h_c = h_n;
         mu_c = mu_n;
         M_c = M_n;
-        rs_n = ps;
-        rl_n = pl;
-        rc_n = pc;
-        rs_r = prs;
-        rl_r = prl;
-        rc_r = prc;
+        if (PIPE) {
+            rs_n = ps;
+            rl_n = pl;
+            rc_n = pc;
+            rs_r = prs;
+            rl_r = prl;
+            rc_r = prc;
+        }
     }
     if (REPORT)
         block_sum3<NT>(acc, partials + 3 * (size_t)blockIdx.x);
+#undef PLANE
 }

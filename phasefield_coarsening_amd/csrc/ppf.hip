@@ -56,6 +56,7 @@ struct ppf_ctx {
     int kernel;           // resolved PPF_KERNEL_*
     TileCfg tile;
     int tz;
+    int order;  // workgroup order: bit 0 z-chunk fastest, bit 1 XCD-contiguous
     const char *kname;
     char err[512];
 };
@@ -172,19 +173,27 @@ static KConst32 make_kconst32(const ppf_params &p)
 template <class P>
 using tiled_fn = void (*)(const typename P::T *, const typename P::T *, const typename P::T *,
                           typename P::T *, typename P::T *, typename P::T *, int, int, int, int,
-                          int, int, typename P::K, double *);
+                          int, int, int, typename P::K, double *);
 typedef void (*naive_fn)(const double *, const double *, const double *, double *, double *,
                          double *, int, int, int, KConst, double *);
 
-template <class P, int TX, int TY>
-static tiled_fn<P> pick_tiled(int bc, bool conly, bool report)
+template <class P, int TX, int TY, bool PIPE>
+static tiled_fn<P> pick_tiled_p(int bc, bool conly, bool report)
 {
 #define PPF_T(BC, C, R) \
-    if (bc == BC && conly == C && report == R) return k_tiled<P, TX, TY, BC, C, R>;
+    if (bc == BC && conly == C && report == R) return k_tiled<P, TX, TY, BC, C, R, PIPE>;
     PPF_T(0, false, false) PPF_T(0, false, true) PPF_T(0, true, false) PPF_T(0, true, true)
     PPF_T(1, false, false) PPF_T(1, false, true) PPF_T(1, true, false) PPF_T(1, true, true)
 #undef PPF_T
     return nullptr;
+}
+
+// order bit 2 selects the unpipelined march
+template <class P, int TX, int TY>
+static tiled_fn<P> pick_tiled(int bc, bool conly, bool report, int order)
+{
+    return (order & 4) ? pick_tiled_p<P, TX, TY, false>(bc, conly, report)
+                       : pick_tiled_p<P, TX, TY, true>(bc, conly, report);
 }
 
 static naive_fn pick_naive(int bc, bool eqa, bool conly, bool report)
@@ -203,20 +212,20 @@ static naive_fn pick_naive(int bc, bool eqa, bool conly, bool report)
 static const TileCfg k_tiles[] = {{64, 8}, {64, 16}, {64, 4}, {32, 16}, {32, 8}, {16, 16}};
 
 template <class P>
-static tiled_fn<P> pick_tiled_shape(TileCfg t, int bc, bool conly, bool report)
+static tiled_fn<P> pick_tiled_shape(TileCfg t, int bc, bool conly, bool report, int order)
 {
     if (t.tx == 64 && t.ty == 8)
-        return pick_tiled<P, 64, 8>(bc, conly, report);
+        return pick_tiled<P, 64, 8>(bc, conly, report, order);
     if (t.tx == 64 && t.ty == 16)
-        return pick_tiled<P, 64, 16>(bc, conly, report);
+        return pick_tiled<P, 64, 16>(bc, conly, report, order);
     if (t.tx == 64 && t.ty == 4)
-        return pick_tiled<P, 64, 4>(bc, conly, report);
+        return pick_tiled<P, 64, 4>(bc, conly, report, order);
     if (t.tx == 32 && t.ty == 16)
-        return pick_tiled<P, 32, 16>(bc, conly, report);
+        return pick_tiled<P, 32, 16>(bc, conly, report, order);
     if (t.tx == 32 && t.ty == 8)
-        return pick_tiled<P, 32, 8>(bc, conly, report);
+        return pick_tiled<P, 32, 8>(bc, conly, report, order);
     if (t.tx == 16 && t.ty == 16)
-        return pick_tiled<P, 16, 16>(bc, conly, report);
+        return pick_tiled<P, 16, 16>(bc, conly, report, order);
     return nullptr;
 }
 
@@ -256,7 +265,8 @@ static int launch_tiled(ppf_ctx *ctx, const typename P::K &k, bool conly, double
     const int cs = ctx->cur[0], cl = ctx->cur[1], cc = ctx->cur[2];
     int ntx, nty, nzc;
     tiled_grid(ctx, ntx, nty, nzc);
-    tiled_fn<P> f = pick_tiled_shape<P>(ctx->tile, ctx->d.bc, conly, part != nullptr);
+    tiled_fn<P> f =
+        pick_tiled_shape<P>(ctx->tile, ctx->d.bc, conly, part != nullptr, ctx->order);
     if (!f)
         return set_err(ctx, PPF_EARG, "no kernel for tile %dx%d", ctx->tile.tx, ctx->tile.ty);
     hipLaunchKernelGGL(f, dim3((unsigned)((size_t)ntx * nty * nzc)),
@@ -264,7 +274,32 @@ static int launch_tiled(ppf_ctx *ctx, const typename P::K &k, bool conly, double
                        (const T *)ctx->buf[cs][0], (const T *)ctx->buf[cl][1],
                        (const T *)ctx->buf[cc][2], (T *)ctx->buf[1 - cs][0],
                        (T *)ctx->buf[1 - cl][1], (T *)ctx->buf[1 - cc][2], ctx->d.nx, ctx->d.ny,
-                       ctx->d.nz, ctx->tz, ntx, nty, k, part);
+                       ctx->d.nz, ctx->tz, ntx, nty, ctx->order, k, part);
+    return PPF_OK;
+}
+
+// Time `reps` sweeps of the current state without advancing it: the kernel
+// reads copy cur and writes copy 1-cur, and cur is left as it was.
+static int time_tiling(ppf_ctx *ctx, int reps, double *ms)
+{
+    int rc = PPF_OK;
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int r = 0; r < reps && rc == PPF_OK; r++) {
+        if (ctx->d.precision == PPF_F32)
+            rc = launch_tiled<PhysF32>(ctx, ctx->k32, false, nullptr);
+        else if (ctx->eqa)
+            rc = launch_tiled<PhysF64<true>>(ctx, ctx->k, false, nullptr);
+        else
+            rc = launch_tiled<PhysF64<false>>(ctx, ctx->k, false, nullptr);
+    }
+    if (rc)
+        return rc;
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
+    float f = 0.0f;
+    HIPCHK(ctx, hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+    *ms = (double)f / reps;
     return PPF_OK;
 }
 
@@ -351,27 +386,22 @@ static int set_target_from_device(ppf_ctx *ctx)
     return PPF_OK;
 }
 
+// Default tiling, from sweeps on MI355X at 512^3: x rows of 64 sites and
+// 32-plane z chunks; FP64 pipelined 64x8 tiles in tile-fastest order, FP32
+// unpipelined 64x16 tiles with the z chunks of a tile adjacent.
+// ppf_set_tiling or ppf_autotune override it.
 static void choose_tiles(ppf_ctx *ctx)
 {
     const int nx = ctx->d.nx;
+    const bool f32 = ctx->d.precision == PPF_F32;
     if (nx >= 64)
-        ctx->tile = {64, 8};
+        ctx->tile = f32 ? TileCfg{64, 16} : TileCfg{64, 8};
     else if (nx >= 32)
         ctx->tile = {32, 8};
     else
         ctx->tile = {16, 16};
-    int ntx = (ctx->d.nx + ctx->tile.tx - 1) / ctx->tile.tx;
-    int nty = (ctx->d.ny + ctx->tile.ty - 1) / ctx->tile.ty;
-    // Enough workgroups to fill 256 CUs several times over, but chunks long
-    // enough that the two extra planes per chunk stay a few percent.
-    const long tiles = (long)ntx * nty;
-    long chunks = (8192 + tiles - 1) / tiles;
-    long maxchunks = (ctx->d.nz + 15) / 16;
-    if (chunks > maxchunks)
-        chunks = maxchunks;
-    if (chunks < 1)
-        chunks = 1;
-    ctx->tz = (int)((ctx->d.nz + chunks - 1) / chunks);
+    ctx->order = f32 ? 5 : 0;
+    ctx->tz = ctx->d.nz < 32 ? ctx->d.nz : 32;
 }
 
 extern "C" int ppf_create(ppf_ctx **out, const ppf_desc *desc)
@@ -395,6 +425,9 @@ extern "C" int ppf_create(ppf_ctx **out, const ppf_desc *desc)
     if (d.kernel < PPF_KERNEL_AUTO || d.kernel > PPF_KERNEL_TILED)
         return set_err(nullptr, PPF_EARG, "bad kernel %d", d.kernel);
     const size_t n = (size_t)d.nz * d.ny * d.nx;
+    if ((size_t)d.ny * d.nx >= ((size_t)1 << 31))
+        return set_err(nullptr, PPF_EARG, "an x-y plane of %d x %d sites is too large", d.ny,
+                       d.nx);
     if ((n >> 36) != 0)
         return set_err(nullptr, PPF_EARG, "grid too large");
     int ndev = 0;
@@ -765,7 +798,7 @@ extern "C" int ppf_spheres_generate(uint64_t seed, int n, int nz, int ny, int nx
     return PPF_OK;
 }
 
-extern "C" int ppf_set_tiling(ppf_ctx *ctx, int tx, int ty, int tz)
+extern "C" int ppf_set_tiling(ppf_ctx *ctx, int tx, int ty, int tz, int order)
 {
     if (!ctx)
         return PPF_EARG;
@@ -778,15 +811,66 @@ extern "C" int ppf_set_tiling(ppf_ctx *ctx, int tx, int ty, int tz)
         return set_err(ctx, PPF_EARG, "tile %dx%d is not compiled", tx, ty);
     if (tz < 1)
         return set_err(ctx, PPF_EARG, "tz must be >= 1");
+    if (order < 0 || order > 7)
+        return set_err(ctx, PPF_EARG, "order must be 0..7");
+    ctx->order = order;
     ctx->tile = {tx, ty};
     ctx->tz = tz > ctx->d.nz ? ctx->d.nz : tz;
     return PPF_OK;
 }
 
-extern "C" int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz)
+extern "C" int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms)
 {
-    if (!ctx || !tx || !ty || !tz)
+    if (!ctx || reps < 1 || !best_ms)
+        return ctx ? set_err(ctx, PPF_EARG, "bad autotune arguments") : PPF_EARG;
+    if (ctx->kernel != PPF_KERNEL_TILED)
+        return set_err(ctx, PPF_ESTATE, "autotuning applies to the tiled kernels only");
+    HIPCHK(ctx, hipSetDevice(ctx->d.device));
+    const TileCfg keep_tile = ctx->tile;
+    const int keep_tz = ctx->tz, keep_order = ctx->order;
+    static const int tzs[] = {16, 32, 64};
+    static const int orders[] = {0, 1, 2, 4, 5, 6};
+    double best = 1e300;
+    TileCfg bt = keep_tile;
+    int btz = keep_tz, bo = keep_order;
+    for (const TileCfg &c : k_tiles) {
+        if (c.tx > 2 * ctx->d.nx && c.tx > 16)
+            continue;  // mostly idle lanes
+        for (int tz : tzs)
+            for (int o : orders) {
+                ctx->tile = c;
+                ctx->tz = tz > ctx->d.nz ? ctx->d.nz : tz;
+                ctx->order = o;
+                double ms = 0.0;
+                int rc = time_tiling(ctx, 1, &ms);  // warm
+                if (rc == PPF_OK)
+                    rc = time_tiling(ctx, reps, &ms);
+                if (rc) {
+                    ctx->tile = keep_tile;
+                    ctx->tz = keep_tz;
+                    ctx->order = keep_order;
+                    return rc;
+                }
+                if (ms < best) {
+                    best = ms;
+                    bt = c;
+                    btz = ctx->tz;
+                    bo = o;
+                }
+            }
+    }
+    ctx->tile = bt;
+    ctx->tz = btz;
+    ctx->order = bo;
+    *best_ms = best;
+    return PPF_OK;
+}
+
+extern "C" int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz, int *order)
+{
+    if (!ctx || !tx || !ty || !tz || !order)
         return PPF_EARG;
+    *order = ctx->order;
     *tx = ctx->tile.tx;
     *ty = ctx->tile.ty;
     *tz = ctx->tz;

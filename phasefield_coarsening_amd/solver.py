@@ -85,13 +85,14 @@ SIGNATURES = {
     "ppf_spheres_generate": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                             ctypes.c_double, _dp, _dp]),
-    "ppf_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
-    "ppf_get_tiling": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3),
+    "ppf_set_tiling": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 4),
+    "ppf_get_tiling": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 4),
     "ppf_particles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
                                      ctypes.POINTER(ParticleStats),
                                      ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]),
     "ppf_set_mass_correction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ppf_mass_target": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64)]),
+    "ppf_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
     "ppf_timer_start": (ctypes.c_int, [ctypes.c_void_p]),
     "ppf_timer_stop": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "ppf_sync": (ctypes.c_int, [ctypes.c_void_p]),
@@ -228,11 +229,19 @@ class Solver:
     def sync(self):
         self._check(lib().ppf_sync(self._h))
 
-    def set_tiling(self, tx, ty, tz):
-        self._check(lib().ppf_set_tiling(self._h, int(tx), int(ty), int(tz)))
+    def set_tiling(self, tx, ty, tz, order=0):
+        self._check(lib().ppf_set_tiling(self._h, int(tx), int(ty), int(tz), int(order)))
+
+    def autotune(self, reps=3):
+        """Pick the fastest tiling for this grid on this device; the state is
+        not advanced.  Returns (tiling, ms per sweep)."""
+        ms = ctypes.c_double()
+        self._check(lib().ppf_autotune(self._h, int(reps), ctypes.byref(ms)))
+        return self.get_tiling(), ms.value
 
     def get_tiling(self):
-        v = [ctypes.c_int() for _ in range(3)]
+        """(tx, ty, tz, order)"""
+        v = [ctypes.c_int() for _ in range(4)]
         self._check(lib().ppf_get_tiling(self._h, *(ctypes.byref(a) for a in v)))
         return tuple(a.value for a in v)
 

@@ -191,3 +191,34 @@ def test_particles_match_cpu_labelling(shape, bc):
     np.testing.assert_array_equal(sizes, want)
     assert st["solid_sites"] == (s > 0.5).sum()
     assert abs(st["mean_radius"] - mean_radius(want)) < 1e-12
+
+
+@pytest.mark.parametrize("precision", [S.F64, S.F32])
+def test_autotune_keeps_state_and_results(precision, tmp_path, monkeypatch):
+    """ppf_autotune times trial sweeps without advancing the state, and the
+    tiling it picks gives the same result as the default (FP64: bitwise,
+    also against the oracle)."""
+    from phasefield_coarsening_amd import tuning
+    monkeypatch.setenv("PPF_TUNING_CACHE", str(tmp_path / "t.json"))
+    shape = (40, 48, 72)
+    (s0, l0, c0), _, _ = _spheres_state(shape, 6, 3.0, 7.0, 8)
+    with S.Solver(shape, precision=precision) as a:
+        a.upload(s0, l0, c0)
+        before = a.download()
+        t = tuning.tuned(a, reps=1)
+        after = a.download()
+        for x, y in zip(before, after):
+            np.testing.assert_array_equal(x, y)
+        a.step(4)
+        got = a.download()
+    assert tuning.load()[tuning.key(shape, precision, S.BC_PERIODIC)]["tx"] == t["tx"]
+    with S.Solver(shape, precision=precision) as b:
+        b.upload(s0, l0, c0)
+        b.step(4)
+        ref = b.download()
+    for x, y in zip(got, ref):
+        np.testing.assert_array_equal(x, y)
+    if precision == S.F64:
+        want = O.run(s0, l0, c0, 4)
+        for x, y in zip(got, want[:3]):
+            np.testing.assert_array_equal(x, y)
