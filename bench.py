@@ -98,6 +98,42 @@ def cpu_baseline(S, shape, nsteps):
                       f"OpenMP {threads} threads"}
 
 
+def reference_serial_baseline(S, shape, nsteps=3):
+    """The reference program itself (oracle/_ref, its serial path compiled
+    for 512^3 periodic), one core: wall time of a 3-step run minus a 0-step
+    run on the same input file, i.e. the stepping loop without the HDF5
+    read.  None when the binaries or HDF5 are not present."""
+    import subprocess
+    import tempfile
+    ref = os.path.join(HERE, "oracle", "_ref")
+    exe3, exe0 = (os.path.join(ref, f"ppf_ref_p{shape[0]}_s{k}") for k in (nsteps, 0))
+    if not (os.path.exists(exe3) and os.path.exists(exe0)):
+        return None
+    try:
+        from phasefield_coarsening_amd import h5
+        cen, rad = S.spheres(2, 2000, shape, 6.0, 14.0)
+        with S.Solver(shape) as sv:
+            sv.init_spheres(cen, rad)
+            s, l, c = sv.download()
+        with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+            h5.write(os.path.join(tmp, "in.h5"), s, l, c)
+            del s, l, c
+            times = []
+            for exe in (exe0, exe3):
+                t0 = time.perf_counter()
+                subprocess.run([exe, "in.h5"], cwd=tmp, check=True, capture_output=True,
+                               timeout=600)
+                times.append(time.perf_counter() - t0)
+    except Exception as e:  # reported, not fatal: the port baseline stands
+        return {"error": str(e)[:200]}
+    sec = times[1] - times[0]
+    n = shape[0] * shape[1] * shape[2]
+    return {"value": n * nsteps / sec / 1e9, "unit": "GLUPS", "cores": 1, "kind": "reference",
+            "sample": f"{nsteps} sweeps of the {shape[0]}^3 workload by the reference program "
+                      f"(phasefield.c serial path, gcc -O2), {sec:.1f} s = run of {nsteps} "
+                      f"steps ({times[1]:.1f} s) minus run of 0 steps ({times[0]:.1f} s)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +200,9 @@ def main():
         out["roofline"]["frac_of_copy_ceiling"] = round(achieved / cc, 4)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(S, shape, a.cpu_steps)
+        ref = reference_serial_baseline(S, shape)
+        if ref is not None:
+            out["cpu_baseline_reference_serial"] = ref
     print(json.dumps(out), flush=True)
 
 
