@@ -205,6 +205,44 @@ const char *ppf_kernel_name(ppf_ctx *ctx);
 
 const char *ppf_last_error(ppf_ctx *ctx);
 
+/* ------------------------------------------------------------------------
+ * Ensembles: many independent grids of uneven size stepped together, one
+ * persistent launch per step over a work list of (member, tile, z chunk)
+ * items.  Each member evolves exactly as a ppf_ctx of the same grid,
+ * precision and tiling rules would (same per-site arithmetic).  No
+ * counterpart in the reference, which steps one grid per MPI job.
+ * ------------------------------------------------------------------------ */
+typedef struct ppf_ens ppf_ens;
+
+/* ndom members; dims_zyx holds (nz, ny, nx) per member (1..4096 each). */
+int ppf_ens_create(ppf_ens **out, int ndom, const int *dims_zyx, int bc, int precision,
+                   int device, const ppf_params *params);
+int ppf_ens_destroy(ppf_ens *e);
+int ppf_ens_upload(ppf_ens *e, int dom, const double *s, const double *l, const double *c);
+int ppf_ens_download(ppf_ens *e, int dom, double *s, double *l, double *c);
+/* The member's fields as float32 (snapshots for training data). */
+int ppf_ens_download_f32(ppf_ens *e, int dom, float *s, float *l, float *c);
+int ppf_ens_init_spheres(ppf_ens *e, int dom, int n, const double *centers_zyx,
+                         const double *radii, double c_liq, double c_span);
+int ppf_ens_step(ppf_ens *e, int64_t nsteps);
+int ppf_ens_step_c_only(ppf_ens *e, int64_t nsteps);
+/* Member tiling: shape 0 = 64x4, 1 = 32x8, 2 = 16x16 column tiles (256
+ * threads each), tz planes per chunk. */
+int ppf_ens_set_tiling(ppf_ens *e, int dom, int shape, int tz);
+int ppf_ens_get_tiling(ppf_ens *e, int dom, int *shape, int *tz);
+/* Per grid size, time every shape x tz in {8,16,32,64} on those members
+ * alone and keep the fastest.  The state is not advanced. */
+int ppf_ens_autotune(ppf_ens *e, int reps);
+int ppf_ens_reduce(ppf_ens *e, int dom, ppf_stats *out);
+/* FP32 members keep an FP64 solute target each, as a ppf_ctx does. */
+int ppf_ens_set_mass_correction(ppf_ens *e, int every);
+int ppf_ens_mass_target(ppf_ens *e, int dom, double *target);
+int ppf_ens_timer_start(ppf_ens *e);
+int ppf_ens_timer_stop(ppf_ens *e, double *ms);
+/* Total sites over members, work items per step, persistent workgroups. */
+int ppf_ens_info(ppf_ens *e, int64_t *total_sites, int *nitems, int *grid);
+const char *ppf_ens_last_error(ppf_ens *e);
+
 #ifdef __cplusplus
 }
 #endif

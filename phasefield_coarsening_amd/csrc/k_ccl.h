@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_ccl_merge(int *__restrict__ L, int nx, 
 }
 
 // Point every solid site at its root and count the sites of each root.
-__global__ __launch_bounds__(256) void k_ccl_flatten(int *__restrict__ L, int *__restrict__ cnt,
+static __global__ __launch_bounds__(256) void k_ccl_flatten(int *__restrict__ L, int *__restrict__ cnt,
                                                      size_t n)
 {
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_ccl_flatten(int *__restrict__ L, int *_
 
 // Number of roots (first pass, list == nullptr), or append each root's size
 // to list (second pass; *nroots restarts from zero, cap bounds the writes).
-__global__ __launch_bounds__(256) void k_ccl_collect(const int *__restrict__ L,
+static __global__ __launch_bounds__(256) void k_ccl_collect(const int *__restrict__ L,
                                                      const int *__restrict__ cnt, size_t n,
                                                      unsigned long long *__restrict__ nroots,
                                                      int *__restrict__ list, unsigned long long cap)

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void k_stats(const T *__restrict__ s, const T 
 }
 
 // Combine nb stat records (one workgroup of 256, fixed order) into out[0..7].
-__global__ __launch_bounds__(256) void k_stats_final(const double *__restrict__ p, int nb,
+static __global__ __launch_bounds__(256) void k_stats_final(const double *__restrict__ p, int nb,
                                                      double *__restrict__ out)
 {
     DD sc = {0.0, 0.0};
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_stats_final(const double *__restrict__ 
 }
 
 // Sum nb triples (one workgroup of 256, fixed order) into out[0..2].
-__global__ __launch_bounds__(256) void k_sum3_final(const double *__restrict__ p, int nb,
+static __global__ __launch_bounds__(256) void k_sum3_final(const double *__restrict__ p, int nb,
                                                     double *__restrict__ out)
 {
     double a[3] = {0.0, 0.0, 0.0};
@@ -211,10 +211,11 @@ __global__ __launch_bounds__(256) void k_sum_dd(const T *__restrict__ x, size_t 
 }
 
 // Combine nb (hi, lo) pairs; out[1] = the sum, out[0] = (target - sum)/n,
-// the per-site shift that puts sum(c) back on the FP64 target.
-__global__ __launch_bounds__(256) void k_mass_delta(const double *__restrict__ p, int nb,
-                                                    double target, double n,
-                                                    double *__restrict__ out)
+// the per-site shift that puts sum(c) back on the FP64 target.  Run by one
+// workgroup of 256 threads.
+__device__ __forceinline__ void k_mass_delta_body(const double *__restrict__ p, int nb,
+                                                  double target, double n,
+                                                  double *__restrict__ out)
 {
     DD acc = {0.0, 0.0};
     for (int j = threadIdx.x; j < nb; j += 256)
@@ -240,6 +241,13 @@ __global__ __launch_bounds__(256) void k_mass_delta(const double *__restrict__ p
     }
 }
 
+static __global__ __launch_bounds__(256) void k_mass_delta(const double *__restrict__ p, int nb,
+                                                    double target, double n,
+                                                    double *__restrict__ out)
+{
+    k_mass_delta_body(p, nb, target, n, out);
+}
+
 // 64-bit mix (SplitMix64 finaliser) for counter-based random numbers.
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
@@ -248,19 +256,18 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
-// c += delta[0] with stochastic rounding to FP32: the exact value t = c +
-// delta lies between two adjacent floats lo <= t <= hi; pick hi with
-// probability (t - lo)/(hi - lo) from a hash of (site, seed).  The expected
-// new sum is exactly sum(c) + n*delta, so a shift far below one ulp per site
-// still moves the total (round-to-nearest would drop it).
-__global__ __launch_bounds__(256) void k_mass_shift(float *__restrict__ c, size_t n,
-                                                    const double *__restrict__ delta,
-                                                    uint64_t seed)
+// c += d with stochastic rounding to FP32: the exact value t = c + d lies
+// between two adjacent floats lo <= t <= hi; pick hi with probability
+// (t - lo)/(hi - lo) from a hash of (site, seed).  The expected new sum is
+// exactly sum(c) + n*d, so a shift far below one ulp per site still moves
+// the total (round-to-nearest would drop it).  Sites start+threadIdx.x,
+// then every stride.
+__device__ __forceinline__ void mass_shift_body(float *__restrict__ c, size_t n, double d,
+                                                uint64_t seed, size_t start, size_t stride)
 {
-    const double d = delta[0];
     if (d == 0.0 || !isfinite(d))
         return;
-    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+    for (size_t q = start + threadIdx.x; q < n; q += stride) {
         const double t = (double)c[q] + d;
         const float f = (float)t;
         const double fd = (double)f;
@@ -282,4 +289,11 @@ __global__ __launch_bounds__(256) void k_mass_shift(float *__restrict__ c, size_
             0x1.0p-53;
         c[q] = r < p ? hi : lo;
     }
+}
+
+static __global__ __launch_bounds__(256) void k_mass_shift(float *__restrict__ c, size_t n,
+                                                    const double *__restrict__ delta,
+                                                    uint64_t seed)
+{
+    mass_shift_body(c, n, delta[0], seed, (size_t)blockIdx.x * 256, (size_t)gridDim.x * 256);
 }

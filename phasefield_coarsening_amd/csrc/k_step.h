@@ -109,37 +109,23 @@ __global__ __launch_bounds__(256) void k_naive_f64(
 // each field per site; ring reads and the extra planes of each tz chunk
 // come mostly from L2.
 // ---------------------------------------------------------------------------
+// One workgroup's march over the column tile [x0, x0+TX) x [y0, y0+TY) and
+// the planes [z0, zend) of one grid.  sh is the workgroup's LDS,
+// 2 x 4 x (TY+2) x (TX+2) values; acc gets the REPORT sums of squared
+// increments.  Every thread of the workgroup must call it.
 template <class P, int TX, int TY, int BC, bool CONLY, bool REPORT, bool PIPE>
-__global__ __launch_bounds__(TX *TY) void k_tiled(
+__device__ __forceinline__ void tile_march(
     const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
     const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
     typename P::T *__restrict__ l_out, typename P::T *__restrict__ c_out, int nx, int ny, int nz,
-    int tz, int ntx, int nty, int order, typename P::K k, double *__restrict__ partials)
+    int x0, int y0, int z0, int zend, const typename P::K &k,
+    typename P::T (*__restrict__ sh)[4][TY + 2][TX + 2], double acc[3])
 {
-    constexpr int NT = TX * TY;
-    constexpr int PX = TX + 2;
-    constexpr int PY = TY + 2;
     constexpr int NRING = 2 * TX + 2 * TY;
-    static_assert(NRING <= NT, "ring larger than the workgroup");
+    static_assert(NRING <= TX * TY, "ring larger than the workgroup");
     using T = typename P::T;
-    __shared__ T sh[2][4][PY][PX];
-
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
-    const int ntile = ntx * nty;
-    const int nblk = gridDim.x;
-    int b = blockIdx.x;
-    // Workgroups are dealt round-robin to the 8 XCDs (b % 8); with the XCD
-    // bit of `order` set, each XCD instead gets one contiguous run of the
-    // logical order, so tiles that share ring and halo planes share an L2.
-    if ((order & 2) && (nblk & 7) == 0)
-        b = (b & 7) * (nblk >> 3) + (b >> 3);
-    // order bit 0: z chunk fastest (all chunks of one tile adjacent);
-    // otherwise tiles fastest (one z chunk of every tile adjacent).
-    const int tile = (order & 1) ? b / ((nblk + ntile - 1) / ntile) : b % ntile;
-    const int zc = (order & 1) ? b % ((nblk + ntile - 1) / ntile) : b / ntile;
-    const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY, z0 = zc * tz;
-    const int zend = min(z0 + tz, nz);
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x < nx && y < ny;
     const size_t plane = (size_t)nx * ny;
@@ -168,8 +154,6 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
     const bool ring = tid < NRING;
     const unsigned rcol = (unsigned)ghost_any<BC>(y0 + ry - 1, ny) * (unsigned)nx +
                           (unsigned)ghost_any<BC>(x0 + rx - 1, nx);
-
-    double acc[3] = {0.0, 0.0, 0.0};
 
     // Prologue: planes z0-1 and z0 computed, plane z0+1 and the ring of z0
     // loaded raw.
@@ -296,7 +280,36 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
             rc_r = prc;
         }
     }
+#undef PLANE
+}
+
+template <class P, int TX, int TY, int BC, bool CONLY, bool REPORT, bool PIPE>
+__global__ __launch_bounds__(TX *TY) void k_tiled(
+    const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
+    const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
+    typename P::T *__restrict__ l_out, typename P::T *__restrict__ c_out, int nx, int ny, int nz,
+    int tz, int ntx, int nty, int order, typename P::K k, double *__restrict__ partials)
+{
+    constexpr int NT = TX * TY;
+    using T = typename P::T;
+    __shared__ T sh[2][4][TY + 2][TX + 2];
+    const int ntile = ntx * nty;
+    const int nblk = gridDim.x;
+    int b = blockIdx.x;
+    // Workgroups are dealt round-robin to the 8 XCDs (b % 8); with the XCD
+    // bit of `order` set, each XCD instead gets one contiguous run of the
+    // logical order, so tiles that share ring and halo planes share an L2.
+    if ((order & 2) && (nblk & 7) == 0)
+        b = (b & 7) * (nblk >> 3) + (b >> 3);
+    // order bit 0: z chunk fastest (all chunks of one tile adjacent);
+    // otherwise tiles fastest (one z chunk of every tile adjacent).
+    const int tile = (order & 1) ? b / ((nblk + ntile - 1) / ntile) : b % ntile;
+    const int zc = (order & 1) ? b % ((nblk + ntile - 1) / ntile) : b / ntile;
+    const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY, z0 = zc * tz;
+    const int zend = min(z0 + tz, nz);
+    double acc[3] = {0.0, 0.0, 0.0};
+    tile_march<P, TX, TY, BC, CONLY, REPORT, PIPE>(s_in, l_in, c_in, s_out, l_out, c_out, nx, ny,
+                                                   nz, x0, y0, z0, zend, k, sh, acc);
     if (REPORT)
         block_sum3<NT>(acc, partials + 3 * (size_t)blockIdx.x);
-#undef PLANE
 }

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include "ppf.h"
+#include "ppf_host.h"
 #include "k_step.h"
 #include "k_misc.h"
 #include "k_ccl.h"
@@ -80,33 +81,6 @@ static int set_err(ppf_ctx *ctx, int code, const char *fmt, ...)
             return set_err((ctx), PPF_EHIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
-static KConst make_kconst(const ppf_params &p)
-{
-    KConst k;
-    k.AS = p.AS;
-    k.AL = p.AL;
-    k.dA = p.AL - p.AS;
-    k.tmp5 = p.AS * p.AL;
-    k.tmp6 = p.cL0 - p.cS0;
-    k.cL0 = p.cL0;
-    k.cS0 = p.cS0;
-    k.ML = p.ML;
-    k.dM = p.MS - p.ML;
-    k.tmp1 = -1.0 * p.L * p.dt;
-    k.tmp2 = (p.AS * p.cS0 - p.AL * p.cL0);
-    k.tmp3 = p.dt * 0.5 * p.h2;
-    k.hAS = 0.5 * p.AS;
-    k.hAL = 0.5 * p.AL;
-    k.CS = p.CS;
-    k.CL = p.CL;
-    k.m = p.m;
-    k.g2 = 2.0 * p.gamma;
-    k.kh2 = p.kappa * p.h2;
-    k.K5 = k.tmp5 / p.AS;
-    k.invA = 1.0 / p.AS;
-    return k;
-}
-
 extern "C" int ppf_params_default(ppf_params *p)
 {
     if (!p)
@@ -146,25 +120,6 @@ extern "C" int ppf_params_default(ppf_params *p)
     p->gamma = gamma;
     p->kappa = kappa;
     return PPF_OK;
-}
-
-static KConst32 make_kconst32(const ppf_params &p)
-{
-    // the dimensionless FP32 constants (ppf_physics.h, PhysF32); valid for
-    // AS == AL, which ppf_create checks
-    const double tmp1 = -1.0 * p.L * p.dt;
-    const double tmp3 = p.dt * 0.5 * p.h2;
-    KConst32 k;
-    k.cL0 = (float)p.cL0;
-    k.tmp6 = (float)(p.cL0 - p.cS0);
-    k.rM = (float)((p.MS - p.ML) / p.ML);
-    k.a1 = (float)(tmp1 * p.m);
-    k.a2 = (float)(-tmp1 * p.kappa * p.h2);
-    k.g2 = (float)(2.0 * p.gamma);
-    k.b0 = (float)(tmp1 * (p.CS - p.CL));
-    k.b1 = (float)(tmp1 * p.AL * (p.cL0 - p.cS0));
-    k.d0 = (float)(tmp3 * p.ML * p.AL);
-    return k;
 }
 
 // ----------------------------------------------------------------------------

@@ -1,0 +1,53 @@
+// ppf_host.h -- host-side helpers shared by the translation units of
+// libppf.so: the hoisted kernel constants.
+#pragma once
+
+#include "ppf.h"
+#include "ppf_physics.h"
+
+static inline KConst make_kconst(const ppf_params &p)
+{
+    KConst k;
+    k.AS = p.AS;
+    k.AL = p.AL;
+    k.dA = p.AL - p.AS;
+    k.tmp5 = p.AS * p.AL;
+    k.tmp6 = p.cL0 - p.cS0;
+    k.cL0 = p.cL0;
+    k.cS0 = p.cS0;
+    k.ML = p.ML;
+    k.dM = p.MS - p.ML;
+    k.tmp1 = -1.0 * p.L * p.dt;
+    k.tmp2 = (p.AS * p.cS0 - p.AL * p.cL0);
+    k.tmp3 = p.dt * 0.5 * p.h2;
+    k.hAS = 0.5 * p.AS;
+    k.hAL = 0.5 * p.AL;
+    k.CS = p.CS;
+    k.CL = p.CL;
+    k.m = p.m;
+    k.g2 = 2.0 * p.gamma;
+    k.kh2 = p.kappa * p.h2;
+    k.K5 = k.tmp5 / p.AS;
+    k.invA = 1.0 / p.AS;
+    return k;
+}
+
+static inline KConst32 make_kconst32(const ppf_params &p)
+{
+    // the dimensionless FP32 constants (ppf_physics.h, PhysF32); valid for
+    // AS == AL, which ppf_create checks
+    const double tmp1 = -1.0 * p.L * p.dt;
+    const double tmp3 = p.dt * 0.5 * p.h2;
+    KConst32 k;
+    k.cL0 = (float)p.cL0;
+    k.tmp6 = (float)(p.cL0 - p.cS0);
+    k.rM = (float)((p.MS - p.ML) / p.ML);
+    k.a1 = (float)(tmp1 * p.m);
+    k.a2 = (float)(-tmp1 * p.kappa * p.h2);
+    k.g2 = (float)(2.0 * p.gamma);
+    k.b0 = (float)(tmp1 * (p.CS - p.CL));
+    k.b1 = (float)(tmp1 * p.AL * (p.cL0 - p.cS0));
+    k.d0 = (float)(tmp3 * p.ML * p.AL);
+    return k;
+}
+

@@ -99,6 +99,30 @@ SIGNATURES = {
     "ppf_steps_taken": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "ppf_kernel_name": (ctypes.c_char_p, [ctypes.c_void_p]),
     "ppf_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "ppf_ens_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.POINTER(Params)]),
+    "ppf_ens_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "ppf_ens_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]),
+    "ppf_ens_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]),
+    "ppf_ens_download_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] +
+                             [ctypes.POINTER(ctypes.c_float)] * 3),
+    "ppf_ens_init_spheres": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                            ctypes.c_double, ctypes.c_double]),
+    "ppf_ens_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "ppf_ens_step_c_only": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "ppf_ens_set_tiling": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 3),
+    "ppf_ens_get_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] +
+                           [ctypes.POINTER(ctypes.c_int)] * 2),
+    "ppf_ens_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ppf_ens_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Stats)]),
+    "ppf_ens_set_mass_correction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ppf_ens_mass_target": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
+    "ppf_ens_timer_start": (ctypes.c_int, [ctypes.c_void_p]),
+    "ppf_ens_timer_stop": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "ppf_ens_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "ppf_ens_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
 }
 
 
@@ -280,3 +304,112 @@ class Solver:
         v = ctypes.c_int64()
         self._check(lib().ppf_steps_taken(self._h, ctypes.byref(v)))
         return v.value
+
+
+class Ensemble:
+    """Many independent grids stepped together by one persistent launch per
+    step.  dims: list of (nz, ny, nx)."""
+
+    def __init__(self, dims, bc=BC_PERIODIC, precision=F32, device=0, params=None):
+        L = lib()
+        dims = [tuple(int(v) for v in d) for d in dims]
+        arr = (ctypes.c_int * (3 * len(dims)))(*[v for d in dims for v in d])
+        p = params if params is not None else default_params()
+        h = ctypes.c_void_p()
+        rc = L.ppf_ens_create(ctypes.byref(h), len(dims), arr, int(bc), int(precision),
+                              int(device), ctypes.byref(p))
+        if rc:
+            raise PpfError(rc, L.ppf_ens_last_error(None).decode(errors="replace"))
+        self._h = h
+        self.dims = dims
+        self.bc = bc
+        self.precision = precision
+
+    def _check(self, rc):
+        if rc:
+            raise PpfError(rc, lib().ppf_ens_last_error(self._h).decode(errors="replace"))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ppf_ens_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, dom, s, l, c):
+        a = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (s, l, c)]
+        for x in a:
+            if x is not None and x.shape != self.dims[dom]:
+                raise ValueError(f"member {dom}: shape {x.shape} != {self.dims[dom]}")
+        self._check(lib().ppf_ens_upload(self._h, int(dom), _ptr(a[0]),
+                                         _ptr(a[1]) if a[1] is not None else None, _ptr(a[2])))
+
+    def download(self, dom, f32=False):
+        if f32:
+            out = [np.empty(self.dims[dom], np.float32) for _ in range(3)]
+            fp = ctypes.POINTER(ctypes.c_float)
+            self._check(lib().ppf_ens_download_f32(self._h, int(dom),
+                                                   *(a.ctypes.data_as(fp) for a in out)))
+        else:
+            out = [np.empty(self.dims[dom], np.float64) for _ in range(3)]
+            self._check(lib().ppf_ens_download(self._h, int(dom), *(_ptr(a) for a in out)))
+        return tuple(out)
+
+    def init_spheres(self, dom, centers, radii, c_liq=0.1, c_span=0.8):
+        centers = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        radii = np.ascontiguousarray(radii, dtype=np.float64).reshape(-1)
+        self._check(lib().ppf_ens_init_spheres(self._h, int(dom), len(radii), _ptr(centers),
+                                               _ptr(radii), float(c_liq), float(c_span)))
+
+    def step(self, nsteps=1):
+        self._check(lib().ppf_ens_step(self._h, int(nsteps)))
+
+    def step_c_only(self, nsteps=1):
+        self._check(lib().ppf_ens_step_c_only(self._h, int(nsteps)))
+
+    def set_tiling(self, dom, shape, tz):
+        self._check(lib().ppf_ens_set_tiling(self._h, int(dom), int(shape), int(tz)))
+
+    def get_tiling(self, dom):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().ppf_ens_get_tiling(self._h, int(dom), ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, reps=3):
+        self._check(lib().ppf_ens_autotune(self._h, int(reps)))
+
+    def reduce(self, dom):
+        st = Stats()
+        self._check(lib().ppf_ens_reduce(self._h, int(dom), ctypes.byref(st)))
+        return st.as_dict()
+
+    def set_mass_correction(self, every):
+        self._check(lib().ppf_ens_set_mass_correction(self._h, int(every)))
+
+    def mass_target(self, dom):
+        t = ctypes.c_double()
+        self._check(lib().ppf_ens_mass_target(self._h, int(dom), ctypes.byref(t)))
+        return t.value
+
+    def timer_start(self):
+        self._check(lib().ppf_ens_timer_start(self._h))
+
+    def timer_stop(self):
+        ms = ctypes.c_double()
+        self._check(lib().ppf_ens_timer_stop(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def info(self):
+        n, it, g = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        self._check(lib().ppf_ens_info(self._h, ctypes.byref(n), ctypes.byref(it), ctypes.byref(g)))
+        return {"total_sites": n.value, "items": it.value, "grid": g.value}
