@@ -1,0 +1,78 @@
+/*
+ * ppf_unet.h -- learned K-step jump for the phase-field state: a 3-level
+ * 3-D convolutional U-Net on MI355X (part of libppf.so).
+ *
+ * No counterpart in the reference, which only steps explicitly
+ * (phasefield.c:178-258).  The network maps (etaS, etaL, c) at step t to the
+ * state at step t+K:
+ *
+ *   level 1 (N^3):    conv3 3->B, conv3 B->B                 (skip)
+ *   level 2 (N^3/8):  avgpool2, conv3 B->2B, conv3 2B->2B   (skip)
+ *   level 3 (N^3/64): avgpool2, conv3 2B->4B, conv3 4B->4B
+ *   decoder:          nearest x2, concat skip, conv3 6B->2B, conv3 2B->2B,
+ *                     nearest x2, concat skip, conv3 3B->B,  conv3 B->B,
+ *   head:             1x1x1 conv B->3; output = input + head
+ *
+ * with B = 32 base channels, 3x3x3 kernels with periodic padding, leaky
+ * ReLU (slope 0.1) after every 3x3x3 conv.  Weights and activations are
+ * BF16 on the MFMA units with FP32 accumulation; master weights, gradients
+ * and the Adam moments are FP32.  Grids must have every dimension a
+ * multiple of 4 (two poolings).
+ *
+ * Parameters, flat FP32, layer by layer: W[27][Cout][Cin] then b[Cout] for
+ * the ten 3x3x3 convolutions (tap = (dz+1)*9 + (dy+1)*3 + (dx+1): the weight
+ * of input site (z+dz, y+dy, x+dx)), then the head's W[3][B] and b[3].
+ *
+ * Fields crossing this ABI are float32, [batch][nz][ny][nx][3] with the
+ * channels etaS, etaL, c last.  Every call returns 0 or a negative PPF_E*.
+ */
+#ifndef PPF_UNET_H
+#define PPF_UNET_H
+
+#include <stdint.h>
+
+#include "ppf.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ppf_unet ppf_unet;
+
+/* base: channels at level 1 (32 or 64); seed: the seeded uniform init
+ * (He, for leaky ReLU); the head starts at zero, so an untrained network
+ * returns its input. */
+int ppf_unet_create(ppf_unet **out, int base, int device, uint64_t seed);
+int ppf_unet_destroy(ppf_unet *net);
+int ppf_unet_nparams(ppf_unet *net, int64_t *n);
+int ppf_unet_get_params(ppf_unet *net, float *params);
+int ppf_unet_set_params(ppf_unet *net, const float *params);
+
+/* y = net(x) for a batch of grids (host arrays). */
+int ppf_unet_forward(ppf_unet *net, int batch, int nz, int ny, int nx, const float *x, float *y);
+
+/* loss = mean((net(x) - y)^2) over batch, sites and 3 channels, and its
+ * gradient with respect to every parameter (flat, as get_params); no
+ * update.  grads may be NULL. */
+int ppf_unet_loss_grad(ppf_unet *net, int batch, int nz, int ny, int nx, const float *x,
+                       const float *y, float *loss, float *grads);
+
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on that loss. */
+int ppf_unet_train_step(ppf_unet *net, int batch, int nz, int ny, int nx, const float *x,
+                        const float *y, float lr, float *loss);
+
+/* Apply the network to a stepping context's current state in place (a
+ * jump of K steps).  c is then shifted uniformly so that sum(c) equals its
+ * value before the jump (FP64 sums), conserving solute exactly. */
+int ppf_unet_apply(ppf_unet *net, ppf_ctx *ctx);
+
+/* Milliseconds of the last forward pass, measured with HIP events, and
+ * its MFMA work in FLOP (2*27*Cin*Cout per site per 3x3x3 conv). */
+int ppf_unet_last_forward(ppf_unet *net, double *ms, double *flop);
+
+const char *ppf_unet_last_error(ppf_unet *net);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -39,7 +39,7 @@ def _run(cmd, cwd=REPO):
 
 def build_libppf():
     out = os.path.join(PKG, "libppf.so")
-    srcs = [os.path.join(CSRC, "ppf.hip"), os.path.join(CSRC, "ppf_ens.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("ppf.hip", "ppf_ens.hip", "unet.hip")]
     _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{CSRC}", *srcs, "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out

@@ -21,46 +21,7 @@
 #include <algorithm>
 #include <vector>
 
-namespace {
-
 constexpr int MAX_DIM = 1 << 16;
-
-struct TileCfg {
-    int tx, ty;
-};
-
-}  // namespace
-
-struct ppf_ctx {
-    ppf_desc d;
-    KConst k;
-    KConst32 k32;
-    bool eqa;
-    // FP32 solute bookkeeping: FP64 target of sum(c), and a mean shift of c
-    // back onto it every shift_every steps (0: never).
-    double c_target;
-    int shift_every;
-    int since_shift;
-    int64_t n_shifts;
-    int *ccl_labels;  // per-site labels and counts for ppf_particles, on first use
-    int *ccl_counts;
-    hipStream_t stream;
-    hipEvent_t ev0, ev1;
-    void *buf[2][3];  // [copy][field]: etaS, etaL, c
-    int cur[3];       // copy holding the current value of each field
-    size_t n;
-    size_t esz;
-    double *partials;
-    size_t partials_cap;  // in doubles
-    double *dres;         // 16 doubles of results
-    int64_t steps;
-    int kernel;           // resolved PPF_KERNEL_*
-    TileCfg tile;
-    int tz;
-    int order;  // workgroup order: bit 0 z-chunk fastest, bit 1 XCD-contiguous
-    const char *kname;
-    char err[512];
-};
 
 static char g_create_err[512];
 

@@ -1,0 +1,1185 @@
+// unet.hip -- the learned K-step jump (include/ppf_unet.h): a 3-level 3-D
+// convolutional U-Net with hand-written forward, backward and Adam kernels.
+//
+// Convolutions are implicit GEMMs on the gfx950 matrix cores
+// (v_mfma_f32_16x16x32_bf16: A 16x32 bf16, B 32x16 bf16, C 16x16 f32).  For a
+// 3x3x3 periodic convolution Y[v][co] = sum_tap sum_ci X[v+s_tap][ci] *
+// W[tap][co][ci], the M dimension is sites (16 per wave), N is output
+// channels and K runs over (tap, input channel).  Activations are stored
+// channels-last in BF16, so each lane's A operand (8 consecutive channels of
+// one site) and B operand (8 consecutive input channels of one weight row)
+// are single 16-byte loads.  The input gradient is the same convolution
+// with the weights flipped in space and transposed in channels; the weight
+// gradient contracts over sites (K = sites) and writes per-split partial
+// sums that a second pass adds in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "ppf_host.h"
+#include "ppf_unet.h"
+#include "k_misc.h"
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr float LEAK = 0.1f;
+constexpr int NCONV = 10;
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Site index -> (b, z, y, x) for a [B][D][H][W] grid.
+struct Coord {
+    int b, z, y, x;
+};
+
+__device__ __forceinline__ Coord coord_of(size_t v, size_t V, int H, int W)
+{
+    Coord c;
+    c.b = (int)(v / V);
+    const size_t r = v - (size_t)c.b * V;
+    const size_t hw = (size_t)H * W;
+    c.z = (int)(r / hw);
+    const size_t r2 = r - (size_t)c.z * hw;
+    c.y = (int)(r2 / W);
+    c.x = (int)(r2 - (size_t)c.y * W);
+    return c;
+}
+
+// ----------------------------------------------------------------------------
+// Convolution: MODE 0 stores bf16(leaky(acc + bias)); MODE 1 stores acc as
+// FP32; MODE 2 adds acc into FP32.  Each wave computes 16 sites x NJ*16
+// output channels; a workgroup is 4 waves (64 consecutive sites).  Cin is a
+// multiple of 32, CoutW (the padded weight rows) a multiple of 16.
+// ----------------------------------------------------------------------------
+template <int NJ, int MODE, int TAPS>
+__global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ X, int ldx, int xoff,
+                                              int Cin, const bf16 *__restrict__ Wt, int CoutW,
+                                              const float *__restrict__ bias, int Cout,
+                                              bf16 *__restrict__ Yb, float *__restrict__ Yf,
+                                              int ldy, int yoff, int B, int D, int H, int Wd)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t V = (size_t)D * H * Wd, total = V * B;
+    const size_t vbase = ((size_t)blockIdx.x * 4 + wave) * 16;
+    const int n0 = blockIdx.y * (NJ * 16);
+    const size_t va = vbase + (lane & 15);
+    const bool va_ok = va < total;
+    const Coord c = coord_of(va_ok ? va : 0, V, H, Wd);
+    const int kq = (lane >> 4) * 8;
+    f32x4 acc[NJ];
+    for (int j = 0; j < NJ; j++)
+        acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < TAPS; ++tap) {
+        int dz = 0, dy = 0, dx = 0;
+        if (TAPS == 27) {
+            dz = tap / 9 - 1;
+            dy = (tap / 3) % 3 - 1;
+            dx = tap % 3 - 1;
+        }
+        const size_t nv = (size_t)c.b * V +
+                          ((size_t)wrapi(c.z + dz, D) * H + wrapi(c.y + dy, H)) * Wd +
+                          wrapi(c.x + dx, Wd);
+        const bf16 *xa = X + nv * ldx + xoff + kq;
+        const bf16 *wb = Wt + ((size_t)tap * CoutW + n0 + (lane & 15)) * Cin + kq;
+        for (int kc = 0; kc < Cin; kc += 32) {
+            bf16x8 a;
+            if (va_ok)
+                a = *(const bf16x8 *)(xa + kc);
+            else
+                for (int j = 0; j < 8; j++)
+                    a[j] = (bf16)0.0f;
+#pragma unroll
+            for (int j = 0; j < NJ; j++) {
+                if (n0 + 16 * j < CoutW) {
+                    const bf16x8 bw = *(const bf16x8 *)(wb + (size_t)16 * j * Cin + kc);
+                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[j], 0, 0, 0);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int ch = n0 + 16 * j + (lane & 15);
+        if (ch >= Cout)
+            continue;
+        for (int r = 0; r < 4; r++) {
+            const size_t vox = vbase + (lane >> 4) * 4 + r;
+            if (vox >= total)
+                continue;
+            float v = acc[j][r];
+            const size_t o = vox * ldy + yoff + ch;
+            if (MODE == 0) {
+                v += bias[ch];
+                v = v > 0.f ? v : LEAK * v;
+                Yb[o] = (bf16)v;
+            } else if (MODE == 1) {
+                Yf[o] = v;
+            } else {
+                Yf[o] += v;
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Weight gradient partials: part[split][tap][co][ci] = sum over the split's
+// sites of G[v][co] * X[v+s_tap][ci].  A = G^T (16 co x 32 sites), B = the
+// shifted input (32 sites x 16 ci), gathered per lane.  Workgroup: one tap,
+// 64 output channels (16 per wave), 32 input channels (two B tiles).
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_wgrad(const bf16 *__restrict__ G, int ldg, int Cout,
+                                               const bf16 *__restrict__ X, int ldx, int xoff,
+                                               int CinStore, float *__restrict__ part, int nsplit,
+                                               size_t vps, int B, int D, int H, int Wd)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tap = blockIdx.x / nsplit, split = blockIdx.x % nsplit;
+    const int co0 = blockIdx.y * 64 + wave * 16;
+    const int ci0 = blockIdx.z * 32;
+    if (co0 >= Cout)
+        return;  // whole wave; no workgroup barrier follows
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const size_t V = (size_t)D * H * Wd, total = V * B;
+    const size_t vstart = (size_t)split * vps;
+    const size_t vend = std::min(total, vstart + vps);
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    const int co = co0 + (lane & 15);
+    const int ci = ci0 + (lane & 15);
+    for (size_t vb = vstart; vb < vend; vb += 32) {
+        bf16x8 a, b0, b1;
+        size_t v0 = vb + (lane >> 4) * 8;
+        Coord c = coord_of(v0 < total ? v0 : 0, V, H, Wd);
+        for (int j = 0; j < 8; j++) {
+            const size_t v = v0 + j;
+            if (v < vend) {
+                a[j] = G[v * ldg + co];
+                const size_t nv = (size_t)c.b * V +
+                                  ((size_t)wrapi(c.z + dz, D) * H + wrapi(c.y + dy, H)) * Wd +
+                                  wrapi(c.x + dx, Wd);
+                b0[j] = X[nv * ldx + xoff + ci];
+                b1[j] = X[nv * ldx + xoff + ci + 16];
+            } else {
+                a[j] = (bf16)0.0f;
+                b0[j] = (bf16)0.0f;
+                b1[j] = (bf16)0.0f;
+            }
+            // next site in x-fastest order
+            if (++c.x == Wd) {
+                c.x = 0;
+                if (++c.y == H) {
+                    c.y = 0;
+                    if (++c.z == D) {
+                        c.z = 0;
+                        ++c.b;
+                    }
+                }
+            }
+        }
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc1, 0, 0, 0);
+    }
+    for (int r = 0; r < 4; r++) {
+        const int o = co0 + (lane >> 4) * 4 + r;
+        if (o >= Cout)
+            continue;
+        float *p = part + (((size_t)split * 27 + tap) * Cout + o) * CinStore;
+        const int i0 = ci0 + (lane & 15);
+        if (i0 < CinStore)
+            p[i0] = acc0[r];
+        if (i0 + 16 < CinStore)
+            p[i0 + 16] = acc1[r];
+    }
+}
+
+// out[i] = sum_s part[s*n + i], fixed order.
+__global__ __launch_bounds__(256) void k_sum_splits(const float *__restrict__ part, int nsplit,
+                                                    size_t n, float *__restrict__ out)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        float s = 0.f;
+        for (int k = 0; k < nsplit; k++)
+            s += part[(size_t)k * n + i];
+        out[i] = s;
+    }
+}
+
+// Per-block column sums of G[v][c] (bias gradients): part[block][c].
+__global__ __launch_bounds__(256) void k_colsum(const bf16 *__restrict__ G, int ld, int C,
+                                                size_t total, size_t chunk,
+                                                float *__restrict__ part)
+{
+    const size_t v0 = (size_t)blockIdx.x * chunk, v1 = std::min(total, v0 + chunk);
+    for (int c = threadIdx.x; c < C; c += 256) {
+        float s = 0.f;
+        for (size_t v = v0; v < v1; v++)
+            s += (float)G[v * ld + c];
+        part[(size_t)blockIdx.x * C + c] = s;
+    }
+}
+
+// Gb[v][c] = bf16(g[v][c] * leaky'(y[v][c])); y > 0 exactly when the
+// pre-activation is, since the slope is positive.
+__global__ __launch_bounds__(256) void k_act_bwd(const float *__restrict__ g, int ldg, int goff,
+                                                 const bf16 *__restrict__ y, int ldy, int yoff,
+                                                 int C, size_t total, bf16 *__restrict__ Gb)
+{
+    const size_t n = total * C;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const size_t v = i / C;
+        const int c = (int)(i - v * C);
+        const float d = (float)y[v * ldy + yoff + c] > 0.f ? 1.f : LEAK;
+        Gb[i] = (bf16)(g[v * ldg + goff + c] * d);
+    }
+}
+
+// 2x2x2 average pool (coarse grid [B][D/2][H/2][W/2]).
+__global__ __launch_bounds__(256) void k_pool_fwd(const bf16 *__restrict__ S, int lds, int soff,
+                                                  int C, bf16 *__restrict__ P, int B, int D,
+                                                  int H, int W)
+{
+    const int D2 = D / 2, H2 = H / 2, W2 = W / 2;
+    const size_t V2 = (size_t)D2 * H2 * W2, n = V2 * B * C;
+    const size_t V = (size_t)D * H * W;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const size_t p = i / C;
+        const int c = (int)(i - p * C);
+        const Coord q = coord_of(p, V2, H2, W2);
+        float s = 0.f;
+        for (int k = 0; k < 8; k++) {
+            const size_t v = (size_t)q.b * V +
+                             ((size_t)(2 * q.z + (k >> 2)) * H + 2 * q.y + ((k >> 1) & 1)) * W +
+                             2 * q.x + (k & 1);
+            s += (float)S[v * lds + soff + c];
+        }
+        P[i] = (bf16)(s * 0.125f);
+    }
+}
+
+// gS[child][c] += gP[parent][c] / 8
+__global__ __launch_bounds__(256) void k_pool_bwd(const float *__restrict__ gP, int C,
+                                                  float *__restrict__ gS, int lds, int soff,
+                                                  int B, int D, int H, int W)
+{
+    const size_t V = (size_t)D * H * W, n = V * B * C;
+    const int H2 = H / 2, W2 = W / 2, D2 = D / 2;
+    const size_t V2 = (size_t)D2 * H2 * W2;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const size_t v = i / C;
+        const int c = (int)(i - v * C);
+        const Coord q = coord_of(v, V, H, W);
+        const size_t p = (size_t)q.b * V2 + ((size_t)(q.z >> 1) * H2 + (q.y >> 1)) * W2 + (q.x >> 1);
+        gS[v * lds + soff + c] += gP[p * C + c] * 0.125f;
+    }
+}
+
+// Nearest x2 upsample of S [B][D/2][H/2][W/2][C] into U (ldu, uoff).
+__global__ __launch_bounds__(256) void k_up_fwd(const bf16 *__restrict__ S, int C,
+                                                bf16 *__restrict__ U, int ldu, int uoff, int B,
+                                                int D, int H, int W)
+{
+    const size_t V = (size_t)D * H * W, n = V * B * C;
+    const int H2 = H / 2, W2 = W / 2, D2 = D / 2;
+    const size_t V2 = (size_t)D2 * H2 * W2;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const size_t v = i / C;
+        const int c = (int)(i - v * C);
+        const Coord q = coord_of(v, V, H, W);
+        const size_t p = (size_t)q.b * V2 + ((size_t)(q.z >> 1) * H2 + (q.y >> 1)) * W2 + (q.x >> 1);
+        U[v * ldu + uoff + c] = S[p * C + c];
+    }
+}
+
+// gS[parent][c] = sum over the 8 children of gU[child][c]
+__global__ __launch_bounds__(256) void k_up_bwd(const float *__restrict__ gU, int ldu, int uoff,
+                                                int C, float *__restrict__ gS, int B, int D,
+                                                int H, int W)
+{
+    const int D2 = D / 2, H2 = H / 2, W2 = W / 2;
+    const size_t V2 = (size_t)D2 * H2 * W2, n = V2 * B * C;
+    const size_t V = (size_t)D * H * W;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const size_t p = i / C;
+        const int c = (int)(i - p * C);
+        const Coord q = coord_of(p, V2, H2, W2);
+        float s = 0.f;
+        for (int k = 0; k < 8; k++) {
+            const size_t v = (size_t)q.b * V +
+                             ((size_t)(2 * q.z + (k >> 2)) * H + 2 * q.y + ((k >> 1) & 1)) * W +
+                             2 * q.x + (k & 1);
+            s += gU[v * ldu + uoff + c];
+        }
+        gS[i] = s;
+    }
+}
+
+// X0[v][c] = bf16(x[v][c]) for c < 3, 0 for 3 <= c < 32.
+__global__ __launch_bounds__(256) void k_pack_in(const float *__restrict__ x, size_t total,
+                                                 bf16 *__restrict__ X0)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total * 32;
+         i += (size_t)gridDim.x * 256) {
+        const size_t v = i >> 5;
+        const int c = (int)(i & 31);
+        X0[i] = c < 3 ? (bf16)x[v * 3 + c] : (bf16)0.0f;
+    }
+}
+
+// Head: out[v][c] = x[v][c] + b[c] + sum_ci W[c][ci] * F[v][ci]  (ci < CB)
+__global__ __launch_bounds__(256) void k_head_fwd(const bf16 *__restrict__ F, int CB,
+                                                  const float *__restrict__ hw,
+                                                  const float *__restrict__ hb,
+                                                  const float *__restrict__ x, size_t total,
+                                                  float *__restrict__ out)
+{
+    for (size_t v = (size_t)blockIdx.x * 256 + threadIdx.x; v < total; v += (size_t)gridDim.x * 256) {
+        float a0 = hb[0], a1 = hb[1], a2 = hb[2];
+        for (int ci = 0; ci < CB; ci++) {
+            const float f = (float)F[v * CB + ci];
+            a0 += hw[ci] * f;
+            a1 += hw[CB + ci] * f;
+            a2 += hw[2 * CB + ci] * f;
+        }
+        out[v * 3 + 0] = x[v * 3 + 0] + a0;
+        out[v * 3 + 1] = x[v * 3 + 1] + a1;
+        out[v * 3 + 2] = x[v * 3 + 2] + a2;
+    }
+}
+
+// gOut = 2 (out - y) / N; per-block loss partials; gF[v][ci] = sum_c gOut W[c][ci];
+// head partials: part[block][c*CB + ci] and part[block][3*CB + c].
+__global__ __launch_bounds__(256) void k_head_bwd(const float *__restrict__ out,
+                                                  const float *__restrict__ y,
+                                                  const bf16 *__restrict__ F, int CB,
+                                                  const float *__restrict__ hw, size_t total,
+                                                  size_t chunk, float inv_n,
+                                                  float *__restrict__ gF,
+                                                  float *__restrict__ part,
+                                                  float *__restrict__ lossp)
+{
+    __shared__ float s_g[256][3];
+    __shared__ float s_red[256];
+    const size_t v0 = (size_t)blockIdx.x * chunk, v1 = std::min(total, v0 + chunk);
+    const int np = 3 * CB + 3;
+    float acc = 0.f;   // thread t < np accumulates one head partial
+    float loss = 0.f;
+    for (size_t base = v0; base < v1; base += 256) {
+        const size_t v = base + threadIdx.x;
+        float g[3] = {0.f, 0.f, 0.f};
+        if (v < v1) {
+            for (int c = 0; c < 3; c++) {
+                const float d = out[v * 3 + c] - y[v * 3 + c];
+                loss += d * d;
+                g[c] = 2.f * d * inv_n;
+            }
+            for (int ci = 0; ci < CB; ci++)
+                gF[v * CB + ci] = g[0] * hw[ci] + g[1] * hw[CB + ci] + g[2] * hw[2 * CB + ci];
+        }
+        s_g[threadIdx.x][0] = g[0];
+        s_g[threadIdx.x][1] = g[1];
+        s_g[threadIdx.x][2] = g[2];
+        __syncthreads();
+        const int t = threadIdx.x;
+        if (t < np) {
+            const int cnt = (int)std::min<size_t>(256, v1 - base);
+            if (t < 3 * CB) {
+                const int c = t / CB, ci = t - c * CB;
+                for (int k = 0; k < cnt; k++)
+                    acc += s_g[k][c] * (float)F[(base + k) * CB + ci];
+            } else {
+                const int c = t - 3 * CB;
+                for (int k = 0; k < cnt; k++)
+                    acc += s_g[k][c];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < np)
+        part[(size_t)blockIdx.x * np + threadIdx.x] = acc;
+    s_red[threadIdx.x] = loss;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            s_red[threadIdx.x] += s_red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        lossp[blockIdx.x] = s_red[0];
+}
+
+// Adam on the flat FP32 parameters.
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float *__restrict__ g,
+                                              float *__restrict__ m, float *__restrict__ v,
+                                              size_t n, float lr, float b1, float b2, float eps,
+                                              float c1, float c2)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// BF16 weights of one 3x3x3 layer, [27][Cout][CinPad], and the flipped,
+// channel-transposed copy for the input gradient, [27][CinPad][Cout].
+__global__ __launch_bounds__(256) void k_prep_w(const float *__restrict__ w, int Cout, int CinLog,
+                                                int CinPad, bf16 *__restrict__ wb,
+                                                bf16 *__restrict__ wt)
+{
+    const size_t n = (size_t)27 * Cout * CinPad;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int ci = (int)(i % CinPad);
+        const int co = (int)((i / CinPad) % Cout);
+        const int tap = (int)(i / ((size_t)CinPad * Cout));
+        const float val = ci < CinLog ? w[((size_t)tap * Cout + co) * CinLog + ci] : 0.f;
+        wb[i] = (bf16)val;
+        wt[((size_t)(26 - tap) * CinPad + ci) * Cout + co] = (bf16)val;
+    }
+}
+
+// Pack a stepping context's current fields into x [V][3] (FP32).
+template <typename T>
+__global__ __launch_bounds__(256) void k_ctx_to_x(const T *__restrict__ s, const T *__restrict__ l,
+                                                  const T *__restrict__ c, size_t n,
+                                                  float *__restrict__ x)
+{
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        x[3 * q] = (float)s[q];
+        x[3 * q + 1] = (float)l[q];
+        x[3 * q + 2] = (float)c[q];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_y_to_ctx(const float *__restrict__ y, size_t n,
+                                                  T *__restrict__ s, T *__restrict__ l,
+                                                  T *__restrict__ c)
+{
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        s[q] = (T)y[3 * q];
+        l[q] = (T)y[3 * q + 1];
+        c[q] = (T)y[3 * q + 2];
+    }
+}
+
+// c += d[0] in FP64 (the solute projection of an FP64 context)
+__global__ __launch_bounds__(256) void k_add_delta(double *__restrict__ c, size_t n,
+                                                   const double *__restrict__ d)
+{
+    const double v = d[0];
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        c[q] += v;
+}
+
+struct Layer {
+    int cin_log, cin, cout;
+    size_t w_off, b_off;   // in the flat FP32 parameters
+    size_t wb_off;         // in the BF16 weight arrays
+};
+
+unsigned grid_for(size_t n) { return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 65536)); }
+
+}  // namespace
+
+struct ppf_unet {
+    int base, device;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    Layer L[NCONV];
+    size_t head_w, head_b;
+    int64_t np;
+    size_t nwb;
+    float *p, *g, *m, *v;   // FP32 parameters, gradient, Adam moments
+    bf16 *wb, *wt;          // BF16 weights and their flipped transposes
+    int64_t t;              // Adam steps
+    // workspace for one (batch, grid) shape
+    int wsB, wsD, wsH, wsW;
+    std::vector<void *> ws;
+    bf16 *X0, *E1a, *U1, *P1, *E2a, *U2, *P2, *E3a, *E3b, *D2a, *D2b, *D1a, *D1b, *Gb;
+    float *xin, *yout, *ytgt;
+    float *gD1b, *gD1a, *gU1, *gD2b, *gD2a, *gU2, *gE3b, *gE3a, *gP2, *gE2a, *gP1, *gE1a;
+    float *part;
+    size_t part_cap;
+    double last_ms, last_flop;
+    char err[512];
+};
+
+static char g_unet_err[512];
+
+static int uerr(ppf_unet *n, int code, const char *fmt, ...)
+{
+    char *dst = n ? n->err : g_unet_err;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(dst, 512, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define UHIP(n, call)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return uerr((n), PPF_EHIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+static void free_ws(ppf_unet *n)
+{
+    for (void *p : n->ws)
+        if (p)
+            hipFree(p);
+    n->ws.clear();
+    n->wsB = n->wsD = n->wsH = n->wsW = 0;
+}
+
+template <typename T>
+static int ws_alloc(ppf_unet *n, T **ptr, size_t count)
+{
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess)
+        return uerr(n, PPF_ENOMEM, "workspace allocation of %zu bytes failed", count * sizeof(T));
+    n->ws.push_back(p);
+    *ptr = (T *)p;
+    return PPF_OK;
+}
+
+static int ensure_ws(ppf_unet *n, int B, int D, int H, int W, bool train)
+{
+    if (n->wsB == B && n->wsD == D && n->wsH == H && n->wsW == W && (!train || n->gD1b))
+        return PPF_OK;
+    UHIP(n, hipStreamSynchronize(n->stream));
+    free_ws(n);
+    n->gD1b = nullptr;
+    const int C = n->base;
+    const size_t V1 = (size_t)B * D * H * W, V2 = V1 / 8, V3 = V1 / 64;
+    int rc = PPF_OK;
+#define A(ptr, cnt) \
+    if (!rc) rc = ws_alloc(n, &n->ptr, (cnt))
+    A(X0, V1 * C);
+    A(E1a, V1 * C);
+    A(U1, V1 * 3 * C);
+    A(P1, V2 * C);
+    A(E2a, V2 * 2 * C);
+    A(U2, V2 * 6 * C);
+    A(P2, V3 * 2 * C);
+    A(E3a, V3 * 4 * C);
+    A(E3b, V3 * 4 * C);
+    A(D2a, V2 * 2 * C);
+    A(D2b, V2 * 2 * C);
+    A(D1a, V1 * C);
+    A(D1b, V1 * C);
+    A(xin, V1 * 3);
+    A(yout, V1 * 3);
+    if (train) {
+        A(ytgt, V1 * 3);
+        A(Gb, std::max(V1 * C, std::max(V2 * 2 * C, V3 * 4 * C)));
+        A(gD1b, V1 * C);
+        A(gD1a, V1 * C);
+        A(gU1, V1 * 3 * C);
+        A(gD2b, V2 * 2 * C);
+        A(gD2a, V2 * 2 * C);
+        A(gU2, V2 * 6 * C);
+        A(gE3b, V3 * 4 * C);
+        A(gE3a, V3 * 4 * C);
+        A(gP2, V3 * 2 * C);
+        A(gE2a, V2 * 2 * C);
+        A(gP1, V2 * C);
+        A(gE1a, V1 * C);
+    }
+#undef A
+    if (rc) {
+        free_ws(n);
+        n->gD1b = nullptr;
+        return rc;
+    }
+    n->wsB = B;
+    n->wsD = D;
+    n->wsH = H;
+    n->wsW = W;
+    return PPF_OK;
+}
+
+static int ensure_part(ppf_unet *n, size_t count)
+{
+    if (count <= n->part_cap)
+        return PPF_OK;
+    if (n->part)
+        UHIP(n, hipFree(n->part));
+    n->part = nullptr;
+    n->part_cap = 0;
+    UHIP(n, hipMalloc(&n->part, count * sizeof(float)));
+    n->part_cap = count;
+    return PPF_OK;
+}
+
+static int prep_weights(ppf_unet *n)
+{
+    for (int i = 0; i < NCONV; i++) {
+        const Layer &l = n->L[i];
+        hipLaunchKernelGGL(k_prep_w, dim3(grid_for((size_t)27 * l.cout * l.cin)), dim3(256), 0,
+                           n->stream, n->p + l.w_off, l.cout, l.cin_log, l.cin, n->wb + l.wb_off,
+                           n->wt + l.wb_off);
+    }
+    UHIP(n, hipGetLastError());
+    return PPF_OK;
+}
+
+// Launch one 3x3x3 convolution of layer geometry (cin, cout) on the grid
+// [B][D][H][W].
+static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int cin,
+                 const bf16 *Wt, int coutw, const float *bias, int cout, bf16 *Yb, float *Yf,
+                 int ldy, int yoff, int B, int D, int H, int W)
+{
+    const size_t total = (size_t)B * D * H * W;
+    const dim3 grid((unsigned)((total + 63) / 64), (unsigned)((coutw + 63) / 64));
+    if (mode == 0)
+        hipLaunchKernelGGL((k_conv<4, 0, 27>), grid, dim3(256), 0, n->stream, X, ldx, xoff, cin,
+                           Wt, coutw, bias, cout, Yb, Yf, ldy, yoff, B, D, H, W);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_conv<4, 1, 27>), grid, dim3(256), 0, n->stream, X, ldx, xoff, cin,
+                           Wt, coutw, bias, cout, Yb, Yf, ldy, yoff, B, D, H, W);
+    else
+        hipLaunchKernelGGL((k_conv<4, 2, 27>), grid, dim3(256), 0, n->stream, X, ldx, xoff, cin,
+                           Wt, coutw, bias, cout, Yb, Yf, ldy, yoff, B, D, H, W);
+}
+
+static int forward(ppf_unet *n, int B, int D, int H, int W)
+{
+    const int C = n->base;
+    const size_t V1 = (size_t)B * D * H * W;
+    const int D2 = D / 2, H2 = H / 2, W2 = W / 2, D3 = D / 4, H3 = H / 4, W3 = W / 4;
+    const Layer *L = n->L;
+    auto wb = [&](int i) { return n->wb + L[i].wb_off; };
+    auto bias = [&](int i) { return n->p + L[i].b_off; };
+    hipLaunchKernelGGL(k_pack_in, dim3(grid_for(V1 * 32)), dim3(256), 0, n->stream, n->xin, V1,
+                       n->X0);
+    // level 1
+    conv(n, 0, n->X0, C, 0, L[0].cin, wb(0), L[0].cout, bias(0), L[0].cout, n->E1a, nullptr, C,
+         0, B, D, H, W);
+    conv(n, 0, n->E1a, C, 0, C, wb(1), C, bias(1), C, n->U1, nullptr, 3 * C, 2 * C, B, D, H, W);
+    hipLaunchKernelGGL(k_pool_fwd, dim3(grid_for(V1 / 8 * C)), dim3(256), 0, n->stream, n->U1,
+                       3 * C, 2 * C, C, n->P1, B, D, H, W);
+    // level 2
+    conv(n, 0, n->P1, C, 0, C, wb(2), 2 * C, bias(2), 2 * C, n->E2a, nullptr, 2 * C, 0, B, D2, H2,
+         W2);
+    conv(n, 0, n->E2a, 2 * C, 0, 2 * C, wb(3), 2 * C, bias(3), 2 * C, n->U2, nullptr, 6 * C, 4 * C,
+         B, D2, H2, W2);
+    hipLaunchKernelGGL(k_pool_fwd, dim3(grid_for(V1 / 64 * 2 * C)), dim3(256), 0, n->stream,
+                       n->U2, 6 * C, 4 * C, 2 * C, n->P2, B, D2, H2, W2);
+    // level 3
+    conv(n, 0, n->P2, 2 * C, 0, 2 * C, wb(4), 4 * C, bias(4), 4 * C, n->E3a, nullptr, 4 * C, 0, B,
+         D3, H3, W3);
+    conv(n, 0, n->E3a, 4 * C, 0, 4 * C, wb(5), 4 * C, bias(5), 4 * C, n->E3b, nullptr, 4 * C, 0, B,
+         D3, H3, W3);
+    // decoder
+    hipLaunchKernelGGL(k_up_fwd, dim3(grid_for(V1 / 8 * 4 * C)), dim3(256), 0, n->stream, n->E3b,
+                       4 * C, n->U2, 6 * C, 0, B, D2, H2, W2);
+    conv(n, 0, n->U2, 6 * C, 0, 6 * C, wb(6), 2 * C, bias(6), 2 * C, n->D2a, nullptr, 2 * C, 0, B,
+         D2, H2, W2);
+    conv(n, 0, n->D2a, 2 * C, 0, 2 * C, wb(7), 2 * C, bias(7), 2 * C, n->D2b, nullptr, 2 * C, 0, B,
+         D2, H2, W2);
+    hipLaunchKernelGGL(k_up_fwd, dim3(grid_for(V1 * 2 * C)), dim3(256), 0, n->stream, n->D2b,
+                       2 * C, n->U1, 3 * C, 0, B, D, H, W);
+    conv(n, 0, n->U1, 3 * C, 0, 3 * C, wb(8), C, bias(8), C, n->D1a, nullptr, C, 0, B, D, H, W);
+    conv(n, 0, n->D1a, C, 0, C, wb(9), C, bias(9), C, n->D1b, nullptr, C, 0, B, D, H, W);
+    hipLaunchKernelGGL(k_head_fwd, dim3(grid_for(V1)), dim3(256), 0, n->stream, n->D1b, C,
+                       n->p + n->head_w, n->p + n->head_b, n->xin, V1, n->yout);
+    UHIP(n, hipGetLastError());
+    return PPF_OK;
+}
+
+// Weight and bias gradient of layer i into n->g, from Gb (bf16 [sites][cout])
+// and the layer input X (ldx, xoff).
+static int layer_wgrad(ppf_unet *n, int i, const bf16 *X, int ldx, int xoff, int B, int D, int H,
+                       int W)
+{
+    const Layer &l = n->L[i];
+    const size_t total = (size_t)B * D * H * W;
+    const int ncit = l.cin / 32, ncot = (l.cout + 63) / 64;
+    // splits: enough workgroups to fill the device, at least 32 sites each
+    int nsplit = (int)std::max<size_t>(1, std::min<size_t>((total + 31) / 32,
+                                                          4096 / (27 * ncit * ncot) + 1));
+    size_t vps = ((total + nsplit - 1) / nsplit + 31) / 32 * 32;
+    nsplit = (int)((total + vps - 1) / vps);
+    const size_t nw = (size_t)27 * l.cout * l.cin_log;
+    int rc = ensure_part(n, (size_t)nsplit * nw);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_wgrad, dim3(27 * nsplit, ncot, ncit), dim3(256), 0, n->stream, n->Gb,
+                       l.cout, l.cout, X, ldx, xoff, l.cin_log, n->part, nsplit, vps, B, D, H, W);
+    hipLaunchKernelGGL(k_sum_splits, dim3(grid_for(nw)), dim3(256), 0, n->stream, n->part, nsplit,
+                       nw, n->g + l.w_off);
+    // bias: column sums of Gb
+    const size_t chunk = 4096;
+    const int nb = (int)((total + chunk - 1) / chunk);
+    rc = ensure_part(n, (size_t)nb * l.cout);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, n->stream, n->Gb, l.cout, l.cout, total,
+                       chunk, n->part);
+    hipLaunchKernelGGL(k_sum_splits, dim3(grid_for(l.cout)), dim3(256), 0, n->stream, n->part, nb,
+                       (size_t)l.cout, n->g + l.b_off);
+    UHIP(n, hipGetLastError());
+    return PPF_OK;
+}
+
+// Gb = g * leaky'(y) for layer output y (bf16, ldy, yoff) with C channels.
+static void act_bwd(ppf_unet *n, const float *g, int ldg, int goff, const bf16 *y, int ldy,
+                    int yoff, int C, size_t total)
+{
+    hipLaunchKernelGGL(k_act_bwd, dim3(grid_for(total * C)), dim3(256), 0, n->stream, g, ldg, goff,
+                       y, ldy, yoff, C, total, n->Gb);
+}
+
+// Backward through the whole network after forward(); loss into *loss.
+static int backward(ppf_unet *n, int B, int D, int H, int W, float *loss)
+{
+    const int C = n->base;
+    const size_t V1 = (size_t)B * D * H * W, V2 = V1 / 8, V3 = V1 / 64;
+    const int D2 = D / 2, H2 = H / 2, W2 = W / 2, D3 = D / 4, H3 = H / 4, W3 = W / 4;
+    const Layer *L = n->L;
+    auto wt = [&](int i) { return n->wt + L[i].wb_off; };
+    // head
+    const size_t chunk = 4096;
+    const int nb = (int)((V1 + chunk - 1) / chunk);
+    const int np = 3 * C + 3;
+    int rc = ensure_part(n, (size_t)nb * np + nb);
+    if (rc)
+        return rc;
+    float *lossp = n->part + (size_t)nb * np;
+    hipLaunchKernelGGL(k_head_bwd, dim3(nb), dim3(256), 0, n->stream, n->yout, n->ytgt, n->D1b, C,
+                       n->p + n->head_w, V1, chunk, 1.0f / (float)(V1 * 3), n->gD1b, n->part,
+                       lossp);
+    hipLaunchKernelGGL(k_sum_splits, dim3(grid_for(np)), dim3(256), 0, n->stream, n->part, nb,
+                       (size_t)np, n->g + n->head_w);
+    UHIP(n, hipGetLastError());
+    std::vector<float> lp(nb);
+    UHIP(n, hipMemcpyAsync(lp.data(), lossp, nb * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    // L9: D1a -> D1b
+    act_bwd(n, n->gD1b, C, 0, n->D1b, C, 0, C, V1);
+    if ((rc = layer_wgrad(n, 9, n->D1a, C, 0, B, D, H, W)))
+        return rc;
+    conv(n, 1, n->Gb, C, 0, C, wt(9), C, nullptr, C, nullptr, n->gD1a, C, 0, B, D, H, W);
+    // L8: U1 -> D1a
+    act_bwd(n, n->gD1a, C, 0, n->D1a, C, 0, C, V1);
+    if ((rc = layer_wgrad(n, 8, n->U1, 3 * C, 0, B, D, H, W)))
+        return rc;
+    conv(n, 1, n->Gb, C, 0, C, wt(8), 3 * C, nullptr, 3 * C, nullptr, n->gU1, 3 * C, 0, B, D, H, W);
+    // up1
+    hipLaunchKernelGGL(k_up_bwd, dim3(grid_for(V2 * 2 * C)), dim3(256), 0, n->stream, n->gU1, 3 * C,
+                       0, 2 * C, n->gD2b, B, D, H, W);
+    // L7: D2a -> D2b
+    act_bwd(n, n->gD2b, 2 * C, 0, n->D2b, 2 * C, 0, 2 * C, V2);
+    if ((rc = layer_wgrad(n, 7, n->D2a, 2 * C, 0, B, D2, H2, W2)))
+        return rc;
+    conv(n, 1, n->Gb, 2 * C, 0, 2 * C, wt(7), 2 * C, nullptr, 2 * C, nullptr, n->gD2a, 2 * C, 0, B,
+         D2, H2, W2);
+    // L6: U2 -> D2a
+    act_bwd(n, n->gD2a, 2 * C, 0, n->D2a, 2 * C, 0, 2 * C, V2);
+    if ((rc = layer_wgrad(n, 6, n->U2, 6 * C, 0, B, D2, H2, W2)))
+        return rc;
+    conv(n, 1, n->Gb, 2 * C, 0, 2 * C, wt(6), 6 * C, nullptr, 6 * C, nullptr, n->gU2, 6 * C, 0, B,
+         D2, H2, W2);
+    // up2
+    hipLaunchKernelGGL(k_up_bwd, dim3(grid_for(V3 * 4 * C)), dim3(256), 0, n->stream, n->gU2, 6 * C,
+                       0, 4 * C, n->gE3b, B, D2, H2, W2);
+    // L5: E3a -> E3b
+    act_bwd(n, n->gE3b, 4 * C, 0, n->E3b, 4 * C, 0, 4 * C, V3);
+    if ((rc = layer_wgrad(n, 5, n->E3a, 4 * C, 0, B, D3, H3, W3)))
+        return rc;
+    conv(n, 1, n->Gb, 4 * C, 0, 4 * C, wt(5), 4 * C, nullptr, 4 * C, nullptr, n->gE3a, 4 * C, 0, B,
+         D3, H3, W3);
+    // L4: P2 -> E3a
+    act_bwd(n, n->gE3a, 4 * C, 0, n->E3a, 4 * C, 0, 4 * C, V3);
+    if ((rc = layer_wgrad(n, 4, n->P2, 2 * C, 0, B, D3, H3, W3)))
+        return rc;
+    conv(n, 1, n->Gb, 4 * C, 0, 4 * C, wt(4), 2 * C, nullptr, 2 * C, nullptr, n->gP2, 2 * C, 0, B,
+         D3, H3, W3);
+    // pool2: into the skip part of gU2
+    hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for(V2 * 2 * C)), dim3(256), 0, n->stream, n->gP2,
+                       2 * C, n->gU2, 6 * C, 4 * C, B, D2, H2, W2);
+    // L3: E2a -> E2b (= U2[:, 4C:6C])
+    act_bwd(n, n->gU2, 6 * C, 4 * C, n->U2, 6 * C, 4 * C, 2 * C, V2);
+    if ((rc = layer_wgrad(n, 3, n->E2a, 2 * C, 0, B, D2, H2, W2)))
+        return rc;
+    conv(n, 1, n->Gb, 2 * C, 0, 2 * C, wt(3), 2 * C, nullptr, 2 * C, nullptr, n->gE2a, 2 * C, 0, B,
+         D2, H2, W2);
+    // L2: P1 -> E2a
+    act_bwd(n, n->gE2a, 2 * C, 0, n->E2a, 2 * C, 0, 2 * C, V2);
+    if ((rc = layer_wgrad(n, 2, n->P1, C, 0, B, D2, H2, W2)))
+        return rc;
+    conv(n, 1, n->Gb, 2 * C, 0, 2 * C, wt(2), C, nullptr, C, nullptr, n->gP1, C, 0, B, D2, H2, W2);
+    // pool1: into the skip part of gU1
+    hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for(V1 * C)), dim3(256), 0, n->stream, n->gP1, C,
+                       n->gU1, 3 * C, 2 * C, B, D, H, W);
+    // L1: E1a -> E1b (= U1[:, 2C:3C])
+    act_bwd(n, n->gU1, 3 * C, 2 * C, n->U1, 3 * C, 2 * C, C, V1);
+    if ((rc = layer_wgrad(n, 1, n->E1a, C, 0, B, D, H, W)))
+        return rc;
+    conv(n, 1, n->Gb, C, 0, C, wt(1), C, nullptr, C, nullptr, n->gE1a, C, 0, B, D, H, W);
+    // L0: X0 -> E1a (no input gradient needed)
+    act_bwd(n, n->gE1a, C, 0, n->E1a, C, 0, C, V1);
+    if ((rc = layer_wgrad(n, 0, n->X0, C, 0, B, D, H, W)))
+        return rc;
+    UHIP(n, hipGetLastError());
+    UHIP(n, hipStreamSynchronize(n->stream));
+    double s = 0.0;
+    for (float v : lp)
+        s += v;
+    *loss = (float)(s / (double)(V1 * 3));
+    return PPF_OK;
+}
+
+static bool dims_ok(int B, int D, int H, int W)
+{
+    return B >= 1 && D >= 4 && H >= 4 && W >= 4 && D % 4 == 0 && H % 4 == 0 && W % 4 == 0 &&
+           (size_t)B * D * H * W < ((size_t)1 << 31);
+}
+
+extern "C" int ppf_unet_destroy(ppf_unet *n)
+{
+    if (!n)
+        return PPF_OK;
+    hipSetDevice(n->device);
+    if (n->stream)
+        hipStreamSynchronize(n->stream);
+    free_ws(n);
+    void *bufs[] = {n->p, n->g, n->m, n->v, n->wb, n->wt, n->part};
+    for (void *b : bufs)
+        if (b)
+            hipFree(b);
+    if (n->ev0)
+        hipEventDestroy(n->ev0);
+    if (n->ev1)
+        hipEventDestroy(n->ev1);
+    if (n->stream)
+        hipStreamDestroy(n->stream);
+    delete n;
+    return PPF_OK;
+}
+
+extern "C" int ppf_unet_create(ppf_unet **out, int base, int device, uint64_t seed)
+{
+    if (!out || (base != 32 && base != 64))
+        return uerr(nullptr, PPF_EARG, "base channels must be 32 or 64");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return uerr(nullptr, PPF_ENODEV, "device %d not present", device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return uerr(nullptr, PPF_ENODEV, "device %d is not gfx950", device);
+    if (hipSetDevice(device) != hipSuccess)
+        return uerr(nullptr, PPF_EHIP, "hipSetDevice failed");
+    ppf_unet *n = new ppf_unet();
+    n->base = base;
+    n->device = device;
+    const int C = base;
+    // (cin_log, cin padded, cout) of the ten 3x3x3 convolutions
+    const int geo[NCONV][3] = {{3, 32, C},          {C, C, C},         {C, C, 2 * C},
+                               {2 * C, 2 * C, 2 * C}, {2 * C, 2 * C, 4 * C}, {4 * C, 4 * C, 4 * C},
+                               {6 * C, 6 * C, 2 * C}, {2 * C, 2 * C, 2 * C}, {3 * C, 3 * C, C},
+                               {C, C, C}};
+    size_t off = 0, woff = 0;
+    for (int i = 0; i < NCONV; i++) {
+        Layer &l = n->L[i];
+        l.cin_log = geo[i][0];
+        l.cin = geo[i][1];
+        l.cout = geo[i][2];
+        l.w_off = off;
+        off += (size_t)27 * l.cout * l.cin_log;
+        l.b_off = off;
+        off += l.cout;
+        l.wb_off = woff;
+        woff += (size_t)27 * l.cout * l.cin;
+    }
+    n->head_w = off;
+    off += 3 * (size_t)C;
+    n->head_b = off;
+    off += 3;
+    n->np = (int64_t)off;
+    n->nwb = woff;
+    // seeded init: He-uniform for the 3x3x3 layers (gain for leaky ReLU),
+    // zero biases, zero head (the untrained network returns its input)
+    std::vector<float> h(off, 0.0f);
+    uint64_t st = seed;
+    auto u01 = [&]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * 0x1.0p-53;
+    };
+    for (int i = 0; i < NCONV; i++) {
+        const Layer &l = n->L[i];
+        const double fan_in = 27.0 * l.cin_log;
+        const double a = sqrt(6.0 / ((1.0 + LEAK * LEAK) * fan_in));
+        for (size_t q = 0; q < (size_t)27 * l.cout * l.cin_log; q++)
+            h[l.w_off + q] = (float)((2.0 * u01() - 1.0) * a);
+    }
+    hipError_t e = hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipEventCreate(&n->ev0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&n->ev1);
+    if (e == hipSuccess)
+        e = hipMalloc(&n->p, off * sizeof(float));
+    if (e == hipSuccess)
+        e = hipMalloc(&n->g, off * sizeof(float));
+    if (e == hipSuccess)
+        e = hipMalloc(&n->m, off * sizeof(float));
+    if (e == hipSuccess)
+        e = hipMalloc(&n->v, off * sizeof(float));
+    if (e == hipSuccess)
+        e = hipMalloc(&n->wb, woff * sizeof(bf16));
+    if (e == hipSuccess)
+        e = hipMalloc(&n->wt, woff * sizeof(bf16));
+    if (e == hipSuccess)
+        e = hipMemcpy(n->p, h.data(), off * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemset(n->m, 0, off * sizeof(float));
+    if (e == hipSuccess)
+        e = hipMemset(n->v, 0, off * sizeof(float));
+    if (e == hipSuccess)
+        e = hipMemset(n->g, 0, off * sizeof(float));
+    if (e != hipSuccess) {
+        uerr(nullptr, PPF_ENOMEM, "network allocation failed: %s", hipGetErrorString(e));
+        ppf_unet_destroy(n);
+        return PPF_ENOMEM;
+    }
+    int rc = prep_weights(n);
+    if (rc == PPF_OK && hipStreamSynchronize(n->stream) != hipSuccess)
+        rc = PPF_EHIP;
+    if (rc) {
+        uerr(nullptr, rc, "weight preparation failed");
+        ppf_unet_destroy(n);
+        return rc;
+    }
+    *out = n;
+    return PPF_OK;
+}
+
+extern "C" int ppf_unet_nparams(ppf_unet *n, int64_t *np)
+{
+    if (!n || !np)
+        return PPF_EARG;
+    *np = n->np;
+    return PPF_OK;
+}
+
+extern "C" int ppf_unet_get_params(ppf_unet *n, float *params)
+{
+    if (!n || !params)
+        return PPF_EARG;
+    UHIP(n, hipSetDevice(n->device));
+    UHIP(n, hipStreamSynchronize(n->stream));
+    UHIP(n, hipMemcpy(params, n->p, n->np * sizeof(float), hipMemcpyDeviceToHost));
+    return PPF_OK;
+}
+
+extern "C" int ppf_unet_set_params(ppf_unet *n, const float *params)
+{
+    if (!n || !params)
+        return PPF_EARG;
+    UHIP(n, hipSetDevice(n->device));
+    UHIP(n, hipStreamSynchronize(n->stream));
+    UHIP(n, hipMemcpy(n->p, params, n->np * sizeof(float), hipMemcpyHostToDevice));
+    int rc = prep_weights(n);
+    if (rc)
+        return rc;
+    UHIP(n, hipStreamSynchronize(n->stream));
+    return PPF_OK;
+}
+
+static double forward_flop(const ppf_unet *n, size_t V1)
+{
+    double f = 0.0;
+    const size_t vox[NCONV] = {V1, V1, V1 / 8, V1 / 8, V1 / 64, V1 / 64, V1 / 8, V1 / 8, V1, V1};
+    for (int i = 0; i < NCONV; i++)
+        f += 2.0 * 27.0 * n->L[i].cin * n->L[i].cout * (double)vox[i];
+    return f;
+}
+
+extern "C" int ppf_unet_forward(ppf_unet *n, int B, int D, int H, int W, const float *x, float *y)
+{
+    if (!n || !x || !y || !dims_ok(B, D, H, W))
+        return n ? uerr(n, PPF_EARG, "bad forward arguments") : PPF_EARG;
+    UHIP(n, hipSetDevice(n->device));
+    int rc = ensure_ws(n, B, D, H, W, false);
+    if (rc)
+        return rc;
+    const size_t V1 = (size_t)B * D * H * W;
+    UHIP(n, hipMemcpyAsync(n->xin, x, V1 * 3 * sizeof(float), hipMemcpyHostToDevice, n->stream));
+    UHIP(n, hipEventRecord(n->ev0, n->stream));
+    if ((rc = forward(n, B, D, H, W)))
+        return rc;
+    UHIP(n, hipEventRecord(n->ev1, n->stream));
+    UHIP(n, hipMemcpyAsync(y, n->yout, V1 * 3 * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    UHIP(n, hipStreamSynchronize(n->stream));
+    float ms = 0.f;
+    UHIP(n, hipEventElapsedTime(&ms, n->ev0, n->ev1));
+    n->last_ms = ms;
+    n->last_flop = forward_flop(n, V1);
+    return PPF_OK;
+}
+
+static int loss_grad(ppf_unet *n, int B, int D, int H, int W, const float *x, const float *y,
+                     float *loss)
+{
+    int rc = ensure_ws(n, B, D, H, W, true);
+    if (rc)
+        return rc;
+    const size_t V1 = (size_t)B * D * H * W;
+    UHIP(n, hipMemcpyAsync(n->xin, x, V1 * 3 * sizeof(float), hipMemcpyHostToDevice, n->stream));
+    UHIP(n, hipMemcpyAsync(n->ytgt, y, V1 * 3 * sizeof(float), hipMemcpyHostToDevice, n->stream));
+    if ((rc = forward(n, B, D, H, W)))
+        return rc;
+    return backward(n, B, D, H, W, loss);
+}
+
+extern "C" int ppf_unet_loss_grad(ppf_unet *n, int B, int D, int H, int W, const float *x,
+                                  const float *y, float *loss, float *grads)
+{
+    if (!n || !x || !y || !loss || !dims_ok(B, D, H, W))
+        return n ? uerr(n, PPF_EARG, "bad arguments") : PPF_EARG;
+    UHIP(n, hipSetDevice(n->device));
+    int rc = loss_grad(n, B, D, H, W, x, y, loss);
+    if (rc)
+        return rc;
+    if (grads)
+        UHIP(n, hipMemcpy(grads, n->g, n->np * sizeof(float), hipMemcpyDeviceToHost));
+    return PPF_OK;
+}
+
+extern "C" int ppf_unet_train_step(ppf_unet *n, int B, int D, int H, int W, const float *x,
+                                   const float *y, float lr, float *loss)
+{
+    if (!n || !x || !y || !loss || !dims_ok(B, D, H, W) || !(lr >= 0.f))
+        return n ? uerr(n, PPF_EARG, "bad arguments") : PPF_EARG;
+    UHIP(n, hipSetDevice(n->device));
+    int rc = loss_grad(n, B, D, H, W, x, y, loss);
+    if (rc)
+        return rc;
+    n->t++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)n->t), c2 = 1.f - powf(b2, (float)n->t);
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(n->np)), dim3(256), 0, n->stream, n->p, n->g, n->m,
+                       n->v, (size_t)n->np, lr, b1, b2, 1e-8f, c1, c2);
+    UHIP(n, hipGetLastError());
+    if ((rc = prep_weights(n)))
+        return rc;
+    UHIP(n, hipStreamSynchronize(n->stream));
+    return PPF_OK;
+}
+
+extern "C" int ppf_unet_apply(ppf_unet *n, ppf_ctx *ctx)
+{
+    if (!n || !ctx)
+        return n ? uerr(n, PPF_EARG, "null context") : PPF_EARG;
+    const int D = ctx->d.nz, H = ctx->d.ny, W = ctx->d.nx;
+    if (!dims_ok(1, D, H, W))
+        return uerr(n, PPF_EARG, "grid %dx%dx%d: every dimension must be a multiple of 4", D, H,
+                    W);
+    if (ctx->d.device != n->device)
+        return uerr(n, PPF_EARG, "network and context are on different devices");
+    UHIP(n, hipSetDevice(n->device));
+    int rc = ensure_ws(n, 1, D, H, W, false);
+    if (rc)
+        return rc;
+    // the context's queued steps finish before the network reads its fields
+    UHIP(n, hipStreamSynchronize(ctx->stream));
+    const size_t N = ctx->n;
+    void *fs = ctx->buf[ctx->cur[0]][0], *fl = ctx->buf[ctx->cur[1]][1],
+         *fc = ctx->buf[ctx->cur[2]][2];
+    const bool f64 = ctx->esz == 8;
+    // solute before the jump (FP64, double-double) into dres[0..1]
+    double *tmp = nullptr;
+    UHIP(n, hipMalloc(&tmp, (2 * 1024 + 4) * sizeof(double)));
+    if (f64)
+        hipLaunchKernelGGL(k_sum_dd<double>, dim3(1024), dim3(256), 0, n->stream,
+                           (const double *)fc, N, tmp);
+    else
+        hipLaunchKernelGGL(k_sum_dd<float>, dim3(1024), dim3(256), 0, n->stream,
+                           (const float *)fc, N, tmp);
+    hipLaunchKernelGGL(k_mass_delta, dim3(1), dim3(256), 0, n->stream, tmp, 1024, 0.0, 1.0,
+                       tmp + 2048);
+    if (f64)
+        hipLaunchKernelGGL(k_ctx_to_x<double>, dim3(grid_for(N)), dim3(256), 0, n->stream,
+                           (const double *)fs, (const double *)fl, (const double *)fc, N, n->xin);
+    else
+        hipLaunchKernelGGL(k_ctx_to_x<float>, dim3(grid_for(N)), dim3(256), 0, n->stream,
+                           (const float *)fs, (const float *)fl, (const float *)fc, N, n->xin);
+    hipEventRecord(n->ev0, n->stream);
+    rc = forward(n, 1, D, H, W);
+    hipEventRecord(n->ev1, n->stream);
+    if (rc) {
+        hipFree(tmp);
+        return rc;
+    }
+    if (f64)
+        hipLaunchKernelGGL(k_y_to_ctx<double>, dim3(grid_for(N)), dim3(256), 0, n->stream, n->yout,
+                           N, (double *)fs, (double *)fl, (double *)fc);
+    else
+        hipLaunchKernelGGL(k_y_to_ctx<float>, dim3(grid_for(N)), dim3(256), 0, n->stream, n->yout,
+                           N, (float *)fs, (float *)fl, (float *)fc);
+    // project c back onto the pre-jump total
+    double before[2];
+    hipError_t e = hipMemcpyAsync(before, tmp + 2048, sizeof before, hipMemcpyDeviceToHost,
+                                  n->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(n->stream);
+    if (e == hipSuccess) {
+        if (f64) {
+            hipLaunchKernelGGL(k_sum_dd<double>, dim3(1024), dim3(256), 0, n->stream,
+                               (const double *)fc, N, tmp);
+            hipLaunchKernelGGL(k_mass_delta, dim3(1), dim3(256), 0, n->stream, tmp, 1024,
+                               before[1], (double)N, tmp + 2048);
+            hipLaunchKernelGGL(k_add_delta, dim3(grid_for(N)), dim3(256), 0, n->stream,
+                               (double *)fc, N, (const double *)(tmp + 2048));
+        } else {
+            hipLaunchKernelGGL(k_sum_dd<float>, dim3(1024), dim3(256), 0, n->stream,
+                               (const float *)fc, N, tmp);
+            hipLaunchKernelGGL(k_mass_delta, dim3(1), dim3(256), 0, n->stream, tmp, 1024,
+                               before[1], (double)N, tmp + 2048);
+            hipLaunchKernelGGL(k_mass_shift, dim3(grid_for(N)), dim3(256), 0, n->stream,
+                               (float *)fc, N, (const double *)(tmp + 2048), (uint64_t)ctx->steps);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(n->stream);
+    hipFree(tmp);
+    if (e != hipSuccess)
+        return uerr(n, PPF_EHIP, "apply failed: %s", hipGetErrorString(e));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, n->ev0, n->ev1);
+    n->last_ms = ms;
+    n->last_flop = forward_flop(n, N);
+    return PPF_OK;
+}
+
+extern "C" int ppf_unet_last_forward(ppf_unet *n, double *ms, double *flop)
+{
+    if (!n || !ms || !flop)
+        return PPF_EARG;
+    *ms = n->last_ms;
+    *flop = n->last_flop;
+    return PPF_OK;
+}
+
+extern "C" const char *ppf_unet_last_error(ppf_unet *n) { return n ? n->err : g_unet_err; }

@@ -1,0 +1,110 @@
+"""The GPU U-Net (BF16 MFMA forward, backward, Adam) against the torch CPU
+float64 reference of the same network (tests/unet_ref.py)."""
+import numpy as np
+import pytest
+
+from phasefield_coarsening_amd import solver as S
+from phasefield_coarsening_amd.unet import UNet, layer_shapes
+from tests import unet_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(seed, head=0.05):
+    """He-scaled random weights (the regime the network runs in), small
+    random biases, a head of scale `head`."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for name, s in layer_shapes(32):
+        if name.startswith("w"):
+            out.append(rng.normal(0, np.sqrt(2.0 / (27 * s[2])), int(np.prod(s))))
+        elif name == "head_w":
+            out.append(rng.normal(0, head, int(np.prod(s))))
+        else:
+            out.append(rng.normal(0, 0.01, int(np.prod(s))))
+    return np.concatenate(out).astype(np.float32)
+
+
+def _data(seed, shape=(2, 16, 16, 16)):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1, shape + (3,)).astype(np.float32)
+    y = (x + rng.normal(0, 0.05, x.shape)).astype(np.float32)
+    return x, y
+
+
+def test_forward_matches_reference():
+    p = _params(1)
+    x, _ = _data(2)
+    with UNet(seed=3) as net:
+        net.set_params(p)
+        got = net.forward(x)
+    want = unet_ref.predict(p, x, bf16=True)
+    d = np.abs(got - want)
+    scale = np.abs(want - x).max()
+    print("forward: max |d| %.3e, mean |d| %.3e, head scale %.3e" % (d.max(), d.mean(), scale))
+    # same bf16 rounding points; FP32 vs FP64 accumulation can flip a bf16
+    # rounding here and there
+    assert d.max() < 2e-2 * scale and d.mean() < 3e-3 * scale
+    # and close to the unrounded float64 network
+    assert np.abs(got - unet_ref.predict(p, x, bf16=False)).mean() < 2e-2 * scale
+
+
+def test_gradient_matches_reference():
+    p = _params(4)
+    x, y = _data(5)
+    with UNet(seed=3) as net:
+        net.set_params(p)
+        loss, g = net.loss_grad(x, y)
+    lr, gr = unet_ref.loss_and_grad(p, x, y, bf16=True)
+    assert abs(loss - lr) / lr < 1e-3
+    o = 0
+    for name, shp in layer_shapes(32):
+        n = int(np.prod(shp))
+        a, b = g[o:o + n].astype(np.float64), gr[o:o + n]
+        o += n
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
+        print(f"{name}: rel {rel:.2e} cos {cos:.6f}")
+        assert rel < 5e-2 and cos > 0.998, name
+
+
+def _pairs(shape, seeds, steps):
+    xs, ys = [], []
+    for sd in seeds:
+        cen, rad = S.spheres(sd, 6, shape, 3.0, 6.0)
+        with S.Solver(shape, precision=S.F32) as sv:
+            sv.init_spheres(cen, rad)
+            xs.append(np.stack(sv.download(), axis=-1))
+            sv.step(steps)
+            ys.append(np.stack(sv.download(), axis=-1))
+    return np.array(xs, np.float32), np.array(ys, np.float32)
+
+
+def test_training_reduces_loss():
+    """Real phase-field pairs: sphere states and the same states 20 FP32
+    steps later.  Adam with the library's own kernels must cut the loss."""
+    x, y = _pairs((32, 32, 32), (11, 12), 20)
+    with UNet(seed=7) as net:
+        l0 = net.train_step(x, y, 1e-3)
+        for _ in range(60):
+            l1 = net.train_step(x, y, 1e-3)
+    print("loss", l0, "->", l1)
+    assert l1 < 0.7 * l0
+
+
+def test_apply_jumps_state_and_conserves_solute():
+    shape = (32, 32, 32)
+    cen, rad = S.spheres(3, 8, shape, 3.0, 6.0)
+    for prec in (S.F64, S.F32):
+        with S.Solver(shape, precision=prec) as sv, UNet(seed=8) as net:
+            sv.init_spheres(cen, rad)
+            before = sv.download()
+            c0 = sv.reduce()["sum_c"]
+            net.set_params(_params(9, 0.01))
+            net.apply(sv)
+            after = sv.download()
+            c1 = sv.reduce()["sum_c"]
+            assert np.abs(after[0] - before[0]).max() > 1e-4
+            assert abs(c1 - c0) / c0 < (1e-13 if prec == S.F64 else 1e-9)
+            ms, flop = net.last_forward()
+            assert ms > 0 and flop > 0
