@@ -188,6 +188,12 @@ int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms);
 int ppf_set_mass_correction(ppf_ctx *ctx, int every);
 int ppf_mass_target(ppf_ctx *ctx, double *target, int64_t *n_shifts);
 
+/* Device-to-device copy of the current state (and step count, FP32
+ * solute target) from src into dst; same grid, precision and device.
+ * Checkpoints for the surrogate's exact re-run.  No counterpart in the
+ * reference. */
+int ppf_copy_state(ppf_ctx *dst, ppf_ctx *src);
+
 /* Timing on the context stream: ppf_timer_start records a HIP event,
  * ppf_timer_stop records a second one, synchronises and returns the elapsed
  * milliseconds between them. */

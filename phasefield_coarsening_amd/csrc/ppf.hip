@@ -896,6 +896,25 @@ extern "C" int ppf_mass_target(ppf_ctx *ctx, double *target, int64_t *n_shifts)
     return PPF_OK;
 }
 
+extern "C" int ppf_copy_state(ppf_ctx *dst, ppf_ctx *src)
+{
+    if (!dst || !src)
+        return dst ? set_err(dst, PPF_EARG, "null source") : PPF_EARG;
+    if (dst->n != src->n || dst->esz != src->esz || dst->d.nz != src->d.nz ||
+        dst->d.ny != src->d.ny || dst->d.device != src->d.device)
+        return set_err(dst, PPF_EARG, "contexts differ in grid, precision or device");
+    HIPCHK(dst, hipSetDevice(dst->d.device));
+    HIPCHK(dst, hipStreamSynchronize(src->stream));
+    for (int f = 0; f < 3; f++)
+        HIPCHK(dst, hipMemcpyAsync(dst->buf[dst->cur[f]][f], src->buf[src->cur[f]][f],
+                                   dst->n * dst->esz, hipMemcpyDeviceToDevice, dst->stream));
+    HIPCHK(dst, hipStreamSynchronize(dst->stream));
+    dst->steps = src->steps;
+    dst->c_target = src->c_target;
+    dst->since_shift = src->since_shift;
+    return PPF_OK;
+}
+
 extern "C" int ppf_timer_start(ppf_ctx *ctx)
 {
     if (!ctx)

@@ -93,6 +93,7 @@ SIGNATURES = {
     "ppf_set_mass_correction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ppf_mass_target": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "ppf_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
+    "ppf_copy_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "ppf_timer_start": (ctypes.c_int, [ctypes.c_void_p]),
     "ppf_timer_stop": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "ppf_sync": (ctypes.c_int, [ctypes.c_void_p]),
@@ -279,6 +280,10 @@ class Solver:
                                         int(max_sizes)))
         d = st.as_dict()
         return d, sizes[:min(d["n_particles"], int(max_sizes))]
+
+    def copy_from(self, other):
+        """Device-to-device copy of other's state into this solver."""
+        self._check(lib().ppf_copy_state(self._h, other._h))
 
     def set_mass_correction(self, every):
         """FP32 only: shift c back onto the FP64 solute target every `every`

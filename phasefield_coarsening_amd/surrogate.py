@@ -1,0 +1,169 @@
+"""Learned K-step jumps with an exact re-run fallback.
+
+Pipeline:
+1. ``ensemble_members`` draws the batched ensemble: edges from a set,
+   solid volume fraction phi ~ U[lo, hi], sphere radii R ~ U[rmin, rmax],
+   sphere count = phi * N^3 / E[4/3 pi R^3] (overlaps merge, so the actual
+   fraction is a little lower).
+2. ``trajectories`` steps the ensemble (one persistent launch per step)
+   and keeps snapshots every ``every`` steps, the training data.
+3. ``train`` fits the U-Net to pairs K steps apart with the library's own
+   forward, backward and Adam kernels.
+4. ``Controller.advance`` runs a grid to a target step count: from a
+   checkpoint it jumps K steps with the network (solute projected back),
+   relaxes with ``relax`` explicit steps, and judges the jump by how much
+   the mean particle radius and the solid fraction move during the
+   relaxation.  A jump that moves either by more than its tolerance is
+   undone (checkpoint restored) and the K + relax steps are re-run exactly.
+
+The reference has no counterpart: it only steps explicitly
+(phasefield.c:178-258).
+"""
+import time
+
+import numpy as np
+
+from . import solver as S
+
+
+def ensemble_members(n, seed, edges=(48, 64, 96, 128), phi=(0.1, 0.4), radius=(3.0, 10.0)):
+    """[(dims, centers, radii)] for n members, seeded."""
+    rng = np.random.default_rng(seed)
+    a, b = radius
+    mean_vol = 4.0 / 3.0 * np.pi * (b ** 4 - a ** 4) / (4.0 * (b - a))  # E[4/3 pi R^3]
+    out = []
+    for i in range(n):
+        e = int(rng.choice(edges))
+        f = float(rng.uniform(*phi))
+        count = max(1, int(round(f * e ** 3 / mean_vol)))
+        cen, rad = S.spheres(int(seed) * 100003 + i, count, (e, e, e), a, b)
+        out.append(((e, e, e), cen, rad))
+    return out
+
+
+def trajectories(members, steps, every, precision=S.F32, autotune=True, keep=None):
+    """Step all members together; return (snapshots, stats).  snapshots[i] is
+    a float32 array [T, D, H, W, 3] (T = steps // every + 1) for the members
+    in ``keep`` (all when None), and stats the ensemble throughput."""
+    keep = range(len(members)) if keep is None else keep
+    ens = S.Ensemble([m[0] for m in members], precision=precision)
+    try:
+        for i, (_, cen, rad) in enumerate(members):
+            ens.init_spheres(i, cen, rad)
+        if autotune:
+            ens.autotune(reps=2)
+        snaps = {i: [np.stack(ens.download(i, f32=True), axis=-1)] for i in keep}
+        info = ens.info()
+        t_step = 0.0
+        done = 0
+        while done < steps:
+            k = min(every, steps - done)
+            ens.timer_start()
+            ens.step(k)
+            t_step += ens.timer_stop() * 1e-3
+            done += k
+            if done % every == 0:
+                for i in keep:
+                    snaps[i].append(np.stack(ens.download(i, f32=True), axis=-1))
+        stats = {"members": len(members), "total_sites": info["total_sites"],
+                 "work_items": info["items"], "persistent_workgroups": info["grid"],
+                 "steps": steps, "step_seconds": t_step,
+                 "glups": info["total_sites"] * steps / t_step / 1e9 if t_step > 0 else 0.0}
+        return {i: np.array(v, np.float32) for i, v in snaps.items()}, stats
+    finally:
+        ens.close()
+
+
+def pairs(snaps, k_snap=1):
+    """Training pairs (x, y) k_snap snapshots apart, grouped by grid shape."""
+    by_shape = {}
+    for traj in snaps.values():
+        for t in range(len(traj) - k_snap):
+            by_shape.setdefault(traj.shape[1:4], []).append((traj[t], traj[t + k_snap]))
+    return {s: (np.array([p[0] for p in v]), np.array([p[1] for p in v]))
+            for s, v in by_shape.items()}
+
+
+def train(net, data, steps, lr=1e-3, batch=4, seed=0, log_every=0):
+    """Adam on random minibatches of one shape at a time; returns losses."""
+    rng = np.random.default_rng(seed)
+    shapes = sorted(data)
+    losses = []
+    for it in range(steps):
+        s = shapes[it % len(shapes)]
+        x, y = data[s]
+        idx = rng.choice(len(x), size=min(batch, len(x)), replace=False)
+        # a cosine decay of the step size over the run
+        lr_t = lr * 0.5 * (1.0 + np.cos(np.pi * it / max(steps, 1)))
+        losses.append(net.train_step(x[idx], y[idx], lr_t))
+        if log_every and (it % log_every == 0 or it == steps - 1):
+            print(f"train step {it}: loss {losses[-1]:.3e}", flush=True)
+    return losses
+
+
+def eval_loss(net, data, batch=4):
+    """Mean loss of the network and of 'no change' (persistence) over data."""
+    ln, lp, n = 0.0, 0.0, 0
+    for x, y in data.values():
+        for i in range(0, len(x), batch):
+            xb, yb = x[i:i + batch], y[i:i + batch]
+            pred = net.forward(xb)
+            ln += float(np.mean((pred - yb) ** 2)) * len(xb)
+            lp += float(np.mean((xb - yb) ** 2)) * len(xb)
+            n += len(xb)
+    return ln / n, lp / n
+
+
+class Controller:
+    """Advance ``solver`` with learned jumps of ``k`` steps, each followed by
+    ``relax`` explicit steps and judged; rejected jumps are re-run exactly."""
+
+    def __init__(self, solver, net, k=200, relax=20, tol_radius=0.02, tol_fraction=0.02,
+                 threshold=0.5):
+        self.sv = solver
+        self.net = net
+        self.k = k
+        self.relax = relax
+        self.tol_r = tol_radius
+        self.tol_f = tol_fraction
+        self.thr = threshold
+        self.ckpt = S.Solver(solver.shape, bc=solver.bc, precision=solver.precision)
+        self.log = []
+
+    def close(self):
+        self.ckpt.close()
+
+    def _measure(self):
+        st = self.sv.reduce()
+        p, _ = self.sv.particles(self.thr, max_sizes=1)
+        return p["mean_radius"], st["sum_h"] / st["n_sites"], st["n_nonfinite"]
+
+    def advance(self, nsteps):
+        """Run nsteps of model time; returns a summary dict."""
+        t0 = time.perf_counter()
+        done = acc = rej = exact = 0
+        while nsteps - done >= self.k + self.relax:
+            self.ckpt.copy_from(self.sv)
+            self.net.apply(self.sv)
+            r_j, f_j, bad_j = self._measure()
+            self.sv.step(self.relax)
+            r_r, f_r, bad_r = self._measure()
+            ok = (bad_j == 0 and bad_r == 0 and r_j > 0 and
+                  abs(r_r - r_j) <= self.tol_r * r_j and abs(f_r - f_j) <= self.tol_f * max(f_j, 1e-12))
+            if ok:
+                acc += 1
+            else:
+                rej += 1
+                self.sv.copy_from(self.ckpt)
+                self.sv.step(self.k + self.relax)
+                exact += self.k + self.relax
+            self.log.append({"step": done, "accepted": bool(ok), "R_jump": r_j, "R_relax": r_r,
+                             "frac_jump": f_j, "frac_relax": f_r})
+            done += self.k + self.relax
+        if nsteps > done:
+            self.sv.step(nsteps - done)
+            exact += nsteps - done
+            done = nsteps
+        self.sv.sync()
+        return {"steps": done, "accepted": acc, "rejected": rej, "exact_steps": exact,
+                "seconds": time.perf_counter() - t0}
