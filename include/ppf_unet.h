@@ -70,6 +70,10 @@ int ppf_unet_apply(ppf_unet *net, ppf_ctx *ctx);
  * its MFMA work in FLOP (2*27*Cin*Cout per site per 3x3x3 conv). */
 int ppf_unet_last_forward(ppf_unet *net, double *ms, double *flop);
 
+/* Convolution kernel: 1 (default) the LDS-tiled form, 0 the direct form
+ * (every operand straight from global memory), kept for cross-checks. */
+int ppf_unet_set_conv_kernel(ppf_unet *net, int lds);
+
 const char *ppf_unet_last_error(ppf_unet *net);
 
 #ifdef __cplusplus

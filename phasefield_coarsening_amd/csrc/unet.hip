@@ -133,6 +133,123 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ X, int ld
 }
 
 // ----------------------------------------------------------------------------
+// LDS-tiled convolution, same contract as k_conv.  A workgroup computes a
+// brick of 16 (x) x 8 (y) sites of one z plane for 64 output channels.  Per
+// chunk of 32 input channels it stages the brick's halo (3 x 10 x 18 sites)
+// in LDS, then per row of taps (dz, dy) the weights of the 3 taps dx; each
+// wave multiplies 2 site rows (2 M tiles) by 4 N tiles: 8 MFMAs per 6 LDS
+// fragment reads.  Site and weight rows are padded to 40 bf16 (80 bytes) so
+// the 16 lanes of a fragment read hit distinct banks.
+// ----------------------------------------------------------------------------
+constexpr int CL_BX = 16, CL_BY = 8, CL_HX = CL_BX + 2, CL_HY = CL_BY + 2;
+constexpr int CL_HALO = 3 * CL_HY * CL_HX;  // 540 sites
+constexpr int CL_PAD = 40;                   // bf16 per staged site / weight row
+constexpr int CL_CT = 64;                    // output channels per workgroup
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_conv_lds(const bf16 *__restrict__ X, int ldx, int xoff,
+                                                  int Cin, const bf16 *__restrict__ Wt, int CoutW,
+                                                  const float *__restrict__ bias, int Cout,
+                                                  bf16 *__restrict__ Yb, float *__restrict__ Yf,
+                                                  int ldy, int yoff, int B, int D, int H, int Wd)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sin[CL_HALO * CL_PAD];
+    __shared__ __attribute__((aligned(16))) bf16 sw[3 * CL_CT * CL_PAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nbx = (Wd + CL_BX - 1) / CL_BX, nby = (H + CL_BY - 1) / CL_BY;
+    int r = blockIdx.x;
+    const int bx = r % nbx;
+    r /= nbx;
+    const int by = r % nby;
+    r /= nby;
+    const int z = r % D;
+    const int b = r / D;
+    const int x0 = bx * CL_BX, y0 = by * CL_BY;
+    const int n0 = blockIdx.y * CL_CT;
+    const size_t V = (size_t)D * H * Wd;
+    const int ntile = min(4, (CoutW - n0 + 15) / 16);  // N tiles that exist
+    const int kq = (lane >> 4) * 8;
+    f32x4 acc[2][4];
+    for (int m = 0; m < 2; m++)
+        for (int j = 0; j < 4; j++)
+            acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < Cin; kc += 32) {
+        __syncthreads();  // the previous chunk's reads of sin are done
+        for (int i = tid; i < CL_HALO * 4; i += 256) {
+            const int s = i >> 2, part = i & 3;
+            const int hz = s / (CL_HY * CL_HX), hy = (s / CL_HX) % CL_HY, hx = s % CL_HX;
+            const size_t nv = (size_t)b * V +
+                              ((size_t)wrapi(z + hz - 1, D) * H + wrapi(y0 + hy - 1, H)) * Wd +
+                              wrapi(x0 + hx - 1, Wd);
+            *(bf16x8 *)(sin + s * CL_PAD + part * 8) =
+                *(const bf16x8 *)(X + nv * ldx + xoff + kc + part * 8);
+        }
+        for (int g = 0; g < 9; g++) {
+            const int dz = g / 3 - 1, dy = g % 3 - 1;
+            __syncthreads();  // sin staged / the previous group's reads of sw done
+            for (int i = tid; i < 3 * CL_CT * 4; i += 256) {
+                const int part = i & 3, row = i >> 2;  // row = dxi*64 + co
+                const int dxi = row / CL_CT, co = n0 + row % CL_CT;
+                bf16x8 v;
+                if (co < CoutW)
+                    v = *(const bf16x8 *)(Wt + ((size_t)(g * 3 + dxi) * CoutW + co) * Cin + kc +
+                                          part * 8);
+                else
+                    for (int e = 0; e < 8; e++)
+                        v[e] = (bf16)0.0f;
+                *(bf16x8 *)(sw + row * CL_PAD + part * 8) = v;
+            }
+            __syncthreads();
+            for (int dxi = 0; dxi < 3; dxi++) {
+                bf16x8 a[2];
+                for (int m = 0; m < 2; m++) {
+                    const int hy = 2 * wave + m + dy + 1, hx = (lane & 15) + dxi;
+                    a[m] = *(const bf16x8 *)(sin + (((dz + 1) * CL_HY + hy) * CL_HX + hx) * CL_PAD +
+                                             kq);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (j < ntile) {
+                        const bf16x8 w =
+                            *(const bf16x8 *)(sw + (dxi * CL_CT + j * 16 + (lane & 15)) * CL_PAD + kq);
+                        acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], w, acc[0][j], 0, 0, 0);
+                        acc[1][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], w, acc[1][j], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+    // lane holds C[site 4*(lane>>4)+rr][channel lane&15] of each (m, j) tile
+    for (int m = 0; m < 2; m++) {
+        const int y = y0 + 2 * wave + m;
+        if (y >= H)
+            continue;
+        for (int j = 0; j < 4; j++) {
+            const int ch = n0 + j * 16 + (lane & 15);
+            if (j >= ntile || ch >= Cout)
+                continue;
+            for (int rr = 0; rr < 4; rr++) {
+                const int x = x0 + (lane >> 4) * 4 + rr;
+                if (x >= Wd)
+                    continue;
+                const size_t vox = (size_t)b * V + ((size_t)z * H + y) * Wd + x;
+                float v = acc[m][j][rr];
+                const size_t o = vox * ldy + yoff + ch;
+                if (MODE == 0) {
+                    v += bias[ch];
+                    v = v > 0.f ? v : LEAK * v;
+                    Yb[o] = (bf16)v;
+                } else if (MODE == 1) {
+                    Yf[o] = v;
+                } else {
+                    Yf[o] += v;
+                }
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
 // Weight gradient partials: part[split][tap][co][ci] = sum over the split's
 // sites of G[v][co] * X[v+s_tap][ci].  A = G^T (16 co x 32 sites), B = the
 // shifted input (32 sites x 16 ci), gathered per lane.  Workgroup: one tap,
@@ -514,6 +631,7 @@ struct ppf_unet {
     float *part;
     size_t part_cap;
     double last_ms, last_flop;
+    int conv_lds;  // 1: LDS-tiled convolution kernel, 0: the direct one
     char err[512];
 };
 
@@ -645,6 +763,20 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
                  int ldy, int yoff, int B, int D, int H, int W)
 {
     const size_t total = (size_t)B * D * H * W;
+    if (n->conv_lds) {
+        const size_t bricks = (size_t)B * D * ((H + CL_BY - 1) / CL_BY) * ((W + CL_BX - 1) / CL_BX);
+        const dim3 g2((unsigned)bricks, (unsigned)((coutw + CL_CT - 1) / CL_CT));
+        if (mode == 0)
+            hipLaunchKernelGGL(k_conv_lds<0>, g2, dim3(256), 0, n->stream, X, ldx, xoff, cin, Wt,
+                               coutw, bias, cout, Yb, Yf, ldy, yoff, B, D, H, W);
+        else if (mode == 1)
+            hipLaunchKernelGGL(k_conv_lds<1>, g2, dim3(256), 0, n->stream, X, ldx, xoff, cin, Wt,
+                               coutw, bias, cout, Yb, Yf, ldy, yoff, B, D, H, W);
+        else
+            hipLaunchKernelGGL(k_conv_lds<2>, g2, dim3(256), 0, n->stream, X, ldx, xoff, cin, Wt,
+                               coutw, bias, cout, Yb, Yf, ldy, yoff, B, D, H, W);
+        return;
+    }
     const dim3 grid((unsigned)((total + 63) / 64), (unsigned)((coutw + 63) / 64));
     if (mode == 0)
         hipLaunchKernelGGL((k_conv<4, 0, 27>), grid, dim3(256), 0, n->stream, X, ldx, xoff, cin,
@@ -888,6 +1020,7 @@ extern "C" int ppf_unet_create(ppf_unet **out, int base, int device, uint64_t se
         return uerr(nullptr, PPF_EHIP, "hipSetDevice failed");
     ppf_unet *n = new ppf_unet();
     n->base = base;
+    n->conv_lds = 1;
     n->device = device;
     const int C = base;
     // (cin_log, cin padded, cout) of the ten 3x3x3 convolutions
@@ -1179,6 +1312,14 @@ extern "C" int ppf_unet_last_forward(ppf_unet *n, double *ms, double *flop)
         return PPF_EARG;
     *ms = n->last_ms;
     *flop = n->last_flop;
+    return PPF_OK;
+}
+
+extern "C" int ppf_unet_set_conv_kernel(ppf_unet *n, int lds)
+{
+    if (!n || (lds != 0 && lds != 1))
+        return n ? uerr(n, PPF_EARG, "kernel must be 0 (direct) or 1 (LDS-tiled)") : PPF_EARG;
+    n->conv_lds = lds;
     return PPF_OK;
 }
 

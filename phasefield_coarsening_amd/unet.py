@@ -27,6 +27,7 @@ _SIG = {
     "ppf_unet_apply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "ppf_unet_last_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(ctypes.c_double)]),
+    "ppf_unet_set_conv_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ppf_unet_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
 }
 
@@ -149,6 +150,10 @@ class UNet:
     def apply(self, solver):
         """Jump the solver's state in place (solute projected back)."""
         self._check(lib().ppf_unet_apply(self._h, solver._h))
+
+    def set_conv_kernel(self, lds):
+        """1: LDS-tiled convolutions (default); 0: the direct form."""
+        self._check(lib().ppf_unet_set_conv_kernel(self._h, int(bool(lds))))
 
     def last_forward(self):
         ms, fl = ctypes.c_double(), ctypes.c_double()

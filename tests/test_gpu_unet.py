@@ -32,10 +32,15 @@ def _data(seed, shape=(2, 16, 16, 16)):
     return x, y
 
 
-def test_forward_matches_reference():
+@pytest.mark.parametrize("lds", [1, 0])
+@pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 24, 12, 40)])
+def test_forward_matches_reference(lds, shape):
+    """Both convolution kernels (LDS-tiled and direct), on a cube and on a
+    grid whose x and y are not multiples of the LDS brick."""
     p = _params(1)
-    x, _ = _data(2)
+    x, _ = _data(2, shape)
     with UNet(seed=3) as net:
+        net.set_conv_kernel(lds)
         net.set_params(p)
         got = net.forward(x)
     want = unet_ref.predict(p, x, bf16=True)
@@ -49,10 +54,12 @@ def test_forward_matches_reference():
     assert np.abs(got - unet_ref.predict(p, x, bf16=False)).mean() < 2e-2 * scale
 
 
-def test_gradient_matches_reference():
+@pytest.mark.parametrize("lds", [1, 0])
+def test_gradient_matches_reference(lds):
     p = _params(4)
     x, y = _data(5)
     with UNet(seed=3) as net:
+        net.set_conv_kernel(lds)
         net.set_params(p)
         loss, g = net.loss_grad(x, y)
     lr, gr = unet_ref.loss_and_grad(p, x, y, bf16=True)
