@@ -134,6 +134,50 @@ def reference_serial_baseline(S, shape, nsteps=3):
                       f"steps ({times[1]:.1f} s) minus run of 0 steps ({times[0]:.1f} s)"}
 
 
+def unet_line(grid=256, reps=3):
+    """U-Net forward (BF16 MFMA) at grid^3: time from HIP events and the
+    MFMA work 2*27*Cin*Cout per site of each 3x3x3 conv."""
+    import numpy as np
+
+    from phasefield_coarsening_amd.unet import UNet
+    with UNet(seed=1) as net:
+        x = np.random.default_rng(0).uniform(0, 1, (1, grid, grid, grid, 3)).astype(np.float32)
+        net.forward(x)
+        best = None
+        for _ in range(reps):
+            net.forward(x)
+            ms, flop = net.last_forward()
+            best = ms if best is None else min(best, ms)
+    tf = flop / best / 1e9
+    return {"grid": [grid] * 3, "forward_ms": round(best, 3), "TFLOPs": round(tf, 1),
+            "mfma_bf16_peak_TFLOPs": 2500.0, "mfma_frac": round(tf / 2500.0, 4)}
+
+
+def ensemble_line(S, members=256, steps=1000, seed=4):
+    """Batched ensemble (configs[3] shape): members with edges in {48, 64, 96,
+    128}, solid fraction U[0.1, 0.4], radii U[3, 10], FP32, autotuned, one
+    persistent launch per step."""
+    from phasefield_coarsening_amd import surrogate as SG
+    mem = SG.ensemble_members(members, seed)
+    ens = S.Ensemble([m[0] for m in mem], precision=S.F32)
+    try:
+        for i, (_, cen, rad) in enumerate(mem):
+            ens.init_spheres(i, cen, rad)
+        ens.autotune(reps=2)
+        ens.step(10)
+        ens.timer_start()
+        ens.step(steps)
+        ms = ens.timer_stop()
+        info = ens.info()
+    finally:
+        ens.close()
+    g = info["total_sites"] * steps / (ms * 1e-3) / 1e9
+    return {"members": members, "total_sites": info["total_sites"], "steps": steps,
+            "GLUPS": round(g, 2), "roofline_achieved_GBs": round(g * 24, 1),
+            "roofline_frac": round(g * 24 / HBM_PEAK_GBS, 4),
+            "work_items": info["items"], "persistent_workgroups": info["grid"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +188,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32", action="store_true")
     ap.add_argument("--no-copy", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the ensemble and U-Net lines")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -194,6 +240,9 @@ def main():
                       "roofline_frac": round(ach32 / HBM_PEAK_GBS, 4),
                       "solute_vs_fp64_target_rel":
                           abs(r32["stats1"]["sum_c"] - r32["c_target"]) / abs(r32["c_target"])}
+    if not a.no_extras:
+        out["ensemble"] = ensemble_line(S)
+        out["unet"] = unet_line()
     if not a.no_copy:
         cc = copy_ceiling()
         out["roofline"]["copy_ceiling_GBs"] = round(cc, 1)

@@ -17,12 +17,18 @@ __device__ __forceinline__ int ccl_load(const int *L, int x)
     return __hip_atomic_load(L + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ int ccl_find(const int *L, int x)
+// Root of x with path halving: each visited node is re-pointed at its
+// grandparent (atomicMin, so a label only ever decreases and still names an
+// ancestor), which keeps the chains short in large merged components.
+__device__ __forceinline__ int ccl_find(int *L, int x)
 {
     int y = ccl_load(L, x);
     while (y != x) {
+        const int z = ccl_load(L, y);
+        if (z < y)
+            atomicMin(L + x, z);
         x = y;
-        y = ccl_load(L, x);
+        y = z;
     }
     return x;
 }
@@ -86,14 +92,30 @@ __global__ __launch_bounds__(256) void k_ccl_merge(int *__restrict__ L, int nx, 
 }
 
 // Point every solid site at its root and count the sites of each root.
+// Lanes of a wave with the same root add their count with one atomic (one
+// large component would otherwise serialise millions of atomics on one
+// counter).
 static __global__ __launch_bounds__(256) void k_ccl_flatten(int *__restrict__ L, int *__restrict__ cnt,
                                                      size_t n)
 {
-    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
-        if (L[q] < 0)
-            continue;
-        const int r = ccl_find(L, (int)q);
-        atomicAdd(cnt + r, 1);
+    const size_t stride = (size_t)gridDim.x * 256;
+    // every lane runs the same number of iterations so the ballots see the
+    // whole wave
+    const size_t iters = (n + stride - 1) / stride;
+    for (size_t it = 0; it < iters; it++) {
+        const size_t q = it * stride + (size_t)blockIdx.x * 256 + threadIdx.x;
+        int r = -1;
+        if (q < n && L[q] >= 0)
+            r = ccl_find(L, (int)q);
+        unsigned long long pending = __ballot(r >= 0);
+        while (pending) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const int rl = __shfl(r, leader, 64);
+            const unsigned long long same = __ballot(r == rl);
+            if ((int)(threadIdx.x & 63) == leader)
+                atomicAdd(cnt + rl, (int)__popcll(same));
+            pending &= ~same;
+        }
     }
 }
 

@@ -48,7 +48,8 @@ def main(argv=None):
     ap.add_argument("--relax", type=int, default=20)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--save", default="")
-    ap.add_argument("--load", default="")
+    ap.add_argument("--load", default="",
+                    help="weights .npy (float16 or float32); skips the ensemble and training")
     a = ap.parse_args(argv)
 
     net = UNet(seed=a.seed)
