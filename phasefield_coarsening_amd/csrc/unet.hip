@@ -38,6 +38,14 @@ constexpr int NCONV = 10;
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// Any coordinate to [0, n): a staged brick can be larger than the grid (the
+// coarse levels of small inputs), so its halo may lie several periods out.
+__device__ __forceinline__ int wrapm(int i, int n)
+{
+    const int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
 // Site index -> (b, z, y, x) for a [B][D][H][W] grid.
 struct Coord {
     int b, z, y, x;
@@ -179,8 +187,8 @@ __global__ __launch_bounds__(256) void k_conv_lds(const bf16 *__restrict__ X, in
             const int s = i >> 2, part = i & 3;
             const int hz = s / (CL_HY * CL_HX), hy = (s / CL_HX) % CL_HY, hx = s % CL_HX;
             const size_t nv = (size_t)b * V +
-                              ((size_t)wrapi(z + hz - 1, D) * H + wrapi(y0 + hy - 1, H)) * Wd +
-                              wrapi(x0 + hx - 1, Wd);
+                              ((size_t)wrapm(z + hz - 1, D) * H + wrapm(y0 + hy - 1, H)) * Wd +
+                              wrapm(x0 + hx - 1, Wd);
             *(bf16x8 *)(sin + s * CL_PAD + part * 8) =
                 *(const bf16x8 *)(X + nv * ldx + xoff + kc + part * 8);
         }
