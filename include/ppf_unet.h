@@ -70,9 +70,16 @@ int ppf_unet_apply(ppf_unet *net, ppf_ctx *ctx);
  * its MFMA work in FLOP (2*27*Cin*Cout per site per 3x3x3 conv). */
 int ppf_unet_last_forward(ppf_unet *net, double *ms, double *flop);
 
-/* Convolution kernel: 1 (default) the LDS-tiled form, 0 the direct form
- * (every operand straight from global memory), kept for cross-checks. */
-int ppf_unet_set_conv_kernel(ppf_unet *net, int lds);
+/* Convolution kernel:
+ *   2 (default) row-blocked: a wave owns 16 x 4 or 16 x 8 sites and reuses
+ *     each loaded input row for the three dy taps; no LDS;
+ *   3, 4 row-blocked with fixed (rows, N tiles) of (8, 2) and (8, 4 where
+ *     Cout allows), for tuning;
+ *   1 LDS-tiled: a 16 x 8 brick's halo staged in LDS;
+ *   0 direct: every operand straight from global memory.
+ * The kinds add the (tap, 32-channel) products into the FP32 accumulators
+ * in different orders, so they agree to FP32 rounding, not bitwise. */
+int ppf_unet_set_conv_kernel(ppf_unet *net, int kind);
 
 const char *ppf_unet_last_error(ppf_unet *net);
 

@@ -258,6 +258,175 @@ __global__ __launch_bounds__(256) void k_conv_lds(const bf16 *__restrict__ X, in
 }
 
 // ----------------------------------------------------------------------------
+// Row-blocked convolution, same contract as k_conv; no LDS, no barriers.
+// A wave owns 16 consecutive x sites of R consecutive rows (one z plane) and
+// NJ*16 output channels.  For one (dz, dx) and 32 input channels it loads
+// the R+2 input rows y0-1 .. y0+R as A fragments; row r+dy serves output row
+// r for tap dy, so each row is loaded once for all three dy taps: per (dz,
+// dx) group R+2 A and 3*NJ B fragments feed 3*R*NJ MFMAs, and the next
+// group's fragments load while they run.  Every
+// coordinate is wrapped to the grid (a tile may be larger than a coarse
+// grid); sites outside the grid are computed on wrapped data and not stored.
+// Waves are ordered N block, x tile, y tile, z, batch, so the four waves of
+// a workgroup share their input rows in L1.
+// ----------------------------------------------------------------------------
+template <int R, int NJ, int MODE, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_conv_rows(
+    const bf16 *__restrict__ X, int ldx, int xoff, int Cin, const bf16 *__restrict__ Wt, int CoutW,
+    const float *__restrict__ bias, int Cout, bf16 *__restrict__ Yb, float *__restrict__ Yf,
+    int ldy, int yoff, int B, int D, int H, int Wd, int nnb, int nxt, int nyt)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
+    long g = (long)blockIdx.x * 4 + wave;
+    const int nb = (int)(g % nnb);
+    g /= nnb;
+    const int xt = (int)(g % nxt);
+    g /= nxt;
+    const int yt = (int)(g % nyt);
+    g /= nyt;
+    const int z = (int)(g % D);
+    const int b = (int)(g / D);
+    if (b >= B)
+        return;  // whole wave; the kernel has no barrier
+    const int xl = lane & 15, q = lane >> 4;
+    const int n0 = nb * NJ * 16;
+    const int y0 = yt * R;
+    const int rowlen = Wd * ldx;  // elements per grid row; a plane is < 2^31 elements
+    // per-lane offsets (elements) for dx = -1, 0, +1, with the channel quad;
+    // everything else in an address is uniform (scalar base + vector offset)
+    int xo[3];  // bytes
+#pragma unroll
+    for (int dx = 0; dx < 3; dx++)
+        xo[dx] = (wrapm(xt * 16 + xl + dx - 1, Wd) * ldx + xoff + 8 * q) * 2;
+    // Loads are buffer loads: a descriptor per source in scalar registers, a
+    // per-lane byte offset (VGPR) and a uniform byte offset (SGPR), so no
+    // 64-bit address is kept per load.  Each descriptor's extent is its
+    // source's true size (a plane, the weight rows); a load outside it
+    // would read zeros, not fault.
+    // planes z-1, z, z+1 and rows y0-1 .. y0+R
+    const int planebytes = H * rowlen * 2;
+    __amdgpu_buffer_rsrc_t pl[3];
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++)
+        pl[dz] = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(X + ((size_t)b * D + wrapm(z + dz - 1, D)) * H * (size_t)rowlen), (short)0,
+            planebytes, 0x00020000);
+    int ro[R + 2];
+#pragma unroll
+    for (int r = 0; r < R + 2; r++)
+        ro[r] = wrapm(y0 + r - 1, H) * rowlen;
+    // lane's weight row in N tile j: Wt[tap][n0+16j+xl][kc+8q]; CoutW is a
+    // multiple of NJ*16, so every N tile exists
+    const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(Wt + (size_t)n0 * Cin), (short)0, (27 * CoutW - n0) * Cin * 2, 0x00020000);
+    const int wo = (xl * Cin + 8 * q) * 2;
+    const int tapstride = CoutW * Cin, jstride = 16 * Cin;
+    f32x4 acc[R][NJ];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int j = 0; j < NJ; j++)
+            acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // Fragments of one (32-channel chunk, dz, dx) group.
+    auto ld = [&](bf16x8(&a)[R + 2], bf16x8(&w)[3][NJ], int kc, int grp) {
+        const int dz = grp / 3, dx = grp % 3;
+#pragma unroll
+        for (int r = 0; r < R + 2; r++)
+            a[r] = __builtin_bit_cast(
+                bf16x8, __builtin_amdgcn_raw_buffer_load_b128(pl[dz], xo[dx], (ro[r] + kc) * 2, 0));
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+            for (int j = 0; j < NJ; j++)
+                w[dy][j] = __builtin_bit_cast(
+                    bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
+                                wr, wo, ((dz * 9 + dy * 3 + dx) * tapstride + j * jstride + kc) * 2, 0));
+    };
+    auto mm = [&](const bf16x8(&a)[R + 2], const bf16x8(&w)[3][NJ]) {
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int j = 0; j < NJ; j++)
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r + dy], w[dy][j],
+                                                                        acc[r][j], 0, 0, 0);
+    };
+    // Two register buffers.  Each group's MFMAs run while the next group's
+    // loads are in flight; scheduling fences keep the loads ahead of the
+    // MFMAs (left alone, the compiler moves each load next to its use).
+    // A chunk has 9 groups, so the buffers swap roles from one chunk to the
+    // next and the loop body covers two chunks.  Past the last chunk the
+    // prefetch re-reads the last chunk (in range, unused).
+    bf16x8 a0[R + 2], w0[3][NJ], a1[R + 2], w1[3][NJ];
+    const int klast = Cin - 32;
+    ld(a0, w0, 0, 0);
+    for (int kc = 0; kc < Cin; kc += 64) {
+        const int k1 = min(kc + 32, klast), k2 = min(kc + 64, klast);
+#pragma unroll
+        for (int grp = 0; grp < 9; grp++) {
+            if (grp % 2 == 0) {
+                ld(a1, w1, grp < 8 ? kc : k1, grp < 8 ? grp + 1 : 0);
+                __builtin_amdgcn_sched_barrier(0);
+                mm(a0, w0);
+            } else {
+                ld(a0, w0, kc, grp + 1);
+                __builtin_amdgcn_sched_barrier(0);
+                mm(a1, w1);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (kc + 32 >= Cin)
+            break;
+#pragma unroll
+        for (int grp = 0; grp < 9; grp++) {
+            if (grp % 2 == 0) {
+                ld(a0, w0, grp < 8 ? k1 : k2, grp < 8 ? grp + 1 : 0);
+                __builtin_amdgcn_sched_barrier(0);
+                mm(a1, w1);
+            } else {
+                ld(a1, w1, k1, grp + 1);
+                __builtin_amdgcn_sched_barrier(0);
+                mm(a0, w0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // lane holds C[site 4q+rr][channel xl] of each (r, j) tile
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + r;
+        if (y >= H)
+            break;
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int ch = n0 + 16 * j + xl;
+            if (ch >= Cout)
+                continue;
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int x = xt * 16 + 4 * q + rr;
+                if (x >= Wd)
+                    continue;
+                const size_t vox = (((size_t)b * D + z) * H + y) * Wd + x;
+                float v = acc[r][j][rr];
+                const size_t o = vox * ldy + yoff + ch;
+                if (MODE == 0) {
+                    v += bias[ch];
+                    v = v > 0.f ? v : LEAK * v;
+                    Yb[o] = (bf16)v;
+                } else if (MODE == 1) {
+                    Yf[o] = v;
+                } else {
+                    Yf[o] += v;
+                }
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
 // Weight gradient partials: part[split][tap][co][ci] = sum over the split's
 // sites of G[v][co] * X[v+s_tap][ci].  A = G^T (16 co x 32 sites), B = the
 // shifted input (32 sites x 16 ci), gathered per lane.  Workgroup: one tap,
@@ -368,27 +537,107 @@ __global__ __launch_bounds__(256) void k_act_bwd(const float *__restrict__ g, in
     }
 }
 
-// 2x2x2 average pool (coarse grid [B][D/2][H/2][W/2]).
+// Forward glue kernels.  One thread moves 8 channels (16 bytes); a block
+// covers part of one grid row, so the only divisions are per block and per
+// channel octet.  Channel counts, strides and offsets are multiples of 8.
+__device__ __forceinline__ void row_split(int nch, int &row, int &t)
+{
+    row = blockIdx.x / nch;
+    t = (blockIdx.x - row * nch) * 256 + threadIdx.x;
+}
+
+// 2x2x2 average pooling of channels [soff, soff+C) of S (row stride lds) on
+// the grid [B][D][H][W] into P [B][D/2][H/2][W/2][C]; the 8 fine sites are
+// summed in (dz, dy, dx) order in FP32 and scaled by 1/8.
 __global__ __launch_bounds__(256) void k_pool_fwd(const bf16 *__restrict__ S, int lds, int soff,
                                                   int C, bf16 *__restrict__ P, int B, int D,
                                                   int H, int W)
 {
-    const int D2 = D / 2, H2 = H / 2, W2 = W / 2;
-    const size_t V2 = (size_t)D2 * H2 * W2, n = V2 * B * C;
-    const size_t V = (size_t)D * H * W;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        const size_t p = i / C;
-        const int c = (int)(i - p * C);
-        const Coord q = coord_of(p, V2, H2, W2);
-        float s = 0.f;
-        for (int k = 0; k < 8; k++) {
-            const size_t v = (size_t)q.b * V +
-                             ((size_t)(2 * q.z + (k >> 2)) * H + 2 * q.y + ((k >> 1) & 1)) * W +
-                             2 * q.x + (k & 1);
-            s += (float)S[v * lds + soff + c];
-        }
-        P[i] = (bf16)(s * 0.125f);
+    const int D2 = D / 2, H2 = H / 2, W2 = W / 2, oct = C >> 3;
+    const int per_row = W2 * oct;
+    int row, t;
+    row_split((per_row + 255) / 256, row, t);
+    if (t >= per_row)
+        return;
+    const int x2 = t / oct, o = t - x2 * oct;
+    const int y2 = row % H2, bz = row / H2, z2 = bz % D2, b = bz / D2;
+    float s[8];
+    for (int e = 0; e < 8; e++)
+        s[e] = 0.f;
+    for (int k = 0; k < 8; k++) {
+        const size_t v =
+            ((size_t)(b * D + 2 * z2 + (k >> 2)) * H + 2 * y2 + ((k >> 1) & 1)) * W + 2 * x2 + (k & 1);
+        const bf16x8 a = *(const bf16x8 *)(S + v * lds + soff + 8 * o);
+        for (int e = 0; e < 8; e++)
+            s[e] += (float)a[e];
     }
+    bf16x8 r;
+    for (int e = 0; e < 8; e++)
+        r[e] = (bf16)(s[e] * 0.125f);
+    *(bf16x8 *)(P + ((size_t)row * W2 + x2) * C + 8 * o) = r;
+}
+
+// Nearest x2 upsampling of S [B][D/2][H/2][W/2][C] into channels
+// [uoff, uoff+C) of U (row stride ldu) on the grid [B][D][H][W].
+__global__ __launch_bounds__(256) void k_up_fwd(const bf16 *__restrict__ S, int C,
+                                                bf16 *__restrict__ U, int ldu, int uoff, int B,
+                                                int D, int H, int W)
+{
+    const int oct = C >> 3, per_row = W * oct;
+    int row, t;
+    row_split((per_row + 255) / 256, row, t);
+    if (t >= per_row)
+        return;
+    const int x = t / oct, o = t - x * oct;
+    const int y = row % H, bz = row / H, z = bz % D, b = bz / D;
+    const size_t src = ((size_t)(b * (D / 2) + z / 2) * (H / 2) + y / 2) * (W / 2) + x / 2;
+    *(bf16x8 *)(U + ((size_t)row * W + x) * ldu + uoff + 8 * o) =
+        *(const bf16x8 *)(S + src * C + 8 * o);
+}
+
+// Network input: x [total][3] FP32 -> X0 [total][32] BF16, channels 3..31 zero.
+__global__ __launch_bounds__(256) void k_pack_in(const float *__restrict__ x, size_t total,
+                                                 bf16 *__restrict__ X0)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total * 4)
+        return;
+    const size_t v = i >> 2;
+    bf16x8 r;
+    for (int e = 0; e < 8; e++)
+        r[e] = (bf16)0.0f;
+    if ((i & 3) == 0) {
+        r[0] = (bf16)x[v * 3];
+        r[1] = (bf16)x[v * 3 + 1];
+        r[2] = (bf16)x[v * 3 + 2];
+    }
+    *(bf16x8 *)(X0 + i * 8) = r;
+}
+
+// Head: out = x + hb + hw F (1x1x1, CB -> 3 channels), summed over ci in order.
+__global__ __launch_bounds__(256) void k_head_fwd(const bf16 *__restrict__ F, int CB,
+                                                  const float *__restrict__ hw,
+                                                  const float *__restrict__ hb,
+                                                  const float *__restrict__ x, size_t total,
+                                                  float *__restrict__ out)
+{
+    const size_t v = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= total)
+        return;
+    float a0 = hb[0], a1 = hb[1], a2 = hb[2];
+    for (int c8 = 0; c8 < CB; c8 += 8) {
+        const bf16x8 f8 = *(const bf16x8 *)(F + v * CB + c8);
+        for (int e = 0; e < 8; e++) {
+            const int ci = c8 + e;
+            const float f = (float)f8[e];
+            a0 += hw[ci] * f;
+            a1 += hw[CB + ci] * f;
+            a2 += hw[2 * CB + ci] * f;
+        }
+    }
+    out[v * 3 + 0] = x[v * 3 + 0] + a0;
+    out[v * 3 + 1] = x[v * 3 + 1] + a1;
+    out[v * 3 + 2] = x[v * 3 + 2] + a2;
 }
 
 // gS[child][c] += gP[parent][c] / 8
@@ -405,23 +654,6 @@ __global__ __launch_bounds__(256) void k_pool_bwd(const float *__restrict__ gP, 
         const Coord q = coord_of(v, V, H, W);
         const size_t p = (size_t)q.b * V2 + ((size_t)(q.z >> 1) * H2 + (q.y >> 1)) * W2 + (q.x >> 1);
         gS[v * lds + soff + c] += gP[p * C + c] * 0.125f;
-    }
-}
-
-// Nearest x2 upsample of S [B][D/2][H/2][W/2][C] into U (ldu, uoff).
-__global__ __launch_bounds__(256) void k_up_fwd(const bf16 *__restrict__ S, int C,
-                                                bf16 *__restrict__ U, int ldu, int uoff, int B,
-                                                int D, int H, int W)
-{
-    const size_t V = (size_t)D * H * W, n = V * B * C;
-    const int H2 = H / 2, W2 = W / 2, D2 = D / 2;
-    const size_t V2 = (size_t)D2 * H2 * W2;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        const size_t v = i / C;
-        const int c = (int)(i - v * C);
-        const Coord q = coord_of(v, V, H, W);
-        const size_t p = (size_t)q.b * V2 + ((size_t)(q.z >> 1) * H2 + (q.y >> 1)) * W2 + (q.x >> 1);
-        U[v * ldu + uoff + c] = S[p * C + c];
     }
 }
 
@@ -445,39 +677,6 @@ __global__ __launch_bounds__(256) void k_up_bwd(const float *__restrict__ gU, in
             s += gU[v * ldu + uoff + c];
         }
         gS[i] = s;
-    }
-}
-
-// X0[v][c] = bf16(x[v][c]) for c < 3, 0 for 3 <= c < 32.
-__global__ __launch_bounds__(256) void k_pack_in(const float *__restrict__ x, size_t total,
-                                                 bf16 *__restrict__ X0)
-{
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total * 32;
-         i += (size_t)gridDim.x * 256) {
-        const size_t v = i >> 5;
-        const int c = (int)(i & 31);
-        X0[i] = c < 3 ? (bf16)x[v * 3 + c] : (bf16)0.0f;
-    }
-}
-
-// Head: out[v][c] = x[v][c] + b[c] + sum_ci W[c][ci] * F[v][ci]  (ci < CB)
-__global__ __launch_bounds__(256) void k_head_fwd(const bf16 *__restrict__ F, int CB,
-                                                  const float *__restrict__ hw,
-                                                  const float *__restrict__ hb,
-                                                  const float *__restrict__ x, size_t total,
-                                                  float *__restrict__ out)
-{
-    for (size_t v = (size_t)blockIdx.x * 256 + threadIdx.x; v < total; v += (size_t)gridDim.x * 256) {
-        float a0 = hb[0], a1 = hb[1], a2 = hb[2];
-        for (int ci = 0; ci < CB; ci++) {
-            const float f = (float)F[v * CB + ci];
-            a0 += hw[ci] * f;
-            a1 += hw[CB + ci] * f;
-            a2 += hw[2 * CB + ci] * f;
-        }
-        out[v * 3 + 0] = x[v * 3 + 0] + a0;
-        out[v * 3 + 1] = x[v * 3 + 1] + a1;
-        out[v * 3 + 2] = x[v * 3 + 2] + a2;
     }
 }
 
@@ -616,6 +815,10 @@ struct Layer {
 };
 
 unsigned grid_for(size_t n) { return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 65536)); }
+// one thread per item, no grid-stride loop
+unsigned blocks_for(size_t n) { return (unsigned)std::max<size_t>(1, (n + 255) / 256); }
+// rows of per_row items each, a whole number of blocks per row (row_split)
+unsigned row_blocks(int rows, int per_row) { return (unsigned)rows * (unsigned)((per_row + 255) / 256); }
 
 }  // namespace
 
@@ -639,7 +842,7 @@ struct ppf_unet {
     float *part;
     size_t part_cap;
     double last_ms, last_flop;
-    int conv_lds;  // 1: LDS-tiled convolution kernel, 0: the direct one
+    int conv_kind;  // 2-4: row-blocked variants, 1: LDS-tiled, 0: direct
     char err[512];
 };
 
@@ -771,7 +974,47 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
                  int ldy, int yoff, int B, int D, int H, int W)
 {
     const size_t total = (size_t)B * D * H * W;
-    if (n->conv_lds) {
+    if (n->conv_kind >= 2 && coutw % 32 == 0) {
+        // (R, NJ): 8 rows x 2 N tiles, or 4 rows x 4 N tiles when the
+        // channel count allows (kind 2) / 8 x 4 (kind 4)
+        // NJ*16 must divide coutw (the kernel has no partial N tile); every
+        // layer's padded channel count is a multiple of 32
+        const bool wide = coutw % 64 == 0 && n->conv_kind != 3;
+        const int R = wide && n->conv_kind == 2 ? 4 : 8, NJ = wide ? 4 : 2;
+        const int nnb = (coutw + NJ * 16 - 1) / (NJ * 16), nxt = (W + 15) / 16,
+                  nyt = (H + R - 1) / R;
+        const size_t waves = (size_t)B * D * nyt * nxt * nnb;
+        const dim3 g3((unsigned)((waves + 3) / 4));
+#define ROWS(RR, JJ, MM)                                                                      \
+    hipLaunchKernelGGL((k_conv_rows<RR, JJ, MM, (RR * JJ > 16 ? 1 : 2)>), g3, dim3(256), 0,       \
+                       n->stream, X, ldx, xoff, cin, Wt, coutw, bias, cout, Yb, Yf, ldy, yoff, B, \
+                       D, H, W, nnb, nxt, nyt)
+        if (R == 8 && NJ == 2) {
+            if (mode == 0)
+                ROWS(8, 2, 0);
+            else if (mode == 1)
+                ROWS(8, 2, 1);
+            else
+                ROWS(8, 2, 2);
+        } else if (R == 4) {
+            if (mode == 0)
+                ROWS(4, 4, 0);
+            else if (mode == 1)
+                ROWS(4, 4, 1);
+            else
+                ROWS(4, 4, 2);
+        } else {
+            if (mode == 0)
+                ROWS(8, 4, 0);
+            else if (mode == 1)
+                ROWS(8, 4, 1);
+            else
+                ROWS(8, 4, 2);
+        }
+#undef ROWS
+        return;
+    }
+    if (n->conv_kind >= 1) {
         const size_t bricks = (size_t)B * D * ((H + CL_BY - 1) / CL_BY) * ((W + CL_BX - 1) / CL_BX);
         const dim3 g2((unsigned)bricks, (unsigned)((coutw + CL_CT - 1) / CL_CT));
         if (mode == 0)
@@ -805,38 +1048,38 @@ static int forward(ppf_unet *n, int B, int D, int H, int W)
     const Layer *L = n->L;
     auto wb = [&](int i) { return n->wb + L[i].wb_off; };
     auto bias = [&](int i) { return n->p + L[i].b_off; };
-    hipLaunchKernelGGL(k_pack_in, dim3(grid_for(V1 * 32)), dim3(256), 0, n->stream, n->xin, V1,
+    hipLaunchKernelGGL(k_pack_in, dim3(blocks_for(V1 * 4)), dim3(256), 0, n->stream, n->xin, V1,
                        n->X0);
     // level 1
     conv(n, 0, n->X0, C, 0, L[0].cin, wb(0), L[0].cout, bias(0), L[0].cout, n->E1a, nullptr, C,
          0, B, D, H, W);
     conv(n, 0, n->E1a, C, 0, C, wb(1), C, bias(1), C, n->U1, nullptr, 3 * C, 2 * C, B, D, H, W);
-    hipLaunchKernelGGL(k_pool_fwd, dim3(grid_for(V1 / 8 * C)), dim3(256), 0, n->stream, n->U1,
-                       3 * C, 2 * C, C, n->P1, B, D, H, W);
+    hipLaunchKernelGGL(k_pool_fwd, dim3(row_blocks(B * D2 * H2, W2 * C / 8)), dim3(256), 0,
+                       n->stream, n->U1, 3 * C, 2 * C, C, n->P1, B, D, H, W);
     // level 2
     conv(n, 0, n->P1, C, 0, C, wb(2), 2 * C, bias(2), 2 * C, n->E2a, nullptr, 2 * C, 0, B, D2, H2,
          W2);
     conv(n, 0, n->E2a, 2 * C, 0, 2 * C, wb(3), 2 * C, bias(3), 2 * C, n->U2, nullptr, 6 * C, 4 * C,
          B, D2, H2, W2);
-    hipLaunchKernelGGL(k_pool_fwd, dim3(grid_for(V1 / 64 * 2 * C)), dim3(256), 0, n->stream,
-                       n->U2, 6 * C, 4 * C, 2 * C, n->P2, B, D2, H2, W2);
+    hipLaunchKernelGGL(k_pool_fwd, dim3(row_blocks(B * D3 * H3, W3 * 2 * C / 8)), dim3(256), 0,
+                       n->stream, n->U2, 6 * C, 4 * C, 2 * C, n->P2, B, D2, H2, W2);
     // level 3
     conv(n, 0, n->P2, 2 * C, 0, 2 * C, wb(4), 4 * C, bias(4), 4 * C, n->E3a, nullptr, 4 * C, 0, B,
          D3, H3, W3);
     conv(n, 0, n->E3a, 4 * C, 0, 4 * C, wb(5), 4 * C, bias(5), 4 * C, n->E3b, nullptr, 4 * C, 0, B,
          D3, H3, W3);
     // decoder
-    hipLaunchKernelGGL(k_up_fwd, dim3(grid_for(V1 / 8 * 4 * C)), dim3(256), 0, n->stream, n->E3b,
-                       4 * C, n->U2, 6 * C, 0, B, D2, H2, W2);
+    hipLaunchKernelGGL(k_up_fwd, dim3(row_blocks(B * D2 * H2, W2 * 4 * C / 8)), dim3(256), 0,
+                       n->stream, n->E3b, 4 * C, n->U2, 6 * C, 0, B, D2, H2, W2);
     conv(n, 0, n->U2, 6 * C, 0, 6 * C, wb(6), 2 * C, bias(6), 2 * C, n->D2a, nullptr, 2 * C, 0, B,
          D2, H2, W2);
     conv(n, 0, n->D2a, 2 * C, 0, 2 * C, wb(7), 2 * C, bias(7), 2 * C, n->D2b, nullptr, 2 * C, 0, B,
          D2, H2, W2);
-    hipLaunchKernelGGL(k_up_fwd, dim3(grid_for(V1 * 2 * C)), dim3(256), 0, n->stream, n->D2b,
-                       2 * C, n->U1, 3 * C, 0, B, D, H, W);
+    hipLaunchKernelGGL(k_up_fwd, dim3(row_blocks(B * D * H, W * 2 * C / 8)), dim3(256), 0,
+                       n->stream, n->D2b, 2 * C, n->U1, 3 * C, 0, B, D, H, W);
     conv(n, 0, n->U1, 3 * C, 0, 3 * C, wb(8), C, bias(8), C, n->D1a, nullptr, C, 0, B, D, H, W);
     conv(n, 0, n->D1a, C, 0, C, wb(9), C, bias(9), C, n->D1b, nullptr, C, 0, B, D, H, W);
-    hipLaunchKernelGGL(k_head_fwd, dim3(grid_for(V1)), dim3(256), 0, n->stream, n->D1b, C,
+    hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(V1)), dim3(256), 0, n->stream, n->D1b, C,
                        n->p + n->head_w, n->p + n->head_b, n->xin, V1, n->yout);
     UHIP(n, hipGetLastError());
     return PPF_OK;
@@ -1028,7 +1271,7 @@ extern "C" int ppf_unet_create(ppf_unet **out, int base, int device, uint64_t se
         return uerr(nullptr, PPF_EHIP, "hipSetDevice failed");
     ppf_unet *n = new ppf_unet();
     n->base = base;
-    n->conv_lds = 1;
+    n->conv_kind = 2;
     n->device = device;
     const int C = base;
     // (cin_log, cin padded, cout) of the ten 3x3x3 convolutions
@@ -1323,11 +1566,11 @@ extern "C" int ppf_unet_last_forward(ppf_unet *n, double *ms, double *flop)
     return PPF_OK;
 }
 
-extern "C" int ppf_unet_set_conv_kernel(ppf_unet *n, int lds)
+extern "C" int ppf_unet_set_conv_kernel(ppf_unet *n, int kind)
 {
-    if (!n || (lds != 0 && lds != 1))
-        return n ? uerr(n, PPF_EARG, "kernel must be 0 (direct) or 1 (LDS-tiled)") : PPF_EARG;
-    n->conv_lds = lds;
+    if (!n || kind < 0 || kind > 4)
+        return n ? uerr(n, PPF_EARG, "convolution kernel must be 0..4") : PPF_EARG;
+    n->conv_kind = kind;
     return PPF_OK;
 }
 

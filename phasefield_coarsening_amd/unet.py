@@ -151,9 +151,10 @@ class UNet:
         """Jump the solver's state in place (solute projected back)."""
         self._check(lib().ppf_unet_apply(self._h, solver._h))
 
-    def set_conv_kernel(self, lds):
-        """1: LDS-tiled convolutions (default); 0: the direct form."""
-        self._check(lib().ppf_unet_set_conv_kernel(self._h, int(bool(lds))))
+    def set_conv_kernel(self, kind):
+        """2: row-blocked convolutions (default); 3, 4: row-blocked with fixed
+        tiles; 1: LDS-tiled; 0: the direct form."""
+        self._check(lib().ppf_unet_set_conv_kernel(self._h, int(kind)))
 
     def last_forward(self):
         ms, fl = ctypes.c_double(), ctypes.c_double()

@@ -32,15 +32,21 @@ def _data(seed, shape=(2, 16, 16, 16)):
     return x, y
 
 
-@pytest.mark.parametrize("lds", [1, 0])
+# convolution kernels: 2 row-blocked (default), 3 and 4 its fixed-tile
+# variants, 1 LDS-tiled, 0 direct
+KINDS = [2, 3, 4, 1, 0]
+
+
+@pytest.mark.parametrize("kind", KINDS)
 @pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 24, 12, 40)])
-def test_forward_matches_reference(lds, shape):
-    """Both convolution kernels (LDS-tiled and direct), on a cube and on a
-    grid whose x and y are not multiples of the LDS brick."""
+def test_forward_matches_reference(kind, shape):
+    """Every convolution kernel, on a cube and on a grid whose x and y are
+    not multiples of the kernels' tiles (coarse levels 6 x 3 x 10, smaller
+    than a tile)."""
     p = _params(1)
     x, _ = _data(2, shape)
     with UNet(seed=3) as net:
-        net.set_conv_kernel(lds)
+        net.set_conv_kernel(kind)
         net.set_params(p)
         got = net.forward(x)
     want = unet_ref.predict(p, x, bf16=True)
@@ -54,12 +60,12 @@ def test_forward_matches_reference(lds, shape):
     assert np.abs(got - unet_ref.predict(p, x, bf16=False)).mean() < 2e-2 * scale
 
 
-@pytest.mark.parametrize("lds", [1, 0])
-def test_gradient_matches_reference(lds):
+@pytest.mark.parametrize("kind", KINDS)
+def test_gradient_matches_reference(kind):
     p = _params(4)
     x, y = _data(5)
     with UNet(seed=3) as net:
-        net.set_conv_kernel(lds)
+        net.set_conv_kernel(kind)
         net.set_params(p)
         loss, g = net.loss_grad(x, y)
     lr, gr = unet_ref.loss_and_grad(p, x, y, bf16=True)
