@@ -79,6 +79,10 @@ typedef struct ppf_stats {
     double min_c, max_c; /* range of c */
     double n_sites;      /* nz*ny*nx */
     double n_nonfinite;  /* sites where any field is NaN or Inf */
+    double sum_h1h;      /* sum of hS (1 - hS): an interface measure.  For a tanh
+                          * interface phi, |grad phi| is proportional to phi (1 - phi),
+                          * so 3 sum_h / sum_h1h is proportional to the mean
+                          * volume-to-area radius of the solid */
 } ppf_stats;
 
 /* Sums of the last step's squared increments over the interior

@@ -7,8 +7,9 @@
 
 // Per-workgroup partial statistics of the state.  Record per block:
 //   [0,1] sum c as double-double, [2] sum h, [3] count etaS > 0.5,
-//   [4] sum etaS, [5] min c, [6] max c, [7] non-finite sites.
-constexpr int STAT_W = 8;
+//   [4] sum etaS, [5] min c, [6] max c, [7] non-finite sites,
+//   [8] sum h (1 - h), an interface measure.
+constexpr int STAT_W = 9;
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_stats(const T *__restrict__ s, const T *__restrict__ l,
@@ -16,7 +17,7 @@ __global__ __launch_bounds__(256) void k_stats(const T *__restrict__ s, const T 
                                                double *__restrict__ partials)
 {
     DD sc = {0.0, 0.0};
-    double sh = 0.0, ns = 0.0, ss = 0.0, mn = INFINITY, mx = -INFINITY, nf = 0.0;
+    double sh = 0.0, ns = 0.0, ss = 0.0, mn = INFINITY, mx = -INFINITY, nf = 0.0, si = 0.0;
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
         const double sv = (double)s[q], lv = (double)l[q], cv = (double)c[q];
         if (!isfinite(sv) || !isfinite(lv) || !isfinite(cv)) {
@@ -24,7 +25,9 @@ __global__ __launch_bounds__(256) void k_stats(const T *__restrict__ s, const T 
             continue;
         }
         sc = dd_add1(sc, cv);
-        sh += sv * sv / (sv * sv + lv * lv);
+        const double hv = sv * sv / (sv * sv + lv * lv);
+        sh += hv;
+        si += hv * (1.0 - hv);
         ns += sv > 0.5 ? 1.0 : 0.0;
         ss += sv;
         mn = fmin(mn, cv);
@@ -37,6 +40,7 @@ __global__ __launch_bounds__(256) void k_stats(const T *__restrict__ s, const T 
     mn = wave_min(mn);
     mx = wave_max(mx);
     nf = wave_sum(nf);
+    si = wave_sum(si);
     __shared__ double part[STAT_W][4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) {
@@ -48,12 +52,13 @@ __global__ __launch_bounds__(256) void k_stats(const T *__restrict__ s, const T 
         part[5][w] = mn;
         part[6][w] = mx;
         part[7][w] = nf;
+        part[8][w] = si;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         DD t = {part[0][0], part[1][0]};
         double a = part[2][0], b = part[3][0], d = part[4][0], e = part[5][0], f = part[6][0],
-               g = part[7][0];
+               g = part[7][0], h = part[8][0];
         for (int j = 1; j < 4; j++) {
             t = dd_add(t, DD{part[0][j], part[1][j]});
             a += part[2][j];
@@ -62,6 +67,7 @@ __global__ __launch_bounds__(256) void k_stats(const T *__restrict__ s, const T 
             e = fmin(e, part[5][j]);
             f = fmax(f, part[6][j]);
             g += part[7][j];
+            h += part[8][j];
         }
         double *o = partials + (size_t)blockIdx.x * STAT_W;
         o[0] = t.hi;
@@ -72,6 +78,7 @@ __global__ __launch_bounds__(256) void k_stats(const T *__restrict__ s, const T 
         o[5] = e;
         o[6] = f;
         o[7] = g;
+        o[8] = h;
     }
 }
 
@@ -80,7 +87,7 @@ static __global__ __launch_bounds__(256) void k_stats_final(const double *__rest
                                                      double *__restrict__ out)
 {
     DD sc = {0.0, 0.0};
-    double sh = 0.0, ns = 0.0, ss = 0.0, mn = INFINITY, mx = -INFINITY, nf = 0.0;
+    double sh = 0.0, ns = 0.0, ss = 0.0, mn = INFINITY, mx = -INFINITY, nf = 0.0, si = 0.0;
     for (int j = threadIdx.x; j < nb; j += 256) {
         const double *r = p + (size_t)j * STAT_W;
         sc = dd_add(sc, DD{r[0], r[1]});
@@ -90,6 +97,7 @@ static __global__ __launch_bounds__(256) void k_stats_final(const double *__rest
         mn = fmin(mn, r[5]);
         mx = fmax(mx, r[6]);
         nf += r[7];
+        si += r[8];
     }
     sc = wave_dd_sum(sc);
     sh = wave_sum(sh);
@@ -98,6 +106,7 @@ static __global__ __launch_bounds__(256) void k_stats_final(const double *__rest
     mn = wave_min(mn);
     mx = wave_max(mx);
     nf = wave_sum(nf);
+    si = wave_sum(si);
     __shared__ double part[STAT_W][4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) {
@@ -109,12 +118,13 @@ static __global__ __launch_bounds__(256) void k_stats_final(const double *__rest
         part[5][w] = mn;
         part[6][w] = mx;
         part[7][w] = nf;
+        part[8][w] = si;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         DD t = {part[0][0], part[1][0]};
         double a = part[2][0], b = part[3][0], d = part[4][0], e = part[5][0], f = part[6][0],
-               g = part[7][0];
+               g = part[7][0], h = part[8][0];
         for (int j = 1; j < 4; j++) {
             t = dd_add(t, DD{part[0][j], part[1][j]});
             a += part[2][j];
@@ -123,6 +133,7 @@ static __global__ __launch_bounds__(256) void k_stats_final(const double *__rest
             e = fmin(e, part[5][j]);
             f = fmax(f, part[6][j]);
             g += part[7][j];
+            h += part[8][j];
         }
         out[0] = t.hi + t.lo;
         out[1] = a;
@@ -131,6 +142,7 @@ static __global__ __launch_bounds__(256) void k_stats_final(const double *__rest
         out[4] = e;
         out[5] = f;
         out[6] = g;
+        out[7] = h;
     }
 }
 

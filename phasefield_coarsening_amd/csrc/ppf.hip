@@ -627,7 +627,7 @@ extern "C" int ppf_reduce(ppf_ctx *ctx, ppf_stats *out)
                        ctx->dres);
     HIPCHK(ctx, hipGetLastError());
     double r[8];
-    HIPCHK(ctx, hipMemcpyAsync(r, ctx->dres, 7 * sizeof(double), hipMemcpyDeviceToHost,
+    HIPCHK(ctx, hipMemcpyAsync(r, ctx->dres, 8 * sizeof(double), hipMemcpyDeviceToHost,
                                ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     out->sum_c = r[0];
@@ -637,6 +637,7 @@ extern "C" int ppf_reduce(ppf_ctx *ctx, ppf_stats *out)
     out->min_c = r[4];
     out->max_c = r[5];
     out->n_nonfinite = r[6];
+    out->sum_h1h = r[7];
     out->n_sites = (double)ctx->n;
     return PPF_OK;
 }

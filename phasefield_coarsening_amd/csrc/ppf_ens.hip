@@ -598,7 +598,7 @@ extern "C" int ppf_ens_reduce(ppf_ens *e, int dom, ppf_stats *out)
                            (const float *)m.f[e->parity][0], (const float *)m.f[e->parity][1],
                            (const float *)m.f[e->parity][2], e->nsite[dom], part);
     hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, e->stream, part, nb, e->dres);
-    double r[7];
+    double r[8];
     hipError_t h = hipGetLastError();
     if (h == hipSuccess)
         h = hipMemcpyAsync(r, e->dres, sizeof r, hipMemcpyDeviceToHost, e->stream);
@@ -614,6 +614,7 @@ extern "C" int ppf_ens_reduce(ppf_ens *e, int dom, ppf_stats *out)
     out->min_c = r[4];
     out->max_c = r[5];
     out->n_nonfinite = r[6];
+    out->sum_h1h = r[7];
     out->n_sites = (double)e->nsite[dom];
     return PPF_OK;
 }

@@ -43,7 +43,7 @@ class Desc(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("sum_c", "sum_h", "n_solid", "sum_s", "min_c", "max_c", "n_sites",
-                 "n_nonfinite")]
+                 "n_nonfinite", "sum_h1h")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

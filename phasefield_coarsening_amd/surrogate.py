@@ -12,9 +12,9 @@ Pipeline:
 4. ``Controller.advance`` runs a grid to a target step count: from a
    checkpoint it jumps K steps with the network (solute projected back),
    relaxes with ``relax`` explicit steps, and judges the jump by how much
-   the mean particle radius and the solid fraction move during the
-   relaxation.  A jump that moves either by more than its tolerance is
-   undone (checkpoint restored) and the K + relax steps are re-run exactly.
+   a radius measure and the solid fraction move during the relaxation.  A
+   jump that moves either by more than its tolerance is undone (checkpoint
+   restored) and the K + relax steps are re-run exactly.
 
 The reference has no counterpart: it only steps explicitly
 (phasefield.c:178-258).
@@ -116,10 +116,23 @@ def eval_loss(net, data, batch=4):
 
 class Controller:
     """Advance ``solver`` with learned jumps of ``k`` steps, each followed by
-    ``relax`` explicit steps and judged; rejected jumps are re-run exactly."""
+    ``relax`` explicit steps and judged; rejected jumps are re-run exactly.
+
+    Judge (``judge``):
+    - ``"area"`` (default): the volume-to-area radius 3 sum(h) / sum(h (1-h))
+      and the solid fraction, both from one reduction pass (``ppf_reduce``);
+    - ``"particles"``: the mean particle radius from connected components
+      (``ppf_particles``) and the solid fraction, two labelling passes per
+      jump.
+    After a rejection the next ``wait`` cycles run exactly without a trial
+    (``wait`` = 1, doubling per consecutive rejection up to ``max_wait``), so
+    a fast transient does not cost a network pass per cycle.
+    """
 
     def __init__(self, solver, net, k=200, relax=20, tol_radius=0.02, tol_fraction=0.02,
-                 threshold=0.5):
+                 threshold=0.5, judge="area", max_wait=8, timing=False):
+        if judge not in ("area", "particles"):
+            raise ValueError("judge must be 'area' or 'particles'")
         self.sv = solver
         self.net = net
         self.k = k
@@ -127,43 +140,78 @@ class Controller:
         self.tol_r = tol_radius
         self.tol_f = tol_fraction
         self.thr = threshold
+        self.judge = judge
+        self.max_wait = max_wait
+        self.timing = timing
         self.ckpt = S.Solver(solver.shape, bc=solver.bc, precision=solver.precision)
         self.log = []
+        self.phase_seconds = {"checkpoint": 0.0, "jump": 0.0, "judge": 0.0, "relax": 0.0,
+                              "exact": 0.0}
 
     def close(self):
         self.ckpt.close()
 
     def _measure(self):
         st = self.sv.reduce()
-        p, _ = self.sv.particles(self.thr, max_sizes=1)
-        return p["mean_radius"], st["sum_h"] / st["n_sites"], st["n_nonfinite"]
+        frac = st["sum_h"] / st["n_sites"]
+        if self.judge == "particles":
+            p, _ = self.sv.particles(self.thr, max_sizes=1)
+            r = p["mean_radius"]
+        else:
+            r = 3.0 * st["sum_h"] / st["sum_h1h"] if st["sum_h1h"] > 0 else 0.0
+        return r, frac, st["n_nonfinite"]
+
+    def _timed(self, phase, fn, *args):
+        if not self.timing:
+            return fn(*args)
+        self.sv.sync()
+        t0 = time.perf_counter()
+        out = fn(*args)
+        self.sv.sync()
+        self.phase_seconds[phase] += time.perf_counter() - t0
+        return out
 
     def advance(self, nsteps):
         """Run nsteps of model time; returns a summary dict."""
         t0 = time.perf_counter()
-        done = acc = rej = exact = 0
-        while nsteps - done >= self.k + self.relax:
-            self.ckpt.copy_from(self.sv)
-            self.net.apply(self.sv)
-            r_j, f_j, bad_j = self._measure()
-            self.sv.step(self.relax)
-            r_r, f_r, bad_r = self._measure()
+        done = acc = rej = skipped = exact = 0
+        skip, wait = 0, 1
+        cyc = self.k + self.relax
+        while nsteps - done >= cyc:
+            if skip > 0:
+                skip -= 1
+                skipped += 1
+                self._timed("exact", self.sv.step, cyc)
+                exact += cyc
+                self.log.append({"step": done, "trial": False})
+                done += cyc
+                continue
+            self._timed("checkpoint", self.ckpt.copy_from, self.sv)
+            self._timed("jump", self.net.apply, self.sv)
+            r_j, f_j, bad_j = self._timed("judge", self._measure)
+            self._timed("relax", self.sv.step, self.relax)
+            r_r, f_r, bad_r = self._timed("judge", self._measure)
             ok = (bad_j == 0 and bad_r == 0 and r_j > 0 and
                   abs(r_r - r_j) <= self.tol_r * r_j and abs(f_r - f_j) <= self.tol_f * max(f_j, 1e-12))
             if ok:
                 acc += 1
+                wait = 1
             else:
                 rej += 1
                 self.sv.copy_from(self.ckpt)
-                self.sv.step(self.k + self.relax)
-                exact += self.k + self.relax
-            self.log.append({"step": done, "accepted": bool(ok), "R_jump": r_j, "R_relax": r_r,
-                             "frac_jump": f_j, "frac_relax": f_r})
-            done += self.k + self.relax
+                self._timed("exact", self.sv.step, cyc)
+                exact += cyc
+                skip, wait = wait, min(2 * wait, self.max_wait)
+            self.log.append({"step": done, "trial": True, "accepted": bool(ok), "R_jump": r_j,
+                             "R_relax": r_r, "frac_jump": f_j, "frac_relax": f_r})
+            done += cyc
         if nsteps > done:
-            self.sv.step(nsteps - done)
+            self._timed("exact", self.sv.step, nsteps - done)
             exact += nsteps - done
             done = nsteps
         self.sv.sync()
-        return {"steps": done, "accepted": acc, "rejected": rej, "exact_steps": exact,
-                "seconds": time.perf_counter() - t0}
+        out = {"steps": done, "accepted": acc, "rejected": rej, "skipped": skipped,
+               "exact_steps": exact, "seconds": time.perf_counter() - t0, "judge": self.judge}
+        if self.timing:
+            out["phase_seconds"] = dict(self.phase_seconds)
+        return out

@@ -13,8 +13,9 @@
    "no change" (persistence) on the held-out pairs.
 4. Surrogate at ``--grid``^3 (``--spheres`` spheres, R ~ U[6, 12]): the
    exact FP64 run to ``--t-end`` steps against the controller (jump K, 20
-   FP32 relax steps, judge, exact re-run on rejection), compared on mean
-   particle radius and solid fraction at t_end; wall times give the speedup.
+   FP32 relax steps, judge ``--judge``, exact re-run on rejection), compared
+   on mean particle radius (connected components) and solid fraction at
+   t_end; wall times give the speedup.
 
 Weights go to ``--save`` (float32 .npy) when given; ``--load`` skips
 training.  One JSON line per stage is printed.
@@ -46,6 +47,12 @@ def main(argv=None):
     ap.add_argument("--spheres", type=int, default=500)
     ap.add_argument("--t-end", type=int, default=2200)
     ap.add_argument("--relax", type=int, default=20)
+    ap.add_argument("--judge", default="area", choices=("area", "particles"),
+                    help="area: volume-to-area radius from one reduction; particles: "
+                         "connected-component mean radius")
+    ap.add_argument("--timing", action="store_true",
+                    help="synchronise around each controller phase and report its seconds "
+                         "(adds a little time)")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--save", default="")
     ap.add_argument("--load", default="",
@@ -95,7 +102,7 @@ def main(argv=None):
     with S.Solver(shape, precision=S.F32) as sv:
         sv.upload(*s0)
         del s0
-        ctl = SG.Controller(sv, net, k=a.k, relax=a.relax)
+        ctl = SG.Controller(sv, net, k=a.k, relax=a.relax, judge=a.judge, timing=a.timing)
         summ = ctl.advance(a.t_end)
         pr, _ = sv.particles(0.5, max_sizes=1)
         st = sv.reduce()
@@ -106,7 +113,9 @@ def main(argv=None):
                       "relax": a.relax, "exact_fp64_seconds": round(t_exact, 4),
                       "surrogate_seconds": round(summ["seconds"], 4),
                       "speedup": round(t_exact / summ["seconds"], 3),
-                      "accepted": summ["accepted"], "rejected": summ["rejected"],
+                      "judge": a.judge, "accepted": summ["accepted"],
+                      "rejected": summ["rejected"], "skipped": summ["skipped"],
+                      "phase_seconds": summ.get("phase_seconds"),
                       "exact_steps_in_surrogate": summ["exact_steps"],
                       "R_exact": r_exact, "R_surrogate": r_sur,
                       "R_err_rel": abs(r_sur - r_exact) / r_exact,

@@ -138,6 +138,7 @@ def test_stats_match_numpy():
     h = s * s / (s * s + l * l)
     np.testing.assert_allclose(st["sum_c"], np.sum(c), rtol=1e-14)
     np.testing.assert_allclose(st["sum_h"], h.sum(), rtol=1e-12)
+    np.testing.assert_allclose(st["sum_h1h"], (h * (1.0 - h)).sum(), rtol=1e-12)
     np.testing.assert_allclose(st["sum_s"], s.sum(), rtol=1e-12)
     assert st["n_solid"] == (s > 0.5).sum()
     assert st["min_c"] == c.min() and st["max_c"] == c.max()

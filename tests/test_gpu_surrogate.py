@@ -33,10 +33,11 @@ def test_trained_jump_beats_no_jump_on_fresh_states():
     assert mse_net < mse_keep / 2
 
 
-def test_controller_accuracy_vs_exact_fp64():
+@pytest.mark.parametrize("judge", ["area", "particles"])
+def test_controller_accuracy_vs_exact_fp64(judge):
     """128^3, 60 spheres: jumps + 20 FP32 relax steps (rejected jumps re-run
-    exactly) to t_end = 880; mean particle radius and solid fraction within
-    2% of the exact FP64 run, total solute on target."""
+    exactly) to t_end = 880, with either judge; mean particle radius and
+    solid fraction within 2% of the exact FP64 run, total solute on target."""
     shape = (128, 128, 128)
     cen, rad = S.spheres(21, 60, shape, 6.0, 12.0)
     with S.Solver(shape) as ref:
@@ -48,13 +49,14 @@ def test_controller_accuracy_vs_exact_fp64():
     with S.Solver(shape, precision=S.F32) as sv, _net() as net:
         sv.upload(*s0)
         target = sv.mass_target()[0]
-        ctl = SG.Controller(sv, net)
+        ctl = SG.Controller(sv, net, judge=judge)
         summ = ctl.advance(880)
         ps, _ = sv.particles(0.5, 1)
         ss = sv.reduce()
         ctl.close()
     print(summ, pr["mean_radius"], ps["mean_radius"])
-    assert summ["steps"] == 880 and summ["accepted"] + summ["rejected"] == 4
+    assert summ["steps"] == 880
+    assert summ["accepted"] + summ["rejected"] + summ["skipped"] == 4
     assert abs(ps["mean_radius"] - pr["mean_radius"]) / pr["mean_radius"] < 0.02
     f_ref, f_sur = st["sum_h"] / st["n_sites"], ss["sum_h"] / ss["n_sites"]
     assert abs(f_sur - f_ref) / f_ref < 0.02
