@@ -173,6 +173,21 @@ int ppf_spheres_generate(uint64_t seed, int n, int nz, int ny, int nx, double rm
 int ppf_set_tiling(ppf_ctx *ctx, int tx, int ty, int tz, int order);
 int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz, int *order);
 
+/* Two steps per launch (periodic contexts of the tiled kernels, FP64 or
+ * FP32): one z march computes step t+1 on each tile plus a one-cell ring
+ * and keeps it in LDS, then step t+2 on the tile, so each site's state is
+ * read and written once per two steps (24 B per site update in FP64
+ * instead of 48).  Every site is updated with the same operands in the same
+ * order as by a single step, so results are bit for bit those of single
+ * steps; ppf_step runs pairs and a single step for an odd remainder, and
+ * never fuses across an FP32 mass shift.  steps_per_launch is 1 (off) or 2;
+ * (tx, ty) in {(64,8), (32,8), (32,16), (64,16)} or (0,0) to keep; tz > 0
+ * or 0 to keep; order bits 0-1 as ppf_set_tiling, negative to keep.
+ * ppf_get_fusion reports 2 only while the context can fuse.  No counterpart
+ * in the reference, which sweeps once per step (phasefield.c:178-215). */
+int ppf_set_fusion(ppf_ctx *ctx, int steps_per_launch, int tx, int ty, int tz, int order);
+int ppf_get_fusion(ppf_ctx *ctx, int *steps_per_launch, int *tx, int *ty, int *tz, int *order);
+
 /* Time every compiled tiling (tile shape x tz in {16,32,64} x order in
  * {0,1,2,4,5,6}) on the current state, `reps` sweeps each, and keep the
  * fastest (ppf_get_tiling reads it back; *best_ms gets its time per

@@ -313,3 +313,271 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
     if (REPORT)
         block_sum3<NT>(acc, partials + 3 * (size_t)blockIdx.x);
 }
+
+// ---------------------------------------------------------------------------
+// Two steps per launch (periodic grids).  The new state of step t+2 is
+// computed from step t in one z march, with step t+1 kept on chip:
+//   stage 1, plane z:   step t -> t+1 on the tile and its one-cell face ring
+//                        (the ring sites are the in-plane neighbours stage 2
+//                        needs); step t of the tile, of that ring, of the
+//                        two-cell face ring and of the four one-ring corners
+//                        sits in LDS (A1);
+//   stage 2, plane z-1: step t+1 -> t+2 on the tile, from A2 (step t+1 of
+//                        the tile and its ring) and registers (z neighbours).
+// Every site, ring sites included, is updated at its periodic image with the
+// same operands in the same order as a single step, so two steps here equal
+// two single steps bit for bit.  (Mirror boundaries would reorder the two x
+// or y neighbours of an image site, so they are not offered.)  Each output
+// plane is read and written once for two steps: 24 B per site update in
+// FP64 instead of 48.  A z chunk [z0, zend) runs stage 1 over
+// [z0-1, zend] and so reads step t over [z0-2, zend+1].
+// Per plane: A1 written, barrier, stage 1 (tile and ring) and stage 2,
+// barrier, A2 written.  The barriers separate every write of A1 or A2 from
+// the reads of the previous plane, so both are single buffered.
+// Threads: tid < NR1 also own a ring site (its own z column of step t);
+// NR1 <= tid < NR1 + NO2 load one outer site (two-cell ring or corner) of
+// the current plane.
+// ---------------------------------------------------------------------------
+template <class P, int TX, int TY>
+__device__ __forceinline__ void tile_march2(
+    const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
+    const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
+    typename P::T *__restrict__ l_out, typename P::T *__restrict__ c_out, int nx, int ny, int nz,
+    int x0, int y0, int z0, int zend, const typename P::K &k,
+    typename P::T (*__restrict__ a1)[TY + 4][TX + 4], typename P::T (*__restrict__ a2)[TY + 2][TX + 2])
+{
+    constexpr int NR1 = 2 * TX + 2 * TY, NO2 = 2 * TX + 2 * TY + 4;
+    static_assert(NR1 + NO2 <= TX * TY, "ring sites exceed the workgroup");
+    using T = typename P::T;
+    const int tid = threadIdx.x;
+    const int tx = tid % TX, ty = tid / TX;
+    const bool active = x0 + tx < nx && y0 + ty < ny;
+    const size_t plane = (size_t)nx * ny;
+#define PLANE(p, zz) uniform_ptr((p) + (size_t)(zz) * plane)
+    // site (x0 + ax, y0 + ay) of the tile frame -> offset within a plane
+    auto colof = [&](int ax, int ay) {
+        return (unsigned)ghost_any<0>(y0 + ay, ny) * (unsigned)nx + (unsigned)ghost_any<0>(x0 + ax, nx);
+    };
+    const unsigned col = colof(tx, ty);
+    const unsigned ocol = (unsigned)(y0 + ty) * (unsigned)nx + (unsigned)(x0 + tx);
+
+    // one-cell ring site (A2 frame rx, ry; tile frame rx-1, ry-1)
+    int rx = 0, ry = 0;
+    if (tid < TY) {
+        rx = 0;
+        ry = tid + 1;
+    } else if (tid < 2 * TY) {
+        rx = TX + 1;
+        ry = tid - TY + 1;
+    } else if (tid < 2 * TY + TX) {
+        rx = tid - 2 * TY + 1;
+        ry = 0;
+    } else if (tid < NR1) {
+        rx = tid - 2 * TY - TX + 1;
+        ry = TY + 1;
+    }
+    const bool ring = tid < NR1;
+    const unsigned rcol = colof(rx - 1, ry - 1);
+    // outer site (A1 frame ox, oy; tile frame ox-2, oy-2)
+    int ox = 0, oy = 0;
+    {
+        const int j = tid - NR1;
+        if (j < 0) {
+        } else if (j < TY) {
+            ox = 0;
+            oy = j + 2;
+        } else if (j < 2 * TY) {
+            ox = TX + 3;
+            oy = j - TY + 2;
+        } else if (j < 2 * TY + TX) {
+            ox = j - 2 * TY + 2;
+            oy = 0;
+        } else if (j < 2 * TY + 2 * TX) {
+            ox = j - 2 * TY - TX + 2;
+            oy = TY + 3;
+        } else if (j < NO2) {
+            const int c = j - 2 * TY - 2 * TX;
+            ox = (c & 1) ? TX + 2 : 1;
+            oy = (c & 2) ? TY + 2 : 1;
+        }
+    }
+    const bool outer = tid >= NR1 && tid < NR1 + NO2;
+    const unsigned o2col = colof(ox - 2, oy - 2);
+
+    // step t columns of the tile site (p: z-1, c: z, n: z+1 raw) and of the
+    // ring site (q*)
+    T s_p, l_p, mu_p, M_p, s_c, l_c, c_c, h_c, mu_c, M_c;
+    T qs_p = 0, ql_p = 0, qmu_p = 0, qM_p = 0, qs_c = 0, ql_c = 0, qc_c = 0, qh_c = 0, qmu_c = 0,
+      qM_c = 0;
+    {
+        const int zp = ghost_any<0>(z0 - 2, nz), zc = ghost_any<0>(z0 - 1, nz);
+        T hp;
+        s_p = PLANE(s_in, zp)[col];
+        l_p = PLANE(l_in, zp)[col];
+        P::point(s_p, l_p, PLANE(c_in, zp)[col], k, hp, mu_p, M_p);
+        s_c = PLANE(s_in, zc)[col];
+        l_c = PLANE(l_in, zc)[col];
+        c_c = PLANE(c_in, zc)[col];
+        P::point(s_c, l_c, c_c, k, h_c, mu_c, M_c);
+        if (ring) {
+            qs_p = PLANE(s_in, zp)[rcol];
+            ql_p = PLANE(l_in, zp)[rcol];
+            T qh;
+            P::point(qs_p, ql_p, PLANE(c_in, zp)[rcol], k, qh, qmu_p, qM_p);
+            qs_c = PLANE(s_in, zc)[rcol];
+            ql_c = PLANE(l_in, zc)[rcol];
+            qc_c = PLANE(c_in, zc)[rcol];
+            P::point(qs_c, ql_c, qc_c, k, qh_c, qmu_c, qM_c);
+        }
+    }
+    // step t+1 column of the tile site: u*_p plane z-2, u*_c plane z-1
+    T us_p = 0, ul_p = 0, umu_p = 0, uM_p = 0, us_c = 0, ul_c = 0, uc_c = 0, uh_c = 0, umu_c = 0,
+      uM_c = 0;
+
+    for (int zs = z0 - 1; zs <= zend; ++zs) {
+        const int zc = ghost_any<0>(zs, nz), zn = ghost_any<0>(zs + 1, nz);
+        // step t, plane zs+1 (raw) of the tile and ring columns; plane zs of
+        // the outer site
+        const T rs_n = PLANE(s_in, zn)[col], rl_n = PLANE(l_in, zn)[col],
+                rc_n = PLANE(c_in, zn)[col];
+        T qs_n = 0, ql_n = 0, qc_n = 0;
+        if (ring) {
+            qs_n = PLANE(s_in, zn)[rcol];
+            ql_n = PLANE(l_in, zn)[rcol];
+            qc_n = PLANE(c_in, zn)[rcol];
+        }
+        // A1: step t of plane zs
+        a1[0][ty + 2][tx + 2] = s_c;
+        a1[1][ty + 2][tx + 2] = l_c;
+        a1[2][ty + 2][tx + 2] = mu_c;
+        a1[3][ty + 2][tx + 2] = M_c;
+        if (ring) {
+            a1[0][ry + 1][rx + 1] = qs_c;
+            a1[1][ry + 1][rx + 1] = ql_c;
+            a1[2][ry + 1][rx + 1] = qmu_c;
+            a1[3][ry + 1][rx + 1] = qM_c;
+        }
+        if (outer) {
+            const T os = PLANE(s_in, zc)[o2col], ol = PLANE(l_in, zc)[o2col];
+            T oh, omu, oM;
+            P::point(os, ol, PLANE(c_in, zc)[o2col], k, oh, omu, oM);
+            a1[0][oy][ox] = os;
+            a1[1][oy][ox] = ol;
+            a1[2][oy][ox] = omu;
+            a1[3][oy][ox] = oM;
+        }
+        T h_n, mu_n, M_n, qh_n = 0, qmu_n = 0, qM_n = 0;
+        P::point(rs_n, rl_n, rc_n, k, h_n, mu_n, M_n);
+        if (ring)
+            P::point(qs_n, ql_n, qc_n, k, qh_n, qmu_n, qM_n);
+        __syncthreads();
+
+        // stage 1: step t+1 of plane zs at the tile site
+        T s1, l1, c1, h1, mu1, M1;
+        {
+            const int ax = tx + 2, ay = ty + 2;
+            const T sn[6] = {rs_n, s_p, a1[0][ay + 1][ax], a1[0][ay - 1][ax], a1[0][ay][ax + 1], a1[0][ay][ax - 1]};
+            const T ln[6] = {rl_n, l_p, a1[1][ay + 1][ax], a1[1][ay - 1][ax], a1[1][ay][ax + 1], a1[1][ay][ax - 1]};
+            const T mun[6] = {mu_n, mu_p, a1[2][ay + 1][ax], a1[2][ay - 1][ax], a1[2][ay][ax + 1], a1[2][ay][ax - 1]};
+            const T Mn[6] = {M_n, M_p, a1[3][ay + 1][ax], a1[3][ay - 1][ax], a1[3][ay][ax + 1], a1[3][ay][ax - 1]};
+            double ds, dl, dc;
+            P::update(s_c, l_c, c_c, h_c, mu_c, M_c, sn, ln, mun, Mn, k, s1, l1, c1, ds, dl, dc);
+            P::point(s1, l1, c1, k, h1, mu1, M1);
+        }
+        // ... and at the ring site
+        T qs1 = 0, ql1 = 0, qmu1 = 0, qM1 = 0;
+        if (ring) {
+            const int ax = rx + 1, ay = ry + 1;
+            const T sn[6] = {qs_n, qs_p, a1[0][ay + 1][ax], a1[0][ay - 1][ax], a1[0][ay][ax + 1], a1[0][ay][ax - 1]};
+            const T ln[6] = {ql_n, ql_p, a1[1][ay + 1][ax], a1[1][ay - 1][ax], a1[1][ay][ax + 1], a1[1][ay][ax - 1]};
+            const T mun[6] = {qmu_n, qmu_p, a1[2][ay + 1][ax], a1[2][ay - 1][ax], a1[2][ay][ax + 1], a1[2][ay][ax - 1]};
+            const T Mn[6] = {qM_n, qM_p, a1[3][ay + 1][ax], a1[3][ay - 1][ax], a1[3][ay][ax + 1], a1[3][ay][ax - 1]};
+            T qc1, qh1;
+            double ds, dl, dc;
+            P::update(qs_c, ql_c, qc_c, qh_c, qmu_c, qM_c, sn, ln, mun, Mn, k, qs1, ql1, qc1, ds, dl, dc);
+            P::point(qs1, ql1, qc1, k, qh1, qmu1, qM1);
+        }
+        // stage 2: step t+2 of plane zs-1 at the tile site (A2 holds step t+1
+        // of plane zs-1)
+        if (zs >= z0 + 1 && active) {
+            const int bx = tx + 1, by = ty + 1;
+            const T sn[6] = {s1, us_p, a2[0][by + 1][bx], a2[0][by - 1][bx], a2[0][by][bx + 1], a2[0][by][bx - 1]};
+            const T ln[6] = {l1, ul_p, a2[1][by + 1][bx], a2[1][by - 1][bx], a2[1][by][bx + 1], a2[1][by][bx - 1]};
+            const T mun[6] = {mu1, umu_p, a2[2][by + 1][bx], a2[2][by - 1][bx], a2[2][by][bx + 1], a2[2][by][bx - 1]};
+            const T Mn[6] = {M1, uM_p, a2[3][by + 1][bx], a2[3][by - 1][bx], a2[3][by][bx + 1], a2[3][by][bx - 1]};
+            T so, lo, co;
+            double ds, dl, dc;
+            P::update(us_c, ul_c, uc_c, uh_c, umu_c, uM_c, sn, ln, mun, Mn, k, so, lo, co, ds, dl, dc);
+            __builtin_nontemporal_store(so, PLANE(s_out, zs - 1) + ocol);
+            __builtin_nontemporal_store(lo, PLANE(l_out, zs - 1) + ocol);
+            __builtin_nontemporal_store(co, PLANE(c_out, zs - 1) + ocol);
+        }
+        __syncthreads();
+        // A2: step t+1 of plane zs
+        a2[0][ty + 1][tx + 1] = s1;
+        a2[1][ty + 1][tx + 1] = l1;
+        a2[2][ty + 1][tx + 1] = mu1;
+        a2[3][ty + 1][tx + 1] = M1;
+        if (ring) {
+            a2[0][ry][rx] = qs1;
+            a2[1][ry][rx] = ql1;
+            a2[2][ry][rx] = qmu1;
+            a2[3][ry][rx] = qM1;
+        }
+        // shift the columns
+        s_p = s_c;
+        l_p = l_c;
+        mu_p = mu_c;
+        M_p = M_c;
+        s_c = rs_n;
+        l_c = rl_n;
+        c_c = rc_n;
+        h_c = h_n;
+        mu_c = mu_n;
+        M_c = M_n;
+        qs_p = qs_c;
+        ql_p = ql_c;
+        qmu_p = qmu_c;
+        qM_p = qM_c;
+        qs_c = qs_n;
+        ql_c = ql_n;
+        qc_c = qc_n;
+        qh_c = qh_n;
+        qmu_c = qmu_n;
+        qM_c = qM_n;
+        us_p = us_c;
+        ul_p = ul_c;
+        umu_p = umu_c;
+        uM_p = uM_c;
+        us_c = s1;
+        ul_c = l1;
+        uc_c = c1;
+        uh_c = h1;
+        umu_c = mu1;
+        uM_c = M1;
+    }
+#undef PLANE
+}
+
+template <class P, int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_tiled2(
+    const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
+    const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
+    typename P::T *__restrict__ l_out, typename P::T *__restrict__ c_out, int nx, int ny, int nz,
+    int tz, int ntx, int nty, int order, typename P::K k)
+{
+    using T = typename P::T;
+    __shared__ T a1[4][TY + 4][TX + 4];
+    __shared__ T a2[4][TY + 2][TX + 2];
+    const int ntile = ntx * nty;
+    const int nblk = gridDim.x;
+    int b = blockIdx.x;
+    if ((order & 2) && (nblk & 7) == 0)
+        b = (b & 7) * (nblk >> 3) + (b >> 3);
+    const int tile = (order & 1) ? b / ((nblk + ntile - 1) / ntile) : b % ntile;
+    const int zc = (order & 1) ? b % ((nblk + ntile - 1) / ntile) : b / ntile;
+    const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY, z0 = zc * tz;
+    const int zend = min(z0 + tz, nz);
+    tile_march2<P, TX, TY>(s_in, l_in, c_in, s_out, l_out, c_out, nx, ny, nz, x0, y0, z0, zend, k,
+                           a1, a2);
+}

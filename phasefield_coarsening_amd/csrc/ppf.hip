@@ -194,6 +194,54 @@ static int launch_tiled(ppf_ctx *ctx, const typename P::K &k, bool conly, double
     return PPF_OK;
 }
 
+// Two steps per launch (k_tiled2, periodic only).
+template <class P>
+using fused_fn = void (*)(const typename P::T *, const typename P::T *, const typename P::T *,
+                          typename P::T *, typename P::T *, typename P::T *, int, int, int, int,
+                          int, int, int, typename P::K);
+
+static const TileCfg k_tiles2[] = {{64, 8}, {32, 8}, {32, 16}, {64, 16}};
+
+template <class P>
+static fused_fn<P> pick_fused(TileCfg t)
+{
+    if (t.tx == 64 && t.ty == 8)
+        return k_tiled2<P, 64, 8>;
+    if (t.tx == 32 && t.ty == 8)
+        return k_tiled2<P, 32, 8>;
+    if (t.tx == 32 && t.ty == 16)
+        return k_tiled2<P, 32, 16>;
+    if (t.tx == 64 && t.ty == 16)
+        return k_tiled2<P, 64, 16>;
+    return nullptr;
+}
+
+static bool fused_available(const ppf_ctx *ctx)
+{
+    return ctx->fuse == 2 && ctx->d.bc == PPF_BC_PERIODIC && ctx->kernel == PPF_KERNEL_TILED;
+}
+
+template <class P>
+static int launch_fused(ppf_ctx *ctx, const typename P::K &k)
+{
+    using T = typename P::T;
+    const int cs = ctx->cur[0], cl = ctx->cur[1], cc = ctx->cur[2];
+    const int ntx = (ctx->d.nx + ctx->tile2.tx - 1) / ctx->tile2.tx;
+    const int nty = (ctx->d.ny + ctx->tile2.ty - 1) / ctx->tile2.ty;
+    const int nzc = (ctx->d.nz + ctx->tz2 - 1) / ctx->tz2;
+    fused_fn<P> f = pick_fused<P>(ctx->tile2);
+    if (!f)
+        return set_err(ctx, PPF_EARG, "no two-step kernel for tile %dx%d", ctx->tile2.tx,
+                       ctx->tile2.ty);
+    hipLaunchKernelGGL(f, dim3((unsigned)((size_t)ntx * nty * nzc)),
+                       dim3(ctx->tile2.tx * ctx->tile2.ty), 0, ctx->stream,
+                       (const T *)ctx->buf[cs][0], (const T *)ctx->buf[cl][1],
+                       (const T *)ctx->buf[cc][2], (T *)ctx->buf[1 - cs][0],
+                       (T *)ctx->buf[1 - cl][1], (T *)ctx->buf[1 - cc][2], ctx->d.nx, ctx->d.ny,
+                       ctx->d.nz, ctx->tz2, ntx, nty, ctx->order2, k);
+    return PPF_OK;
+}
+
 // Time `reps` sweeps of the current state without advancing it: the kernel
 // reads copy cur and writes copy 1-cur, and cur is left as it was.
 static int time_tiling(ppf_ctx *ctx, int reps, double *ms)
@@ -250,6 +298,26 @@ static int launch_step(ppf_ctx *ctx, bool conly, bool report)
     }
     ctx->cur[2] ^= 1;
     ctx->steps++;
+    return PPF_OK;
+}
+
+// Two sweeps in one launch (fused_available), from copy cur to 1-cur.
+static int launch_step2(ppf_ctx *ctx)
+{
+    int rc;
+    if (ctx->d.precision == PPF_F32)
+        rc = launch_fused<PhysF32>(ctx, ctx->k32);
+    else if (ctx->eqa)
+        rc = launch_fused<PhysF64<true>>(ctx, ctx->k);
+    else
+        rc = launch_fused<PhysF64<false>>(ctx, ctx->k);
+    if (rc)
+        return rc;
+    HIPCHK(ctx, hipGetLastError());
+    ctx->cur[0] ^= 1;
+    ctx->cur[1] ^= 1;
+    ctx->cur[2] ^= 1;
+    ctx->steps += 2;
     return PPF_OK;
 }
 
@@ -318,6 +386,10 @@ static void choose_tiles(ppf_ctx *ctx)
         ctx->tile = {16, 16};
     ctx->order = f32 ? 5 : 0;
     ctx->tz = ctx->d.nz < 32 ? ctx->d.nz : 32;
+    ctx->fuse = 1;
+    ctx->tile2 = nx >= 64 ? TileCfg{64, 8} : TileCfg{32, 8};
+    ctx->tz2 = ctx->d.nz < 32 ? ctx->d.nz : 32;
+    ctx->order2 = 0;
 }
 
 extern "C" int ppf_create(ppf_ctx **out, const ppf_desc *desc)
@@ -558,12 +630,30 @@ static int do_steps(ppf_ctx *ctx, int64_t nsteps, bool conly)
     if (nsteps < 0)
         return set_err(ctx, PPF_EARG, "negative step count");
     HIPCHK(ctx, hipSetDevice(ctx->d.device));
-    for (int64_t i = 0; i < nsteps; i++) {
+    const bool fused = !conly && fused_available(ctx);
+    for (int64_t i = 0; i < nsteps;) {
+        // a pair of steps in one launch, unless a FP32 mass shift falls
+        // after the first of them (the shift must see the same state and
+        // step count as with single steps)
+        const bool shift_between = ctx->d.precision == PPF_F32 && ctx->shift_every > 0 &&
+                                   ctx->since_shift + 1 >= ctx->shift_every;
+        if (fused && nsteps - i >= 2 && !shift_between) {
+            int rc = launch_step2(ctx);
+            if (rc == PPF_OK)
+                rc = after_step(ctx);
+            if (rc == PPF_OK)
+                rc = after_step(ctx);
+            if (rc)
+                return rc;
+            i += 2;
+            continue;
+        }
         int rc = launch_step(ctx, conly, false);
         if (rc == PPF_OK)
             rc = after_step(ctx);
         if (rc)
             return rc;
+        i++;
     }
     return PPF_OK;
 }
@@ -791,6 +881,52 @@ extern "C" int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz, int *orde
     *tx = ctx->tile.tx;
     *ty = ctx->tile.ty;
     *tz = ctx->tz;
+    return PPF_OK;
+}
+
+extern "C" int ppf_set_fusion(ppf_ctx *ctx, int steps_per_launch, int tx, int ty, int tz,
+                              int order)
+{
+    if (!ctx)
+        return PPF_EARG;
+    if (steps_per_launch != 1 && steps_per_launch != 2)
+        return set_err(ctx, PPF_EARG, "steps per launch must be 1 or 2");
+    if (steps_per_launch == 2) {
+        if (ctx->kernel != PPF_KERNEL_TILED)
+            return set_err(ctx, PPF_ESTATE, "two steps per launch need the tiled kernels");
+        if (ctx->d.bc != PPF_BC_PERIODIC)
+            return set_err(ctx, PPF_ESTATE, "two steps per launch need periodic boundaries");
+    }
+    if (tx != 0 || ty != 0) {
+        bool known = false;
+        for (const TileCfg &c : k_tiles2)
+            known = known || (c.tx == tx && c.ty == ty);
+        if (!known)
+            return set_err(ctx, PPF_EARG, "two-step tile %dx%d is not compiled", tx, ty);
+        ctx->tile2 = {tx, ty};
+    }
+    if (tz < 0)
+        return set_err(ctx, PPF_EARG, "tz must be >= 0");
+    if (tz > 0)
+        ctx->tz2 = tz > ctx->d.nz ? ctx->d.nz : tz;
+    if (order > 3)
+        return set_err(ctx, PPF_EARG, "order must be 0..3 (or negative to keep)");
+    if (order >= 0)
+        ctx->order2 = order;
+    ctx->fuse = steps_per_launch;
+    return PPF_OK;
+}
+
+extern "C" int ppf_get_fusion(ppf_ctx *ctx, int *steps_per_launch, int *tx, int *ty, int *tz,
+                              int *order)
+{
+    if (!ctx || !steps_per_launch || !tx || !ty || !tz || !order)
+        return PPF_EARG;
+    *steps_per_launch = fused_available(ctx) ? 2 : 1;
+    *tx = ctx->tile2.tx;
+    *ty = ctx->tile2.ty;
+    *tz = ctx->tz2;
+    *order = ctx->order2;
     return PPF_OK;
 }
 

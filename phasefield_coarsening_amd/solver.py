@@ -87,6 +87,8 @@ SIGNATURES = {
                                             ctypes.c_double, _dp, _dp]),
     "ppf_set_tiling": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 4),
     "ppf_get_tiling": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 4),
+    "ppf_set_fusion": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5),
+    "ppf_get_fusion": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 5),
     "ppf_particles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
                                      ctypes.POINTER(ParticleStats),
                                      ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]),
@@ -268,6 +270,19 @@ class Solver:
         """(tx, ty, tz, order)"""
         v = [ctypes.c_int() for _ in range(4)]
         self._check(lib().ppf_get_tiling(self._h, *(ctypes.byref(a) for a in v)))
+        return tuple(a.value for a in v)
+
+    def set_fusion(self, steps_per_launch, tx=0, ty=0, tz=0, order=-1):
+        """Two steps per launch (periodic grids; bitwise equal to single
+        steps): steps_per_launch 1 or 2, with that kernel's tiling ((0, 0, 0,
+        -1) keeps the current one)."""
+        self._check(lib().ppf_set_fusion(self._h, int(steps_per_launch), int(tx), int(ty),
+                                         int(tz), int(order)))
+
+    def get_fusion(self):
+        """(steps per launch in effect, tx, ty, tz, order)"""
+        v = [ctypes.c_int() for _ in range(5)]
+        self._check(lib().ppf_get_fusion(self._h, *(ctypes.byref(a) for a in v)))
         return tuple(a.value for a in v)
 
     def particles(self, threshold=0.5, max_sizes=1 << 20):
