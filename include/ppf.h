@@ -180,7 +180,9 @@ int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz, int *order);
  * instead of 48).  Every site is updated with the same operands in the same
  * order as by a single step, so results are bit for bit those of single
  * steps; ppf_step runs pairs and a single step for an odd remainder, and
- * never fuses across an FP32 mass shift.  steps_per_launch is 1 (off) or 2;
+ * never fuses across an FP32 mass shift.  On by default for periodic FP32
+ * contexts (faster there); off for FP64 (no faster: the FP64 kernel holds
+ * ~185 VGPRs, two waves per SIMD).  steps_per_launch is 1 (off) or 2;
  * (tx, ty) in {(64,8), (32,8), (32,16), (64,16)} or (0,0) to keep; tz > 0
  * or 0 to keep; order bits 0-1 as ppf_set_tiling, negative to keep.
  * ppf_get_fusion reports 2 only while the context can fuse.  No counterpart
@@ -190,9 +192,12 @@ int ppf_get_fusion(ppf_ctx *ctx, int *steps_per_launch, int *tx, int *ty, int *t
 
 /* Time every compiled tiling (tile shape x tz in {16,32,64} x order in
  * {0,1,2,4,5,6}) on the current state, `reps` sweeps each, and keep the
- * fastest (ppf_get_tiling reads it back; *best_ms gets its time per
- * sweep).  The state is not advanced: the trial sweeps write the spare
- * copy only.  Synchronises. */
+ * fastest (ppf_get_tiling reads it back).  On periodic grids also time the
+ * two-step kernel's tilings (ppf_set_fusion shapes x tz in {32,64} x order
+ * in {0,2}) and switch two steps per launch on or off, whichever is faster
+ * per step (ppf_get_fusion reads it back).  *best_ms gets the best time per
+ * step.  The state is not advanced: the trial sweeps write the spare copy
+ * only.  Synchronises. */
 int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms);
 
 /* FP32 solute bookkeeping.  An FP32 context keeps an FP64 target for the
@@ -225,7 +230,8 @@ int ppf_sync(ppf_ctx *ctx);
 /* Steps taken since the last upload. */
 int ppf_steps_taken(ppf_ctx *ctx, int64_t *out);
 
-/* Name of the kernel ppf_step launches (static string). */
+/* Name of the kernel ppf_step launches for multi-step runs (static string):
+ * k_tiled2<...> while two steps run per launch. */
 const char *ppf_kernel_name(ppf_ctx *ctx);
 
 const char *ppf_last_error(ppf_ctx *ctx);

@@ -559,8 +559,10 @@ __device__ __forceinline__ void tile_march2(
 #undef PLANE
 }
 
-template <class P, int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void k_tiled2(
+// WPE: the waves per SIMD the register allocation aims for (FP64 needs a
+// cap: left alone it takes ~185 VGPRs, two waves per SIMD).
+template <class P, int TX, int TY, int WPE>
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_tiled2(
     const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
     const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
     typename P::T *__restrict__ l_out, typename P::T *__restrict__ c_out, int nx, int ny, int nz,
