@@ -124,13 +124,18 @@ class Controller:
     - ``"particles"``: the mean particle radius from connected components
       (``ppf_particles``) and the solid fraction, two labelling passes per
       jump.
-    After a rejection the next ``wait`` cycles run exactly without a trial
-    (``wait`` = 1, doubling per consecutive rejection up to ``max_wait``), so
-    a fast transient does not cost a network pass per cycle.
+    Two gates keep network passes away from fast transients, where jumps
+    fail the judge:
+    - rate: the radius and fraction are measured at every cycle start (one
+      reduction); a jump is tried only when both moved by at most
+      ``max_cycle_change`` (relative) over the previous cycle, so the first
+      cycle always runs exactly;
+    - back-off: after a rejection the next ``wait`` cycles run exactly
+      (``wait`` = 1, doubling per consecutive rejection up to ``max_wait``).
     """
 
     def __init__(self, solver, net, k=200, relax=20, tol_radius=0.02, tol_fraction=0.02,
-                 threshold=0.5, judge="area", max_wait=8, timing=False):
+                 threshold=0.5, judge="area", max_wait=8, max_cycle_change=0.1, timing=False):
         if judge not in ("area", "particles"):
             raise ValueError("judge must be 'area' or 'particles'")
         self.sv = solver
@@ -142,6 +147,7 @@ class Controller:
         self.thr = threshold
         self.judge = judge
         self.max_wait = max_wait
+        self.max_change = max_cycle_change
         self.timing = timing
         self.ckpt = S.Solver(solver.shape, bc=solver.bc, precision=solver.precision)
         self.log = []
@@ -177,13 +183,19 @@ class Controller:
         done = acc = rej = skipped = exact = 0
         skip, wait = 0, 1
         cyc = self.k + self.relax
+        prev = None  # (radius, fraction) at the previous cycle start
         while nsteps - done >= cyc:
-            if skip > 0:
-                skip -= 1
+            r0, f0, _ = self._timed("judge", self._measure)
+            fast = prev is None or r0 <= 0 or prev[0] <= 0 or \
+                abs(r0 - prev[0]) > self.max_change * prev[0] or \
+                abs(f0 - prev[1]) > self.max_change * max(prev[1], 1e-12)
+            prev = (r0, f0)
+            if skip > 0 or fast:
+                skip = max(skip - 1, 0)
                 skipped += 1
                 self._timed("exact", self.sv.step, cyc)
                 exact += cyc
-                self.log.append({"step": done, "trial": False})
+                self.log.append({"step": done, "trial": False, "R": r0, "frac": f0})
                 done += cyc
                 continue
             self._timed("checkpoint", self.ckpt.copy_from, self.sv)
