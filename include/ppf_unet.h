@@ -71,10 +71,12 @@ int ppf_unet_apply(ppf_unet *net, ppf_ctx *ctx);
 int ppf_unet_last_forward(ppf_unet *net, double *ms, double *flop);
 
 /* Convolution kernel:
- *   2 (default) row-blocked: a wave owns 16 x 4 or 16 x 8 sites and reuses
- *     each loaded input row for the three dy taps; no LDS;
- *   3, 4 row-blocked with fixed (rows, N tiles) of (8, 2) and (8, 4 where
- *     Cout allows), for tuning;
+ *   2 (default) row-blocked: a wave owns 16 x-sites of 8 rows and 32 or 64
+ *     output channels and reuses each loaded input row for the three dy
+ *     taps; no LDS; the first layer runs as one GEMM over its 81 real (tap,
+ *     channel) pairs instead of 27 taps x 32 padded channels;
+ *   3, 4 the same with other tiles: 8 rows x 32 channels everywhere (3), or
+ *     4 rows x 64 channels where Cout allows (4), for tuning;
  *   1 LDS-tiled: a 16 x 8 brick's halo staged in LDS;
  *   0 direct: every operand straight from global memory.
  * The kinds add the (tap, 32-channel) products into the FP32 accumulators

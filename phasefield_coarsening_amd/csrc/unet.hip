@@ -427,6 +427,117 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 }
 
 // ----------------------------------------------------------------------------
+// First layer (3 input channels) without padding each tap to 32 channels.
+// K runs over the 9 (dz, dy) groups x 16 slots; slots 0..8 are (dx, c) of
+// the 3 taps dx and 3 channels (contiguous in an x row of the staged input),
+// slots 9..15 zero: K = 144 in 5 chunks of 32 (the last half empty), 5 MFMA
+// chunks per site row instead of 27.  A workgroup owns 64 x 8 sites of one
+// z plane (one 16-site column of 8 rows per wave); it stages the 3 x 10 x 66
+// input halo in LDS as BF16 (the same rounding as the padded input), and
+// each lane gathers its 8 K values per chunk from there.  Weights (5 chunks x
+// Cout x 32, prepared by k_prep_w0) stay in registers.  Output as k_conv
+// MODE 0: bf16(leaky(acc + bias)).
+// ----------------------------------------------------------------------------
+constexpr int C0_TX = 64, C0_R = 8, C0_HX = C0_TX + 2, C0_HY = C0_R + 2, C0_CH = 5;
+
+template <int NJ>
+__global__ __launch_bounds__(256) void k_conv0(const float *__restrict__ x,
+                                               const bf16 *__restrict__ W0,
+                                               const float *__restrict__ bias, int Cout,
+                                               bf16 *__restrict__ Y, int ldy, int B, int D, int H,
+                                               int Wd)
+{
+    __shared__ bf16 sx[3 * C0_HY * C0_HX * 3];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nbx = (Wd + C0_TX - 1) / C0_TX, nby = (H + C0_R - 1) / C0_R;
+    int r = blockIdx.x;
+    const int bx = r % nbx;
+    r /= nbx;
+    const int by = r % nby;
+    r /= nby;
+    const int z = r % D, b = r / D;
+    const int x0 = bx * C0_TX, y0 = by * C0_R;
+    for (int i = tid; i < 3 * C0_HY * C0_HX; i += 256) {
+        const int hx = i % C0_HX, u = i / C0_HX, hy = u % C0_HY, hz = u / C0_HY;
+        const size_t v = (((size_t)b * D + wrapm(z + hz - 1, D)) * H + wrapm(y0 + hy - 1, H)) * Wd +
+                         wrapm(x0 + hx - 1, Wd);
+        sx[i * 3 + 0] = (bf16)x[v * 3 + 0];
+        sx[i * 3 + 1] = (bf16)x[v * 3 + 1];
+        sx[i * 3 + 2] = (bf16)x[v * 3 + 2];
+    }
+    const int m = lane & 15, q = lane >> 4;
+    bf16x8 w[C0_CH][NJ];
+    for (int c = 0; c < C0_CH; c++)
+        for (int j = 0; j < NJ; j++)
+            w[c][j] = *(const bf16x8 *)(W0 + ((size_t)c * Cout + 16 * j + m) * 32 + 8 * q);
+    __syncthreads();
+    const int sxl = wave * 16 + m;  // staged x of the site's dx = -1 neighbour
+    f32x4 acc[C0_R][NJ];
+    for (int row = 0; row < C0_R; row++)
+        for (int j = 0; j < NJ; j++)
+            acc[row][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int row = 0; row < C0_R; row++) {
+#pragma unroll
+        for (int c = 0; c < C0_CH; c++) {
+            const int g = 2 * c + (q >> 1);  // (dz, dy) group of this lane's 8 K values
+            bf16x8 a;
+            for (int e = 0; e < 8; e++)
+                a[e] = (bf16)0.0f;
+            if (g < 9) {
+                const bf16 *pp = sx + (((g / 3) * C0_HY + row + g % 3) * C0_HX + sxl) * 3;
+                if ((q & 1) == 0) {
+                    for (int e = 0; e < 8; e++)
+                        a[e] = pp[e];
+                } else {
+                    a[0] = pp[8];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NJ; j++)
+                acc[row][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w[c][j], acc[row][j], 0, 0, 0);
+        }
+    }
+    for (int row = 0; row < C0_R; row++) {
+        const int y = y0 + row;
+        if (y >= H)
+            break;
+        for (int j = 0; j < NJ; j++) {
+            const int ch = 16 * j + m;
+            if (ch >= Cout)
+                continue;
+            for (int rr = 0; rr < 4; rr++) {
+                const int xx = x0 + wave * 16 + 4 * q + rr;
+                if (xx >= Wd)
+                    continue;
+                const size_t vox = (((size_t)b * D + z) * H + y) * Wd + xx;
+                float v = acc[row][j][rr] + bias[ch];
+                v = v > 0.f ? v : LEAK * v;
+                Y[vox * ldy + ch] = (bf16)v;
+            }
+        }
+    }
+}
+
+// k_conv0's weights: W0[c][co][kk] = w[tap][co][ci] for K index 32c + kk =
+// 16 g + s with g = (dz+1)*3 + (dy+1) < 9 and s = (dx+1)*3 + ci < 9, else 0;
+// w is the first layer's flat [27][Cout][3].
+__global__ __launch_bounds__(256) void k_prep_w0(const float *__restrict__ w, int Cout,
+                                                 bf16 *__restrict__ W0)
+{
+    const int n = C0_CH * Cout * 32;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const int kk = i % 32, co = (i / 32) % Cout, c = i / (32 * Cout);
+        const int k = 32 * c + kk, g = k / 16, s = k % 16;
+        float v = 0.f;
+        if (g < 9 && s < 9) {
+            const int tap = g * 3 + s / 3, ci = s % 3;
+            v = w[((size_t)tap * Cout + co) * 3 + ci];
+        }
+        W0[i] = (bf16)v;
+    }
+}
+
+// ----------------------------------------------------------------------------
 // Weight gradient partials: part[split][tap][co][ci] = sum over the split's
 // sites of G[v][co] * X[v+s_tap][ci].  A = G^T (16 co x 32 sites), B = the
 // shifted input (32 sites x 16 ci), gathered per lane.  Workgroup: one tap,
@@ -832,6 +943,7 @@ struct ppf_unet {
     size_t nwb;
     float *p, *g, *m, *v;   // FP32 parameters, gradient, Adam moments
     bf16 *wb, *wt;          // BF16 weights and their flipped transposes
+    bf16 *w0;               // first layer's weights for k_conv0, [5][Cout][32]
     int64_t t;              // Adam steps
     // workspace for one (batch, grid) shape
     int wsB, wsD, wsH, wsW;
@@ -963,6 +1075,8 @@ static int prep_weights(ppf_unet *n)
                            n->stream, n->p + l.w_off, l.cout, l.cin_log, l.cin, n->wb + l.wb_off,
                            n->wt + l.wb_off);
     }
+    hipLaunchKernelGGL(k_prep_w0, dim3(grid_for((size_t)C0_CH * n->L[0].cout * 32)), dim3(256), 0,
+                       n->stream, n->p + n->L[0].w_off, n->L[0].cout, n->w0);
     UHIP(n, hipGetLastError());
     return PPF_OK;
 }
@@ -975,12 +1089,13 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
 {
     const size_t total = (size_t)B * D * H * W;
     if (n->conv_kind >= 2 && coutw % 32 == 0) {
-        // (R, NJ): 8 rows x 2 N tiles, or 4 rows x 4 N tiles when the
-        // channel count allows (kind 2) / 8 x 4 (kind 4)
-        // NJ*16 must divide coutw (the kernel has no partial N tile); every
-        // layer's padded channel count is a multiple of 32
+        // (R, NJ): 8 rows x 2 N tiles; where the channel count is a multiple
+        // of 64, 8 x 4 (kind 2, fastest measured: 18.0 ms at 256^3) or 4 x 4
+        // (kind 4); kind 3 keeps 8 x 2 everywhere.  NJ*16 must divide coutw
+        // (the kernel has no partial N tile); every layer's padded channel
+        // count is a multiple of 32.
         const bool wide = coutw % 64 == 0 && n->conv_kind != 3;
-        const int R = wide && n->conv_kind == 2 ? 4 : 8, NJ = wide ? 4 : 2;
+        const int R = wide && n->conv_kind == 4 ? 4 : 8, NJ = wide ? 4 : 2;
         const int nnb = (coutw + NJ * 16 - 1) / (NJ * 16), nxt = (W + 15) / 16,
                   nyt = (H + R - 1) / R;
         const size_t waves = (size_t)B * D * nyt * nxt * nnb;
@@ -1040,7 +1155,9 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
                            Wt, coutw, bias, cout, Yb, Yf, ldy, yoff, B, D, H, W);
 }
 
-static int forward(ppf_unet *n, int B, int D, int H, int W)
+// need_x0: also build the padded BF16 input X0 (the first layer's weight
+// gradient reads it); inference with the row-blocked kernels does not.
+static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0)
 {
     const int C = n->base;
     const size_t V1 = (size_t)B * D * H * W;
@@ -1048,11 +1165,25 @@ static int forward(ppf_unet *n, int B, int D, int H, int W)
     const Layer *L = n->L;
     auto wb = [&](int i) { return n->wb + L[i].wb_off; };
     auto bias = [&](int i) { return n->p + L[i].b_off; };
-    hipLaunchKernelGGL(k_pack_in, dim3(blocks_for(V1 * 4)), dim3(256), 0, n->stream, n->xin, V1,
-                       n->X0);
     // level 1
-    conv(n, 0, n->X0, C, 0, L[0].cin, wb(0), L[0].cout, bias(0), L[0].cout, n->E1a, nullptr, C,
-         0, B, D, H, W);
+    if (n->conv_kind >= 2) {
+        const unsigned nb0 = (unsigned)((size_t)B * D * ((H + C0_R - 1) / C0_R) *
+                                        ((W + C0_TX - 1) / C0_TX));
+        if (C == 32)
+            hipLaunchKernelGGL(k_conv0<2>, dim3(nb0), dim3(256), 0, n->stream, n->xin, n->w0,
+                               bias(0), C, n->E1a, C, B, D, H, W);
+        else
+            hipLaunchKernelGGL(k_conv0<4>, dim3(nb0), dim3(256), 0, n->stream, n->xin, n->w0,
+                               bias(0), C, n->E1a, C, B, D, H, W);
+        if (need_x0)
+            hipLaunchKernelGGL(k_pack_in, dim3(blocks_for(V1 * 4)), dim3(256), 0, n->stream,
+                               n->xin, V1, n->X0);
+    } else {
+        hipLaunchKernelGGL(k_pack_in, dim3(blocks_for(V1 * 4)), dim3(256), 0, n->stream, n->xin,
+                           V1, n->X0);
+        conv(n, 0, n->X0, C, 0, L[0].cin, wb(0), L[0].cout, bias(0), L[0].cout, n->E1a, nullptr,
+             C, 0, B, D, H, W);
+    }
     conv(n, 0, n->E1a, C, 0, C, wb(1), C, bias(1), C, n->U1, nullptr, 3 * C, 2 * C, B, D, H, W);
     hipLaunchKernelGGL(k_pool_fwd, dim3(row_blocks(B * D2 * H2, W2 * C / 8)), dim3(256), 0,
                        n->stream, n->U1, 3 * C, 2 * C, C, n->P1, B, D, H, W);
@@ -1241,7 +1372,7 @@ extern "C" int ppf_unet_destroy(ppf_unet *n)
     if (n->stream)
         hipStreamSynchronize(n->stream);
     free_ws(n);
-    void *bufs[] = {n->p, n->g, n->m, n->v, n->wb, n->wt, n->part};
+    void *bufs[] = {n->p, n->g, n->m, n->v, n->wb, n->wt, n->w0, n->part};
     for (void *b : bufs)
         if (b)
             hipFree(b);
@@ -1334,6 +1465,8 @@ extern "C" int ppf_unet_create(ppf_unet **out, int base, int device, uint64_t se
     if (e == hipSuccess)
         e = hipMalloc(&n->wt, woff * sizeof(bf16));
     if (e == hipSuccess)
+        e = hipMalloc(&n->w0, (size_t)C0_CH * C * 32 * sizeof(bf16));
+    if (e == hipSuccess)
         e = hipMemcpy(n->p, h.data(), off * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemset(n->m, 0, off * sizeof(float));
@@ -1396,6 +1529,8 @@ static double forward_flop(const ppf_unet *n, size_t V1)
     const size_t vox[NCONV] = {V1, V1, V1 / 8, V1 / 8, V1 / 64, V1 / 64, V1 / 8, V1 / 8, V1, V1};
     for (int i = 0; i < NCONV; i++)
         f += 2.0 * 27.0 * n->L[i].cin * n->L[i].cout * (double)vox[i];
+    if (n->conv_kind >= 2)  // k_conv0: K = 160 instead of 27 x 32
+        f -= 2.0 * (27.0 * n->L[0].cin - 32.0 * C0_CH) * n->L[0].cout * (double)vox[0];
     return f;
 }
 
@@ -1410,7 +1545,7 @@ extern "C" int ppf_unet_forward(ppf_unet *n, int B, int D, int H, int W, const f
     const size_t V1 = (size_t)B * D * H * W;
     UHIP(n, hipMemcpyAsync(n->xin, x, V1 * 3 * sizeof(float), hipMemcpyHostToDevice, n->stream));
     UHIP(n, hipEventRecord(n->ev0, n->stream));
-    if ((rc = forward(n, B, D, H, W)))
+    if ((rc = forward(n, B, D, H, W, false)))
         return rc;
     UHIP(n, hipEventRecord(n->ev1, n->stream));
     UHIP(n, hipMemcpyAsync(y, n->yout, V1 * 3 * sizeof(float), hipMemcpyDeviceToHost, n->stream));
@@ -1431,7 +1566,7 @@ static int loss_grad(ppf_unet *n, int B, int D, int H, int W, const float *x, co
     const size_t V1 = (size_t)B * D * H * W;
     UHIP(n, hipMemcpyAsync(n->xin, x, V1 * 3 * sizeof(float), hipMemcpyHostToDevice, n->stream));
     UHIP(n, hipMemcpyAsync(n->ytgt, y, V1 * 3 * sizeof(float), hipMemcpyHostToDevice, n->stream));
-    if ((rc = forward(n, B, D, H, W)))
+    if ((rc = forward(n, B, D, H, W, true)))
         return rc;
     return backward(n, B, D, H, W, loss);
 }
@@ -1509,7 +1644,7 @@ extern "C" int ppf_unet_apply(ppf_unet *n, ppf_ctx *ctx)
         hipLaunchKernelGGL(k_ctx_to_x<float>, dim3(grid_for(N)), dim3(256), 0, n->stream,
                            (const float *)fs, (const float *)fl, (const float *)fc, N, n->xin);
     hipEventRecord(n->ev0, n->stream);
-    rc = forward(n, 1, D, H, W);
+    rc = forward(n, 1, D, H, W, false);
     hipEventRecord(n->ev1, n->stream);
     if (rc) {
         hipFree(tmp);
