@@ -4,6 +4,7 @@
  * Replaces the reference's serial reader/writer and output naming:
  *   ppfio_read        read_hdf5            phasefield.c:996-1049
  *   ppfio_write       write_hdf5           phasefield.c:1052-1109
+ *   ppfio_write_chunked  write_hdf5_parallel's chunked layout  phasefield.c:548-624
  *   ppfio_job_stem    argv[1] -> job name  phasefield.c:117-121
  *   ppfio_output_name write_result_file    phasefield.c:1193-1206 (itoa 285-299)
  * ppfio_dims has no counterpart: the reference fixes sizes at compile time
@@ -38,6 +39,14 @@ int ppfio_dims(const char *path, int dims[3]);
 int ppfio_read(const char *path, const int dims[3], double *s, double *l, double *c, int flags);
 int ppfio_write(const char *path, const int dims[3], const double *s, const double *l,
                 const double *c);
+/* ppfio_write with datasets chunked by chunk[3] (z, y, x), the on-disk
+ * layout of the reference's parallel writer, write_hdf5_parallel, whose
+ * chunk is the local block of its MPI decomposition (phasefield.c:548-624).
+ * chunk NULL or all zero: contiguous, as ppfio_write. */
+int ppfio_write_chunked(const char *path, const int dims[3], const double *s, const double *l,
+                        const double *c, const int chunk[3]);
+/* Chunk dims of a dataset, all zero when it is stored contiguously. */
+int ppfio_layout(const char *path, const char *dataset, int chunk[3]);
 int ppfio_job_stem(const char *input, char *stem, int cap);
 int ppfio_output_name(const char *stem, int k, char *out, int cap);
 const char *ppfio_last_error(void);

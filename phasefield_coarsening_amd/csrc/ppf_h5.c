@@ -136,21 +136,43 @@ int ppfio_read(const char *path, const int dims[3], double *s, double *l, double
 int ppfio_write(const char *path, const int dims[3], const double *s, const double *l,
                 const double *c)
 {
+    return ppfio_write_chunked(path, dims, s, l, c, NULL);
+}
+
+/* As ppfio_write, with chunked datasets when chunk is given: the layout of
+ * the reference's parallel writer, which chunks each dataset by the local
+ * block of its MPI decomposition (phasefield.c:576-582).  The values and
+ * dataset shapes are those of the contiguous file; only the storage layout
+ * differs. */
+int ppfio_write_chunked(const char *path, const int dims[3], const double *s, const double *l,
+                        const double *c, const int chunk[3])
+{
     quiet();
     g_err[0] = 0;
     if (!path || !dims || !s || !l || !c)
         return fail(PPFIO_EARG, "null argument%s%s", NULL, NULL);
+    if (chunk && (chunk[0] == 0 && chunk[1] == 0 && chunk[2] == 0))
+        chunk = NULL;
+    if (chunk)
+        for (int i = 0; i < 3; i++)
+            if (chunk[i] < 1 || chunk[i] > dims[i])
+                return fail(PPFIO_EARG, "chunk dims must lie in 1..dims%s%s", NULL, NULL);
     hid_t f = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
     if (f < 0)
         return fail(PPFIO_EOPEN, "cannot create %s%s", path, NULL);
     hsize_t d[3] = {(hsize_t)dims[0], (hsize_t)dims[1], (hsize_t)dims[2]};
     hid_t sp = H5Screate_simple(3, d, NULL);
+    hid_t dcpl = H5P_DEFAULT;
+    if (chunk) {
+        hsize_t cd[3] = {(hsize_t)chunk[0], (hsize_t)chunk[1], (hsize_t)chunk[2]};
+        dcpl = H5Pcreate(H5P_DATASET_CREATE);
+        H5Pset_chunk(dcpl, 3, cd);
+    }
     const char *names[3] = {"etaS", "etaL", "c"};
     const double *bufs[3] = {s, l, c};
     int rc = 0;
     for (int a = 0; a < 3 && !rc; a++) {
-        hid_t ds = H5Dcreate2(f, names[a], H5T_IEEE_F64LE, sp, H5P_DEFAULT, H5P_DEFAULT,
-                              H5P_DEFAULT);
+        hid_t ds = H5Dcreate2(f, names[a], H5T_IEEE_F64LE, sp, H5P_DEFAULT, dcpl, H5P_DEFAULT);
         if (ds < 0) {
             rc = fail(PPFIO_EWRITE, "cannot create dataset %s in %s", names[a], path);
             break;
@@ -159,9 +181,42 @@ int ppfio_write(const char *path, const int dims[3], const double *s, const doub
             rc = fail(PPFIO_EWRITE, "writing dataset %s in %s failed", names[a], path);
         H5Dclose(ds);
     }
+    if (dcpl != H5P_DEFAULT)
+        H5Pclose(dcpl);
     H5Sclose(sp);
     if (H5Fclose(f) < 0 && !rc)
         rc = fail(PPFIO_EWRITE, "closing %s failed%s", path, NULL);
+    return rc;
+}
+
+/* Storage layout of a 3-D dataset: chunk dims, or zeros when contiguous. */
+int ppfio_layout(const char *path, const char *dataset, int chunk[3])
+{
+    quiet();
+    g_err[0] = 0;
+    if (!path || !dataset || !chunk)
+        return fail(PPFIO_EARG, "null argument%s%s", NULL, NULL);
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0)
+        return fail(PPFIO_EOPEN, "cannot open %s for reading%s", path, NULL);
+    int rc = 0;
+    hid_t ds = H5Dopen2(f, dataset, H5P_DEFAULT);
+    if (ds < 0) {
+        rc = fail(PPFIO_EMISSING, "cannot open dataset %s%s", dataset, NULL);
+    } else {
+        hid_t dcpl = H5Dget_create_plist(ds);
+        chunk[0] = chunk[1] = chunk[2] = 0;
+        if (H5Pget_layout(dcpl) == H5D_CHUNKED) {
+            hsize_t cd[3] = {0, 0, 0};
+            if (H5Pget_chunk(dcpl, 3, cd) != 3)
+                rc = fail(PPFIO_EDIMS, "dataset %s chunk is not 3-D%s", dataset, NULL);
+            for (int i = 0; i < 3; i++)
+                chunk[i] = (int)cd[i];
+        }
+        H5Pclose(dcpl);
+        H5Dclose(ds);
+    }
+    H5Fclose(f);
     return rc;
 }
 

@@ -26,6 +26,13 @@
  *   --device=K                [PPF_DEVICE]       default 0
  *   --stats                   [PPF_STATS=1]      one JSON line of throughput
  *                             and solute bookkeeping on stderr at the end
+ *   --chunk=Z,Y,X             [PPF_CHUNK]        write the output datasets
+ *                             chunked by Z x Y x X, the layout of the
+ *                             reference's parallel writer (phasefield.c:
+ *                             576-582); default contiguous, as its serial one
+ *   --blocks=Q                [PPF_BLOCKS]       chunk = the local block of a
+ *                             Q x Q x Q decomposition (Q divides each size):
+ *                             the file an `mpirun -np Q^3` run writes
  *
  * Differences from the reference, on purpose: grid sizes come from the
  * input file (the reference compiles them in, phasefield.c:47-56); a file
@@ -49,6 +56,8 @@ typedef struct {
     int etal_compat;
     int device;
     int stats;
+    int chunk[3];  /* output chunk dims (z, y, x); zeros: contiguous */
+    int blocks;    /* > 0: chunk = dims / blocks */
 } opts_t;
 
 static int parse_long(const char *s, long *out)
@@ -97,6 +106,23 @@ static int set_opt(opts_t *o, const char *key, const char *val)
         o->stats = strcmp(val, "0") != 0;
         return 0;
     }
+    if (!strcmp(key, "chunk")) {
+        int z, y, x;
+        char tail;
+        if (sscanf(val, "%d,%d,%d%c", &z, &y, &x, &tail) != 3 || z < 1 || y < 1 || x < 1)
+            return -1;
+        o->chunk[0] = z;
+        o->chunk[1] = y;
+        o->chunk[2] = x;
+        o->blocks = 0;
+        return 0;
+    }
+    if (!strcmp(key, "blocks")) {
+        if (parse_long(val, &v) || v < 1 || v > 4096)
+            return -1;
+        o->blocks = (int)v;
+        return 0;
+    }
     return -1;
 }
 
@@ -108,7 +134,9 @@ static void env_opts(opts_t *o)
                                    {"PPF_N_ITER_INIT", "n-iter-init"},
                                    {"PPF_ETAL_COMPAT", "etal-compat"},
                                    {"PPF_DEVICE", "device"},
-                                   {"PPF_STATS", "stats"}};
+                                   {"PPF_STATS", "stats"},
+                                   {"PPF_CHUNK", "chunk"},
+                                   {"PPF_BLOCKS", "blocks"}};
     for (size_t i = 0; i < sizeof map / sizeof map[0]; i++) {
         const char *v = getenv(map[i][0]);
         if (v && set_opt(o, map[i][1], v))
@@ -137,7 +165,7 @@ static int die(const char *what, const char *msg)
 
 int main(int argc, char **argv)
 {
-    opts_t o = {PPF_BC_NOFLUX, 5000001L, 50000L, 100000000L, 0, 0, 0};
+    opts_t o = {PPF_BC_NOFLUX, 5000001L, 50000L, 100000000L, 0, 0, 0, {0, 0, 0}, 0};
     env_opts(&o);
     const char *input = NULL;
     for (int a = 1; a < argc; a++) {
@@ -173,6 +201,16 @@ int main(int argc, char **argv)
     int dims[3];
     if (ppfio_dims(input, dims))
         return die(input, ppfio_last_error());
+    if (o.blocks > 0) {
+        for (int i = 0; i < 3; i++) {
+            if (dims[i] % o.blocks)
+                return die("--blocks", "the block count must divide every grid size");
+            o.chunk[i] = dims[i] / o.blocks;
+        }
+    }
+    for (int i = 0; i < 3; i++)
+        if (o.chunk[i] > dims[i])
+            return die("--chunk", "a chunk dimension exceeds the grid");
     const size_t n = (size_t)dims[0] * dims[1] * dims[2];
     double *s = malloc(n * sizeof(double)), *l = malloc(n * sizeof(double)),
            *c = malloc(n * sizeof(double));
@@ -251,7 +289,7 @@ int main(int argc, char **argv)
             fflush(stdout);
             if (ppf_download(ctx, s, l, c))
                 return die("download", ppf_last_error(ctx));
-            if (ppfio_write(fname, dims, s, l, c))
+            if (ppfio_write_chunked(fname, dims, s, l, c, o.chunk))
                 return die(fname, ppfio_last_error());
         }
         i_iter++;

@@ -29,6 +29,9 @@ def _lib_io():
         lib.ppfio_read.argtypes = [ctypes.c_char_p, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
                                    ctypes.c_int]
         lib.ppfio_write.argtypes = [ctypes.c_char_p, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
+        lib.ppfio_write_chunked.argtypes = [ctypes.c_char_p, _c_int_p, _c_dbl_p, _c_dbl_p,
+                                            _c_dbl_p, _c_int_p]
+        lib.ppfio_layout.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _c_int_p]
         lib.ppfio_job_stem.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         lib.ppfio_output_name.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
                                           ctypes.c_int]
@@ -58,13 +61,29 @@ def read(path, etal_compat=False):
     return s, l, c
 
 
-def write(path, s, l, c):
+def write(path, s, l, c, chunk=None):
+    """Write /etaS, /etaL, /c; ``chunk`` (z, y, x) gives the chunked layout of
+    the reference's parallel writer (phasefield.c:576-582), default
+    contiguous as its serial one."""
     lib = _lib_io()
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (s, l, c)]
     if not (arrs[0].shape == arrs[1].shape == arrs[2].shape) or arrs[0].ndim != 3:
         raise ValueError("etaS, etaL, c must be 3-D arrays of one shape")
     dims = (ctypes.c_int * 3)(*arrs[0].shape)
-    _check(lib, lib.ppfio_write(str(path).encode(), dims, *(_dptr(a) for a in arrs)))
+    if chunk is None:
+        _check(lib, lib.ppfio_write(str(path).encode(), dims, *(_dptr(a) for a in arrs)))
+    else:
+        cd = (ctypes.c_int * 3)(*(int(v) for v in chunk))
+        _check(lib, lib.ppfio_write_chunked(str(path).encode(), dims,
+                                            *(_dptr(a) for a in arrs), cd))
+
+
+def layout(path, dataset="etaS"):
+    """Chunk dims (z, y, x) of a dataset, or (0, 0, 0) when contiguous."""
+    lib = _lib_io()
+    cd = (ctypes.c_int * 3)()
+    _check(lib, lib.ppfio_layout(str(path).encode(), dataset.encode(), cd))
+    return tuple(int(v) for v in cd)
 
 
 def job_stem(input_name):

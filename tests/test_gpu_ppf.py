@@ -55,6 +55,31 @@ def test_ppf_output_is_a_restart_input(golden, tmp_path):
     np.testing.assert_array_equal(c, z["c1"])
 
 
+@pytest.mark.parametrize("opt,chunk", [("--blocks=2", (8, 8, 8)), ("--chunk=4,16,8", (4, 16, 8))])
+def test_ppf_chunked_output(golden, tmp_path, opt, chunk):
+    """--blocks=Q writes the file an `mpirun -np Q^3` run of the reference
+    writes (datasets chunked by the local block, phasefield.c:576-582);
+    --chunk picks any chunk.  Values equal the contiguous run's."""
+    _, arrays = golden
+    z = arrays["p16_s10"]
+    h5.write(tmp_path / "in.h5", z["s0"], z["l0"], z["c0"])
+    base = [PPF, "--bc=periodic", "--n-iter=10", "--n-save=9"]
+    subprocess.run(base + [opt, "in.h5"], cwd=tmp_path, check=True, capture_output=True,
+                   timeout=120)
+    assert h5.layout(tmp_path / "in_1.h5") == chunk
+    for a, b in zip(h5.read(tmp_path / "in_1.h5"), (z["s1"], z["l1"], z["c1"])):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_ppf_rejects_blocks_that_do_not_divide(golden, tmp_path):
+    _, arrays = golden
+    z = arrays["pnc_s10"]  # 20 x 16 x 12
+    h5.write(tmp_path / "in.h5", z["s0"], z["l0"], z["c0"])
+    r = subprocess.run([PPF, "--bc=periodic", "--n-iter=2", "--blocks=3", "in.h5"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "--blocks" in r.stderr
+
+
 def test_ppf_stops_on_threshold(tmp_path):
     """A uniform pure-liquid state has dc = 0, so err = 0 < 1e-12 at the
     first save and the loop ends there (phasefield.c:179, 247)."""

@@ -32,7 +32,7 @@ def test_libppfio_exports_every_declared_symbol():
     lib = ctypes.CDLL(os.path.join(_lib.PKG_DIR, "libppfio.so"))
     for n in names:
         assert hasattr(lib, n), n
-    assert len(names) == 6
+    assert len(names) == 8
 
 
 def test_default_params_match_oracle():
@@ -75,6 +75,27 @@ def test_h5_roundtrip_and_compat(tmp_path):
     np.testing.assert_array_equal(s, s3)
     np.testing.assert_array_equal(l3, np.ones_like(l))
     np.testing.assert_array_equal(c, c3)
+
+
+def test_h5_chunked_layout(tmp_path):
+    """The reference's parallel writer chunks each dataset by the local
+    block of its decomposition (phasefield.c:576-582): same values, chunked
+    storage; the default stays contiguous."""
+    rng = np.random.default_rng(1)
+    s, l, c = (rng.random((8, 6, 10)) for _ in range(3))
+    h5.write(tmp_path / "plain.h5", s, l, c)
+    assert h5.layout(tmp_path / "plain.h5") == (0, 0, 0)
+    h5.write(tmp_path / "chunked.h5", s, l, c, chunk=(4, 3, 5))
+    for name in ("etaS", "etaL", "c"):
+        assert h5.layout(tmp_path / "chunked.h5", name) == (4, 3, 5)
+    for a, b in zip(h5.read(tmp_path / "chunked.h5"), (s, l, c)):
+        np.testing.assert_array_equal(a, b)
+    # a chunk that does not divide the grid is legal HDF5 (edge chunks are partial)
+    h5.write(tmp_path / "ragged.h5", s, l, c, chunk=(3, 6, 7))
+    assert h5.layout(tmp_path / "ragged.h5") == (3, 6, 7)
+    np.testing.assert_array_equal(h5.read(tmp_path / "ragged.h5")[2], c)
+    with pytest.raises(h5.PpfIOError):
+        h5.write(tmp_path / "bad.h5", s, l, c, chunk=(9, 6, 10))
 
 
 def test_h5_errors(tmp_path):
