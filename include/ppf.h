@@ -261,8 +261,18 @@ int ppf_ens_step_c_only(ppf_ens *e, int64_t nsteps);
  * threads each), tz planes per chunk. */
 int ppf_ens_set_tiling(ppf_ens *e, int dom, int shape, int tz);
 int ppf_ens_get_tiling(ppf_ens *e, int dom, int *shape, int *tz);
+/* Two steps per launch for periodic members (bit for bit two single
+ * steps; pairs never straddle an FP32 mass shift): steps_per_launch 1 or 2.
+ * On by default for periodic FP32 ensembles.  The two-step kernel has its
+ * own member tiling: shape 0 = 32x8, 1 = 16x16, tz planes per chunk. */
+int ppf_ens_set_fusion(ppf_ens *e, int steps_per_launch);
+int ppf_ens_get_fusion(ppf_ens *e, int *steps_per_launch);
+int ppf_ens_set_tiling2(ppf_ens *e, int dom, int shape, int tz);
+int ppf_ens_get_tiling2(ppf_ens *e, int dom, int *shape, int *tz);
 /* Per grid size, time every shape x tz in {8,16,32,64} on those members
- * alone and keep the fastest.  The state is not advanced. */
+ * alone and keep the fastest; for periodic ensembles also every two-step
+ * shape x tz in {16,32,64}, then keep whichever mode (one or two steps per
+ * launch) is faster per step.  The state is not advanced. */
 int ppf_ens_autotune(ppf_ens *e, int reps);
 int ppf_ens_reduce(ppf_ens *e, int dom, ppf_stats *out);
 /* FP32 members keep an FP64 solute target each, as a ppf_ctx does. */

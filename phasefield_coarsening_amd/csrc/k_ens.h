@@ -127,3 +127,51 @@ static __global__ __launch_bounds__(256) void k_ens_shift(const EnsDom *__restri
     mass_shift_body(c, n, deltas[2 * (size_t)blockIdx.y], seed, (size_t)blockIdx.x * 256,
                     (size_t)gridDim.x * 256);
 }
+
+// ---------------------------------------------------------------------------
+// Two steps per launch (periodic members): the same persistent work list
+// walk with tile_march2 as the item body, so each member advances two steps
+// bit for bit as two single steps.  tile_march2 needs its two rings within
+// the workgroup (4 TX + 4 TY + 4 <= 256), so the shapes are 32x8 and 16x16.
+// ---------------------------------------------------------------------------
+enum { ENS2_32x8 = 0, ENS2_16x16 = 1, ENS2_NSHAPES = 2 };
+
+// LDS in values: A1 (4 x (TY+4) x (TX+4)) then A2 (4 x (TY+2) x (TX+2)), the
+// larger of the two shapes: 32x8 needs 1728 + 1360, 16x16 1600 + 1296.
+constexpr int ENS2_LDS = 4 * 12 * 36 + 4 * 10 * 34;
+
+template <class P>
+__global__ __launch_bounds__(256) void k_ensemble2(const EnsDom *__restrict__ doms,
+                                                   const EnsItem *__restrict__ items, int nitems,
+                                                   int *__restrict__ counter, int parity,
+                                                   typename P::K k)
+{
+    using T = typename P::T;
+    __shared__ T sh[ENS2_LDS];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0)
+            next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = next;
+        __syncthreads();  // every thread has read `next` (and left the last item)
+        if (it >= nitems)
+            return;
+        const EnsItem w = items[it];
+        const EnsDom d = doms[w.dom];
+        const T *si = (const T *)d.f[parity][0];
+        const T *li = (const T *)d.f[parity][1];
+        const T *ci = (const T *)d.f[parity][2];
+        T *so = (T *)d.f[1 - parity][0];
+        T *lo = (T *)d.f[1 - parity][1];
+        T *co = (T *)d.f[1 - parity][2];
+        if (w.shape == ENS2_32x8)
+            tile_march2<P, 32, 8>(si, li, ci, so, lo, co, d.nx, d.ny, d.nz, w.x0, w.y0, w.z0,
+                                  w.zend, k, reinterpret_cast<T(*)[12][36]>(sh),
+                                  reinterpret_cast<T(*)[10][34]>(sh + 4 * 12 * 36));
+        else
+            tile_march2<P, 16, 16>(si, li, ci, so, lo, co, d.nx, d.ny, d.nz, w.x0, w.y0, w.z0,
+                                   w.zend, k, reinterpret_cast<T(*)[20][20]>(sh),
+                                   reinterpret_cast<T(*)[18][18]>(sh + 4 * 20 * 20));
+    }
+}

@@ -33,6 +33,8 @@ struct Tiling {
 
 const int k_shape_tx[ENS_NSHAPES] = {64, 32, 16};
 const int k_shape_ty[ENS_NSHAPES] = {4, 8, 16};
+const int k_shape2_tx[ENS2_NSHAPES] = {32, 16};
+const int k_shape2_ty[ENS2_NSHAPES] = {8, 16};
 
 }  // namespace
 
@@ -52,6 +54,13 @@ struct ppf_ens {
     EnsItem *ditems;
     size_t items_cap;
     int nitems;
+    // two steps per launch (periodic): fuse = 2 turns it on; tiling2 holds
+    // the two-step shapes (ENS2_*), items2 their work list
+    int fuse;
+    std::vector<Tiling> tiling2;
+    EnsItem *ditems2;
+    size_t items2_cap;
+    int nitems2;
     int *counter;
     int grid;
     int parity;
@@ -143,6 +152,64 @@ static int upload_items(ppf_ens *e, const std::vector<EnsItem> &items)
     return PPF_OK;
 }
 
+// Two-step work list (tiling2 shapes), ordered as build_items.
+static std::vector<EnsItem> build_items2(const ppf_ens *e, const std::vector<int> &which)
+{
+    std::vector<int> order = which;
+    if (order.empty()) {
+        order.resize(e->ndom);
+        std::iota(order.begin(), order.end(), 0);
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return e->nsite[a] > e->nsite[b]; });
+    std::vector<EnsItem> items;
+    for (int d : order) {
+        const EnsDom &m = e->doms[d];
+        const Tiling t = e->tiling2[d];
+        const int tx = k_shape2_tx[t.shape], ty = k_shape2_ty[t.shape];
+        for (int y0 = 0; y0 < m.ny; y0 += ty)
+            for (int x0 = 0; x0 < m.nx; x0 += tx)
+                for (int z0 = 0; z0 < m.nz; z0 += t.tz)
+                    items.push_back({d, t.shape, x0, y0, z0, std::min(z0 + t.tz, m.nz)});
+    }
+    return items;
+}
+
+static int upload_items2(ppf_ens *e, const std::vector<EnsItem> &items)
+{
+    if (items.size() > e->items2_cap) {
+        if (e->ditems2)
+            EHIP(e, hipFree(e->ditems2));
+        e->ditems2 = nullptr;
+        e->items2_cap = 0;
+        EHIP(e, hipMalloc(&e->ditems2, items.size() * sizeof(EnsItem)));
+        e->items2_cap = items.size();
+    }
+    EHIP(e, hipMemcpyAsync(e->ditems2, items.data(), items.size() * sizeof(EnsItem),
+                           hipMemcpyHostToDevice, e->stream));
+    EHIP(e, hipStreamSynchronize(e->stream));
+    e->nitems2 = (int)items.size();
+    return PPF_OK;
+}
+
+// One persistent two-step launch over the two-step work list: copy
+// `parity` in, copy 1-parity out, two steps later.
+static int launch2(ppf_ens *e)
+{
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(256), 0, e->stream, e->counter, (size_t)1, 0);
+    if (e->prec == PPF_F32)
+        hipLaunchKernelGGL(k_ensemble2<PhysF32>, dim3(e->grid), dim3(256), 0, e->stream,
+                           e->ddoms, e->ditems2, e->nitems2, e->counter, e->parity, e->k32);
+    else if (e->eqa)
+        hipLaunchKernelGGL(k_ensemble2<PhysF64<true>>, dim3(e->grid), dim3(256), 0, e->stream,
+                           e->ddoms, e->ditems2, e->nitems2, e->counter, e->parity, e->k);
+    else
+        hipLaunchKernelGGL(k_ensemble2<PhysF64<false>>, dim3(e->grid), dim3(256), 0, e->stream,
+                           e->ddoms, e->ditems2, e->nitems2, e->counter, e->parity, e->k);
+    EHIP(e, hipGetLastError());
+    return PPF_OK;
+}
+
 // One persistent launch over the current work list: copy `parity` in,
 // copy 1-parity out.
 static int launch(ppf_ens *e, bool conly)
@@ -179,8 +246,8 @@ extern "C" int ppf_ens_destroy(ppf_ens *e)
     hipSetDevice(e->device);
     if (e->stream)
         hipStreamSynchronize(e->stream);
-    void *bufs[] = {e->pool, e->ddoms, e->ditems, e->counter, e->dtargets, e->dpartials,
-                    e->ddeltas, e->dres};
+    void *bufs[] = {e->pool, e->ddoms, e->ditems, e->ditems2, e->counter, e->dtargets,
+                    e->dpartials, e->ddeltas, e->dres};
     for (void *b : bufs)
         if (b)
             hipFree(b);
@@ -235,6 +302,10 @@ extern "C" int ppf_ens_create(ppf_ens **out, int ndom, const int *dims_zyx, int 
     e->doms.resize(ndom);
     e->nsite.resize(ndom);
     e->tiling.resize(ndom);
+    e->tiling2.resize(ndom);
+    // two steps per launch by default for periodic FP32 members (faster
+    // there, as for single grids); FP64 steps one step per launch
+    e->fuse = (precision == PPF_F32 && bc == PPF_BC_PERIODIC) ? 2 : 1;
     e->targets.assign(ndom, 0.0);
     // pool: every array 256-byte aligned
     std::vector<size_t> off(ndom);
@@ -250,6 +321,7 @@ extern "C" int ppf_ens_create(ppf_ens **out, int ndom, const int *dims_zyx, int 
         total += 6 * bytes;
         e->tiling[d] = {m.nx >= 64 ? ENS_64x4 : (m.nx >= 32 ? ENS_32x8 : ENS_16x16),
                         m.nz < 32 ? m.nz : 32};
+        e->tiling2[d] = {m.nx >= 32 ? ENS2_32x8 : ENS2_16x16, m.nz < 32 ? m.nz : 32};
     }
     hipError_t h = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (h == hipSuccess)
@@ -312,6 +384,8 @@ extern "C" int ppf_ens_create(ppf_ens **out, int ndom, const int *dims_zyx, int 
     }
     e->grid = prop.multiProcessorCount * (per_cu > 0 ? per_cu : 1);
     int rc = upload_items(e, build_items(e, {}));
+    if (!rc)
+        rc = upload_items2(e, build_items2(e, {}));
     if (rc) {
         memcpy(g_ens_err, e->err, sizeof g_ens_err);
         ppf_ens_destroy(e);
@@ -493,19 +567,56 @@ static int ens_steps(ppf_ens *e, int64_t nsteps, bool conly)
     if (!e || nsteps < 0)
         return e ? eset(e, PPF_EARG, "negative step count") : PPF_EARG;
     EHIP(e, hipSetDevice(e->device));
-    for (int64_t i = 0; i < nsteps; i++) {
+    const bool fused = !conly && e->fuse == 2 && e->bc == PPF_BC_PERIODIC;
+    const bool shifts = e->prec == PPF_F32 && e->shift_every > 0;
+    for (int64_t i = 0; i < nsteps;) {
+        // a pair in one launch unless a mass shift falls after its first step
+        if (fused && nsteps - i >= 2 && !(shifts && e->since + 1 >= e->shift_every)) {
+            int rc = launch2(e);
+            if (rc)
+                return rc;
+            e->parity ^= 1;
+            e->steps += 2;
+            i += 2;
+            if (shifts && (e->since += 2) >= e->shift_every) {
+                e->since = 0;
+                rc = mass_shift(e);
+                if (rc)
+                    return rc;
+            }
+            continue;
+        }
         int rc = launch(e, conly);
         if (rc)
             return rc;
         e->parity ^= 1;
         e->steps++;
-        if (e->prec == PPF_F32 && e->shift_every > 0 && ++e->since >= e->shift_every) {
+        i++;
+        if (shifts && ++e->since >= e->shift_every) {
             e->since = 0;
             rc = mass_shift(e);
             if (rc)
                 return rc;
         }
     }
+    return PPF_OK;
+}
+
+extern "C" int ppf_ens_set_fusion(ppf_ens *e, int steps_per_launch)
+{
+    if (!e || (steps_per_launch != 1 && steps_per_launch != 2))
+        return e ? eset(e, PPF_EARG, "steps per launch must be 1 or 2") : PPF_EARG;
+    if (steps_per_launch == 2 && e->bc != PPF_BC_PERIODIC)
+        return eset(e, PPF_ESTATE, "two steps per launch need periodic members");
+    e->fuse = steps_per_launch;
+    return PPF_OK;
+}
+
+extern "C" int ppf_ens_get_fusion(ppf_ens *e, int *steps_per_launch)
+{
+    if (!e || !steps_per_launch)
+        return e ? eset(e, PPF_EARG, "null argument") : PPF_EARG;
+    *steps_per_launch = (e->fuse == 2 && e->bc == PPF_BC_PERIODIC) ? 2 : 1;
     return PPF_OK;
 }
 
@@ -522,6 +633,23 @@ extern "C" int ppf_ens_set_tiling(ppf_ens *e, int dom, int shape, int tz)
         return e ? eset(e, PPF_EARG, "bad tiling") : PPF_EARG;
     e->tiling[dom] = {shape, std::min(tz, e->doms[dom].nz)};
     return upload_items(e, build_items(e, {}));
+}
+
+extern "C" int ppf_ens_set_tiling2(ppf_ens *e, int dom, int shape, int tz)
+{
+    if (!good_dom(e, dom) || shape < 0 || shape >= ENS2_NSHAPES || tz < 1)
+        return e ? eset(e, PPF_EARG, "bad two-step tiling") : PPF_EARG;
+    e->tiling2[dom] = {shape, std::min(tz, e->doms[dom].nz)};
+    return upload_items2(e, build_items2(e, {}));
+}
+
+extern "C" int ppf_ens_get_tiling2(ppf_ens *e, int dom, int *shape, int *tz)
+{
+    if (!good_dom(e, dom) || !shape || !tz)
+        return e ? eset(e, PPF_EARG, "bad arguments") : PPF_EARG;
+    *shape = e->tiling2[dom].shape;
+    *tz = e->tiling2[dom].tz;
+    return PPF_OK;
 }
 
 extern "C" int ppf_ens_get_tiling(ppf_ens *e, int dom, int *shape, int *tz)
@@ -577,7 +705,54 @@ extern "C" int ppf_ens_autotune(ppf_ens *e, int reps)
         for (int d : members)
             e->tiling[d] = bt;
     }
-    return upload_items(e, build_items(e, {}));
+    int rc = upload_items(e, build_items(e, {}));
+    if (rc || e->bc != PPF_BC_PERIODIC)
+        return rc;
+    // periodic: the two-step shapes per class too, then the faster mode
+    auto time_ms = [&](bool two, int n, float *ms) -> int {
+        int r = two ? launch2(e) : launch(e, false);  // warm
+        if (r)
+            return r;
+        EHIP(e, hipEventRecord(e->ev0, e->stream));
+        for (int i = 0; i < n && r == PPF_OK; i++)
+            r = two ? launch2(e) : launch(e, false);
+        if (r)
+            return r;
+        EHIP(e, hipEventRecord(e->ev1, e->stream));
+        EHIP(e, hipEventSynchronize(e->ev1));
+        EHIP(e, hipEventElapsedTime(ms, e->ev0, e->ev1));
+        return PPF_OK;
+    };
+    static const int tzs2[] = {16, 32, 64};
+    for (auto &kv : classes) {
+        const std::vector<int> &members = kv.second;
+        double best = 1e300;
+        Tiling bt = e->tiling2[members[0]];
+        for (int shape = 0; shape < ENS2_NSHAPES; shape++)
+            for (int tz : tzs2) {
+                const Tiling t = {shape, std::min(tz, std::get<0>(kv.first))};
+                for (int d : members)
+                    e->tiling2[d] = t;
+                if ((rc = upload_items2(e, build_items2(e, members))))
+                    return rc;
+                float ms = 0.0f;
+                if ((rc = time_ms(true, reps, &ms)))
+                    return rc;
+                if (ms < best) {
+                    best = ms;
+                    bt = t;
+                }
+            }
+        for (int d : members)
+            e->tiling2[d] = bt;
+    }
+    if ((rc = upload_items2(e, build_items2(e, {}))))
+        return rc;
+    float ms1 = 0.0f, ms2 = 0.0f;
+    if ((rc = time_ms(false, reps, &ms1)) || (rc = time_ms(true, reps, &ms2)))
+        return rc;
+    e->fuse = ms2 / 2.0f < ms1 ? 2 : 1;
+    return PPF_OK;
 }
 
 extern "C" int ppf_ens_reduce(ppf_ens *e, int dom, ppf_stats *out)

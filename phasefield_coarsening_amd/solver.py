@@ -118,6 +118,11 @@ SIGNATURES = {
     "ppf_ens_get_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] +
                            [ctypes.POINTER(ctypes.c_int)] * 2),
     "ppf_ens_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ppf_ens_set_fusion": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ppf_ens_get_fusion": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "ppf_ens_set_tiling2": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 3),
+    "ppf_ens_get_tiling2": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] +
+                            [ctypes.POINTER(ctypes.c_int)] * 2),
     "ppf_ens_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Stats)]),
     "ppf_ens_set_mass_correction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ppf_ens_mass_target": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
@@ -407,6 +412,25 @@ class Ensemble:
 
     def autotune(self, reps=3):
         self._check(lib().ppf_ens_autotune(self._h, int(reps)))
+
+    def set_fusion(self, steps_per_launch):
+        """Two steps per launch (periodic members): 1 or 2."""
+        self._check(lib().ppf_ens_set_fusion(self._h, int(steps_per_launch)))
+
+    def get_fusion(self):
+        v = ctypes.c_int()
+        self._check(lib().ppf_ens_get_fusion(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_tiling2(self, dom, shape, tz):
+        """Two-step member tiling: shape 0 = 32x8, 1 = 16x16."""
+        self._check(lib().ppf_ens_set_tiling2(self._h, int(dom), int(shape), int(tz)))
+
+    def get_tiling2(self, dom):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().ppf_ens_get_tiling2(self._h, int(dom), ctypes.byref(a),
+                                              ctypes.byref(b)))
+        return a.value, b.value
 
     def reduce(self, dom):
         st = Stats()

@@ -72,3 +72,35 @@ def test_ensemble_fp32_mass_targets():
             st = ens.reduce(i)
             assert abs(st["sum_c"] - targets[i]) / targets[i] < 1e-9
             assert st["n_nonfinite"] == 0
+
+
+@pytest.mark.parametrize("prec", [S.F64, S.F32])
+def test_ensemble_two_step_equals_single_steps(prec):
+    """Two steps per launch (periodic members, both two-step shapes, several
+    z chunks, FP32 mass shifts every 5 steps so pairs split around them):
+    every member bitwise equal to the one-step-per-launch ensemble after an
+    odd number of steps."""
+    print(f"prec={prec}", flush=True)
+    dims = DIMS + [(1, 1, 1), (3, 5, 2)]
+    states = [_member_state(d, 40 + i) for i, d in enumerate(dims)]
+
+    def run(fuse):
+        ens = S.Ensemble(dims, precision=prec)
+        try:
+            for i, st in enumerate(states):
+                ens.upload(i, *st)
+                if fuse:
+                    ens.set_tiling2(i, i % 2, (1, 3, 32)[i % 3])
+            if prec == S.F32:
+                ens.set_mass_correction(5)
+            ens.set_fusion(2 if fuse else 1)
+            assert ens.get_fusion() == (2 if fuse else 1)
+            ens.step(17)
+            return [ens.download(i) for i in range(len(dims))]
+        finally:
+            ens.close()
+
+    a, b = run(False), run(True)
+    for i in range(len(dims)):
+        for x, y in zip(a[i], b[i]):
+            np.testing.assert_array_equal(x, y)
