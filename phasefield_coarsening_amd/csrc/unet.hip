@@ -1181,7 +1181,8 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0)
     } else {
         hipLaunchKernelGGL(k_pack_in, dim3(blocks_for(V1 * 4)), dim3(256), 0, n->stream, n->xin,
                            V1, n->X0);
-        conv(n, 0, n->X0, C, 0, L[0].cin, wb(0), L[0].cout, bias(0), L[0].cout, n->E1a, nullptr,
+        // X0 is [V][32] whatever the base (k_pack_in)
+        conv(n, 0, n->X0, 32, 0, L[0].cin, wb(0), L[0].cout, bias(0), L[0].cout, n->E1a, nullptr,
              C, 0, B, D, H, W);
     }
     conv(n, 0, n->E1a, C, 0, C, wb(1), C, bias(1), C, n->U1, nullptr, 3 * C, 2 * C, B, D, H, W);
@@ -1347,7 +1348,7 @@ static int backward(ppf_unet *n, int B, int D, int H, int W, float *loss)
     conv(n, 1, n->Gb, C, 0, C, wt(1), C, nullptr, C, nullptr, n->gE1a, C, 0, B, D, H, W);
     // L0: X0 -> E1a (no input gradient needed)
     act_bwd(n, n->gE1a, C, 0, n->E1a, C, 0, C, V1);
-    if ((rc = layer_wgrad(n, 0, n->X0, C, 0, B, D, H, W)))
+    if ((rc = layer_wgrad(n, 0, n->X0, 32, 0, B, D, H, W)))  // X0 is [V][32]
         return rc;
     UHIP(n, hipGetLastError());
     UHIP(n, hipStreamSynchronize(n->stream));

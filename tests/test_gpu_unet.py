@@ -10,12 +10,12 @@ from tests import unet_ref
 pytestmark = pytest.mark.gpu
 
 
-def _params(seed, head=0.05):
+def _params(seed, head=0.05, base=32):
     """He-scaled random weights (the regime the network runs in), small
     random biases, a head of scale `head`."""
     rng = np.random.default_rng(seed)
     out = []
-    for name, s in layer_shapes(32):
+    for name, s in layer_shapes(base):
         if name.startswith("w"):
             out.append(rng.normal(0, np.sqrt(2.0 / (27 * s[2])), int(np.prod(s))))
         elif name == "head_w":
@@ -58,6 +58,43 @@ def test_forward_matches_reference(kind, shape):
     assert d.max() < 2e-2 * scale and d.mean() < 3e-3 * scale
     # and close to the unrounded float64 network
     assert np.abs(got - unet_ref.predict(p, x, bf16=False)).mean() < 2e-2 * scale
+
+
+@pytest.mark.parametrize("kind", [2, 0])
+def test_forward_base64_matches_reference(kind):
+    """The 64-channel network (first-layer kernel with four N tiles, 64-wide
+    row tiles at every level) against the reference."""
+    p = _params(6, base=64)
+    x, _ = _data(7, (1, 16, 12, 24))
+    with UNet(base=64, seed=3) as net:
+        net.set_conv_kernel(kind)
+        net.set_params(p)
+        got = net.forward(x)
+    want = unet_ref.predict(p, x, base=64, bf16=True)
+    d = np.abs(got - want)
+    scale = np.abs(want - x).max()
+    print("base 64: max |d| %.3e, mean |d| %.3e, head scale %.3e" % (d.max(), d.mean(), scale))
+    assert d.max() < 2e-2 * scale and d.mean() < 3e-3 * scale
+
+
+def test_gradient_base64_matches_reference():
+    """Every parameter gradient of the 64-channel network (default kernels)."""
+    p = _params(8, base=64)
+    x, y = _data(9, (1, 8, 8, 16))
+    with UNet(base=64, seed=3) as net:
+        net.set_params(p)
+        loss, g = net.loss_grad(x, y)
+    lr, gr = unet_ref.loss_and_grad(p, x, y, base=64, bf16=True)
+    assert abs(loss - lr) / lr < 1e-3
+    o = 0
+    for name, shp in layer_shapes(64):
+        n = int(np.prod(shp))
+        a, b = g[o:o + n].astype(np.float64), gr[o:o + n]
+        o += n
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
+        print(f"{name}: rel {rel:.2e} cos {cos:.6f}")
+        assert rel < 5e-2 and cos > 0.998, name
 
 
 @pytest.mark.parametrize("kind", KINDS)
