@@ -75,8 +75,12 @@ int ppf_unet_last_forward(ppf_unet *net, double *ms, double *flop);
  *     output channels and reuses each loaded input row for the three dy
  *     taps; no LDS; the first layer runs as one GEMM over its 81 real (tap,
  *     channel) pairs instead of 27 taps x 32 padded channels;
+ *     at inference the two decoder convolutions read their upsampled
+ *     channels straight from the coarse tensor (no upsampled copy);
  *   3, 4 the same with other tiles: 8 rows x 32 channels everywhere (3), or
  *     4 rows x 64 channels where Cout allows (4), for tuning;
+ *   5 kind 2 with the upsampled channels materialized, as in training (the
+ *     same sums as kind 2, for cross-checks);
  *   1 LDS-tiled: a 16 x 8 brick's halo staged in LDS;
  *   0 direct: every operand straight from global memory.
  * The kinds add the (tap, 32-channel) products into the FP32 accumulators

@@ -270,11 +270,16 @@ __global__ __launch_bounds__(256) void k_conv_lds(const bf16 *__restrict__ X, in
 // Waves are ordered N block, x tile, y tile, z, batch, so the four waves of
 // a workgroup share their input rows in L1.
 // ----------------------------------------------------------------------------
-template <int R, int NJ, int MODE, int WPE>
+// UP: the first `cup` input channels are the nearest x2 upsampling of Xc
+// ([B][D/2][H/2][W/2][ldc]), read at the coarse site (z/2, y/2, x/2) of each
+// (wrapped) fine neighbour instead of from a materialized copy in X; the
+// values, and so the sums, are the same as with the copy.
+template <int R, int NJ, int MODE, int WPE, bool UP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_conv_rows(
     const bf16 *__restrict__ X, int ldx, int xoff, int Cin, const bf16 *__restrict__ Wt, int CoutW,
     const float *__restrict__ bias, int Cout, bf16 *__restrict__ Yb, float *__restrict__ Yf,
-    int ldy, int yoff, int B, int D, int H, int Wd, int nnb, int nxt, int nyt)
+    int ldy, int yoff, int B, int D, int H, int Wd, int nnb, int nxt, int nyt,
+    const bf16 *__restrict__ Xc, int ldc, int cup)
 {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
@@ -316,6 +321,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 #pragma unroll
     for (int r = 0; r < R + 2; r++)
         ro[r] = wrapm(y0 + r - 1, H) * rowlen;
+    // the coarse source of the upsampled channels (UP)
+    int xoc[3] = {0, 0, 0}, roc[R + 2];
+    __amdgpu_buffer_rsrc_t plc[3];
+    if (UP) {
+        const int Hc = H / 2, rowlen_c = (Wd / 2) * ldc;
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++)
+            xoc[dx] = ((wrapm(xt * 16 + xl + dx - 1, Wd) >> 1) * ldc + 8 * q) * 2;
+#pragma unroll
+        for (int dz = 0; dz < 3; dz++)
+            plc[dz] = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(Xc + ((size_t)b * (D / 2) + (wrapm(z + dz - 1, D) >> 1)) * Hc *
+                                  (size_t)rowlen_c),
+                (short)0, Hc * rowlen_c * 2, 0x00020000);
+#pragma unroll
+        for (int r = 0; r < R + 2; r++)
+            roc[r] = (wrapm(y0 + r - 1, H) >> 1) * rowlen_c;
+    }
     // lane's weight row in N tile j: Wt[tap][n0+16j+xl][kc+8q]; CoutW is a
     // multiple of NJ*16, so every N tile exists
     const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
@@ -331,10 +354,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     // Fragments of one (32-channel chunk, dz, dx) group.
     auto ld = [&](bf16x8(&a)[R + 2], bf16x8(&w)[3][NJ], int kc, int grp) {
         const int dz = grp / 3, dx = grp % 3;
+        if (UP && kc < cup) {  // uniform; the same number of loads either way
 #pragma unroll
-        for (int r = 0; r < R + 2; r++)
-            a[r] = __builtin_bit_cast(
-                bf16x8, __builtin_amdgcn_raw_buffer_load_b128(pl[dz], xo[dx], (ro[r] + kc) * 2, 0));
+            for (int r = 0; r < R + 2; r++)
+                a[r] = __builtin_bit_cast(
+                    bf16x8,
+                    __builtin_amdgcn_raw_buffer_load_b128(plc[dz], xoc[dx], (roc[r] + kc) * 2, 0));
+        } else {
+#pragma unroll
+            for (int r = 0; r < R + 2; r++)
+                a[r] = __builtin_bit_cast(
+                    bf16x8,
+                    __builtin_amdgcn_raw_buffer_load_b128(pl[dz], xo[dx], (ro[r] + kc) * 2, 0));
+        }
 #pragma unroll
         for (int dy = 0; dy < 3; dy++)
 #pragma unroll
@@ -1085,25 +1117,40 @@ static int prep_weights(ppf_unet *n)
 // [B][D][H][W].
 static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int cin,
                  const bf16 *Wt, int coutw, const float *bias, int cout, bf16 *Yb, float *Yf,
-                 int ldy, int yoff, int B, int D, int H, int W)
+                 int ldy, int yoff, int B, int D, int H, int W, const bf16 *Xc = nullptr,
+                 int ldc = 0, int cup = 0)
 {
     const size_t total = (size_t)B * D * H * W;
     if (n->conv_kind >= 2 && coutw % 32 == 0) {
+        const int kind = n->conv_kind == 5 ? 2 : n->conv_kind;  // 5: kind 2's tiles
         // (R, NJ): 8 rows x 2 N tiles; where the channel count is a multiple
         // of 64, 8 x 4 (kind 2, fastest measured: 18.0 ms at 256^3) or 4 x 4
         // (kind 4); kind 3 keeps 8 x 2 everywhere.  NJ*16 must divide coutw
         // (the kernel has no partial N tile); every layer's padded channel
         // count is a multiple of 32.
-        const bool wide = coutw % 64 == 0 && n->conv_kind != 3;
-        const int R = wide && n->conv_kind == 4 ? 4 : 8, NJ = wide ? 4 : 2;
+        const bool wide = coutw % 64 == 0 && kind != 3;
+        const int R = wide && kind == 4 ? 4 : 8, NJ = wide ? 4 : 2;
         const int nnb = (coutw + NJ * 16 - 1) / (NJ * 16), nxt = (W + 15) / 16,
                   nyt = (H + R - 1) / R;
         const size_t waves = (size_t)B * D * nyt * nxt * nnb;
         const dim3 g3((unsigned)((waves + 3) / 4));
 #define ROWS(RR, JJ, MM)                                                                      \
-    hipLaunchKernelGGL((k_conv_rows<RR, JJ, MM, (RR * JJ > 16 ? 1 : 2)>), g3, dim3(256), 0,       \
+    hipLaunchKernelGGL((k_conv_rows<RR, JJ, MM, (RR * JJ > 16 ? 1 : 2), false>), g3, dim3(256),   \
+                       0, n->stream, X, ldx, xoff, cin, Wt, coutw, bias, cout, Yb, Yf, ldy, yoff, \
+                       B, D, H, W, nnb, nxt, nyt, (const bf16 *)nullptr, 0, 0)
+#define ROWS_UP(RR, JJ)                                                                        \
+    hipLaunchKernelGGL((k_conv_rows<RR, JJ, 0, (RR * JJ > 16 ? 1 : 2), true>), g3, dim3(256), 0,  \
                        n->stream, X, ldx, xoff, cin, Wt, coutw, bias, cout, Yb, Yf, ldy, yoff, B, \
-                       D, H, W, nnb, nxt, nyt)
+                       D, H, W, nnb, nxt, nyt, Xc, ldc, cup)
+        if (Xc) {  // forward only
+            if (R == 8 && NJ == 2)
+                ROWS_UP(8, 2);
+            else if (R == 4)
+                ROWS_UP(4, 4);
+            else
+                ROWS_UP(8, 4);
+            return;
+        }
         if (R == 8 && NJ == 2) {
             if (mode == 0)
                 ROWS(8, 2, 0);
@@ -1127,6 +1174,7 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
                 ROWS(8, 4, 2);
         }
 #undef ROWS
+#undef ROWS_UP
         return;
     }
     if (n->conv_kind >= 1) {
@@ -1201,15 +1249,30 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0)
     conv(n, 0, n->E3a, 4 * C, 0, 4 * C, wb(5), 4 * C, bias(5), 4 * C, n->E3b, nullptr, 4 * C, 0, B,
          D3, H3, W3);
     // decoder
-    hipLaunchKernelGGL(k_up_fwd, dim3(row_blocks(B * D2 * H2, W2 * 4 * C / 8)), dim3(256), 0,
-                       n->stream, n->E3b, 4 * C, n->U2, 6 * C, 0, B, D2, H2, W2);
-    conv(n, 0, n->U2, 6 * C, 0, 6 * C, wb(6), 2 * C, bias(6), 2 * C, n->D2a, nullptr, 2 * C, 0, B,
-         D2, H2, W2);
+    // inference with the row-blocked kernels reads the upsampled channels
+    // straight from the coarse tensor; training materializes them (the
+    // weight gradients read the concatenated input)
+    const bool fuse_up = !need_x0 && (n->conv_kind >= 2 && n->conv_kind <= 4);
+    if (fuse_up) {
+        conv(n, 0, n->U2, 6 * C, 0, 6 * C, wb(6), 2 * C, bias(6), 2 * C, n->D2a, nullptr, 2 * C, 0,
+             B, D2, H2, W2, n->E3b, 4 * C, 4 * C);
+    } else {
+        hipLaunchKernelGGL(k_up_fwd, dim3(row_blocks(B * D2 * H2, W2 * 4 * C / 8)), dim3(256), 0,
+                           n->stream, n->E3b, 4 * C, n->U2, 6 * C, 0, B, D2, H2, W2);
+        conv(n, 0, n->U2, 6 * C, 0, 6 * C, wb(6), 2 * C, bias(6), 2 * C, n->D2a, nullptr, 2 * C, 0,
+             B, D2, H2, W2);
+    }
     conv(n, 0, n->D2a, 2 * C, 0, 2 * C, wb(7), 2 * C, bias(7), 2 * C, n->D2b, nullptr, 2 * C, 0, B,
          D2, H2, W2);
-    hipLaunchKernelGGL(k_up_fwd, dim3(row_blocks(B * D * H, W * 2 * C / 8)), dim3(256), 0,
-                       n->stream, n->D2b, 2 * C, n->U1, 3 * C, 0, B, D, H, W);
-    conv(n, 0, n->U1, 3 * C, 0, 3 * C, wb(8), C, bias(8), C, n->D1a, nullptr, C, 0, B, D, H, W);
+    if (fuse_up) {
+        conv(n, 0, n->U1, 3 * C, 0, 3 * C, wb(8), C, bias(8), C, n->D1a, nullptr, C, 0, B, D, H, W,
+             n->D2b, 2 * C, 2 * C);
+    } else {
+        hipLaunchKernelGGL(k_up_fwd, dim3(row_blocks(B * D * H, W * 2 * C / 8)), dim3(256), 0,
+                           n->stream, n->D2b, 2 * C, n->U1, 3 * C, 0, B, D, H, W);
+        conv(n, 0, n->U1, 3 * C, 0, 3 * C, wb(8), C, bias(8), C, n->D1a, nullptr, C, 0, B, D, H,
+             W);
+    }
     conv(n, 0, n->D1a, C, 0, C, wb(9), C, bias(9), C, n->D1b, nullptr, C, 0, B, D, H, W);
     hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(V1)), dim3(256), 0, n->stream, n->D1b, C,
                        n->p + n->head_w, n->p + n->head_b, n->xin, V1, n->yout);
@@ -1704,8 +1767,8 @@ extern "C" int ppf_unet_last_forward(ppf_unet *n, double *ms, double *flop)
 
 extern "C" int ppf_unet_set_conv_kernel(ppf_unet *n, int kind)
 {
-    if (!n || kind < 0 || kind > 4)
-        return n ? uerr(n, PPF_EARG, "convolution kernel must be 0..4") : PPF_EARG;
+    if (!n || kind < 0 || kind > 5)
+        return n ? uerr(n, PPF_EARG, "convolution kernel must be 0..5") : PPF_EARG;
     n->conv_kind = kind;
     return PPF_OK;
 }

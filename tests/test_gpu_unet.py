@@ -33,8 +33,23 @@ def _data(seed, shape=(2, 16, 16, 16)):
 
 
 # convolution kernels: 2 row-blocked (default), 3 and 4 its fixed-tile
-# variants, 1 LDS-tiled, 0 direct
-KINDS = [2, 3, 4, 1, 0]
+# variants, 5 kind 2 with materialized upsampling, 1 LDS-tiled, 0 direct
+KINDS = [2, 3, 4, 5, 1, 0]
+
+
+@pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 24, 12, 40)])
+def test_fused_upsampling_is_bitwise_the_materialized_one(shape):
+    """At inference the decoder convolutions read the upsampled channels from
+    the coarse tensor (kind 2); kind 5 runs the same kernels on a
+    materialized upsampled copy.  Same values, same sums: bitwise equal."""
+    p = _params(2)
+    x, _ = _data(3, shape)
+    with UNet(seed=3) as net:
+        net.set_params(p)
+        a = net.forward(x)
+        net.set_conv_kernel(5)
+        b = net.forward(x)
+    np.testing.assert_array_equal(a, b)
 
 
 @pytest.mark.parametrize("kind", KINDS)
