@@ -63,7 +63,10 @@ int ppf_unet_train_step(ppf_unet *net, int batch, int nz, int ny, int nx, const 
 
 /* Apply the network to a stepping context's current state in place (a
  * jump of K steps).  c is then shifted uniformly so that sum(c) equals its
- * value before the jump (FP64 sums), conserving solute exactly. */
+ * value before the jump (FP64 sums), conserving solute exactly.  Queued in
+ * stream order with the context's work (no host synchronisation): steps
+ * issued on the context before the call run first, and the context's later
+ * calls see the jumped state. */
 int ppf_unet_apply(ppf_unet *net, ppf_ctx *ctx);
 
 /* Milliseconds of the last forward pass, measured with HIP events, and

@@ -472,9 +472,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 // ----------------------------------------------------------------------------
 constexpr int C0_TX = 64, C0_R = 8, C0_HX = C0_TX + 2, C0_HY = C0_R + 2, C0_CH = 5;
 
-template <int NJ>
-__global__ __launch_bounds__(256) void k_conv0(const float *__restrict__ x,
-                                               const bf16 *__restrict__ W0,
+// The network's input: packed FP32 [V][3] (InPacked), or a stepping
+// context's three field arrays of type T (InFields; the value is rounded to
+// FP32 first, as the packed copy would be).
+struct InPacked {
+    const float *x;
+    __device__ float get(size_t v, int ch) const { return x[v * 3 + ch]; }
+};
+template <typename T>
+struct InFields {
+    const T *s, *l, *c;
+    __device__ float get(size_t v, int ch) const
+    {
+        return (float)(ch == 0 ? s[v] : (ch == 1 ? l[v] : c[v]));
+    }
+};
+
+template <int NJ, class Src>
+__global__ __launch_bounds__(256) void k_conv0(const Src src, const bf16 *__restrict__ W0,
                                                const float *__restrict__ bias, int Cout,
                                                bf16 *__restrict__ Y, int ldy, int B, int D, int H,
                                                int Wd)
@@ -493,9 +508,9 @@ __global__ __launch_bounds__(256) void k_conv0(const float *__restrict__ x,
         const int hx = i % C0_HX, u = i / C0_HX, hy = u % C0_HY, hz = u / C0_HY;
         const size_t v = (((size_t)b * D + wrapm(z + hz - 1, D)) * H + wrapm(y0 + hy - 1, H)) * Wd +
                          wrapm(x0 + hx - 1, Wd);
-        sx[i * 3 + 0] = (bf16)x[v * 3 + 0];
-        sx[i * 3 + 1] = (bf16)x[v * 3 + 1];
-        sx[i * 3 + 2] = (bf16)x[v * 3 + 2];
+        sx[i * 3 + 0] = (bf16)src.get(v, 0);
+        sx[i * 3 + 1] = (bf16)src.get(v, 1);
+        sx[i * 3 + 2] = (bf16)src.get(v, 2);
     }
     const int m = lane & 15, q = lane >> 4;
     bf16x8 w[C0_CH][NJ];
@@ -736,6 +751,43 @@ __global__ __launch_bounds__(256) void k_up_fwd(const bf16 *__restrict__ S, int 
     const size_t src = ((size_t)(b * (D / 2) + z / 2) * (H / 2) + y / 2) * (W / 2) + x / 2;
     *(bf16x8 *)(U + ((size_t)row * W + x) * ldu + uoff + 8 * o) =
         *(const bf16x8 *)(S + src * C + 8 * o);
+}
+
+// Head straight into a stepping context: the residual input is the
+// context's own fields, and out = x + head replaces them in place (each
+// thread reads and writes its own site only).  The sums are k_head_fwd's.
+template <typename T>
+__global__ __launch_bounds__(256) void k_head_ctx(const bf16 *__restrict__ F, int CB,
+                                                  const float *__restrict__ hw,
+                                                  const float *__restrict__ hb, size_t total,
+                                                  T *__restrict__ s, T *__restrict__ l,
+                                                  T *__restrict__ c)
+{
+    const size_t v = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= total)
+        return;
+    float a0 = hb[0], a1 = hb[1], a2 = hb[2];
+    for (int c8 = 0; c8 < CB; c8 += 8) {
+        const bf16x8 f8 = *(const bf16x8 *)(F + v * CB + c8);
+        for (int e = 0; e < 8; e++) {
+            const int ci = c8 + e;
+            const float f = (float)f8[e];
+            a0 += hw[ci] * f;
+            a1 += hw[CB + ci] * f;
+            a2 += hw[2 * CB + ci] * f;
+        }
+    }
+    s[v] = (T)((float)s[v] + a0);
+    l[v] = (T)((float)l[v] + a1);
+    c[v] = (T)((float)c[v] + a2);
+}
+
+// k_mass_delta with the target read from the device (the pre-jump sum).
+__global__ __launch_bounds__(256) void k_mass_delta_dev(const double *__restrict__ p, int nb,
+                                                        const double *__restrict__ target,
+                                                        double n, double *__restrict__ out)
+{
+    k_mass_delta_body(p, nb, target[0], n, out);
 }
 
 // Network input: x [total][3] FP32 -> X0 [total][32] BF16, channels 3..31 zero.
@@ -986,6 +1038,9 @@ struct ppf_unet {
     float *part;
     size_t part_cap;
     double last_ms, last_flop;
+    bool timing_pending;    // ev0/ev1 bracket an apply not yet read back
+    hipEvent_t evc, evd;    // ordering with a stepping context's stream (ppf_unet_apply)
+    double *tmp;            // solute sums of ppf_unet_apply, 2 * 1024 + 8 doubles
     int conv_kind;  // 2-4: row-blocked variants, 1: LDS-tiled, 0: direct
     char err[512];
 };
@@ -1203,9 +1258,18 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
                            Wt, coutw, bias, cout, Yb, Yf, ldy, yoff, B, D, H, W);
 }
 
+// A stepping context's fields as the network's input and output (apply).
+struct CtxIO {
+    void *s, *l, *c;
+    bool f64;
+};
+
 // need_x0: also build the padded BF16 input X0 (the first layer's weight
 // gradient reads it); inference with the row-blocked kernels does not.
-static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0)
+// io (row-blocked kernels only): read the input from a context's fields and
+// write the output back into them instead of xin / yout.
+static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
+                   const CtxIO *io = nullptr)
 {
     const int C = n->base;
     const size_t V1 = (size_t)B * D * H * W;
@@ -1217,12 +1281,30 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0)
     if (n->conv_kind >= 2) {
         const unsigned nb0 = (unsigned)((size_t)B * D * ((H + C0_R - 1) / C0_R) *
                                         ((W + C0_TX - 1) / C0_TX));
-        if (C == 32)
-            hipLaunchKernelGGL(k_conv0<2>, dim3(nb0), dim3(256), 0, n->stream, n->xin, n->w0,
-                               bias(0), C, n->E1a, C, B, D, H, W);
-        else
-            hipLaunchKernelGGL(k_conv0<4>, dim3(nb0), dim3(256), 0, n->stream, n->xin, n->w0,
-                               bias(0), C, n->E1a, C, B, D, H, W);
+#define CONV0(NJ, SRC)                                                                        \
+    hipLaunchKernelGGL((k_conv0<NJ, decltype(SRC)>), dim3(nb0), dim3(256), 0, n->stream, SRC,     \
+                       n->w0, bias(0), C, n->E1a, C, B, D, H, W)
+        const InPacked packed{n->xin};
+        if (io && io->f64) {
+            const InFields<double> src{(const double *)io->s, (const double *)io->l,
+                                       (const double *)io->c};
+            if (C == 32)
+                CONV0(2, src);
+            else
+                CONV0(4, src);
+        } else if (io) {
+            const InFields<float> src{(const float *)io->s, (const float *)io->l,
+                                      (const float *)io->c};
+            if (C == 32)
+                CONV0(2, src);
+            else
+                CONV0(4, src);
+        } else if (C == 32) {
+            CONV0(2, packed);
+        } else {
+            CONV0(4, packed);
+        }
+#undef CONV0
         if (need_x0)
             hipLaunchKernelGGL(k_pack_in, dim3(blocks_for(V1 * 4)), dim3(256), 0, n->stream,
                                n->xin, V1, n->X0);
@@ -1274,8 +1356,17 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0)
              W);
     }
     conv(n, 0, n->D1a, C, 0, C, wb(9), C, bias(9), C, n->D1b, nullptr, C, 0, B, D, H, W);
-    hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(V1)), dim3(256), 0, n->stream, n->D1b, C,
-                       n->p + n->head_w, n->p + n->head_b, n->xin, V1, n->yout);
+    if (io && io->f64)
+        hipLaunchKernelGGL(k_head_ctx<double>, dim3(blocks_for(V1)), dim3(256), 0, n->stream,
+                           n->D1b, C, n->p + n->head_w, n->p + n->head_b, V1, (double *)io->s,
+                           (double *)io->l, (double *)io->c);
+    else if (io)
+        hipLaunchKernelGGL(k_head_ctx<float>, dim3(blocks_for(V1)), dim3(256), 0, n->stream,
+                           n->D1b, C, n->p + n->head_w, n->p + n->head_b, V1, (float *)io->s,
+                           (float *)io->l, (float *)io->c);
+    else
+        hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(V1)), dim3(256), 0, n->stream, n->D1b, C,
+                           n->p + n->head_w, n->p + n->head_b, n->xin, V1, n->yout);
     UHIP(n, hipGetLastError());
     return PPF_OK;
 }
@@ -1436,7 +1527,7 @@ extern "C" int ppf_unet_destroy(ppf_unet *n)
     if (n->stream)
         hipStreamSynchronize(n->stream);
     free_ws(n);
-    void *bufs[] = {n->p, n->g, n->m, n->v, n->wb, n->wt, n->w0, n->part};
+    void *bufs[] = {n->p, n->g, n->m, n->v, n->wb, n->wt, n->w0, n->tmp, n->part};
     for (void *b : bufs)
         if (b)
             hipFree(b);
@@ -1444,6 +1535,10 @@ extern "C" int ppf_unet_destroy(ppf_unet *n)
         hipEventDestroy(n->ev0);
     if (n->ev1)
         hipEventDestroy(n->ev1);
+    if (n->evc)
+        hipEventDestroy(n->evc);
+    if (n->evd)
+        hipEventDestroy(n->evd);
     if (n->stream)
         hipStreamDestroy(n->stream);
     delete n;
@@ -1531,6 +1626,12 @@ extern "C" int ppf_unet_create(ppf_unet **out, int base, int device, uint64_t se
     if (e == hipSuccess)
         e = hipMalloc(&n->w0, (size_t)C0_CH * C * 32 * sizeof(bf16));
     if (e == hipSuccess)
+        e = hipMalloc(&n->tmp, (2 * 1024 + 8) * sizeof(double));
+    if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&n->evc, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&n->evd, hipEventDisableTiming);
+    if (e == hipSuccess)
         e = hipMemcpy(n->p, h.data(), off * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemset(n->m, 0, off * sizeof(float));
@@ -1617,6 +1718,7 @@ extern "C" int ppf_unet_forward(ppf_unet *n, int B, int D, int H, int W, const f
     float ms = 0.f;
     UHIP(n, hipEventElapsedTime(&ms, n->ev0, n->ev1));
     n->last_ms = ms;
+    n->timing_pending = false;
     n->last_flop = forward_flop(n, V1);
     return PPF_OK;
 }
@@ -1684,15 +1786,16 @@ extern "C" int ppf_unet_apply(ppf_unet *n, ppf_ctx *ctx)
     int rc = ensure_ws(n, 1, D, H, W, false);
     if (rc)
         return rc;
-    // the context's queued steps finish before the network reads its fields
-    UHIP(n, hipStreamSynchronize(ctx->stream));
+    // stream order: the context's queued steps, then this, then the
+    // context's later work (events; no host synchronisation)
+    UHIP(n, hipEventRecord(n->evc, ctx->stream));
+    UHIP(n, hipStreamWaitEvent(n->stream, n->evc, 0));
     const size_t N = ctx->n;
     void *fs = ctx->buf[ctx->cur[0]][0], *fl = ctx->buf[ctx->cur[1]][1],
          *fc = ctx->buf[ctx->cur[2]][2];
     const bool f64 = ctx->esz == 8;
-    // solute before the jump (FP64, double-double) into dres[0..1]
-    double *tmp = nullptr;
-    UHIP(n, hipMalloc(&tmp, (2 * 1024 + 4) * sizeof(double)));
+    double *tmp = n->tmp;
+    // solute before the jump (FP64, double-double): tmp[2049]
     if (f64)
         hipLaunchKernelGGL(k_sum_dd<double>, dim3(1024), dim3(256), 0, n->stream,
                            (const double *)fc, N, tmp);
@@ -1701,57 +1804,53 @@ extern "C" int ppf_unet_apply(ppf_unet *n, ppf_ctx *ctx)
                            (const float *)fc, N, tmp);
     hipLaunchKernelGGL(k_mass_delta, dim3(1), dim3(256), 0, n->stream, tmp, 1024, 0.0, 1.0,
                        tmp + 2048);
-    if (f64)
-        hipLaunchKernelGGL(k_ctx_to_x<double>, dim3(grid_for(N)), dim3(256), 0, n->stream,
-                           (const double *)fs, (const double *)fl, (const double *)fc, N, n->xin);
-    else
-        hipLaunchKernelGGL(k_ctx_to_x<float>, dim3(grid_for(N)), dim3(256), 0, n->stream,
-                           (const float *)fs, (const float *)fl, (const float *)fc, N, n->xin);
-    hipEventRecord(n->ev0, n->stream);
-    rc = forward(n, 1, D, H, W, false);
-    hipEventRecord(n->ev1, n->stream);
-    if (rc) {
-        hipFree(tmp);
+    // the row-blocked kernels read and write the context's fields directly;
+    // the others go through the packed FP32 copies
+    const bool direct = n->conv_kind >= 2;
+    const CtxIO io{fs, fl, fc, f64};
+    if (!direct) {
+        if (f64)
+            hipLaunchKernelGGL(k_ctx_to_x<double>, dim3(grid_for(N)), dim3(256), 0, n->stream,
+                               (const double *)fs, (const double *)fl, (const double *)fc, N,
+                               n->xin);
+        else
+            hipLaunchKernelGGL(k_ctx_to_x<float>, dim3(grid_for(N)), dim3(256), 0, n->stream,
+                               (const float *)fs, (const float *)fl, (const float *)fc, N, n->xin);
+    }
+    UHIP(n, hipEventRecord(n->ev0, n->stream));
+    rc = forward(n, 1, D, H, W, false, direct ? &io : nullptr);
+    UHIP(n, hipEventRecord(n->ev1, n->stream));
+    if (rc)
         return rc;
+    if (!direct) {
+        if (f64)
+            hipLaunchKernelGGL(k_y_to_ctx<double>, dim3(grid_for(N)), dim3(256), 0, n->stream,
+                               n->yout, N, (double *)fs, (double *)fl, (double *)fc);
+        else
+            hipLaunchKernelGGL(k_y_to_ctx<float>, dim3(grid_for(N)), dim3(256), 0, n->stream,
+                               n->yout, N, (float *)fs, (float *)fl, (float *)fc);
     }
-    if (f64)
-        hipLaunchKernelGGL(k_y_to_ctx<double>, dim3(grid_for(N)), dim3(256), 0, n->stream, n->yout,
-                           N, (double *)fs, (double *)fl, (double *)fc);
-    else
-        hipLaunchKernelGGL(k_y_to_ctx<float>, dim3(grid_for(N)), dim3(256), 0, n->stream, n->yout,
-                           N, (float *)fs, (float *)fl, (float *)fc);
-    // project c back onto the pre-jump total
-    double before[2];
-    hipError_t e = hipMemcpyAsync(before, tmp + 2048, sizeof before, hipMemcpyDeviceToHost,
-                                  n->stream);
-    if (e == hipSuccess)
-        e = hipStreamSynchronize(n->stream);
-    if (e == hipSuccess) {
-        if (f64) {
-            hipLaunchKernelGGL(k_sum_dd<double>, dim3(1024), dim3(256), 0, n->stream,
-                               (const double *)fc, N, tmp);
-            hipLaunchKernelGGL(k_mass_delta, dim3(1), dim3(256), 0, n->stream, tmp, 1024,
-                               before[1], (double)N, tmp + 2048);
-            hipLaunchKernelGGL(k_add_delta, dim3(grid_for(N)), dim3(256), 0, n->stream,
-                               (double *)fc, N, (const double *)(tmp + 2048));
-        } else {
-            hipLaunchKernelGGL(k_sum_dd<float>, dim3(1024), dim3(256), 0, n->stream,
-                               (const float *)fc, N, tmp);
-            hipLaunchKernelGGL(k_mass_delta, dim3(1), dim3(256), 0, n->stream, tmp, 1024,
-                               before[1], (double)N, tmp + 2048);
-            hipLaunchKernelGGL(k_mass_shift, dim3(grid_for(N)), dim3(256), 0, n->stream,
-                               (float *)fc, N, (const double *)(tmp + 2048), (uint64_t)ctx->steps);
-        }
-        e = hipGetLastError();
+    // project c back onto the pre-jump total: delta = (before - after) / N
+    // into tmp[2052], then the shift (FP32: stochastically rounded)
+    if (f64) {
+        hipLaunchKernelGGL(k_sum_dd<double>, dim3(1024), dim3(256), 0, n->stream,
+                           (const double *)fc, N, tmp);
+        hipLaunchKernelGGL(k_mass_delta_dev, dim3(1), dim3(256), 0, n->stream, tmp, 1024,
+                           (const double *)(tmp + 2049), (double)N, tmp + 2052);
+        hipLaunchKernelGGL(k_add_delta, dim3(grid_for(N)), dim3(256), 0, n->stream, (double *)fc,
+                           N, (const double *)(tmp + 2052));
+    } else {
+        hipLaunchKernelGGL(k_sum_dd<float>, dim3(1024), dim3(256), 0, n->stream,
+                           (const float *)fc, N, tmp);
+        hipLaunchKernelGGL(k_mass_delta_dev, dim3(1), dim3(256), 0, n->stream, tmp, 1024,
+                           (const double *)(tmp + 2049), (double)N, tmp + 2052);
+        hipLaunchKernelGGL(k_mass_shift, dim3(grid_for(N)), dim3(256), 0, n->stream, (float *)fc,
+                           N, (const double *)(tmp + 2052), (uint64_t)ctx->steps);
     }
-    if (e == hipSuccess)
-        e = hipStreamSynchronize(n->stream);
-    hipFree(tmp);
-    if (e != hipSuccess)
-        return uerr(n, PPF_EHIP, "apply failed: %s", hipGetErrorString(e));
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, n->ev0, n->ev1);
-    n->last_ms = ms;
+    UHIP(n, hipGetLastError());
+    UHIP(n, hipEventRecord(n->evd, n->stream));
+    UHIP(n, hipStreamWaitEvent(ctx->stream, n->evd, 0));
+    n->timing_pending = true;
     n->last_flop = forward_flop(n, N);
     return PPF_OK;
 }
@@ -1760,6 +1859,13 @@ extern "C" int ppf_unet_last_forward(ppf_unet *n, double *ms, double *flop)
 {
     if (!n || !ms || !flop)
         return PPF_EARG;
+    if (n->timing_pending) {  // an apply's forward: read its events now
+        UHIP(n, hipEventSynchronize(n->ev1));
+        float f = 0.f;
+        UHIP(n, hipEventElapsedTime(&f, n->ev0, n->ev1));
+        n->last_ms = f;
+        n->timing_pending = false;
+    }
     *ms = n->last_ms;
     *flop = n->last_flop;
     return PPF_OK;

@@ -173,3 +173,30 @@ def test_apply_jumps_state_and_conserves_solute():
             assert abs(c1 - c0) / c0 < (1e-13 if prec == S.F64 else 1e-9)
             ms, flop = net.last_forward()
             assert ms > 0 and flop > 0
+
+
+@pytest.mark.parametrize("kind", [2, 0])
+def test_apply_equals_the_host_forward(kind):
+    """apply reads and writes the context's fields (row-blocked kernels) or
+    goes through packed FP32 copies (the others); either way etaS and etaL
+    are bitwise the network's output on the same state (forward from host
+    arrays), and c is that output shifted uniformly onto the old total."""
+    shape = (16, 24, 32)
+    cen, rad = S.spheres(4, 6, shape, 3.0, 6.0)
+    for prec in (S.F64, S.F32):
+        with S.Solver(shape, precision=prec) as sv, UNet(seed=8) as net:
+            net.set_conv_kernel(kind)
+            net.set_params(_params(10, 0.02))
+            sv.init_spheres(cen, rad)
+            sv.step(3)
+            x = np.stack(sv.download(), axis=-1).astype(np.float32)[None]
+            y = net.forward(x)[0]
+            net.apply(sv)
+            s, l, c = sv.download()
+            cast = np.float64 if prec == S.F64 else np.float32
+            np.testing.assert_array_equal(s, y[..., 0].astype(cast).astype(np.float64))
+            np.testing.assert_array_equal(l, y[..., 1].astype(cast).astype(np.float64))
+            shift = c - y[..., 2].astype(cast).astype(np.float64)
+            # one uniform shift (FP32: stochastically rounded, within an ulp)
+            tol = 0.0 if prec == S.F64 else 2.0 * np.spacing(np.float32(np.abs(c).max()))
+            assert np.ptp(shift) <= tol + 1e-15
