@@ -345,12 +345,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
         (void *)(Wt + (size_t)n0 * Cin), (short)0, (27 * CoutW - n0) * Cin * 2, 0x00020000);
     const int wo = (xl * Cin + 8 * q) * 2;
     const int tapstride = CoutW * Cin, jstride = 16 * Cin;
+    // MODE 0: the accumulators start at the bias of the lane's output
+    // channel, so the store loop holds no load: on gfx9 a load there makes
+    // every store wait for the one before it (vmcnt counts stores)
     f32x4 acc[R][NJ];
 #pragma unroll
-    for (int r = 0; r < R; r++)
+    for (int j = 0; j < NJ; j++) {
+        const float bj = (MODE == 0 && n0 + 16 * j + xl < Cout) ? bias[n0 + 16 * j + xl] : 0.f;
 #pragma unroll
-        for (int j = 0; j < NJ; j++)
-            acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int r = 0; r < R; r++)
+            acc[r][j] = f32x4{bj, bj, bj, bj};
+    }
     // Fragments of one (32-channel chunk, dz, dx) group.
     auto ld = [&](bf16x8(&a)[R + 2], bf16x8(&w)[3][NJ], int kc, int grp) {
         const int dz = grp / 3, dx = grp % 3;
@@ -445,13 +450,142 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
                 float v = acc[r][j][rr];
                 const size_t o = vox * ldy + yoff + ch;
                 if (MODE == 0) {
-                    v += bias[ch];
                     v = v > 0.f ? v : LEAK * v;
                     Yb[o] = (bf16)v;
                 } else if (MODE == 1) {
                     Yf[o] = v;
                 } else {
                     Yf[o] += v;
+                }
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// z-marching convolution for Cin = Cout = 32 (the level-1 layers), forward
+// (MODE 0).  A workgroup owns a 16 (x) x 32 (y) tile and marches a chunk of
+// z planes; wave w computes rows 8w .. 8w+7.  Each input plane is staged
+// once in LDS (tile + one-site halo, 34 x 18 sites, 32 channels padded to
+// 40 so that the 16-byte fragment reads of 8 consecutive sites hit distinct
+// banks) and serves the three output planes that need it; three slots hold
+// planes z-1, z, z+1 while the next plane is loaded into registers.  All 27
+// taps' weights (27 x 2 fragments) stay in registers.  Per output plane a
+// wave runs 9 (dz, dx) groups of 10 LDS fragment reads and 48 MFMAs.
+// ----------------------------------------------------------------------------
+constexpr int ZM_TX = 16, ZM_TY = 32, ZM_HX = ZM_TX + 2, ZM_HY = ZM_TY + 2;
+constexpr int ZM_SITES = ZM_HX * ZM_HY;          // 612 per plane
+constexpr int ZM_PAD = 40;                       // bf16 per staged site
+constexpr int ZM_CHUNKS = ZM_SITES * 4;          // 16-byte chunks per plane
+constexpr int ZM_PER_THREAD = (ZM_CHUNKS + 255) / 256;  // 10
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_conv_z32(
+    const bf16 *__restrict__ X, int ldx, int xoff, const bf16 *__restrict__ Wt,
+    const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy, int yoff, int B, int D, int H,
+    int Wd, int zc, int ntx, int nty)
+{
+    __shared__ __attribute__((aligned(16))) bf16 zsh[3 * ZM_SITES * ZM_PAD];  // 146,880 bytes
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int r = blockIdx.x;
+    const int tx = r % ntx;
+    r /= ntx;
+    const int ty = r % nty;
+    r /= nty;
+    const int nzc = (D + zc - 1) / zc;
+    const int zci = r % nzc, b = r / nzc;
+    const int x0 = tx * ZM_TX, y0 = ty * ZM_TY, z0 = zci * zc, zend = min(z0 + zc, D);
+    const int xl = lane & 15, q = lane >> 4;
+    // weights: w[tap][j] = Wt[tap][16j + xl][8q .. 8q+7]
+    bf16x8 w[27][2];
+#pragma unroll
+    for (int tap = 0; tap < 27; tap++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            w[tap][j] = *(const bf16x8 *)(Wt + ((size_t)tap * 32 + 16 * j + xl) * 32 + 8 * q);
+    // this thread's staging chunks: site s = c / 4 (row-major 34 x 18), part c % 4
+    auto plane_src = [&](int i, int zz) -> const bf16 * {
+        const int c = tid + 256 * i, s = c >> 2, part = c & 3;
+        const int hy = s / ZM_HX, hx = s - hy * ZM_HX;
+        const size_t v = (((size_t)b * D + wrapm(zz, D)) * H + wrapm(y0 + hy - 1, H)) * Wd +
+                         wrapm(x0 + hx - 1, Wd);
+        return X + v * ldx + xoff + 8 * part;
+    };
+    // bias of the lane's two output channels: the accumulators start there
+    // (no load in the store loop; see k_conv_rows)
+    const float bias0 = bias[xl], bias1 = bias[16 + xl];
+    bf16x8 pf[ZM_PER_THREAD];
+    auto load_plane = [&](int zz) {
+#pragma unroll
+        for (int i = 0; i < ZM_PER_THREAD; i++)
+            if (tid + 256 * i < ZM_CHUNKS)
+                pf[i] = *(const bf16x8 *)plane_src(i, zz);
+    };
+    auto store_plane = [&](int slot) {
+#pragma unroll
+        for (int i = 0; i < ZM_PER_THREAD; i++) {
+            const int c = tid + 256 * i;
+            if (c < ZM_CHUNKS)
+                *(bf16x8 *)(zsh + ((size_t)slot * ZM_SITES + (c >> 2)) * ZM_PAD + 8 * (c & 3)) = pf[i];
+        }
+    };
+    // planes z0-1 and z0 staged, z0+1 in registers
+    load_plane(z0 - 1);
+    store_plane(0);
+    load_plane(z0);
+    store_plane(1);
+    load_plane(z0 + 1);
+    for (int z = z0; z < zend; z++) {
+        const int s_m = (z - z0) % 3, s_c = (z - z0 + 1) % 3, s_p = (z - z0 + 2) % 3;
+        __syncthreads();  // the slot of plane z-2 is no longer read
+        store_plane(s_p);  // plane z+1
+        __syncthreads();
+        if (z + 1 < zend)
+            load_plane(z + 2);  // in flight during this plane's MFMAs
+        f32x4 acc[8][2];
+#pragma unroll
+        for (int rr = 0; rr < 8; rr++) {
+            acc[rr][0] = f32x4{bias0, bias0, bias0, bias0};
+            acc[rr][1] = f32x4{bias1, bias1, bias1, bias1};
+        }
+        const int slots[3] = {s_m, s_c, s_p};
+#pragma unroll
+        for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) {
+                bf16x8 a[10];
+                const bf16 *base = zsh + ((size_t)slots[dz] * ZM_SITES + (8 * wave) * ZM_HX + xl + dx) *
+                                             ZM_PAD + 8 * q;
+#pragma unroll
+                for (int rr = 0; rr < 10; rr++)
+                    a[rr] = *(const bf16x8 *)(base + rr * ZM_HX * ZM_PAD);
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+                    for (int rr = 0; rr < 8; rr++)
+#pragma unroll
+                        for (int j = 0; j < 2; j++)
+                            acc[rr][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                a[rr + dy], w[dz * 9 + dy * 3 + dx][j], acc[rr][j], 0, 0, 0);
+            }
+        // lane holds C[site 4q+e][channel xl] of each (row, j) tile
+#pragma unroll
+        for (int rr = 0; rr < 8; rr++) {
+            const int y = y0 + 8 * wave + rr;
+            if (y >= H)
+                continue;
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int ch = 16 * j + xl;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int x = x0 + 4 * q + e;
+                    if (x >= Wd)
+                        continue;
+                    const size_t vox = (((size_t)b * D + z) * H + y) * Wd + x;
+                    float v = acc[rr][j][e];
+                    v = v > 0.f ? v : LEAK * v;
+                    Y[vox * ldy + yoff + ch] = (bf16)v;
                 }
             }
         }
@@ -1176,6 +1310,15 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
                  int ldc = 0, int cup = 0)
 {
     const size_t total = (size_t)B * D * H * W;
+    if ((n->conv_kind == 2 || n->conv_kind == 5) && mode == 0 && !Xc && cin == 32 &&
+        coutw == 32 && cout == 32) {
+        // level-1 layers: the z-marching kernel (same sums as k_conv_rows)
+        const int zc = 16, ntx = (W + ZM_TX - 1) / ZM_TX, nty = (H + ZM_TY - 1) / ZM_TY;
+        const unsigned nblk = (unsigned)((size_t)B * ((D + zc - 1) / zc) * nty * ntx);
+        hipLaunchKernelGGL(k_conv_z32, dim3(nblk), dim3(256), 0, n->stream, X, ldx, xoff, Wt,
+                           bias, Yb, ldy, yoff, B, D, H, W, zc, ntx, nty);
+        return;
+    }
     if (n->conv_kind >= 2 && coutw % 32 == 0) {
         const int kind = n->conv_kind == 5 ? 2 : n->conv_kind;  // 5: kind 2's tiles
         // (R, NJ): 8 rows x 2 N tiles; where the channel count is a multiple

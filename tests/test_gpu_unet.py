@@ -37,6 +37,24 @@ def _data(seed, shape=(2, 16, 16, 16)):
 KINDS = [2, 3, 4, 5, 1, 0]
 
 
+@pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 24, 12, 40), (1, 8, 36, 20)])
+def test_row_blocked_kinds_agree_bitwise(shape):
+    """Kinds 2-5 use different tiles (8 or 4 rows, 32 or 64 channels per
+    wave), the z-marching level-1 kernel (2, 5) or not (3, 4), and fused or
+    materialized upsampling, but every output sums the same (chunk, dz, dx,
+    dy) products in the same order: their forwards are bitwise equal."""
+    p = _params(5)
+    x, _ = _data(6, shape)
+    out = {}
+    with UNet(seed=3) as net:
+        net.set_params(p)
+        for kind in (2, 3, 4, 5):
+            net.set_conv_kernel(kind)
+            out[kind] = net.forward(x)
+    for kind in (3, 4, 5):
+        np.testing.assert_array_equal(out[kind], out[2])
+
+
 @pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 24, 12, 40)])
 def test_fused_upsampling_is_bitwise_the_materialized_one(shape):
     """At inference the decoder convolutions read the upsampled channels from
@@ -147,14 +165,17 @@ def _pairs(shape, seeds, steps):
 
 def test_training_reduces_loss():
     """Real phase-field pairs: sphere states and the same states 20 FP32
-    steps later.  Adam with the library's own kernels must cut the loss."""
+    steps later.  Adam with the library's own kernels must cut the loss.
+    At lr 5e-4 the 60-step loss is 0.035-0.037 of the first one, whatever
+    the accumulation order of the forward; at 1e-3 this small problem's
+    loss spikes (to 20x) on the way and lands anywhere in 0.5-1.2x."""
     x, y = _pairs((32, 32, 32), (11, 12), 20)
     with UNet(seed=7) as net:
-        l0 = net.train_step(x, y, 1e-3)
+        l0 = net.train_step(x, y, 5e-4)
         for _ in range(60):
-            l1 = net.train_step(x, y, 1e-3)
+            l1 = net.train_step(x, y, 5e-4)
     print("loss", l0, "->", l1)
-    assert l1 < 0.7 * l0
+    assert l1 < 0.2 * l0
 
 
 def test_apply_jumps_state_and_conserves_solute():
