@@ -30,10 +30,34 @@
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
 constexpr float LEAK = 0.1f;
+
+// The convolution kernels run the MFMA as C^T = W X^T: the weight fragment
+// is the A operand (rows = output channels) and the input fragment the B
+// operand (columns = sites), so lane (l % 16, l / 16) ends with channels
+// 4(l/16) .. 4(l/16)+3 of site l % 16, four consecutive values of one
+// channels-last row, written with one 8-byte (bf16) or 16-byte (fp32) store.
+// MODE 0 outputs: bf16(leaky(bias + sum)), the accumulator starting at the
+// bias (a load of the bias in the store loop would make each store wait for
+// the stores before it: vmcnt counts both).
+__device__ __forceinline__ f32x4 bias4(const float *bias, int c0)
+{
+    return f32x4{bias[c0], bias[c0 + 1], bias[c0 + 2], bias[c0 + 3]};
+}
+
+__device__ __forceinline__ bf16x4 leaky4(f32x4 v)
+{
+    bf16x4 o;
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+        o[e] = (bf16)(v[e] > 0.f ? v[e] : LEAK * v[e]);
+    return o;
+}
+
 constexpr int NCONV = 10;
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -345,16 +369,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
         (void *)(Wt + (size_t)n0 * Cin), (short)0, (27 * CoutW - n0) * Cin * 2, 0x00020000);
     const int wo = (xl * Cin + 8 * q) * 2;
     const int tapstride = CoutW * Cin, jstride = 16 * Cin;
-    // MODE 0: the accumulators start at the bias of the lane's output
-    // channel, so the store loop holds no load: on gfx9 a load there makes
-    // every store wait for the one before it (vmcnt counts stores)
+    // MODE 0: the accumulators start at the bias of the lane's four output
+    // channels
     f32x4 acc[R][NJ];
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        const float bj = (MODE == 0 && n0 + 16 * j + xl < Cout) ? bias[n0 + 16 * j + xl] : 0.f;
+        const int c0 = n0 + 16 * j + 4 * q;
+        const f32x4 bj = (MODE == 0 && c0 < Cout) ? bias4(bias, c0) : f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int r = 0; r < R; r++)
-            acc[r][j] = f32x4{bj, bj, bj, bj};
+            acc[r][j] = bj;
     }
     // Fragments of one (32-channel chunk, dz, dx) group.
     auto ld = [&](bf16x8(&a)[R + 2], bf16x8(&w)[3][NJ], int kc, int grp) {
@@ -387,7 +411,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
             for (int r = 0; r < R; r++)
 #pragma unroll
                 for (int j = 0; j < NJ; j++)
-                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r + dy], w[dy][j],
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[dy][j], a[r + dy],
                                                                         acc[r][j], 0, 0, 0);
     };
     // Two register buffers.  Each group's MFMAs run while the next group's
@@ -430,33 +454,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
             __builtin_amdgcn_sched_barrier(0);
         }
     }
-    // lane holds C[site 4q+rr][channel xl] of each (r, j) tile
+    // lane holds channels n0+16j+4q .. +3 of site (xt*16+xl, y0+r) in acc[r][j]
+    const int x = xt * 16 + xl;
+    if (x >= Wd)
+        return;
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int y = y0 + r;
         if (y >= H)
             break;
+        const size_t vox = (((size_t)b * D + z) * H + y) * Wd + x;
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
-            const int ch = n0 + 16 * j + xl;
-            if (ch >= Cout)
+            const int c0 = n0 + 16 * j + 4 * q;
+            if (c0 >= Cout)
                 continue;
-#pragma unroll
-            for (int rr = 0; rr < 4; rr++) {
-                const int x = xt * 16 + 4 * q + rr;
-                if (x >= Wd)
-                    continue;
-                const size_t vox = (((size_t)b * D + z) * H + y) * Wd + x;
-                float v = acc[r][j][rr];
-                const size_t o = vox * ldy + yoff + ch;
-                if (MODE == 0) {
-                    v = v > 0.f ? v : LEAK * v;
-                    Yb[o] = (bf16)v;
-                } else if (MODE == 1) {
-                    Yf[o] = v;
-                } else {
-                    Yf[o] += v;
-                }
+            const size_t o = vox * ldy + yoff + c0;
+            if (MODE == 0) {
+                *(bf16x4 *)(Yb + o) = leaky4(acc[r][j]);
+            } else if (MODE == 1) {
+                *(f32x4 *)(Yf + o) = acc[r][j];
+            } else {
+                *(f32x4 *)(Yf + o) += acc[r][j];
             }
         }
     }
@@ -511,9 +530,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                          wrapm(x0 + hx - 1, Wd);
         return X + v * ldx + xoff + 8 * part;
     };
-    // bias of the lane's two output channels: the accumulators start there
-    // (no load in the store loop; see k_conv_rows)
-    const float bias0 = bias[xl], bias1 = bias[16 + xl];
+    // bias of the lane's output channels 4q .. 4q+3 and 16+4q .. 16+4q+3:
+    // the accumulators start there
+    const f32x4 bias0 = bias4(bias, 4 * q), bias1 = bias4(bias, 16 + 4 * q);
     bf16x8 pf[ZM_PER_THREAD];
     auto load_plane = [&](int zz) {
 #pragma unroll
@@ -545,8 +564,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         f32x4 acc[8][2];
 #pragma unroll
         for (int rr = 0; rr < 8; rr++) {
-            acc[rr][0] = f32x4{bias0, bias0, bias0, bias0};
-            acc[rr][1] = f32x4{bias1, bias1, bias1, bias1};
+            acc[rr][0] = bias0;
+            acc[rr][1] = bias1;
         }
         const int slots[3] = {s_m, s_c, s_p};
 #pragma unroll
@@ -566,27 +585,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
                         for (int j = 0; j < 2; j++)
                             acc[rr][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                a[rr + dy], w[dz * 9 + dy * 3 + dx][j], acc[rr][j], 0, 0, 0);
+                                w[dz * 9 + dy * 3 + dx][j], a[rr + dy], acc[rr][j], 0, 0, 0);
             }
-        // lane holds C[site 4q+e][channel xl] of each (row, j) tile
+        // lane holds channels 16j+4q .. +3 of site (x0+xl, y0+8w+rr) in acc[rr][j]
+        const int x = x0 + xl;
+        if (x < Wd) {
 #pragma unroll
-        for (int rr = 0; rr < 8; rr++) {
-            const int y = y0 + 8 * wave + rr;
-            if (y >= H)
-                continue;
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int ch = 16 * j + xl;
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const int x = x0 + 4 * q + e;
-                    if (x >= Wd)
-                        continue;
-                    const size_t vox = (((size_t)b * D + z) * H + y) * Wd + x;
-                    float v = acc[rr][j][e];
-                    v = v > 0.f ? v : LEAK * v;
-                    Y[vox * ldy + yoff + ch] = (bf16)v;
-                }
+            for (int rr = 0; rr < 8; rr++) {
+                const int y = y0 + 8 * wave + rr;
+                if (y >= H)
+                    break;
+                const size_t o = ((((size_t)b * D + z) * H + y) * Wd + x) * ldy + yoff + 4 * q;
+                *(bf16x4 *)(Y + o) = leaky4(acc[rr][0]);
+                *(bf16x4 *)(Y + o + 16) = leaky4(acc[rr][1]);
             }
         }
     }
@@ -653,10 +664,14 @@ __global__ __launch_bounds__(256) void k_conv0(const Src src, const bf16 *__rest
             w[c][j] = *(const bf16x8 *)(W0 + ((size_t)c * Cout + 16 * j + m) * 32 + 8 * q);
     __syncthreads();
     const int sxl = wave * 16 + m;  // staged x of the site's dx = -1 neighbour
+    // the accumulators start at the bias of the lane's channels 16j+4q .. +3
     f32x4 acc[C0_R][NJ];
-    for (int row = 0; row < C0_R; row++)
-        for (int j = 0; j < NJ; j++)
-            acc[row][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < NJ; j++) {
+        const int c0 = 16 * j + 4 * q;
+        const f32x4 bj = c0 < Cout ? bias4(bias, c0) : f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int row = 0; row < C0_R; row++)
+            acc[row][j] = bj;
+    }
     for (int row = 0; row < C0_R; row++) {
 #pragma unroll
         for (int c = 0; c < C0_CH; c++) {
@@ -675,26 +690,22 @@ __global__ __launch_bounds__(256) void k_conv0(const Src src, const bf16 *__rest
             }
 #pragma unroll
             for (int j = 0; j < NJ; j++)
-                acc[row][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w[c][j], acc[row][j], 0, 0, 0);
+                acc[row][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[c][j], a, acc[row][j], 0, 0, 0);
         }
     }
+    // lane holds channels 16j+4q .. +3 of site (x0+16w+m, y0+row) in acc[row][j]
+    const int xx = x0 + wave * 16 + m;
+    if (xx >= Wd)
+        return;
     for (int row = 0; row < C0_R; row++) {
         const int y = y0 + row;
         if (y >= H)
             break;
+        const size_t vox = (((size_t)b * D + z) * H + y) * Wd + xx;
         for (int j = 0; j < NJ; j++) {
-            const int ch = 16 * j + m;
-            if (ch >= Cout)
-                continue;
-            for (int rr = 0; rr < 4; rr++) {
-                const int xx = x0 + wave * 16 + 4 * q + rr;
-                if (xx >= Wd)
-                    continue;
-                const size_t vox = (((size_t)b * D + z) * H + y) * Wd + xx;
-                float v = acc[row][j][rr] + bias[ch];
-                v = v > 0.f ? v : LEAK * v;
-                Y[vox * ldy + ch] = (bf16)v;
-            }
+            const int c0 = 16 * j + 4 * q;
+            if (c0 < Cout)
+                *(bf16x4 *)(Y + vox * ldy + c0) = leaky4(acc[row][j]);
         }
     }
 }
@@ -1319,7 +1330,9 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
                            bias, Yb, ldy, yoff, B, D, H, W, zc, ntx, nty);
         return;
     }
-    if (n->conv_kind >= 2 && coutw % 32 == 0) {
+    // (the row-blocked kernels store 4 channels at once: aligned when the
+    // channel count, row stride and offset are multiples of 4)
+    if (n->conv_kind >= 2 && coutw % 32 == 0 && cout % 4 == 0 && ldy % 4 == 0 && yoff % 4 == 0) {
         const int kind = n->conv_kind == 5 ? 2 : n->conv_kind;  // 5: kind 2's tiles
         // (R, NJ): 8 rows x 2 N tiles; where the channel count is a multiple
         // of 64, 8 x 4 (kind 2, fastest measured: 18.0 ms at 256^3) or 4 x 4
