@@ -337,8 +337,19 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
 // Threads: tid < NR1 also own a ring site (its own z column of step t);
 // NR1 <= tid < NR1 + NO2 load one outer site (two-cell ring or corner) of
 // the current plane.
+// PF: the raw step-t values a plane needs (tile and ring columns at zs+1,
+// outer site at zs) are loaded one iteration earlier, so the loads stay in
+// flight over a whole plane of arithmetic and both barriers (which wait for
+// LDS only) instead of stalling the plane's first h, mu, M.  Nine more
+// values in registers per thread.
+// DB: A1 and A2 double buffered (a1, a2 hold two sets of 4 arrays each), one
+// barrier per plane.  A1 of plane zs goes to buffer zs&1, last read by stage
+// 1 two planes earlier (the previous plane's barrier has passed); A2 of plane
+// zs is written right after stage 1 into the buffer stage 2 does not read
+// this plane, last read by stage 2 of the previous plane (before this
+// plane's barrier).
 // ---------------------------------------------------------------------------
-template <class P, int TX, int TY>
+template <class P, int TX, int TY, bool PF = false, bool DB = false>
 __device__ __forceinline__ void tile_march2(
     const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
     const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
@@ -430,41 +441,88 @@ __device__ __forceinline__ void tile_march2(
             P::point(qs_c, ql_c, qc_c, k, qh_c, qmu_c, qM_c);
         }
     }
-    // step t+1 column of the tile site: u*_p plane z-2, u*_c plane z-1
+    // step t+1 column of the tile site: u*_p plane zs-2, u*_c plane zs-1
     T us_p = 0, ul_p = 0, umu_p = 0, uM_p = 0, us_c = 0, ul_c = 0, uc_c = 0, uh_c = 0, umu_c = 0,
       uM_c = 0;
-
-    for (int zs = z0 - 1; zs <= zend; ++zs) {
-        const int zc = ghost_any<0>(zs, nz), zn = ghost_any<0>(zs + 1, nz);
-        // step t, plane zs+1 (raw) of the tile and ring columns; plane zs of
-        // the outer site
-        const T rs_n = PLANE(s_in, zn)[col], rl_n = PLANE(l_in, zn)[col],
-                rc_n = PLANE(c_in, zn)[col];
-        T qs_n = 0, ql_n = 0, qc_n = 0;
+    // step t raw values of plane zs+1 (tile and ring columns) and of plane
+    // zs (outer site); with PF the first plane's are loaded here
+    T rs_n = 0, rl_n = 0, rc_n = 0, qs_n = 0, ql_n = 0, qc_n = 0, os = 0, ol = 0, oc = 0;
+    if (PF) {
+        const int zn = ghost_any<0>(z0, nz), zc = ghost_any<0>(z0 - 1, nz);
+        rs_n = PLANE(s_in, zn)[col];
+        rl_n = PLANE(l_in, zn)[col];
+        rc_n = PLANE(c_in, zn)[col];
         if (ring) {
             qs_n = PLANE(s_in, zn)[rcol];
             ql_n = PLANE(l_in, zn)[rcol];
             qc_n = PLANE(c_in, zn)[rcol];
         }
+        if (outer) {
+            os = PLANE(s_in, zc)[o2col];
+            ol = PLANE(l_in, zc)[o2col];
+            oc = PLANE(c_in, zc)[o2col];
+        }
+    }
+
+    for (int zs = z0 - 1; zs <= zend; ++zs) {
+        const int zc = ghost_any<0>(zs, nz), zn = ghost_any<0>(zs + 1, nz);
+        // this plane's A1 and A2 buffers, and the A2 buffer of plane zs-1
+        const int bsel = DB ? ((zs - z0 + 1) & 1) : 0;
+        T(*__restrict__ A1)[TY + 4][TX + 4] = a1 + 4 * bsel;
+        T(*__restrict__ A2)[TY + 2][TX + 2] = a2 + 4 * bsel;
+        const T(*__restrict__ A2p)[TY + 2][TX + 2] = a2 + 4 * (DB ? bsel ^ 1 : 0);
+        if (!PF) {
+            rs_n = PLANE(s_in, zn)[col];
+            rl_n = PLANE(l_in, zn)[col];
+            rc_n = PLANE(c_in, zn)[col];
+            if (ring) {
+                qs_n = PLANE(s_in, zn)[rcol];
+                ql_n = PLANE(l_in, zn)[rcol];
+                qc_n = PLANE(c_in, zn)[rcol];
+            }
+            if (outer) {
+                os = PLANE(s_in, zc)[o2col];
+                ol = PLANE(l_in, zc)[o2col];
+                oc = PLANE(c_in, zc)[o2col];
+            }
+        }
+        // PF: the next iteration's values (plane zs+2 of the columns, zs+1
+        // of the outer site)
+        T fs = 0, fl = 0, fc = 0, fqs = 0, fql = 0, fqc = 0, fos = 0, fol = 0, foc = 0;
+        if (PF && zs < zend) {
+            const int zf = ghost_any<0>(zs + 2, nz);
+            fs = PLANE(s_in, zf)[col];
+            fl = PLANE(l_in, zf)[col];
+            fc = PLANE(c_in, zf)[col];
+            if (ring) {
+                fqs = PLANE(s_in, zf)[rcol];
+                fql = PLANE(l_in, zf)[rcol];
+                fqc = PLANE(c_in, zf)[rcol];
+            }
+            if (outer) {
+                fos = PLANE(s_in, zn)[o2col];
+                fol = PLANE(l_in, zn)[o2col];
+                foc = PLANE(c_in, zn)[o2col];
+            }
+        }
         // A1: step t of plane zs
-        a1[0][ty + 2][tx + 2] = s_c;
-        a1[1][ty + 2][tx + 2] = l_c;
-        a1[2][ty + 2][tx + 2] = mu_c;
-        a1[3][ty + 2][tx + 2] = M_c;
+        A1[0][ty + 2][tx + 2] = s_c;
+        A1[1][ty + 2][tx + 2] = l_c;
+        A1[2][ty + 2][tx + 2] = mu_c;
+        A1[3][ty + 2][tx + 2] = M_c;
         if (ring) {
-            a1[0][ry + 1][rx + 1] = qs_c;
-            a1[1][ry + 1][rx + 1] = ql_c;
-            a1[2][ry + 1][rx + 1] = qmu_c;
-            a1[3][ry + 1][rx + 1] = qM_c;
+            A1[0][ry + 1][rx + 1] = qs_c;
+            A1[1][ry + 1][rx + 1] = ql_c;
+            A1[2][ry + 1][rx + 1] = qmu_c;
+            A1[3][ry + 1][rx + 1] = qM_c;
         }
         if (outer) {
-            const T os = PLANE(s_in, zc)[o2col], ol = PLANE(l_in, zc)[o2col];
             T oh, omu, oM;
-            P::point(os, ol, PLANE(c_in, zc)[o2col], k, oh, omu, oM);
-            a1[0][oy][ox] = os;
-            a1[1][oy][ox] = ol;
-            a1[2][oy][ox] = omu;
-            a1[3][oy][ox] = oM;
+            P::point(os, ol, oc, k, oh, omu, oM);
+            A1[0][oy][ox] = os;
+            A1[1][oy][ox] = ol;
+            A1[2][oy][ox] = omu;
+            A1[3][oy][ox] = oM;
         }
         T h_n, mu_n, M_n, qh_n = 0, qmu_n = 0, qM_n = 0;
         P::point(rs_n, rl_n, rc_n, k, h_n, mu_n, M_n);
@@ -476,10 +534,10 @@ __device__ __forceinline__ void tile_march2(
         T s1, l1, c1, h1, mu1, M1;
         {
             const int ax = tx + 2, ay = ty + 2;
-            const T sn[6] = {rs_n, s_p, a1[0][ay + 1][ax], a1[0][ay - 1][ax], a1[0][ay][ax + 1], a1[0][ay][ax - 1]};
-            const T ln[6] = {rl_n, l_p, a1[1][ay + 1][ax], a1[1][ay - 1][ax], a1[1][ay][ax + 1], a1[1][ay][ax - 1]};
-            const T mun[6] = {mu_n, mu_p, a1[2][ay + 1][ax], a1[2][ay - 1][ax], a1[2][ay][ax + 1], a1[2][ay][ax - 1]};
-            const T Mn[6] = {M_n, M_p, a1[3][ay + 1][ax], a1[3][ay - 1][ax], a1[3][ay][ax + 1], a1[3][ay][ax - 1]};
+            const T sn[6] = {rs_n, s_p, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
+            const T ln[6] = {rl_n, l_p, A1[1][ay + 1][ax], A1[1][ay - 1][ax], A1[1][ay][ax + 1], A1[1][ay][ax - 1]};
+            const T mun[6] = {mu_n, mu_p, A1[2][ay + 1][ax], A1[2][ay - 1][ax], A1[2][ay][ax + 1], A1[2][ay][ax - 1]};
+            const T Mn[6] = {M_n, M_p, A1[3][ay + 1][ax], A1[3][ay - 1][ax], A1[3][ay][ax + 1], A1[3][ay][ax - 1]};
             double ds, dl, dc;
             P::update(s_c, l_c, c_c, h_c, mu_c, M_c, sn, ln, mun, Mn, k, s1, l1, c1, ds, dl, dc);
             P::point(s1, l1, c1, k, h1, mu1, M1);
@@ -488,23 +546,39 @@ __device__ __forceinline__ void tile_march2(
         T qs1 = 0, ql1 = 0, qmu1 = 0, qM1 = 0;
         if (ring) {
             const int ax = rx + 1, ay = ry + 1;
-            const T sn[6] = {qs_n, qs_p, a1[0][ay + 1][ax], a1[0][ay - 1][ax], a1[0][ay][ax + 1], a1[0][ay][ax - 1]};
-            const T ln[6] = {ql_n, ql_p, a1[1][ay + 1][ax], a1[1][ay - 1][ax], a1[1][ay][ax + 1], a1[1][ay][ax - 1]};
-            const T mun[6] = {qmu_n, qmu_p, a1[2][ay + 1][ax], a1[2][ay - 1][ax], a1[2][ay][ax + 1], a1[2][ay][ax - 1]};
-            const T Mn[6] = {qM_n, qM_p, a1[3][ay + 1][ax], a1[3][ay - 1][ax], a1[3][ay][ax + 1], a1[3][ay][ax - 1]};
+            const T sn[6] = {qs_n, qs_p, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
+            const T ln[6] = {ql_n, ql_p, A1[1][ay + 1][ax], A1[1][ay - 1][ax], A1[1][ay][ax + 1], A1[1][ay][ax - 1]};
+            const T mun[6] = {qmu_n, qmu_p, A1[2][ay + 1][ax], A1[2][ay - 1][ax], A1[2][ay][ax + 1], A1[2][ay][ax - 1]};
+            const T Mn[6] = {qM_n, qM_p, A1[3][ay + 1][ax], A1[3][ay - 1][ax], A1[3][ay][ax + 1], A1[3][ay][ax - 1]};
             T qc1, qh1;
             double ds, dl, dc;
             P::update(qs_c, ql_c, qc_c, qh_c, qmu_c, qM_c, sn, ln, mun, Mn, k, qs1, ql1, qc1, ds, dl, dc);
             P::point(qs1, ql1, qc1, k, qh1, qmu1, qM1);
         }
-        // stage 2: step t+2 of plane zs-1 at the tile site (A2 holds step t+1
-        // of plane zs-1)
+        // A2: step t+1 of plane zs (a buffer of its own with DB; otherwise
+        // after stage 2 and a barrier, below)
+        auto store_a2 = [&]() {
+            A2[0][ty + 1][tx + 1] = s1;
+            A2[1][ty + 1][tx + 1] = l1;
+            A2[2][ty + 1][tx + 1] = mu1;
+            A2[3][ty + 1][tx + 1] = M1;
+            if (ring) {
+                A2[0][ry][rx] = qs1;
+                A2[1][ry][rx] = ql1;
+                A2[2][ry][rx] = qmu1;
+                A2[3][ry][rx] = qM1;
+            }
+        };
+        if (DB)
+            store_a2();
+        // stage 2: step t+2 of plane zs-1 at the tile site (A2p holds step
+        // t+1 of plane zs-1)
         if (zs >= z0 + 1 && active) {
             const int bx = tx + 1, by = ty + 1;
-            const T sn[6] = {s1, us_p, a2[0][by + 1][bx], a2[0][by - 1][bx], a2[0][by][bx + 1], a2[0][by][bx - 1]};
-            const T ln[6] = {l1, ul_p, a2[1][by + 1][bx], a2[1][by - 1][bx], a2[1][by][bx + 1], a2[1][by][bx - 1]};
-            const T mun[6] = {mu1, umu_p, a2[2][by + 1][bx], a2[2][by - 1][bx], a2[2][by][bx + 1], a2[2][by][bx - 1]};
-            const T Mn[6] = {M1, uM_p, a2[3][by + 1][bx], a2[3][by - 1][bx], a2[3][by][bx + 1], a2[3][by][bx - 1]};
+            const T sn[6] = {s1, us_p, A2p[0][by + 1][bx], A2p[0][by - 1][bx], A2p[0][by][bx + 1], A2p[0][by][bx - 1]};
+            const T ln[6] = {l1, ul_p, A2p[1][by + 1][bx], A2p[1][by - 1][bx], A2p[1][by][bx + 1], A2p[1][by][bx - 1]};
+            const T mun[6] = {mu1, umu_p, A2p[2][by + 1][bx], A2p[2][by - 1][bx], A2p[2][by][bx + 1], A2p[2][by][bx - 1]};
+            const T Mn[6] = {M1, uM_p, A2p[3][by + 1][bx], A2p[3][by - 1][bx], A2p[3][by][bx + 1], A2p[3][by][bx - 1]};
             T so, lo, co;
             double ds, dl, dc;
             P::update(us_c, ul_c, uc_c, uh_c, umu_c, uM_c, sn, ln, mun, Mn, k, so, lo, co, ds, dl, dc);
@@ -512,17 +586,9 @@ __device__ __forceinline__ void tile_march2(
             __builtin_nontemporal_store(lo, PLANE(l_out, zs - 1) + ocol);
             __builtin_nontemporal_store(co, PLANE(c_out, zs - 1) + ocol);
         }
-        __syncthreads();
-        // A2: step t+1 of plane zs
-        a2[0][ty + 1][tx + 1] = s1;
-        a2[1][ty + 1][tx + 1] = l1;
-        a2[2][ty + 1][tx + 1] = mu1;
-        a2[3][ty + 1][tx + 1] = M1;
-        if (ring) {
-            a2[0][ry][rx] = qs1;
-            a2[1][ry][rx] = ql1;
-            a2[2][ry][rx] = qmu1;
-            a2[3][ry][rx] = qM1;
+        if (!DB) {
+            __syncthreads();
+            store_a2();
         }
         // shift the columns
         s_p = s_c;
@@ -555,13 +621,26 @@ __device__ __forceinline__ void tile_march2(
         uh_c = h1;
         umu_c = mu1;
         uM_c = M1;
+        if (PF) {
+            rs_n = fs;
+            rl_n = fl;
+            rc_n = fc;
+            qs_n = fqs;
+            ql_n = fql;
+            qc_n = fqc;
+            os = fos;
+            ol = fol;
+            oc = foc;
+        }
     }
 #undef PLANE
 }
 
 // WPE: the waves per SIMD the register allocation aims for (FP64 needs a
-// cap: left alone it takes ~185 VGPRs, two waves per SIMD).
-template <class P, int TX, int TY, int WPE>
+// cap: left alone it takes ~185 VGPRs, two waves per SIMD).  PF: prefetch
+// one plane ahead, DB: double-buffered LDS, one barrier per plane
+// (tile_march2).
+template <class P, int TX, int TY, int WPE, bool PF = false, bool DB = false>
 __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_tiled2(
     const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
     const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
@@ -569,8 +648,8 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE
     int tz, int ntx, int nty, int order, typename P::K k)
 {
     using T = typename P::T;
-    __shared__ T a1[4][TY + 4][TX + 4];
-    __shared__ T a2[4][TY + 2][TX + 2];
+    __shared__ T a1[(DB ? 8 : 4)][TY + 4][TX + 4];
+    __shared__ T a2[(DB ? 8 : 4)][TY + 2][TX + 2];
     const int ntile = ntx * nty;
     const int nblk = gridDim.x;
     int b = blockIdx.x;
@@ -580,6 +659,6 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE
     const int zc = (order & 1) ? b % ((nblk + ntile - 1) / ntile) : b / ntile;
     const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY, z0 = zc * tz;
     const int zend = min(z0 + tz, nz);
-    tile_march2<P, TX, TY>(s_in, l_in, c_in, s_out, l_out, c_out, nx, ny, nz, x0, y0, z0, zend, k,
-                           a1, a2);
+    tile_march2<P, TX, TY, PF, DB>(s_in, l_in, c_in, s_out, l_out, c_out, nx, ny, nz, x0, y0, z0,
+                                   zend, k, a1, a2);
 }

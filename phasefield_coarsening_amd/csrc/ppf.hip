@@ -206,16 +206,20 @@ template <class P>
 static fused_fn<P> pick_fused(TileCfg t)
 {
     // FP64 needs ~185 VGPRs: two waves per SIMD (capping it at three spills
-    // and measured slower); FP32 fits five
-    constexpr int W = sizeof(typename P::T) == 8 ? 2 : 5;
+    // and measured slower); FP32 fits five.  At two waves per SIMD the
+    // loads must be issued a plane ahead (PF) to be hidden.  LDS double
+    // buffered (one barrier per plane) except for the FP64 64x16 tile,
+    // whose two buffers would not fit.
+    constexpr bool F64 = sizeof(typename P::T) == 8;
+    constexpr int W = F64 ? 2 : 5;
     if (t.tx == 64 && t.ty == 8)
-        return k_tiled2<P, 64, 8, W>;
+        return k_tiled2<P, 64, 8, W, F64, true>;
     if (t.tx == 32 && t.ty == 8)
-        return k_tiled2<P, 32, 8, W>;
+        return k_tiled2<P, 32, 8, W, F64, true>;
     if (t.tx == 32 && t.ty == 16)
-        return k_tiled2<P, 32, 16, W>;
+        return k_tiled2<P, 32, 16, W, F64, true>;
     if (t.tx == 64 && t.ty == 16)
-        return k_tiled2<P, 64, 16, W == 2 ? 1 : W>;
+        return k_tiled2<P, 64, 16, W == 2 ? 1 : W, F64, !F64>;
     return nullptr;
 }
 
