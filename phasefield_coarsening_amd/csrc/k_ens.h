@@ -137,7 +137,9 @@ static __global__ __launch_bounds__(256) void k_ens_shift(const EnsDom *__restri
 enum { ENS2_32x8 = 0, ENS2_16x16 = 1, ENS2_NSHAPES = 2 };
 
 // LDS in values: A1 (4 x (TY+4) x (TX+4)) then A2 (4 x (TY+2) x (TX+2)), the
-// larger of the two shapes: 32x8 needs 1728 + 1360, 16x16 1600 + 1296.
+// larger of the two shapes: 32x8 needs 1728 + 1360, 16x16 1600 + 1296.  FP32
+// members double-buffer both (one barrier per plane; 24.7 KB per workgroup);
+// FP64 keeps single buffers so that four workgroups fit a CU.
 constexpr int ENS2_LDS = 4 * 12 * 36 + 4 * 10 * 34;
 
 template <class P>
@@ -147,7 +149,9 @@ __global__ __launch_bounds__(256) void k_ensemble2(const EnsDom *__restrict__ do
                                                    typename P::K k)
 {
     using T = typename P::T;
-    __shared__ T sh[ENS2_LDS];
+    constexpr bool DB = sizeof(T) == 4;
+    constexpr int NB = DB ? 2 : 1;
+    __shared__ T sh[NB * ENS2_LDS];
     __shared__ int next;
     for (;;) {
         if (threadIdx.x == 0)
@@ -166,12 +170,12 @@ __global__ __launch_bounds__(256) void k_ensemble2(const EnsDom *__restrict__ do
         T *lo = (T *)d.f[1 - parity][1];
         T *co = (T *)d.f[1 - parity][2];
         if (w.shape == ENS2_32x8)
-            tile_march2<P, 32, 8>(si, li, ci, so, lo, co, d.nx, d.ny, d.nz, w.x0, w.y0, w.z0,
-                                  w.zend, k, reinterpret_cast<T(*)[12][36]>(sh),
-                                  reinterpret_cast<T(*)[10][34]>(sh + 4 * 12 * 36));
+            tile_march2<P, 32, 8, false, DB>(si, li, ci, so, lo, co, d.nx, d.ny, d.nz, w.x0, w.y0,
+                                             w.z0, w.zend, k, reinterpret_cast<T(*)[12][36]>(sh),
+                                             reinterpret_cast<T(*)[10][34]>(sh + NB * 4 * 12 * 36));
         else
-            tile_march2<P, 16, 16>(si, li, ci, so, lo, co, d.nx, d.ny, d.nz, w.x0, w.y0, w.z0,
-                                   w.zend, k, reinterpret_cast<T(*)[20][20]>(sh),
-                                   reinterpret_cast<T(*)[18][18]>(sh + 4 * 20 * 20));
+            tile_march2<P, 16, 16, false, DB>(si, li, ci, so, lo, co, d.nx, d.ny, d.nz, w.x0, w.y0,
+                                              w.z0, w.zend, k, reinterpret_cast<T(*)[20][20]>(sh),
+                                              reinterpret_cast<T(*)[18][18]>(sh + NB * 4 * 20 * 20));
     }
 }
