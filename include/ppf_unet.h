@@ -70,11 +70,13 @@ int ppf_unet_train_step(ppf_unet *net, int batch, int nz, int ny, int nx, const 
 int ppf_unet_apply(ppf_unet *net, ppf_ctx *ctx);
 
 /* Milliseconds of the last forward pass, measured with HIP events, and
- * its MFMA work in FLOP (2*27*Cin*Cout per site per 3x3x3 conv). */
+ * the MFMA work its kernels executed, in FLOP: 2*27*Cin*Cout per site per
+ * 3x3x3 conv, the first layer's K padded to 160, and 8 instead of 27 taps
+ * on the upsampled channels of the polyphase decoder layers (kind 6). */
 int ppf_unet_last_forward(ppf_unet *net, double *ms, double *flop);
 
 /* Convolution kernel:
- *   2 (default) row-blocked: a wave owns 16 x-sites of 8 rows and 32 or 64
+ *   2 row-blocked: a wave owns 16 x-sites of 8 rows and 32 or 64
  *     output channels and reuses each loaded input row for the three dy
  *     taps; no LDS; the first layer runs as one GEMM over its 81 real (tap,
  *     channel) pairs instead of 27 taps x 32 padded channels;
@@ -84,10 +86,17 @@ int ppf_unet_last_forward(ppf_unet *net, double *ms, double *flop);
  *     4 rows x 64 channels where Cout allows (4), for tuning;
  *   5 kind 2 with the upsampled channels materialized, as in training (the
  *     same sums as kind 2, for cross-checks);
+ *   6 (default) kind 2 with the two decoder convolutions in polyphase form at
+ *     inference: for each output parity, the 3x3x3 convolution of the
+ *     nearest-upsampled channels is a 2x2x2 convolution of the coarse
+ *     tensor with summed weights (8 taps instead of 27); the summed weights
+ *     are rounded to BF16 once, so kind 6 agrees with kind 2 to BF16 weight
+ *     rounding, not bitwise; training runs as kind 2;
  *   1 LDS-tiled: a 16 x 8 brick's halo staged in LDS;
  *   0 direct: every operand straight from global memory.
- * The kinds add the (tap, 32-channel) products into the FP32 accumulators
- * in different orders, so they agree to FP32 rounding, not bitwise. */
+ * Kinds 2-5 add the same (tap, 32-channel) products in the same order:
+ * their forwards are bitwise equal.  Kinds 0 and 1 sum in other orders and
+ * agree with them to FP32 rounding. */
 int ppf_unet_set_conv_kernel(ppf_unet *net, int kind);
 
 const char *ppf_unet_last_error(ppf_unet *net);

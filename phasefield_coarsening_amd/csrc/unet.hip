@@ -482,6 +482,246 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 }
 
 // ----------------------------------------------------------------------------
+// Decoder convolution over [nearest x2 upsampling of Xc | skip channels of X],
+// polyphase form (inference).  A fine site x = 2X + p (per dimension) sees
+// its neighbours x-1, x, x+1 of an upsampled tensor at the coarse offsets
+// floor((p + d) / 2): {-1, 0, 0} for p = 0, {0, 0, +1} for p = 1.  So for
+// each output parity (pz, py, px) the 27-tap convolution of the upsampled
+// channels is a 2x2x2 convolution of the coarse tensor, whose weights Wp
+// sum the fine taps that land on the same coarse offset (k_prep_wpoly, from
+// the FP32 weights, rounded to BF16 once): 8 taps instead of 27.  The skip
+// channels (cup .. Cin-1 of X) take the ordinary 27 taps.
+// A wave owns the 16 sites x = 32 xt + 2 l + px (l = lane % 16) of one
+// parity px, R rows (R even, y0 even: row r has py = r & 1) of one z plane,
+// and NJ*16 output channels.  Per 32 upsampled channels and (oz, ox) coarse
+// offset, the R/2 + 2 coarse rows serve all R rows: row r reads coarse rows
+// (r >> 1) + py + oy (oy = 0, 1) with the weights of (py, oy).  Output as
+// k_conv_rows MODE 0.  D, H, W even.
+// ----------------------------------------------------------------------------
+template <int R, int NJ, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_conv_poly(
+    const bf16 *__restrict__ X, int ldx, int Cin, const bf16 *__restrict__ Wt, int CoutW,
+    const bf16 *__restrict__ Wp, const float *__restrict__ bias, int Cout, bf16 *__restrict__ Yb,
+    int ldy, int B, int D, int H, int Wd, int nnb, int nxt2, int nyt, const bf16 *__restrict__ Xc,
+    int ldc, int cup)
+{
+    static_assert(R % 2 == 0, "rows come in parity pairs");
+    constexpr int RC = R / 2 + 2;  // coarse rows per wave
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    long g = (long)blockIdx.x * 4 + wave;
+    const int nb = (int)(g % nnb);
+    g /= nnb;
+    const int xt2 = (int)(g % nxt2);
+    g /= nxt2;
+    const int yt = (int)(g % nyt);
+    g /= nyt;
+    const int z = (int)(g % D);
+    const int b = (int)(g / D);
+    if (b >= B)
+        return;  // whole wave; the kernel has no barrier
+    const int px = xt2 & 1, xt = xt2 >> 1, pz = z & 1;
+    const int xl = lane & 15, q = lane >> 4;
+    const int n0 = nb * NJ * 16;
+    const int y0 = yt * R;
+    const int Dc = D / 2, Hc = H / 2, Wc = Wd / 2;
+    // skip channels: fine planes z-1 .. z+1, rows y0-1 .. y0+R, the lane's
+    // site x = 32 xt + 2 xl + px and its x neighbours
+    const int rowlen = Wd * ldx;
+    int xo[3];
+#pragma unroll
+    for (int dx = 0; dx < 3; dx++)
+        xo[dx] = (wrapm(32 * xt + 2 * xl + px + dx - 1, Wd) * ldx + 8 * q) * 2;
+    const int planebytes = H * rowlen * 2;
+    __amdgpu_buffer_rsrc_t pl[3];
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++)
+        pl[dz] = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(X + ((size_t)b * D + wrapm(z + dz - 1, D)) * H * (size_t)rowlen), (short)0,
+            planebytes, 0x00020000);
+    int ro[R + 2];
+#pragma unroll
+    for (int r = 0; r < R + 2; r++)
+        ro[r] = wrapm(y0 + r - 1, H) * rowlen;
+    // upsampled channels: coarse planes (z >> 1) + pz - 1 + oz, rows
+    // y0/2 - 1 + cr, x (16 xt + xl) + px - 1 + ox
+    const int rowlen_c = Wc * ldc;
+    int xoc[2];
+#pragma unroll
+    for (int ox = 0; ox < 2; ox++)
+        xoc[ox] = (wrapm(16 * xt + xl + px - 1 + ox, Wc) * ldc + 8 * q) * 2;
+    __amdgpu_buffer_rsrc_t plc[2];
+#pragma unroll
+    for (int oz = 0; oz < 2; oz++)
+        plc[oz] = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(Xc + ((size_t)b * Dc + wrapm((z >> 1) + pz - 1 + oz, Dc)) * Hc *
+                              (size_t)rowlen_c),
+            (short)0, Hc * rowlen_c * 2, 0x00020000);
+    int roc[RC];
+#pragma unroll
+    for (int cr = 0; cr < RC; cr++)
+        roc[cr] = wrapm(y0 / 2 - 1 + cr, Hc) * rowlen_c;
+    // weights: Wt[tap][n0+16j+xl][kc+8q] (skip), Wp[par][grp][n0+16j+xl][kc+8q]
+    // (upsampled, par = 4 pz + 2 py + px, grp = 4 oz + 2 oy + ox)
+    const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(Wt + (size_t)n0 * Cin), (short)0, (27 * CoutW - n0) * Cin * 2, 0x00020000);
+    const int wo = (xl * Cin + 8 * q) * 2;
+    const int tapstride = CoutW * Cin, jstride = 16 * Cin;
+    const __amdgpu_buffer_rsrc_t wpr = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)Wp, (short)0, 64 * CoutW * cup * 2, 0x00020000);
+    const int wpo = (xl * cup + 8 * q) * 2;
+    // accumulators start at the bias of the lane's four output channels
+    f32x4 acc[R][NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int c0 = n0 + 16 * j + 4 * q;
+        const f32x4 bj = c0 < Cout ? bias4(bias, c0) : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            acc[r][j] = bj;
+    }
+    // upsampled channels: per 32-channel chunk, four (oz, ox) groups, two
+    // register buffers (the next group's loads in flight during the MFMAs)
+    {
+        bf16x8 c0[RC], q0[2][2][NJ], c1[RC], q1[2][2][NJ];
+        auto ldp = [&](bf16x8(&c)[RC], bf16x8(&w)[2][2][NJ], int kc, int grp) {
+            const int oz = grp >> 1, ox = grp & 1;
+#pragma unroll
+            for (int cr = 0; cr < RC; cr++)
+                c[cr] = __builtin_bit_cast(
+                    bf16x8,
+                    __builtin_amdgcn_raw_buffer_load_b128(plc[oz], xoc[ox], (roc[cr] + kc) * 2, 0));
+#pragma unroll
+            for (int py = 0; py < 2; py++)
+#pragma unroll
+                for (int oy = 0; oy < 2; oy++)
+#pragma unroll
+                    for (int j = 0; j < NJ; j++)
+                        w[py][oy][j] = __builtin_bit_cast(
+                            bf16x8,
+                            __builtin_amdgcn_raw_buffer_load_b128(
+                                wpr, wpo,
+                                ((((4 * pz + 2 * py + px) * 8 + 4 * oz + 2 * oy + ox) * CoutW + n0 +
+                                  16 * j) * cup + kc) * 2,
+                                0));
+        };
+        auto mmp = [&](const bf16x8(&c)[RC], const bf16x8(&w)[2][2][NJ]) {
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int oy = 0; oy < 2; oy++)
+#pragma unroll
+                    for (int j = 0; j < NJ; j++)
+                        acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            w[r & 1][oy][j], c[(r >> 1) + (r & 1) + oy], acc[r][j], 0, 0, 0);
+        };
+        // past the last chunk the prefetch re-reads the last chunk (unused)
+        const int klast = cup - 32;
+        ldp(c0, q0, 0, 0);
+        for (int kc = 0; kc < cup; kc += 32) {
+            const int kn = min(kc + 32, klast);
+            ldp(c1, q1, kc, 1);
+            __builtin_amdgcn_sched_barrier(0);
+            mmp(c0, q0);
+            __builtin_amdgcn_sched_barrier(0);
+            ldp(c0, q0, kc, 2);
+            __builtin_amdgcn_sched_barrier(0);
+            mmp(c1, q1);
+            __builtin_amdgcn_sched_barrier(0);
+            ldp(c1, q1, kc, 3);
+            __builtin_amdgcn_sched_barrier(0);
+            mmp(c0, q0);
+            __builtin_amdgcn_sched_barrier(0);
+            ldp(c0, q0, kn, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            mmp(c1, q1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // skip channels: the 27 taps, as k_conv_rows (9 (dz, dx) groups per
+    // chunk, R + 2 rows serve the three dy taps)
+    if (cup < Cin) {
+        bf16x8 a0[R + 2], w0[3][NJ], a1[R + 2], w1[3][NJ];
+        auto ld = [&](bf16x8(&a)[R + 2], bf16x8(&w)[3][NJ], int kc, int grp) {
+            const int dz = grp / 3, dx = grp % 3;
+#pragma unroll
+            for (int r = 0; r < R + 2; r++)
+                a[r] = __builtin_bit_cast(
+                    bf16x8,
+                    __builtin_amdgcn_raw_buffer_load_b128(pl[dz], xo[dx], (ro[r] + kc) * 2, 0));
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+                for (int j = 0; j < NJ; j++)
+                    w[dy][j] = __builtin_bit_cast(
+                        bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
+                                    wr, wo, ((dz * 9 + dy * 3 + dx) * tapstride + j * jstride + kc) * 2,
+                                    0));
+        };
+        auto mm = [&](const bf16x8(&a)[R + 2], const bf16x8(&w)[3][NJ]) {
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+                for (int r = 0; r < R; r++)
+#pragma unroll
+                    for (int j = 0; j < NJ; j++)
+                        acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[dy][j], a[r + dy],
+                                                                            acc[r][j], 0, 0, 0);
+        };
+        const int klast = Cin - 32;
+        ld(a0, w0, cup, 0);
+        for (int kc = cup; kc < Cin; kc += 64) {
+            const int k1 = min(kc + 32, klast), k2 = min(kc + 64, klast);
+#pragma unroll
+            for (int grp = 0; grp < 9; grp++) {
+                if (grp % 2 == 0) {
+                    ld(a1, w1, grp < 8 ? kc : k1, grp < 8 ? grp + 1 : 0);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mm(a0, w0);
+                } else {
+                    ld(a0, w0, kc, grp + 1);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mm(a1, w1);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (kc + 32 >= Cin)
+                break;
+#pragma unroll
+            for (int grp = 0; grp < 9; grp++) {
+                if (grp % 2 == 0) {
+                    ld(a0, w0, grp < 8 ? k1 : k2, grp < 8 ? grp + 1 : 0);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mm(a1, w1);
+                } else {
+                    ld(a1, w1, k1, grp + 1);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mm(a0, w0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+    // lane holds channels n0+16j+4q .. +3 of site (32 xt + 2 xl + px, y0+r)
+    const int x = 32 * xt + 2 * xl + px;
+    if (x >= Wd)
+        return;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + r;
+        if (y >= H)
+            break;
+        const size_t vox = (((size_t)b * D + z) * H + y) * Wd + x;
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int c0 = n0 + 16 * j + 4 * q;
+            if (c0 < Cout)
+                *(bf16x4 *)(Yb + vox * ldy + c0) = leaky4(acc[r][j]);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
 // z-marching convolution for Cin = Cout = 32 (the level-1 layers), forward
 // (MODE 0).  A workgroup owns a 16 (x) x 32 (y) tile and marches a chunk of
 // z planes; wave w computes rows 8w .. 8w+7.  Each input plane is staged
@@ -726,6 +966,39 @@ __global__ __launch_bounds__(256) void k_prep_w0(const float *__restrict__ w, in
             v = w[((size_t)tap * Cout + co) * 3 + ci];
         }
         W0[i] = (bf16)v;
+    }
+}
+
+// k_conv_poly's weights for the first cup (upsampled) input channels of a
+// layer: Wp[par][grp][co][ci], par = 4 pz + 2 py + px, grp = 4 oz + 2 oy +
+// ox, is the sum of the FP32 weights w[tap][co][ci] over the taps (dz, dy,
+// dx) with floor((p + d) / 2) = p - 1 + o in every dimension, rounded to
+// BF16 once.
+__global__ __launch_bounds__(256) void k_prep_wpoly(const float *__restrict__ w, int Cout,
+                                                    int CinLog, int cup, bf16 *__restrict__ Wp)
+{
+    const size_t n = (size_t)64 * Cout * cup;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int ci = (int)(i % cup);
+        const int co = (int)((i / cup) % Cout);
+        const int grp = (int)((i / ((size_t)cup * Cout)) % 8);
+        const int par = (int)(i / ((size_t)8 * cup * Cout));
+        const int p[3] = {(par >> 2) & 1, (par >> 1) & 1, par & 1};
+        const int o[3] = {(grp >> 2) & 1, (grp >> 1) & 1, grp & 1};
+        float s = 0.f;
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int d[3] = {dz, dy, dx};
+                    bool hit = true;
+                    for (int a = 0; a < 3; a++)  // floor((p + d) / 2) - (p - 1) == o
+                        hit = hit && ((p[a] + d[a] + 2) / 2 - 1 - p[a] + 1 == o[a]);
+                    if (hit) {
+                        const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                        s += w[((size_t)tap * Cout + co) * CinLog + ci];
+                    }
+                }
+        Wp[i] = (bf16)s;
     }
 }
 
@@ -1173,6 +1446,8 @@ struct ppf_unet {
     float *p, *g, *m, *v;   // FP32 parameters, gradient, Adam moments
     bf16 *wb, *wt;          // BF16 weights and their flipped transposes
     bf16 *w0;               // first layer's weights for k_conv0, [5][Cout][32]
+    bf16 *wpoly;            // k_conv_poly's weights of L6 and L8, [8][8][Cout][cup] each
+    size_t wp_off6, wp_off8;
     int64_t t;              // Adam steps
     // workspace for one (batch, grid) shape
     int wsB, wsD, wsH, wsW;
@@ -1186,7 +1461,7 @@ struct ppf_unet {
     bool timing_pending;    // ev0/ev1 bracket an apply not yet read back
     hipEvent_t evc, evd;    // ordering with a stepping context's stream (ppf_unet_apply)
     double *tmp;            // solute sums of ppf_unet_apply, 2 * 1024 + 8 doubles
-    int conv_kind;  // 2-4: row-blocked variants, 1: LDS-tiled, 0: direct
+    int conv_kind;  // 2-6: row-blocked variants (6: polyphase decoder), 1: LDS, 0: direct
     char err[512];
 };
 
@@ -1309,6 +1584,15 @@ static int prep_weights(ppf_unet *n)
     }
     hipLaunchKernelGGL(k_prep_w0, dim3(grid_for((size_t)C0_CH * n->L[0].cout * 32)), dim3(256), 0,
                        n->stream, n->p + n->L[0].w_off, n->L[0].cout, n->w0);
+    const int C = n->base;
+    const int lay[2] = {6, 8}, cups[2] = {4 * C, 2 * C};
+    const size_t offs[2] = {n->wp_off6, n->wp_off8};
+    for (int k = 0; k < 2; k++) {
+        const Layer &l = n->L[lay[k]];
+        hipLaunchKernelGGL(k_prep_wpoly, dim3(grid_for((size_t)64 * l.cout * cups[k])), dim3(256),
+                           0, n->stream, n->p + l.w_off, l.cout, l.cin_log, cups[k],
+                           n->wpoly + offs[k]);
+    }
     UHIP(n, hipGetLastError());
     return PPF_OK;
 }
@@ -1318,10 +1602,29 @@ static int prep_weights(ppf_unet *n)
 static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int cin,
                  const bf16 *Wt, int coutw, const float *bias, int cout, bf16 *Yb, float *Yf,
                  int ldy, int yoff, int B, int D, int H, int W, const bf16 *Xc = nullptr,
-                 int ldc = 0, int cup = 0)
+                 int ldc = 0, int cup = 0, const bf16 *Wp = nullptr)
 {
     const size_t total = (size_t)B * D * H * W;
-    if ((n->conv_kind == 2 || n->conv_kind == 5) && mode == 0 && !Xc && cin == 32 &&
+    if (n->conv_kind == 6 && Xc && Wp && mode == 0 && xoff == 0 && coutw == cout &&
+        coutw % 32 == 0 && ldy % 4 == 0 && yoff == 0 && cup % 32 == 0 && cin % 32 == 0) {
+        // decoder layer over an upsampled tensor: polyphase form (8 x 16
+        // sites x (32 or 64) channels per wave)
+        const bool wide = coutw % 64 == 0;
+        const int NJ = wide ? 4 : 2, R = 8;
+        const int nnb = coutw / (NJ * 16), nxt2 = 2 * ((W + 31) / 32), nyt = (H + R - 1) / R;
+        const size_t waves = (size_t)B * D * nyt * nxt2 * nnb;
+        const dim3 g3((unsigned)((waves + 3) / 4));
+        if (wide)
+            hipLaunchKernelGGL((k_conv_poly<8, 4, 1>), g3, dim3(256), 0, n->stream, X, ldx, cin, Wt,
+                               coutw, Wp, bias, cout, Yb, ldy, B, D, H, W, nnb, nxt2, nyt, Xc, ldc,
+                               cup);
+        else
+            hipLaunchKernelGGL((k_conv_poly<8, 2, 2>), g3, dim3(256), 0, n->stream, X, ldx, cin, Wt,
+                               coutw, Wp, bias, cout, Yb, ldy, B, D, H, W, nnb, nxt2, nyt, Xc, ldc,
+                               cup);
+        return;
+    }
+    if ((n->conv_kind == 2 || n->conv_kind >= 5) && mode == 0 && !Xc && cin == 32 &&
         coutw == 32 && cout == 32) {
         // level-1 layers: the z-marching kernel (same sums as k_conv_rows)
         const int zc = 16, ntx = (W + ZM_TX - 1) / ZM_TX, nty = (H + ZM_TY - 1) / ZM_TY;
@@ -1333,7 +1636,7 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
     // (the row-blocked kernels store 4 channels at once: aligned when the
     // channel count, row stride and offset are multiples of 4)
     if (n->conv_kind >= 2 && coutw % 32 == 0 && cout % 4 == 0 && ldy % 4 == 0 && yoff % 4 == 0) {
-        const int kind = n->conv_kind == 5 ? 2 : n->conv_kind;  // 5: kind 2's tiles
+        const int kind = n->conv_kind >= 5 ? 2 : n->conv_kind;  // 5, 6: kind 2's tiles
         // (R, NJ): 8 rows x 2 N tiles; where the channel count is a multiple
         // of 64, 8 x 4 (kind 2, fastest measured: 18.0 ms at 256^3) or 4 x 4
         // (kind 4); kind 3 keeps 8 x 2 everywhere.  NJ*16 must divide coutw
@@ -1490,10 +1793,11 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
     // inference with the row-blocked kernels reads the upsampled channels
     // straight from the coarse tensor; training materializes them (the
     // weight gradients read the concatenated input)
-    const bool fuse_up = !need_x0 && (n->conv_kind >= 2 && n->conv_kind <= 4);
+    const bool fuse_up = !need_x0 && (n->conv_kind >= 2 && n->conv_kind != 5);
+    const bf16 *wp6 = n->wpoly + n->wp_off6, *wp8 = n->wpoly + n->wp_off8;
     if (fuse_up) {
         conv(n, 0, n->U2, 6 * C, 0, 6 * C, wb(6), 2 * C, bias(6), 2 * C, n->D2a, nullptr, 2 * C, 0,
-             B, D2, H2, W2, n->E3b, 4 * C, 4 * C);
+             B, D2, H2, W2, n->E3b, 4 * C, 4 * C, wp6);
     } else {
         hipLaunchKernelGGL(k_up_fwd, dim3(row_blocks(B * D2 * H2, W2 * 4 * C / 8)), dim3(256), 0,
                            n->stream, n->E3b, 4 * C, n->U2, 6 * C, 0, B, D2, H2, W2);
@@ -1504,7 +1808,7 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
          D2, H2, W2);
     if (fuse_up) {
         conv(n, 0, n->U1, 3 * C, 0, 3 * C, wb(8), C, bias(8), C, n->D1a, nullptr, C, 0, B, D, H, W,
-             n->D2b, 2 * C, 2 * C);
+             n->D2b, 2 * C, 2 * C, wp8);
     } else {
         hipLaunchKernelGGL(k_up_fwd, dim3(row_blocks(B * D * H, W * 2 * C / 8)), dim3(256), 0,
                            n->stream, n->D2b, 2 * C, n->U1, 3 * C, 0, B, D, H, W);
@@ -1683,7 +1987,7 @@ extern "C" int ppf_unet_destroy(ppf_unet *n)
     if (n->stream)
         hipStreamSynchronize(n->stream);
     free_ws(n);
-    void *bufs[] = {n->p, n->g, n->m, n->v, n->wb, n->wt, n->w0, n->tmp, n->part};
+    void *bufs[] = {n->p, n->g, n->m, n->v, n->wb, n->wt, n->w0, n->wpoly, n->tmp, n->part};
     for (void *b : bufs)
         if (b)
             hipFree(b);
@@ -1717,7 +2021,7 @@ extern "C" int ppf_unet_create(ppf_unet **out, int base, int device, uint64_t se
         return uerr(nullptr, PPF_EHIP, "hipSetDevice failed");
     ppf_unet *n = new ppf_unet();
     n->base = base;
-    n->conv_kind = 2;
+    n->conv_kind = 6;
     n->device = device;
     const int C = base;
     // (cin_log, cin padded, cout) of the ten 3x3x3 convolutions
@@ -1744,6 +2048,11 @@ extern "C" int ppf_unet_create(ppf_unet **out, int base, int device, uint64_t se
     off += 3;
     n->np = (int64_t)off;
     n->nwb = woff;
+    // polyphase weights of the fused-upsampling layers: L6 over the 4C
+    // channels of E3b, L8 over the 2C channels of D2b
+    n->wp_off6 = 0;
+    n->wp_off8 = (size_t)64 * n->L[6].cout * 4 * C;
+    const size_t nwp = n->wp_off8 + (size_t)64 * n->L[8].cout * 2 * C;
     // seeded init: He-uniform for the 3x3x3 layers (gain for leaky ReLU),
     // zero biases, zero head (the untrained network returns its input)
     std::vector<float> h(off, 0.0f);
@@ -1781,6 +2090,8 @@ extern "C" int ppf_unet_create(ppf_unet **out, int base, int device, uint64_t se
         e = hipMalloc(&n->wt, woff * sizeof(bf16));
     if (e == hipSuccess)
         e = hipMalloc(&n->w0, (size_t)C0_CH * C * 32 * sizeof(bf16));
+    if (e == hipSuccess)
+        e = hipMalloc(&n->wpoly, nwp * sizeof(bf16));
     if (e == hipSuccess)
         e = hipMalloc(&n->tmp, (2 * 1024 + 8) * sizeof(double));
     if (e == hipSuccess)
@@ -1852,6 +2163,11 @@ static double forward_flop(const ppf_unet *n, size_t V1)
         f += 2.0 * 27.0 * n->L[i].cin * n->L[i].cout * (double)vox[i];
     if (n->conv_kind >= 2)  // k_conv0: K = 160 instead of 27 x 32
         f -= 2.0 * (27.0 * n->L[0].cin - 32.0 * C0_CH) * n->L[0].cout * (double)vox[0];
+    if (n->conv_kind == 6) {  // k_conv_poly: 8 instead of 27 taps on the upsampled channels
+        const int C = n->base;
+        f -= 2.0 * (27.0 - 8.0) * (4.0 * C) * n->L[6].cout * (double)vox[6];
+        f -= 2.0 * (27.0 - 8.0) * (2.0 * C) * n->L[8].cout * (double)vox[8];
+    }
     return f;
 }
 
@@ -2029,8 +2345,8 @@ extern "C" int ppf_unet_last_forward(ppf_unet *n, double *ms, double *flop)
 
 extern "C" int ppf_unet_set_conv_kernel(ppf_unet *n, int kind)
 {
-    if (!n || kind < 0 || kind > 5)
-        return n ? uerr(n, PPF_EARG, "convolution kernel must be 0..5") : PPF_EARG;
+    if (!n || kind < 0 || kind > 6)
+        return n ? uerr(n, PPF_EARG, "convolution kernel must be 0..6") : PPF_EARG;
     n->conv_kind = kind;
     return PPF_OK;
 }

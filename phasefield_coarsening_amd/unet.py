@@ -152,8 +152,10 @@ class UNet:
         self._check(lib().ppf_unet_apply(self._h, solver._h))
 
     def set_conv_kernel(self, kind):
-        """2: row-blocked convolutions (default); 3, 4: row-blocked with fixed
-        tiles; 1: LDS-tiled; 0: the direct form."""
+        """2: row-blocked convolutions; 3, 4: row-blocked with fixed tiles;
+        5: kind 2 with materialized upsampling; 6 (default): kind 2 with the
+        decoder convolutions in polyphase form at inference; 1: LDS-tiled;
+        0: the direct form."""
         self._check(lib().ppf_unet_set_conv_kernel(self._h, int(kind)))
 
     def last_forward(self):

@@ -32,9 +32,10 @@ def _data(seed, shape=(2, 16, 16, 16)):
     return x, y
 
 
-# convolution kernels: 2 row-blocked (default), 3 and 4 its fixed-tile
-# variants, 5 kind 2 with materialized upsampling, 1 LDS-tiled, 0 direct
-KINDS = [2, 3, 4, 5, 1, 0]
+# convolution kernels: 2 row-blocked, 3 and 4 its fixed-tile variants, 5
+# kind 2 with materialized upsampling, 6 (default) kind 2 with polyphase
+# decoder convolutions at inference, 1 LDS-tiled, 0 direct
+KINDS = [2, 3, 4, 5, 6, 1, 0]
 
 
 @pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 24, 12, 40), (1, 8, 36, 20)])
@@ -64,10 +65,33 @@ def test_fused_upsampling_is_bitwise_the_materialized_one(shape):
     x, _ = _data(3, shape)
     with UNet(seed=3) as net:
         net.set_params(p)
+        net.set_conv_kernel(2)
         a = net.forward(x)
         net.set_conv_kernel(5)
         b = net.forward(x)
     np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("base", [32, 64])
+@pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 24, 12, 40)])
+def test_polyphase_decoder_close_to_kind2(shape, base):
+    """Kind 6 runs the two decoder convolutions over upsampled channels as
+    2x2x2 convolutions of the coarse tensor with summed weights.  It differs
+    from kind 2 only by rounding each summed weight to BF16 once instead of
+    each tap's weight: the outputs agree to a small fraction of the head's
+    scale, and most sites agree exactly after the output rounding."""
+    p = _params(4, base=base)
+    x, _ = _data(5, shape)
+    with UNet(base=base, seed=3) as net:
+        net.set_params(p)
+        net.set_conv_kernel(2)
+        a = net.forward(x)
+        net.set_conv_kernel(6)
+        b = net.forward(x)
+    scale = np.abs(a - x).max()
+    d = np.abs(a - b)
+    print("kind 6 vs 2: max |d| %.3e, mean %.3e, scale %.3e" % (d.max(), d.mean(), scale))
+    assert d.max() < 2e-2 * scale and d.mean() < 2e-3 * scale
 
 
 @pytest.mark.parametrize("kind", KINDS)
@@ -93,7 +117,7 @@ def test_forward_matches_reference(kind, shape):
     assert np.abs(got - unet_ref.predict(p, x, bf16=False)).mean() < 2e-2 * scale
 
 
-@pytest.mark.parametrize("kind", [2, 0])
+@pytest.mark.parametrize("kind", [2, 6, 0])
 def test_forward_base64_matches_reference(kind):
     """The 64-channel network (first-layer kernel with four N tiles, 64-wide
     row tiles at every level) against the reference."""
@@ -196,7 +220,7 @@ def test_apply_jumps_state_and_conserves_solute():
             assert ms > 0 and flop > 0
 
 
-@pytest.mark.parametrize("kind", [2, 0])
+@pytest.mark.parametrize("kind", [2, 6, 0])
 def test_apply_equals_the_host_forward(kind):
     """apply reads and writes the context's fields (row-blocked kernels) or
     goes through packed FP32 copies (the others); either way etaS and etaL
