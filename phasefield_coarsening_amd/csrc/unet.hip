@@ -844,6 +844,123 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 }
 
 // ----------------------------------------------------------------------------
+// z-marching convolution for Cin = 32 CK (CK = 1, 2) and Cout = 64, forward
+// (MODE 0): the 128^3-level layers of the base-32 network (L2, L3, L7) and
+// the level-1 layers of the base-64 one.  A workgroup owns a 16 x 16 tile
+// and marches a chunk of z planes; wave w owns output channels 16w .. 16w+15
+// of all 16 rows, with its 27 CK weight fragments in registers (one wave per
+// SIMD).  Each input plane (tile + one-site halo, 18 x 18 sites, channels
+// padded to 32 CK + 8 so that the 16-byte reads of 16 consecutive sites hit
+// distinct banks) is staged once in LDS for the three output planes that
+// need it.  Per output plane and 32-channel chunk a wave runs 9 (dz, dx)
+// groups of 18 LDS fragment reads and 48 MFMAs, in k_conv_rows' order
+// (chunk, dz, dx, dy), so the sums are bitwise those of k_conv_rows.
+// ----------------------------------------------------------------------------
+constexpr int Z64_T = 16, Z64_H = Z64_T + 2, Z64_SITES = Z64_H * Z64_H;  // 324 sites
+
+template <int CK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_conv_z64(
+    const bf16 *__restrict__ X, int ldx, int xoff, const bf16 *__restrict__ Wt,
+    const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy, int yoff, int B, int D, int H,
+    int Wd, int zc, int ntx, int nty)
+{
+    constexpr int CIN = 32 * CK, PAD = CIN + 8;
+    constexpr int PARTS = CIN / 8;                        // 16-byte chunks per site
+    constexpr int CHUNKS = Z64_SITES * PARTS;             // per plane
+    constexpr int PER_THREAD = (CHUNKS + 255) / 256;
+    __shared__ __attribute__((aligned(16))) bf16 zsh[3 * Z64_SITES * PAD];  // 140 KB at CK 2
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int r = blockIdx.x;
+    const int tx = r % ntx;
+    r /= ntx;
+    const int ty = r % nty;
+    r /= nty;
+    const int nzc = (D + zc - 1) / zc;
+    const int zci = r % nzc, b = r / nzc;
+    const int x0 = tx * Z64_T, y0 = ty * Z64_T, z0 = zci * zc, zend = min(z0 + zc, D);
+    const int xl = lane & 15, q = lane >> 4;
+    // weights of this wave's 16 channels: w[tap][k] = Wt[tap][16 wave + xl][32k + 8q ..]
+    bf16x8 w[27][CK];
+#pragma unroll
+    for (int tap = 0; tap < 27; tap++)
+#pragma unroll
+        for (int k = 0; k < CK; k++)
+            w[tap][k] = *(const bf16x8 *)(Wt + ((size_t)tap * 64 + 16 * wave + xl) * CIN + 32 * k + 8 * q);
+    auto plane_src = [&](int i, int zz) -> const bf16 * {
+        const int c = tid + 256 * i, s = c / PARTS, part = c % PARTS;
+        const int hy = s / Z64_H, hx = s - hy * Z64_H;
+        const size_t v = (((size_t)b * D + wrapm(zz, D)) * H + wrapm(y0 + hy - 1, H)) * Wd +
+                         wrapm(x0 + hx - 1, Wd);
+        return X + v * ldx + xoff + 8 * part;
+    };
+    const f32x4 bias0 = bias4(bias, 16 * wave + 4 * q);
+    bf16x8 pf[PER_THREAD];
+    auto load_plane = [&](int zz) {
+#pragma unroll
+        for (int i = 0; i < PER_THREAD; i++)
+            if (tid + 256 * i < CHUNKS)
+                pf[i] = *(const bf16x8 *)plane_src(i, zz);
+    };
+    auto store_plane = [&](int slot) {
+#pragma unroll
+        for (int i = 0; i < PER_THREAD; i++) {
+            const int c = tid + 256 * i;
+            if (c < CHUNKS)
+                *(bf16x8 *)(zsh + ((size_t)slot * Z64_SITES + c / PARTS) * PAD + 8 * (c % PARTS)) = pf[i];
+        }
+    };
+    load_plane(z0 - 1);
+    store_plane(0);
+    load_plane(z0);
+    store_plane(1);
+    load_plane(z0 + 1);
+    for (int z = z0; z < zend; z++) {
+        const int s_m = (z - z0) % 3, s_c = (z - z0 + 1) % 3, s_p = (z - z0 + 2) % 3;
+        __syncthreads();  // the slot of plane z-2 is no longer read
+        store_plane(s_p);  // plane z+1
+        __syncthreads();
+        if (z + 1 < zend)
+            load_plane(z + 2);  // in flight during this plane's MFMAs
+        f32x4 acc[Z64_T];
+#pragma unroll
+        for (int rr = 0; rr < Z64_T; rr++)
+            acc[rr] = bias0;
+        const int slots[3] = {s_m, s_c, s_p};
+#pragma unroll
+        for (int k = 0; k < CK; k++)
+#pragma unroll
+            for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++) {
+                    bf16x8 a[Z64_H];
+                    const bf16 *base = zsh + ((size_t)slots[dz] * Z64_SITES + xl + dx) * PAD + 32 * k + 8 * q;
+#pragma unroll
+                    for (int rr = 0; rr < Z64_H; rr++)
+                        a[rr] = *(const bf16x8 *)(base + rr * Z64_H * PAD);
+#pragma unroll
+                    for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+                        for (int rr = 0; rr < Z64_T; rr++)
+                            acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                w[dz * 9 + dy * 3 + dx][k], a[rr + dy], acc[rr], 0, 0, 0);
+                }
+        // lane holds channels 16 wave + 4q .. +3 of site (x0+xl, y0+rr) in acc[rr]
+        const int x = x0 + xl;
+        if (x < Wd) {
+#pragma unroll
+            for (int rr = 0; rr < Z64_T; rr++) {
+                const int y = y0 + rr;
+                if (y >= H)
+                    break;
+                const size_t o = ((((size_t)b * D + z) * H + y) * Wd + x) * ldy + yoff + 16 * wave + 4 * q;
+                *(bf16x4 *)(Y + o) = leaky4(acc[rr]);
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
 // First layer (3 input channels) without padding each tap to 32 channels.
 // K runs over the 9 (dz, dy) groups x 16 slots; slots 0..8 are (dx, c) of
 // the 3 taps dx and 3 channels (contiguous in an x row of the staged input),
@@ -1622,6 +1739,22 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
             hipLaunchKernelGGL((k_conv_poly<8, 2, 2>), g3, dim3(256), 0, n->stream, X, ldx, cin, Wt,
                                coutw, Wp, bias, cout, Yb, ldy, B, D, H, W, nnb, nxt2, nyt, Xc, ldc,
                                cup);
+        return;
+    }
+    if ((n->conv_kind == 2 || n->conv_kind >= 5) && mode == 0 && !Xc &&
+        (cin == 32 || cin == 64) && coutw == 64 && cout == 64 && ldy % 4 == 0 && yoff % 4 == 0) {
+        // Cout 64: the 16 x 16 z-marching kernel (same sums as k_conv_rows);
+        // z chunks of 8 planes at most, so that a launch has at least ~4
+        // workgroups per CU
+        const int ntx = (W + Z64_T - 1) / Z64_T, nty = (H + Z64_T - 1) / Z64_T;
+        const int zc = D < 8 ? D : 8;
+        const unsigned nblk = (unsigned)((size_t)B * ((D + zc - 1) / zc) * nty * ntx);
+        if (cin == 32)
+            hipLaunchKernelGGL(k_conv_z64<1>, dim3(nblk), dim3(256), 0, n->stream, X, ldx, xoff, Wt,
+                               bias, Yb, ldy, yoff, B, D, H, W, zc, ntx, nty);
+        else
+            hipLaunchKernelGGL(k_conv_z64<2>, dim3(nblk), dim3(256), 0, n->stream, X, ldx, xoff, Wt,
+                               bias, Yb, ldy, yoff, B, D, H, W, zc, ntx, nty);
         return;
     }
     if ((n->conv_kind == 2 || n->conv_kind >= 5) && mode == 0 && !Xc && cin == 32 &&
