@@ -91,7 +91,11 @@ int ppf_unet_last_forward(ppf_unet *net, double *ms, double *flop);
  *     nearest-upsampled channels is a 2x2x2 convolution of the coarse
  *     tensor with summed weights (8 taps instead of 27); the summed weights
  *     are rounded to BF16 once, so kind 6 agrees with kind 2 to BF16 weight
- *     rounding, not bitwise; training runs as kind 2;
+ *     rounding, not bitwise; training runs as kind 2; the last decoder
+ *     convolution of the base-32 network runs z-marching, with its fine and
+ *     coarse inputs staged in LDS;
+ *   7 kind 6 with the row-blocked polyphase kernel for both decoder
+ *     convolutions (the same sums as kind 6, for cross-checks);
  *   1 LDS-tiled: a 16 x 8 brick's halo staged in LDS;
  *   0 direct: every operand straight from global memory.
  * Kinds 2-5 add the same (tap, 32-channel) products in the same order:

@@ -605,11 +605,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
                                   16 * j) * cup + kc) * 2,
                                 0));
         };
+        // oy outermost: each accumulator still takes oy = 0 then 1, and R*NJ
+        // independent MFMAs separate the two
         auto mmp = [&](const bf16x8(&c)[RC], const bf16x8(&w)[2][2][NJ]) {
 #pragma unroll
-            for (int r = 0; r < R; r++)
+            for (int oy = 0; oy < 2; oy++)
 #pragma unroll
-                for (int oy = 0; oy < 2; oy++)
+                for (int r = 0; r < R; r++)
 #pragma unroll
                     for (int j = 0; j < NJ; j++)
                         acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
@@ -721,18 +723,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     }
 }
 
+// LDS layout of the staged planes.  A fragment read (ds_read_b128) has lane
+// (l % 16, l / 16) read 16 bytes at site l % 16 of a row, channel quad
+// l / 16.  Measured on gfx950 (bank-conflict counter over site strides of
+// 64 .. 272 bytes): it is conflict-free exactly when the site stride, in
+// 16-byte units, is 2 mod 4 (96 and 160 bytes: none; 80, 112, 144, 176, 208,
+// 240, 272: 4 extra cycles per read; 128: 12), i.e. the lanes of sites
+// 0..7 and two channel quads are served together.  So staged sites are
+// padded to 48 bf16 (32 channels) or 80 bf16 (64 channels) where the LDS
+// allows it; k_conv_z32 keeps 40 (its 34 x 18 planes would not fit three
+// times at 48, and a shorter tile was slower than the conflicts).
+constexpr int LDS_PAD32 = 48, LDS_PAD64 = 80;
+
 // ----------------------------------------------------------------------------
 // z-marching convolution for Cin = Cout = 32 (the level-1 layers), forward
 // (MODE 0).  A workgroup owns a 16 (x) x 32 (y) tile and marches a chunk of
 // z planes; wave w computes rows 8w .. 8w+7.  Each input plane is staged
 // once in LDS (tile + one-site halo, 34 x 18 sites, 32 channels padded to
-// 40 so that the 16-byte fragment reads of 8 consecutive sites hit distinct
-// banks) and serves the three output planes that need it; three slots hold
+// 40) and serves the three output planes that need it; three slots hold
 // planes z-1, z, z+1 while the next plane is loaded into registers.  All 27
 // taps' weights (27 x 2 fragments) stay in registers.  Per output plane a
 // wave runs 9 (dz, dx) groups of 10 LDS fragment reads and 48 MFMAs.
 // ----------------------------------------------------------------------------
-constexpr int ZM_TX = 16, ZM_TY = 32, ZM_HX = ZM_TX + 2, ZM_HY = ZM_TY + 2;
+constexpr int ZM_TX = 16, ZM_RW = 8, ZM_TY = 4 * ZM_RW, ZM_HX = ZM_TX + 2, ZM_HY = ZM_TY + 2;
 constexpr int ZM_SITES = ZM_HX * ZM_HY;          // 612 per plane
 constexpr int ZM_PAD = 40;                       // bf16 per staged site
 constexpr int ZM_CHUNKS = ZM_SITES * 4;          // 16-byte chunks per plane
@@ -801,9 +814,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         __syncthreads();
         if (z + 1 < zend)
             load_plane(z + 2);  // in flight during this plane's MFMAs
-        f32x4 acc[8][2];
+        f32x4 acc[ZM_RW][2];
 #pragma unroll
-        for (int rr = 0; rr < 8; rr++) {
+        for (int rr = 0; rr < ZM_RW; rr++) {
             acc[rr][0] = bias0;
             acc[rr][1] = bias1;
         }
@@ -812,16 +825,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         for (int dz = 0; dz < 3; dz++)
 #pragma unroll
             for (int dx = 0; dx < 3; dx++) {
-                bf16x8 a[10];
-                const bf16 *base = zsh + ((size_t)slots[dz] * ZM_SITES + (8 * wave) * ZM_HX + xl + dx) *
+                bf16x8 a[ZM_RW + 2];
+                const bf16 *base = zsh + ((size_t)slots[dz] * ZM_SITES + (ZM_RW * wave) * ZM_HX + xl + dx) *
                                              ZM_PAD + 8 * q;
 #pragma unroll
-                for (int rr = 0; rr < 10; rr++)
+                for (int rr = 0; rr < ZM_RW + 2; rr++)
                     a[rr] = *(const bf16x8 *)(base + rr * ZM_HX * ZM_PAD);
 #pragma unroll
                 for (int dy = 0; dy < 3; dy++)
 #pragma unroll
-                    for (int rr = 0; rr < 8; rr++)
+                    for (int rr = 0; rr < ZM_RW; rr++)
 #pragma unroll
                         for (int j = 0; j < 2; j++)
                             acc[rr][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
@@ -831,8 +844,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         const int x = x0 + xl;
         if (x < Wd) {
 #pragma unroll
-            for (int rr = 0; rr < 8; rr++) {
-                const int y = y0 + 8 * wave + rr;
+            for (int rr = 0; rr < ZM_RW; rr++) {
+                const int y = y0 + ZM_RW * wave + rr;
                 if (y >= H)
                     break;
                 const size_t o = ((((size_t)b * D + z) * H + y) * Wd + x) * ldy + yoff + 4 * q;
@@ -844,15 +857,246 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 }
 
 // ----------------------------------------------------------------------------
+// k_conv_poly's sums (upsampled channels in polyphase form, then the 27 taps
+// of the skip channels, in the same order: bitwise the same output), for
+// Cout = 32, 32 skip channels and 64 upsampled channels (L8 of the base-32
+// network), z-marching with both sources staged in LDS.  A workgroup owns a
+// fine 32 (x) x 8 (y) tile and marches a chunk of z planes; wave w computes
+// x parity px = w & 1 (sites x0 + 2 l + px) and output channels 16 (w >> 1)
+// .. +15 of all 8 rows, with its 27 skip-tap weights in registers and the
+// summed weights read through the cache.  LDS holds three fine planes of the
+// skip channels (tile + one-site halo, 10 rows x 34 sites, stored by x
+// parity so that a wave's lanes read consecutive sites; 32 channels padded
+// to 48) and three coarse planes of the upsampled source (6 rows x 18
+// sites, 64 channels padded to 80).  A fine plane z reads coarse planes
+// (z >> 1) + (z & 1) - 1 and the next one, so a new coarse plane is staged
+// every second fine plane.  D, H, W even; chunks start at even z.
+// ----------------------------------------------------------------------------
+constexpr int ZP_TX = 32, ZP_TY = 8;
+constexpr int ZP_FX = 17, ZP_FROWS = ZP_TY + 2, ZP_FPAD = LDS_PAD32;
+constexpr int ZP_FSITES = ZP_FROWS * 2 * ZP_FX;  // 340 per fine plane
+constexpr int ZP_CX = ZP_TX / 2 + 2, ZP_CROWS = ZP_TY / 2 + 2, ZP_CPAD = LDS_PAD64;
+constexpr int ZP_CSITES = ZP_CROWS * ZP_CX;      // 108 per coarse plane
+constexpr int ZP_FCH = ZP_FSITES * 4, ZP_FPT = (ZP_FCH + 255) / 256;  // 16-byte chunks
+constexpr int ZP_CCH = ZP_CSITES * 8, ZP_CPT = (ZP_CCH + 255) / 256;
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_conv_zpoly(
+    const bf16 *__restrict__ X, int ldx, int xoff, const bf16 *__restrict__ Wt,
+    const bf16 *__restrict__ Wp, const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy,
+    int B, int D, int H, int Wd, const bf16 *__restrict__ Xc, int ldc, int zc, int ntx, int nty)
+{
+    __shared__ __attribute__((aligned(16))) bf16 fsh[3 * ZP_FSITES * ZP_FPAD];  // 97,920 bytes
+    __shared__ __attribute__((aligned(16))) bf16 csh[3 * ZP_CSITES * ZP_CPAD];  // 51,840 bytes
+    constexpr int CIN = 96, CUP = 64, COUT = 32;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int px = wave & 1, nt = wave >> 1;
+    int r = blockIdx.x;
+    const int tx = r % ntx;
+    r /= ntx;
+    const int ty = r % nty;
+    r /= nty;
+    const int nzc = (D + zc - 1) / zc;
+    const int zci = r % nzc, b = r / nzc;
+    const int x0 = tx * ZP_TX, y0 = ty * ZP_TY, z0 = zci * zc, zend = min(z0 + zc, D);
+    const int Dc = D / 2, Hc = H / 2, Wc = Wd / 2, Z0 = z0 / 2;
+    const int xl = lane & 15, q = lane >> 4;
+    // skip-tap weights of this wave's channels: ws[tap] = Wt[tap][16 nt + xl][64 + 8q ..]
+    bf16x8 ws[27];
+#pragma unroll
+    for (int tap = 0; tap < 27; tap++)
+        ws[tap] = *(const bf16x8 *)(Wt + ((size_t)tap * COUT + 16 * nt + xl) * CIN + CUP + 8 * q);
+    const __amdgpu_buffer_rsrc_t wpr = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)Wp, (short)0, 64 * COUT * CUP * 2, 0x00020000);
+    const int wpo = ((16 * nt + xl) * CUP + 8 * q) * 2;
+    const f32x4 bias0 = bias4(bias, 16 * nt + 4 * q);
+    // staging: fine chunk c -> site s = c / 4 (LDS order: row, parity, index),
+    // part c % 4; coarse chunk c -> site c / 8, part c % 8.  Every thread
+    // issues the same loads (chunks past the end re-read the last one and
+    // are not stored), so the compiler can count them in its waits.
+    bf16x8 pf[ZP_FPT], pc[ZP_CPT];
+    auto load_fine = [&](int zz) {
+#pragma unroll
+        for (int i = 0; i < ZP_FPT; i++) {
+            const int c = min(tid + 256 * i, ZP_FCH - 1);
+            {
+                const int s = c >> 2, part = c & 3;
+                const int fr = s / (2 * ZP_FX), rem = s - fr * 2 * ZP_FX;
+                const int par = rem / ZP_FX, idx = rem - par * ZP_FX;
+                const int lx = 2 * idx + par;
+                const size_t v = (((size_t)b * D + wrapm(zz, D)) * H + wrapm(y0 + fr - 1, H)) * Wd +
+                                 wrapm(x0 + lx - 1, Wd);
+                pf[i] = *(const bf16x8 *)(X + v * ldx + xoff + 8 * part);
+            }
+        }
+    };
+    auto store_fine = [&](int slot) {
+#pragma unroll
+        for (int i = 0; i < ZP_FPT; i++) {
+            const int c = tid + 256 * i;
+            if (c < ZP_FCH)
+                *(bf16x8 *)(fsh + ((size_t)slot * ZP_FSITES + (c >> 2)) * ZP_FPAD + 8 * (c & 3)) = pf[i];
+        }
+    };
+    auto load_coarse = [&](int zz) {
+#pragma unroll
+        for (int i = 0; i < ZP_CPT; i++) {
+            const int c = min(tid + 256 * i, ZP_CCH - 1);
+            {
+                const int s = c >> 3, part = c & 7;
+                const int cr = s / ZP_CX, cx = s - cr * ZP_CX;
+                const size_t v = (((size_t)b * Dc + wrapm(zz, Dc)) * Hc + wrapm(y0 / 2 + cr - 1, Hc)) *
+                                     Wc + wrapm(x0 / 2 + cx - 1, Wc);
+                pc[i] = *(const bf16x8 *)(Xc + v * ldc + 8 * part);
+            }
+        }
+    };
+    auto store_coarse = [&](int slot) {
+#pragma unroll
+        for (int i = 0; i < ZP_CPT; i++) {
+            const int c = tid + 256 * i;
+            if (c < ZP_CCH)
+                *(bf16x8 *)(csh + ((size_t)slot * ZP_CSITES + (c >> 3)) * ZP_CPAD + 8 * (c & 7)) = pc[i];
+        }
+    };
+    // fine planes z0-1, z0 staged and z0+1 in registers; coarse planes
+    // Z0-1, Z0 staged (coarse plane P lives in slot (P - Z0 + 1) % 3)
+    load_fine(z0 - 1);
+    store_fine(0);
+    load_fine(z0);
+    store_fine(1);
+    load_coarse(Z0 - 1);
+    store_coarse(0);
+    load_coarse(Z0);
+    store_coarse(1);
+    load_fine(z0 + 1);
+    // a plane's summed weights (32 fragments, parity pz), loaded one plane
+    // ahead, after that plane's output stores: vmcnt completes in issue
+    // order, so loads issued after the plane prefetch would make their
+    // first use wait for the prefetch as well
+    bf16x8 wpl[2][4][2][2];
+    auto load_wpl = [&](int pzz) {
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+            for (int grp = 0; grp < 4; grp++)
+#pragma unroll
+                for (int py = 0; py < 2; py++)
+#pragma unroll
+                    for (int oy = 0; oy < 2; oy++)
+                        wpl[k][grp][py][oy] = __builtin_bit_cast(
+                            bf16x8,
+                            __builtin_amdgcn_raw_buffer_load_b128(
+                                wpr, wpo,
+                                (((4 * pzz + 2 * py + px) * 8 + 4 * (grp >> 1) + 2 * oy + (grp & 1)) *
+                                     COUT * CUP + 32 * k) * 2,
+                                0));
+    };
+    load_wpl(z0 & 1);
+    for (int z = z0; z < zend; z++) {
+        const int pz = z & 1;
+        __syncthreads();  // every read of the previous plane is done
+        store_fine((z - z0 + 2) % 3);  // fine plane z+1
+        if (pz)
+            store_coarse(((z >> 1) + 1 - Z0 + 1) % 3);  // coarse plane (z >> 1) + 1
+        __syncthreads();
+        // fine plane z+2, and the coarse plane the next odd plane stores
+        // ((z >> 1) + 1 for even z; odd z loads the plane the following even
+        // one loads again); past the chunk's end these are wrapped planes,
+        // read and not used
+        load_fine(z + 2);
+        load_coarse(((z + 1) >> 1) + 1);
+        // keep the prefetch ahead of the MFMAs (the scheduler would sink the
+        // loads to their stores, and the next plane would wait for them)
+        __builtin_amdgcn_sched_barrier(0);
+        f32x4 acc[ZP_TY];
+#pragma unroll
+        for (int rr = 0; rr < ZP_TY; rr++)
+            acc[rr] = bias0;
+        // 17 groups in k_conv_poly's order: 8 of the upsampled channels
+        // ((chunk, oz, ox): oy, rows), then 9 of the skip channels ((dz, dx):
+        // dy, rows).  Two register buffers: each group's LDS reads are issued
+        // before the previous group's MFMAs (scheduling fences keep them there).
+        const int fsl[3] = {(z - z0) % 3, (z - z0 + 1) % 3, (z - z0 + 2) % 3};
+        auto ldg = [&](bf16x8(&buf)[ZP_FROWS], int g) {
+            if (g < 8) {
+                const int kc = 32 * (g >> 2), oz = (g >> 1) & 1, ox = g & 1;
+                const int cs = ((z >> 1) + pz - 1 + oz - Z0 + 1) % 3;
+                const bf16 *cb = csh + ((size_t)cs * ZP_CSITES + xl + px + ox) * ZP_CPAD + kc + 8 * q;
+#pragma unroll
+                for (int cr = 0; cr < ZP_CROWS; cr++)
+                    buf[cr] = *(const bf16x8 *)(cb + cr * ZP_CX * ZP_CPAD);
+            } else {
+                const int dz = (g - 8) / 3, dx = (g - 8) % 3;
+                // lane's site: local x 2 xl + px + dx, stored at parity
+                // (px + dx) & 1, index xl + ((px + dx) >> 1)
+                const int par = (px + dx) & 1, idx = xl + ((px + dx) >> 1);
+                const bf16 *fb = fsh + ((size_t)fsl[dz] * ZP_FSITES + par * ZP_FX + idx) * ZP_FPAD + 8 * q;
+#pragma unroll
+                for (int rr = 0; rr < ZP_FROWS; rr++)
+                    buf[rr] = *(const bf16x8 *)(fb + rr * 2 * ZP_FX * ZP_FPAD);
+            }
+        };
+        auto mmg = [&](const bf16x8(&buf)[ZP_FROWS], int g) {
+            if (g < 8) {
+                const int k = g >> 2, grp = g & 3;
+#pragma unroll
+                for (int oy = 0; oy < 2; oy++)
+#pragma unroll
+                    for (int rr = 0; rr < ZP_TY; rr++)
+                        acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            wpl[k][grp][rr & 1][oy], buf[(rr >> 1) + (rr & 1) + oy], acc[rr], 0, 0, 0);
+            } else {
+                const int dz = (g - 8) / 3, dx = (g - 8) % 3;
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+                    for (int rr = 0; rr < ZP_TY; rr++)
+                        acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            ws[dz * 9 + dy * 3 + dx], buf[rr + dy], acc[rr], 0, 0, 0);
+            }
+        };
+        bf16x8 b0[ZP_FROWS], b1[ZP_FROWS];
+        ldg(b0, 0);
+#pragma unroll
+        for (int g = 0; g < 17; g++) {
+            if (g % 2 == 0) {
+                if (g + 1 < 17)
+                    ldg(b1, g + 1);
+                __builtin_amdgcn_sched_barrier(0);
+                mmg(b0, g);
+            } else {
+                ldg(b0, g + 1);
+                __builtin_amdgcn_sched_barrier(0);
+                mmg(b1, g);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // lane holds channels 16 nt + 4q .. +3 of site (x0 + 2 xl + px, y0 + rr)
+        const int x = x0 + 2 * xl + px;
+        if (x < Wd) {
+#pragma unroll
+            for (int rr = 0; rr < ZP_TY; rr++) {
+                const int y = y0 + rr;
+                if (y >= H)
+                    break;
+                const size_t o = ((((size_t)b * D + z) * H + y) * Wd + x) * ldy + 16 * nt + 4 * q;
+                *(bf16x4 *)(Y + o) = leaky4(acc[rr]);
+            }
+        }
+        load_wpl(pz ^ 1);  // the next plane's (read and not used after the last)
+    }
+}
+
+// ----------------------------------------------------------------------------
 // z-marching convolution for Cin = 32 CK (CK = 1, 2) and Cout = 64, forward
 // (MODE 0): the 128^3-level layers of the base-32 network (L2, L3, L7) and
 // the level-1 layers of the base-64 one.  A workgroup owns a 16 x 16 tile
 // and marches a chunk of z planes; wave w owns output channels 16w .. 16w+15
 // of all 16 rows, with its 27 CK weight fragments in registers (one wave per
 // SIMD).  Each input plane (tile + one-site halo, 18 x 18 sites, channels
-// padded to 32 CK + 8 so that the 16-byte reads of 16 consecutive sites hit
-// distinct banks) is staged once in LDS for the three output planes that
-// need it.  Per output plane and 32-channel chunk a wave runs 9 (dz, dx)
+// padded to 48 or 80, LDS_PAD32 / LDS_PAD64) is staged once in LDS for the
+// three output planes that need it.  Per output plane and 32-channel chunk a wave runs 9 (dz, dx)
 // groups of 18 LDS fragment reads and 48 MFMAs, in k_conv_rows' order
 // (chunk, dz, dx, dy), so the sums are bitwise those of k_conv_rows.
 // ----------------------------------------------------------------------------
@@ -864,11 +1108,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy, int yoff, int B, int D, int H,
     int Wd, int zc, int ntx, int nty)
 {
-    constexpr int CIN = 32 * CK, PAD = CIN + 8;
+    constexpr int CIN = 32 * CK, PAD = CK == 1 ? LDS_PAD32 : LDS_PAD64;
     constexpr int PARTS = CIN / 8;                        // 16-byte chunks per site
     constexpr int CHUNKS = Z64_SITES * PARTS;             // per plane
     constexpr int PER_THREAD = (CHUNKS + 255) / 256;
-    __shared__ __attribute__((aligned(16))) bf16 zsh[3 * Z64_SITES * PAD];  // 140 KB at CK 2
+    __shared__ __attribute__((aligned(16))) bf16 zsh[3 * Z64_SITES * PAD];  // 155,520 bytes at CK 2
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     int r = blockIdx.x;
@@ -1578,7 +1822,7 @@ struct ppf_unet {
     bool timing_pending;    // ev0/ev1 bracket an apply not yet read back
     hipEvent_t evc, evd;    // ordering with a stepping context's stream (ppf_unet_apply)
     double *tmp;            // solute sums of ppf_unet_apply, 2 * 1024 + 8 doubles
-    int conv_kind;  // 2-6: row-blocked variants (6: polyphase decoder), 1: LDS, 0: direct
+    int conv_kind;  // 2-7: row-blocked / z-marching variants (6, 7: polyphase decoder), 1: LDS, 0: direct
     char err[512];
 };
 
@@ -1722,7 +1966,18 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
                  int ldc = 0, int cup = 0, const bf16 *Wp = nullptr)
 {
     const size_t total = (size_t)B * D * H * W;
-    if (n->conv_kind == 6 && Xc && Wp && mode == 0 && xoff == 0 && coutw == cout &&
+    if (n->conv_kind == 6 && Xc && Wp && mode == 0 && xoff == 0 && cin == 96 && ldx == 96 &&
+        cup == 64 && ldc == 64 && coutw == 32 && cout == 32 && ldy % 4 == 0 && yoff == 0 &&
+        D % 2 == 0 && H % 2 == 0 && W % 2 == 0) {
+        // L8 of the base-32 network: the z-marching polyphase kernel (the
+        // same sums as k_conv_poly), z chunks of 16 planes (even)
+        const int zc = D < 16 ? D : 16, ntx = (W + ZP_TX - 1) / ZP_TX, nty = (H + ZP_TY - 1) / ZP_TY;
+        const unsigned nblk = (unsigned)((size_t)B * ((D + zc - 1) / zc) * nty * ntx);
+        hipLaunchKernelGGL(k_conv_zpoly, dim3(nblk), dim3(256), 0, n->stream, X, ldx, cup, Wt, Wp,
+                           bias, Yb, ldy, B, D, H, W, Xc, ldc, zc, ntx, nty);
+        return;
+    }
+    if ((n->conv_kind == 6 || n->conv_kind == 7) && Xc && Wp && mode == 0 && xoff == 0 && coutw == cout &&
         coutw % 32 == 0 && ldy % 4 == 0 && yoff == 0 && cup % 32 == 0 && cin % 32 == 0) {
         // decoder layer over an upsampled tensor: polyphase form (8 x 16
         // sites x (32 or 64) channels per wave)
@@ -1769,7 +2024,7 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
     // (the row-blocked kernels store 4 channels at once: aligned when the
     // channel count, row stride and offset are multiples of 4)
     if (n->conv_kind >= 2 && coutw % 32 == 0 && cout % 4 == 0 && ldy % 4 == 0 && yoff % 4 == 0) {
-        const int kind = n->conv_kind >= 5 ? 2 : n->conv_kind;  // 5, 6: kind 2's tiles
+        const int kind = n->conv_kind >= 5 ? 2 : n->conv_kind;  // 5-7: kind 2's tiles
         // (R, NJ): 8 rows x 2 N tiles; where the channel count is a multiple
         // of 64, 8 x 4 (kind 2, fastest measured: 18.0 ms at 256^3) or 4 x 4
         // (kind 4); kind 3 keeps 8 x 2 everywhere.  NJ*16 must divide coutw
@@ -2296,7 +2551,7 @@ static double forward_flop(const ppf_unet *n, size_t V1)
         f += 2.0 * 27.0 * n->L[i].cin * n->L[i].cout * (double)vox[i];
     if (n->conv_kind >= 2)  // k_conv0: K = 160 instead of 27 x 32
         f -= 2.0 * (27.0 * n->L[0].cin - 32.0 * C0_CH) * n->L[0].cout * (double)vox[0];
-    if (n->conv_kind == 6) {  // k_conv_poly: 8 instead of 27 taps on the upsampled channels
+    if (n->conv_kind >= 6) {  // polyphase: 8 instead of 27 taps on the upsampled channels
         const int C = n->base;
         f -= 2.0 * (27.0 - 8.0) * (4.0 * C) * n->L[6].cout * (double)vox[6];
         f -= 2.0 * (27.0 - 8.0) * (2.0 * C) * n->L[8].cout * (double)vox[8];
@@ -2478,8 +2733,8 @@ extern "C" int ppf_unet_last_forward(ppf_unet *n, double *ms, double *flop)
 
 extern "C" int ppf_unet_set_conv_kernel(ppf_unet *n, int kind)
 {
-    if (!n || kind < 0 || kind > 6)
-        return n ? uerr(n, PPF_EARG, "convolution kernel must be 0..6") : PPF_EARG;
+    if (!n || kind < 0 || kind > 7)
+        return n ? uerr(n, PPF_EARG, "convolution kernel must be 0..7") : PPF_EARG;
     n->conv_kind = kind;
     return PPF_OK;
 }

@@ -154,8 +154,9 @@ class UNet:
     def set_conv_kernel(self, kind):
         """2: row-blocked convolutions; 3, 4: row-blocked with fixed tiles;
         5: kind 2 with materialized upsampling; 6 (default): kind 2 with the
-        decoder convolutions in polyphase form at inference; 1: LDS-tiled;
-        0: the direct form."""
+        decoder convolutions in polyphase form at inference; 7: kind 6 with
+        the row-blocked polyphase kernel only; 1: LDS-tiled; 0: the direct
+        form."""
         self._check(lib().ppf_unet_set_conv_kernel(self._h, int(kind)))
 
     def last_forward(self):

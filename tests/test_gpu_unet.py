@@ -34,8 +34,9 @@ def _data(seed, shape=(2, 16, 16, 16)):
 
 # convolution kernels: 2 row-blocked, 3 and 4 its fixed-tile variants, 5
 # kind 2 with materialized upsampling, 6 (default) kind 2 with polyphase
-# decoder convolutions at inference, 1 LDS-tiled, 0 direct
-KINDS = [2, 3, 4, 5, 6, 1, 0]
+# decoder convolutions at inference, 7 kind 6 without the z-marching
+# polyphase kernel, 1 LDS-tiled, 0 direct
+KINDS = [2, 3, 4, 5, 6, 7, 1, 0]
 
 
 @pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 24, 12, 40), (1, 8, 36, 20)])
@@ -68,6 +69,24 @@ def test_fused_upsampling_is_bitwise_the_materialized_one(shape):
         net.set_conv_kernel(2)
         a = net.forward(x)
         net.set_conv_kernel(5)
+        b = net.forward(x)
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 24, 12, 40), (1, 8, 36, 20), (1, 36, 20, 68)])
+def test_zmarching_polyphase_is_bitwise_the_row_blocked_one(shape):
+    """Kind 6 runs the last decoder convolution (L8) with the z-marching
+    polyphase kernel (both sources staged in LDS), kind 7 with the
+    row-blocked polyphase kernel: same products, same order, bitwise equal.
+    The shapes cover tiles larger than the grid, ragged x and y (not
+    multiples of the 32 x 8 tile) and z chunks shorter than 16 planes."""
+    p = _params(11)
+    x, _ = _data(12, shape)
+    with UNet(seed=3) as net:
+        net.set_params(p)
+        net.set_conv_kernel(6)
+        a = net.forward(x)
+        net.set_conv_kernel(7)
         b = net.forward(x)
     np.testing.assert_array_equal(a, b)
 
