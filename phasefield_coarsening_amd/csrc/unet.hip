@@ -732,7 +732,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 // 0..7 and two channel quads are served together.  So staged sites are
 // padded to 48 bf16 (32 channels) or 80 bf16 (64 channels) where the LDS
 // allows it; k_conv_z32 keeps 40 (its 34 x 18 planes would not fit three
-// times at 48, and a shorter tile was slower than the conflicts).
+// times at 48, and a 28-row tile was slower than the conflicts, with 4 or 8
+// waves).
 constexpr int LDS_PAD32 = 48, LDS_PAD64 = 80;
 
 // ----------------------------------------------------------------------------
@@ -749,16 +750,24 @@ constexpr int ZM_TX = 16, ZM_RW = 8, ZM_TY = 4 * ZM_RW, ZM_HX = ZM_TX + 2, ZM_HY
 constexpr int ZM_SITES = ZM_HX * ZM_HY;          // 612 per plane
 constexpr int ZM_PAD = 40;                       // bf16 per staged site
 constexpr int ZM_CHUNKS = ZM_SITES * 4;          // 16-byte chunks per plane
-constexpr int ZM_PER_THREAD = (ZM_CHUNKS + 255) / 256;  // 10
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_conv_z32(
+// NW waves per workgroup: 4 (one per SIMD; a wave owns 8 rows and both
+// 16-channel blocks, 54 weight fragments in registers) or 8 (two per SIMD,
+// which the MFMA pipe needs to approach its peak; wave w owns rows 8 (w & 3)
+// .. +7 and channel block w >> 2, 27 weight fragments).  The same products
+// in the same order either way.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 4, NW / 4))) void k_conv_z32(
     const bf16 *__restrict__ X, int ldx, int xoff, const bf16 *__restrict__ Wt,
     const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy, int yoff, int B, int D, int H,
     int Wd, int zc, int ntx, int nty)
 {
+    constexpr int NT = 64 * NW, NJW = NW == 8 ? 1 : 2;
+    constexpr int PER_THREAD = (ZM_CHUNKS + NT - 1) / NT;
     __shared__ __attribute__((aligned(16))) bf16 zsh[3 * ZM_SITES * ZM_PAD];  // 146,880 bytes
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int rg = wave & 3, jb = NW == 8 ? (wave >> 2) : 0;  // row group, first channel block
     int r = blockIdx.x;
     const int tx = r % ntx;
     r /= ntx;
@@ -768,35 +777,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const int zci = r % nzc, b = r / nzc;
     const int x0 = tx * ZM_TX, y0 = ty * ZM_TY, z0 = zci * zc, zend = min(z0 + zc, D);
     const int xl = lane & 15, q = lane >> 4;
-    // weights: w[tap][j] = Wt[tap][16j + xl][8q .. 8q+7]
-    bf16x8 w[27][2];
+    // weights: w[tap][j] = Wt[tap][16 (jb + j) + xl][8q .. 8q+7]
+    bf16x8 w[27][NJW];
 #pragma unroll
     for (int tap = 0; tap < 27; tap++)
 #pragma unroll
-        for (int j = 0; j < 2; j++)
-            w[tap][j] = *(const bf16x8 *)(Wt + ((size_t)tap * 32 + 16 * j + xl) * 32 + 8 * q);
+        for (int j = 0; j < NJW; j++)
+            w[tap][j] = *(const bf16x8 *)(Wt + ((size_t)tap * 32 + 16 * (jb + j) + xl) * 32 + 8 * q);
     // this thread's staging chunks: site s = c / 4 (row-major 34 x 18), part c % 4
     auto plane_src = [&](int i, int zz) -> const bf16 * {
-        const int c = tid + 256 * i, s = c >> 2, part = c & 3;
+        const int c = tid + NT * i, s = c >> 2, part = c & 3;
         const int hy = s / ZM_HX, hx = s - hy * ZM_HX;
         const size_t v = (((size_t)b * D + wrapm(zz, D)) * H + wrapm(y0 + hy - 1, H)) * Wd +
                          wrapm(x0 + hx - 1, Wd);
         return X + v * ldx + xoff + 8 * part;
     };
-    // bias of the lane's output channels 4q .. 4q+3 and 16+4q .. 16+4q+3:
-    // the accumulators start there
-    const f32x4 bias0 = bias4(bias, 4 * q), bias1 = bias4(bias, 16 + 4 * q);
-    bf16x8 pf[ZM_PER_THREAD];
+    // bias of the lane's output channels 16 (jb + j) + 4q .. +3: the
+    // accumulators start there
+    f32x4 bj[NJW];
+#pragma unroll
+    for (int j = 0; j < NJW; j++)
+        bj[j] = bias4(bias, 16 * (jb + j) + 4 * q);
+    bf16x8 pf[PER_THREAD];
     auto load_plane = [&](int zz) {
 #pragma unroll
-        for (int i = 0; i < ZM_PER_THREAD; i++)
-            if (tid + 256 * i < ZM_CHUNKS)
+        for (int i = 0; i < PER_THREAD; i++)
+            if (tid + NT * i < ZM_CHUNKS)
                 pf[i] = *(const bf16x8 *)plane_src(i, zz);
     };
     auto store_plane = [&](int slot) {
 #pragma unroll
-        for (int i = 0; i < ZM_PER_THREAD; i++) {
-            const int c = tid + 256 * i;
+        for (int i = 0; i < PER_THREAD; i++) {
+            const int c = tid + NT * i;
             if (c < ZM_CHUNKS)
                 *(bf16x8 *)(zsh + ((size_t)slot * ZM_SITES + (c >> 2)) * ZM_PAD + 8 * (c & 3)) = pf[i];
         }
@@ -814,19 +826,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         __syncthreads();
         if (z + 1 < zend)
             load_plane(z + 2);  // in flight during this plane's MFMAs
-        f32x4 acc[ZM_RW][2];
+        f32x4 acc[ZM_RW][NJW];
 #pragma unroll
-        for (int rr = 0; rr < ZM_RW; rr++) {
-            acc[rr][0] = bias0;
-            acc[rr][1] = bias1;
-        }
+        for (int rr = 0; rr < ZM_RW; rr++)
+#pragma unroll
+            for (int j = 0; j < NJW; j++)
+                acc[rr][j] = bj[j];
         const int slots[3] = {s_m, s_c, s_p};
 #pragma unroll
         for (int dz = 0; dz < 3; dz++)
 #pragma unroll
             for (int dx = 0; dx < 3; dx++) {
                 bf16x8 a[ZM_RW + 2];
-                const bf16 *base = zsh + ((size_t)slots[dz] * ZM_SITES + (ZM_RW * wave) * ZM_HX + xl + dx) *
+                const bf16 *base = zsh + ((size_t)slots[dz] * ZM_SITES + (ZM_RW * rg) * ZM_HX + xl + dx) *
                                              ZM_PAD + 8 * q;
 #pragma unroll
                 for (int rr = 0; rr < ZM_RW + 2; rr++)
@@ -836,21 +848,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
                     for (int rr = 0; rr < ZM_RW; rr++)
 #pragma unroll
-                        for (int j = 0; j < 2; j++)
+                        for (int j = 0; j < NJW; j++)
                             acc[rr][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                                 w[dz * 9 + dy * 3 + dx][j], a[rr + dy], acc[rr][j], 0, 0, 0);
             }
-        // lane holds channels 16j+4q .. +3 of site (x0+xl, y0+8w+rr) in acc[rr][j]
+        // lane holds channels 16 (jb + j) + 4q .. +3 of site (x0+xl, y0+8 rg+rr)
         const int x = x0 + xl;
         if (x < Wd) {
 #pragma unroll
             for (int rr = 0; rr < ZM_RW; rr++) {
-                const int y = y0 + ZM_RW * wave + rr;
+                const int y = y0 + ZM_RW * rg + rr;
                 if (y >= H)
                     break;
-                const size_t o = ((((size_t)b * D + z) * H + y) * Wd + x) * ldy + yoff + 4 * q;
-                *(bf16x4 *)(Y + o) = leaky4(acc[rr][0]);
-                *(bf16x4 *)(Y + o + 16) = leaky4(acc[rr][1]);
+                const size_t o = ((((size_t)b * D + z) * H + y) * Wd + x) * ldy + yoff + 16 * jb + 4 * q;
+#pragma unroll
+                for (int j = 0; j < NJW; j++)
+                    *(bf16x4 *)(Y + o + 16 * j) = leaky4(acc[rr][j]);
             }
         }
     }
@@ -2017,7 +2030,7 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
         // level-1 layers: the z-marching kernel (same sums as k_conv_rows)
         const int zc = 16, ntx = (W + ZM_TX - 1) / ZM_TX, nty = (H + ZM_TY - 1) / ZM_TY;
         const unsigned nblk = (unsigned)((size_t)B * ((D + zc - 1) / zc) * nty * ntx);
-        hipLaunchKernelGGL(k_conv_z32, dim3(nblk), dim3(256), 0, n->stream, X, ldx, xoff, Wt,
+        hipLaunchKernelGGL(k_conv_z32<8>, dim3(nblk), dim3(512), 0, n->stream, X, ldx, xoff, Wt,
                            bias, Yb, ldy, yoff, B, D, H, W, zc, ntx, nty);
         return;
     }
