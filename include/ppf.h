@@ -166,7 +166,9 @@ int ppf_spheres_generate(uint64_t seed, int n, int nz, int ny, int nx, double rm
  * (else the tiles of one z chunk are); bit 1 set = each XCD gets one
  * contiguous run of that order instead of every 8th workgroup; bit 2 set =
  * the unpipelined z march (fewer registers, loads not issued a plane
- * ahead).  Results do
+ * ahead); bit 3 set (FP64, pipelined march) = each column loaded four
+ * planes ahead instead of two (more loads in flight, 12 more VGPRs).
+ * Results do
  * not depend on the tiling (each site's arithmetic is fixed); only speed
  * does.  No counterpart in the reference, whose cache blocking is compiled
  * out (phasefield.c:14, 58-62). */
@@ -181,8 +183,9 @@ int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz, int *order);
  * order as by a single step, so results are bit for bit those of single
  * steps; ppf_step runs pairs and a single step for an odd remainder, and
  * never fuses across an FP32 mass shift.  On by default for periodic FP32
- * contexts (faster there); off for FP64 (no faster: the FP64 kernel holds
- * ~185 VGPRs, two waves per SIMD).  steps_per_launch is 1 (off) or 2;
+ * contexts (faster there); off for FP64 (level with single steps: the FP64
+ * two-step kernel holds 176 VGPRs, two waves per SIMD).  steps_per_launch
+ * is 1 (off) or 2;
  * (tx, ty) in {(64,8), (32,8), (32,16), (64,16)} or (0,0) to keep; tz > 0
  * or 0 to keep; order bits 0-1 as ppf_set_tiling, negative to keep.
  * ppf_get_fusion reports 2 only while the context can fuse.  No counterpart
@@ -191,7 +194,7 @@ int ppf_set_fusion(ppf_ctx *ctx, int steps_per_launch, int tx, int ty, int tz, i
 int ppf_get_fusion(ppf_ctx *ctx, int *steps_per_launch, int *tx, int *ty, int *tz, int *order);
 
 /* Time every compiled tiling (tile shape x tz in {16,32,64} x order in
- * {0,1,2,4,5,6}) on the current state, `reps` sweeps each, and keep the
+ * {0,1,2,4,5,6,8,10}) on the current state, `reps` sweeps each, and keep the
  * fastest (ppf_get_tiling reads it back).  On periodic grids also time the
  * two-step kernel's tilings (ppf_set_fusion shapes x tz in {32,64} x order
  * in {0,2}) and switch two steps per launch on or off, whichever is faster

@@ -105,15 +105,17 @@ __global__ __launch_bounds__(256) void k_naive_f64(
 // With PIPE the march is software pipelined: the raw (s, l, c) of the
 // column two planes ahead and of the ring one plane ahead are loaded an
 // iteration before they are used, so each wave keeps two planes of loads in
-// flight while it computes (more registers, fewer waves per SIMD).  Compulsory HBM traffic is one read and one write of
-// each field per site; ring reads and the extra planes of each tz chunk
-// come mostly from L2.
+// flight while it computes (more registers, fewer waves per SIMD).  With
+// PIPE, CD > 2 loads the column CD planes ahead instead of two (CD - 2 more
+// planes in flight, 6 VGPRs each in FP64).  Compulsory HBM traffic is one read and one write of each field
+// per site; ring reads and the extra planes of each tz chunk come mostly
+// from L2.
 // ---------------------------------------------------------------------------
 // One workgroup's march over the column tile [x0, x0+TX) x [y0, y0+TY) and
 // the planes [z0, zend) of one grid.  sh is the workgroup's LDS,
 // 2 x 4 x (TY+2) x (TX+2) values; acc gets the REPORT sums of squared
 // increments.  Every thread of the workgroup must call it.
-template <class P, int TX, int TY, int BC, bool CONLY, bool REPORT, bool PIPE>
+template <class P, int TX, int TY, int BC, bool CONLY, bool REPORT, bool PIPE, int CD = 2>
 __device__ __forceinline__ void tile_march(
     const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
     const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
@@ -161,6 +163,8 @@ __device__ __forceinline__ void tile_march(
     T s_c, l_c, c_c, h_c, mu_c, M_c;
     T rs_n, rl_n, rc_n;                // raw column values one plane ahead
     T rs_r = 0, rl_r = 0, rc_r = 0;    // raw ring values of the current plane
+    constexpr int NQ = CD > 2 ? CD - 2 : 1;
+    T qs[NQ], ql[NQ], qc[NQ];          // raw column values 2 .. CD-1 planes ahead
     {
         const int zp = ghost<BC>(z0 - 1, nz), zn = ghost_any<BC>(z0 + 1, nz);
         s_p = PLANE(s_in, zp)[col];
@@ -176,6 +180,16 @@ __device__ __forceinline__ void tile_march(
             rs_r = PLANE(s_in, z0)[rcol];
             rl_r = PLANE(l_in, z0)[rcol];
             rc_r = PLANE(c_in, z0)[rcol];
+        }
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+            qs[k] = ql[k] = qc[k] = 0;
+            if (PIPE && CD > 2 && z0 + 1 + k < zend) {
+                const int z2 = ghost_any<BC>(z0 + 2 + k, nz);
+                qs[k] = PLANE(s_in, z2)[col];
+                ql[k] = PLANE(l_in, z2)[col];
+                qc[k] = PLANE(c_in, z2)[col];
+            }
         }
         T hp;
         P::point(s_p, l_p, c_p, k, hp, mu_p, M_p);
@@ -197,13 +211,15 @@ __device__ __forceinline__ void tile_march(
                 rc_r = PLANE(c_in, z)[rcol];
             }
         }
-        // prefetch: column two planes ahead, ring one plane ahead
+        // prefetch: column CD planes ahead, ring one plane ahead
         T ps = 0, pl = 0, pc = 0, prs = 0, prl = 0, prc = 0;
-        if (PIPE && more) {
-            const int zf = ghost_any<BC>(z + 2, nz);
+        if (PIPE && z + CD - 1 < zend) {
+            const int zf = ghost_any<BC>(z + CD, nz);
             ps = PLANE(s_in, zf)[col];
             pl = PLANE(l_in, zf)[col];
             pc = PLANE(c_in, zf)[col];
+        }
+        if (PIPE && more) {
             if (ring) {
                 prs = PLANE(s_in, z + 1)[rcol];
                 prl = PLANE(l_in, z + 1)[rcol];
@@ -272,9 +288,24 @@ __device__ __forceinline__ void tile_march(
         mu_c = mu_n;
         M_c = M_n;
         if (PIPE) {
-            rs_n = ps;
-            rl_n = pl;
-            rc_n = pc;
+            if (CD > 2) {
+                rs_n = qs[0];
+                rl_n = ql[0];
+                rc_n = qc[0];
+#pragma unroll
+                for (int k = 0; k + 1 < NQ; k++) {
+                    qs[k] = qs[k + 1];
+                    ql[k] = ql[k + 1];
+                    qc[k] = qc[k + 1];
+                }
+                qs[NQ - 1] = ps;
+                ql[NQ - 1] = pl;
+                qc[NQ - 1] = pc;
+            } else {
+                rs_n = ps;
+                rl_n = pl;
+                rc_n = pc;
+            }
             rs_r = prs;
             rl_r = prl;
             rc_r = prc;
@@ -283,7 +314,7 @@ __device__ __forceinline__ void tile_march(
 #undef PLANE
 }
 
-template <class P, int TX, int TY, int BC, bool CONLY, bool REPORT, bool PIPE>
+template <class P, int TX, int TY, int BC, bool CONLY, bool REPORT, bool PIPE, int CD = 2>
 __global__ __launch_bounds__(TX *TY) void k_tiled(
     const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
     const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
@@ -308,7 +339,7 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
     const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY, z0 = zc * tz;
     const int zend = min(z0 + tz, nz);
     double acc[3] = {0.0, 0.0, 0.0};
-    tile_march<P, TX, TY, BC, CONLY, REPORT, PIPE>(s_in, l_in, c_in, s_out, l_out, c_out, nx, ny,
+    tile_march<P, TX, TY, BC, CONLY, REPORT, PIPE, CD>(s_in, l_in, c_in, s_out, l_out, c_out, nx, ny,
                                                    nz, x0, y0, z0, zend, k, sh, acc);
     if (REPORT)
         block_sum3<NT>(acc, partials + 3 * (size_t)blockIdx.x);

@@ -93,23 +93,28 @@ using tiled_fn = void (*)(const typename P::T *, const typename P::T *, const ty
 typedef void (*naive_fn)(const double *, const double *, const double *, double *, double *,
                          double *, int, int, int, KConst, double *);
 
-template <class P, int TX, int TY, bool PIPE>
+template <class P, int TX, int TY, bool PIPE, int CD = 2>
 static tiled_fn<P> pick_tiled_p(int bc, bool conly, bool report)
 {
 #define PPF_T(BC, C, R) \
-    if (bc == BC && conly == C && report == R) return k_tiled<P, TX, TY, BC, C, R, PIPE>;
+    if (bc == BC && conly == C && report == R) return k_tiled<P, TX, TY, BC, C, R, PIPE, CD>;
     PPF_T(0, false, false) PPF_T(0, false, true) PPF_T(0, true, false) PPF_T(0, true, true)
     PPF_T(1, false, false) PPF_T(1, false, true) PPF_T(1, true, false) PPF_T(1, true, true)
 #undef PPF_T
     return nullptr;
 }
 
-// order bit 2 selects the unpipelined march
+// order bit 2 selects the unpipelined march; bit 3 (FP64, pipelined) the
+// column loaded four planes ahead instead of two
 template <class P, int TX, int TY>
 static tiled_fn<P> pick_tiled(int bc, bool conly, bool report, int order)
 {
-    return (order & 4) ? pick_tiled_p<P, TX, TY, false>(bc, conly, report)
-                       : pick_tiled_p<P, TX, TY, true>(bc, conly, report);
+    if (order & 4)
+        return pick_tiled_p<P, TX, TY, false>(bc, conly, report);
+    if constexpr (sizeof(typename P::T) == 8)
+        if (order & 8)
+            return pick_tiled_p<P, TX, TY, true, 4>(bc, conly, report);
+    return pick_tiled_p<P, TX, TY, true>(bc, conly, report);
 }
 
 static naive_fn pick_naive(int bc, bool eqa, bool conly, bool report)
@@ -828,8 +833,8 @@ extern "C" int ppf_set_tiling(ppf_ctx *ctx, int tx, int ty, int tz, int order)
         return set_err(ctx, PPF_EARG, "tile %dx%d is not compiled", tx, ty);
     if (tz < 1)
         return set_err(ctx, PPF_EARG, "tz must be >= 1");
-    if (order < 0 || order > 7)
-        return set_err(ctx, PPF_EARG, "order must be 0..7");
+    if (order < 0 || order > 15)
+        return set_err(ctx, PPF_EARG, "order must be 0..15");
     ctx->order = order;
     ctx->tile = {tx, ty};
     ctx->tz = tz > ctx->d.nz ? ctx->d.nz : tz;
@@ -870,7 +875,7 @@ extern "C" int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms)
     const TileCfg keep_tile = ctx->tile;
     const int keep_tz = ctx->tz, keep_order = ctx->order;
     static const int tzs[] = {16, 32, 64};
-    static const int orders[] = {0, 1, 2, 4, 5, 6};
+    static const int orders[] = {0, 1, 2, 4, 5, 6, 8, 10};
     double best = 1e300;
     TileCfg bt = keep_tile;
     int btz = keep_tz, bo = keep_order;

@@ -223,3 +223,28 @@ def test_autotune_keeps_state_and_results(precision, tmp_path, monkeypatch):
         want = O.run(s0, l0, c0, 4)
         for x, y in zip(got, want[:3]):
             np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("bc", [S.BC_PERIODIC, S.BC_NOFLUX])
+def test_tilings_and_march_variants_bitwise(bc):
+    """Every march variant of the FP64 tiled kernel (order bits: 4 the
+    unpipelined march, 8 the column loaded three planes ahead, 2 the XCD
+    order) on z chunks of 1, 2, 3 and 5 planes (the ends of the load
+    pipeline) and a ragged grid: bitwise the oracle."""
+    shape = (13, 20, 70)
+    print(f"tilings bc={bc}", flush=True)
+    rng = np.random.default_rng(5)
+    s = rng.uniform(0.05, 0.95, shape)
+    l = np.clip(1 - s + rng.uniform(-0.1, 0.1, shape), 0.0, 1.0)
+    c = 0.1 + 0.8 * s + rng.uniform(-0.02, 0.02, shape)
+    want = O.run(s, l, c, 3, bc=bc)
+    with S.Solver(shape, bc=bc) as sv:
+        for tile in [(64, 8), (16, 16)]:
+            for tz in (1, 2, 3, 5):
+                for order in (0, 4, 8, 10):
+                    sv.upload(s, l, c)
+                    sv.set_tiling(tile[0], tile[1], tz, order)
+                    sv.step(3)
+                    got = sv.download()
+                    for g, w in zip(got, want[:3]):
+                        np.testing.assert_array_equal(g, w, err_msg=f"{tile} tz={tz} order={order}")
