@@ -398,11 +398,13 @@ static void choose_tiles(ppf_ctx *ctx)
         ctx->tile = {16, 16};
     ctx->order = f32 ? 5 : 0;
     ctx->tz = ctx->d.nz < 32 ? ctx->d.nz : 32;
-    // two steps per launch: on by default for periodic FP32 (measured 1.22x
-    // the single-step rate at 512^3 with 32x8 tiles, tz 64, XCD order); for
-    // FP64 the two-step kernel holds ~185 VGPRs and is no faster
+    // two steps per launch: on by default for periodic FP32 (1.2x the
+    // single-step rate at 512^3); 32x16 tiles, tz 64, XCD order: level with
+    // 32x8 at 512^3 and 14% faster at 256^3 (233 against 205 GLUPS).  For
+    // FP64 the two-step kernel (176 VGPRs, two waves per SIMD) is level with
+    // single steps.
     ctx->fuse = (f32 && ctx->d.bc == PPF_BC_PERIODIC) ? 2 : 1;
-    ctx->tile2 = {32, 8};
+    ctx->tile2 = {32, 16};
     ctx->tz2 = ctx->d.nz < 64 ? ctx->d.nz : 64;
     ctx->order2 = 2;
 }
