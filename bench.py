@@ -13,8 +13,10 @@ synchronisations.
 
 Prints one JSON line.  value = FP64 GLUPS = N^3 * K / seconds / 1e9.
 roofline: algorithmic bytes per launch (48 B/site in FP64: read etaS, etaL, c
-and write them) over the kernel's average launch duration, from HIP events
-recorded on the stream the kernels run on.  cpu_baseline: the CPU oracle (a
+and write them; a two-step launch, the periodic default, does so once for two
+steps) over the kernel's average launch duration, from HIP events recorded on
+the stream the kernels run on, with the one-step-equivalent rate (48 B per
+site update) beside it; f64_single_step times the one-step kernel alone.  cpu_baseline: the CPU oracle (a
 restatement of the reference algorithm, bit-exact against it) timed on this
 host's cores on a bounded sample of the same workload.
 """
@@ -61,10 +63,13 @@ def copy_ceiling(nbytes=3 << 30, iters=20):
     return 2 * nbytes / (ms.value * 1e-3) / 1e9
 
 
-def run_gpu(S, shape, precision, steps, warmup, seed=2, nsph=2000, rmin=6.0, rmax=14.0):
+def run_gpu(S, shape, precision, steps, warmup, seed=2, nsph=2000, rmin=6.0, rmax=14.0,
+            fuse=None):
     cen, rad = S.spheres(seed, nsph, shape, rmin, rmax)
     sv = S.Solver(shape, precision=precision)
     sv.init_spheres(cen, rad)
+    if fuse is not None:
+        sv.set_fusion(fuse)
     st0 = sv.reduce()
     sv.step(warmup)
     sv.sync()
@@ -243,8 +248,13 @@ def main():
     sites = n ** 3
     r = run_gpu(S, shape, S.F64, a.steps, a.warmup)
     glups = sites * a.steps / r["wall_s"] / 1e9
-    avg_launch_s = r["event_ms"] * 1e-3 / a.steps
+    # steps per launch: 2 = two steps in one z march (periodic default), so a
+    # launch reads and writes the state once for two site updates
+    spl = r["fusion"][0]
+    per_step_s = r["event_ms"] * 1e-3 / a.steps
+    avg_launch_s = per_step_s * spl
     achieved = BYTES_PER_SITE["f64"] * sites / avg_launch_s / 1e9
+    equiv = BYTES_PER_SITE["f64"] * sites / per_step_s / 1e9
     workload = f"{n}^3 periodic, 2000 spheres R~U[6,14], FP64"
     out = {
         "metric": METRIC,
@@ -265,25 +275,43 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(r["kernel"], f"{n}^3-f64"),
                      "bytes_per_site": BYTES_PER_SITE["f64"],
-                     "avg_launch_us": round(avg_launch_s * 1e6, 2)},
+                     "avg_launch_us": round(avg_launch_s * 1e6, 2),
+                     "steps_per_launch": spl,
+                     "hbm_bytes_per_site_update": BYTES_PER_SITE["f64"] / spl,
+                     "one_step_equivalent_GBs": round(equiv, 1),
+                     "one_step_equivalent_frac": round(equiv / HBM_PEAK_GBS, 4)},
         "solute_drift_rel": abs(r["stats1"]["sum_c"] - r["stats0"]["sum_c"]) /
         abs(r["stats0"]["sum_c"]),
     }
-    out["config"]["tiling"] = {"tx": r["tiling"][0], "ty": r["tiling"][1], "tz": r["tiling"][2],
-                               "order": r["tiling"][3]}
+    if spl == 2:
+        f = r["fusion"]
+        out["config"]["tiling"] = {"steps_per_launch": 2, "tx": f[1], "ty": f[2], "tz": f[3],
+                                   "order": f[4]}
+    else:
+        out["config"]["tiling"] = {"tx": r["tiling"][0], "ty": r["tiling"][1],
+                                   "tz": r["tiling"][2], "order": r["tiling"][3]}
+    if spl == 2:
+        r1 = run_gpu(S, shape, S.F64, a.steps, a.warmup, fuse=1)
+        per1_s = r1["event_ms"] * 1e-3 / a.steps
+        ach1 = BYTES_PER_SITE["f64"] * sites / per1_s / 1e9
+        out["f64_single_step"] = {
+            "GLUPS": round(sites * a.steps / r1["wall_s"] / 1e9, 3), "kernel": r1["kernel"],
+            "tiling": list(r1["tiling"]), "avg_launch_us": round(per1_s * 1e6, 2),
+            "achieved_GBs": round(ach1, 1), "achieved_frac": round(ach1 / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(r1["kernel"], f"{n}^3-f64")}
     if not a.no_f32:
         r32 = run_gpu(S, shape, S.F32, a.steps, a.warmup)
         g32 = sites * a.steps / r32["wall_s"] / 1e9
-        spl = r32["fusion"][0]  # steps per launch: 2 = both steps in one z march
-        per_step_s = r32["event_ms"] * 1e-3 / a.steps
+        spl32 = r32["fusion"][0]  # steps per launch: 2 = both steps in one z march
+        per_step32_s = r32["event_ms"] * 1e-3 / a.steps
         # one-step form: 24 B per site update; two steps per launch move the
         # same 24 B per site once for two updates
-        equiv32 = BYTES_PER_SITE["f32"] * sites / per_step_s / 1e9
-        actual32 = equiv32 / spl
+        equiv32 = BYTES_PER_SITE["f32"] * sites / per_step32_s / 1e9
+        actual32 = equiv32 / spl32
         out["f32"] = {"GLUPS": round(g32, 3), "ms_per_step": round(r32["wall_s"] * 1e3 / a.steps, 4),
                       "vs_f64": round(g32 / glups, 3), "kernel": r32["kernel"],
-                      "steps_per_launch": spl,
-                      "hbm_bytes_per_site_update": BYTES_PER_SITE["f32"] / spl,
+                      "steps_per_launch": spl32,
+                      "hbm_bytes_per_site_update": BYTES_PER_SITE["f32"] / spl32,
                       "achieved_GBs": round(actual32, 1),
                       "achieved_frac": round(actual32 / HBM_PEAK_GBS, 4),
                       "one_step_equivalent_GBs": round(equiv32, 1),
@@ -298,6 +326,7 @@ def main():
         cc = copy_ceiling()
         out["roofline"]["copy_ceiling_GBs"] = round(cc, 1)
         out["roofline"]["frac_of_copy_ceiling"] = round(achieved / cc, 4)
+        out["roofline"]["one_step_equivalent_frac_of_copy_ceiling"] = round(equiv / cc, 4)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(S, shape, a.cpu_steps)
         ref = reference_serial_baseline(S, shape)

@@ -205,22 +205,27 @@ using fused_fn = void (*)(const typename P::T *, const typename P::T *, const ty
                           typename P::T *, typename P::T *, typename P::T *, int, int, int, int,
                           int, int, int, typename P::K);
 
-static const TileCfg k_tiles2[] = {{64, 8}, {32, 8}, {32, 16}, {64, 16}};
+static const TileCfg k_tiles2[] = {{64, 8}, {32, 8}, {32, 16}, {64, 16}, {16, 16}};
 
 template <class P>
-static fused_fn<P> pick_fused(TileCfg t)
+static fused_fn<P> pick_fused(TileCfg t, int order)
 {
-    // FP64 needs ~185 VGPRs: two waves per SIMD (capping it at three spills
-    // and measured slower); FP32 fits five.  At two waves per SIMD the
-    // loads must be issued a plane ahead (PF) to be hidden.  LDS double
-    // buffered (one barrier per plane) except for the FP64 64x16 tile,
-    // whose two buffers would not fit.
+    // FP64 needs ~176 VGPRs: two waves per SIMD; at two waves per SIMD the
+    // loads must be issued a plane ahead (PF) to be hidden.  Order bit 2
+    // (FP64, 4-wave tiles 32x8 and 16x16): three waves per SIMD instead,
+    // without PF (164 VGPRs, no spills; 49 KB of LDS, so three workgroups
+    // share a CU).  FP32 fits five waves.  LDS double buffered (one barrier
+    // per plane) except for the FP64 64x16 tile, whose two buffers would
+    // not fit.
     constexpr bool F64 = sizeof(typename P::T) == 8;
     constexpr int W = F64 ? 2 : 5;
+    const bool w3 = F64 && (order & 4);
     if (t.tx == 64 && t.ty == 8)
         return k_tiled2<P, 64, 8, W, F64, true>;
     if (t.tx == 32 && t.ty == 8)
-        return k_tiled2<P, 32, 8, W, F64, true>;
+        return w3 ? k_tiled2<P, 32, 8, 3, false, true> : k_tiled2<P, 32, 8, W, F64, true>;
+    if (t.tx == 16 && t.ty == 16)
+        return w3 ? k_tiled2<P, 16, 16, 3, false, true> : k_tiled2<P, 16, 16, W, F64, true>;
     if (t.tx == 32 && t.ty == 16)
         return k_tiled2<P, 32, 16, W, F64, true>;
     if (t.tx == 64 && t.ty == 16)
@@ -241,7 +246,7 @@ static int launch_fused(ppf_ctx *ctx, const typename P::K &k)
     const int ntx = (ctx->d.nx + ctx->tile2.tx - 1) / ctx->tile2.tx;
     const int nty = (ctx->d.ny + ctx->tile2.ty - 1) / ctx->tile2.ty;
     const int nzc = (ctx->d.nz + ctx->tz2 - 1) / ctx->tz2;
-    fused_fn<P> f = pick_fused<P>(ctx->tile2);
+    fused_fn<P> f = pick_fused<P>(ctx->tile2, ctx->order2);
     if (!f)
         return set_err(ctx, PPF_EARG, "no two-step kernel for tile %dx%d", ctx->tile2.tx,
                        ctx->tile2.ty);
@@ -382,9 +387,10 @@ static int set_target_from_device(ppf_ctx *ctx)
     return PPF_OK;
 }
 
-// Default tiling, from sweeps on MI355X at 512^3: x rows of 64 sites and
-// 32-plane z chunks; FP64 pipelined 64x8 tiles in tile-fastest order, FP32
-// unpipelined 64x16 tiles with the z chunks of a tile adjacent.
+// Default tiling, from sweeps on MI355X at 512^3: single steps in x rows
+// of 64 sites and 32-plane z chunks; FP64 pipelined 64x8 tiles in
+// tile-fastest order, FP32 unpipelined 64x16 tiles with the z chunks of a
+// tile adjacent.
 // ppf_set_tiling or ppf_autotune override it.
 static void choose_tiles(ppf_ctx *ctx)
 {
@@ -398,15 +404,16 @@ static void choose_tiles(ppf_ctx *ctx)
         ctx->tile = {16, 16};
     ctx->order = f32 ? 5 : 0;
     ctx->tz = ctx->d.nz < 32 ? ctx->d.nz : 32;
-    // two steps per launch: on by default for periodic FP32 (1.2x the
-    // single-step rate at 512^3); 32x16 tiles, tz 64, XCD order: level with
-    // 32x8 at 512^3 and 14% faster at 256^3 (233 against 205 GLUPS).  For
-    // FP64 the two-step kernel (176 VGPRs, two waves per SIMD) is level with
-    // single steps.
-    ctx->fuse = (f32 && ctx->d.bc == PPF_BC_PERIODIC) ? 2 : 1;
-    ctx->tile2 = {32, 16};
+    // two steps per launch: on by default for periodic grids.  FP32: 32x16
+    // tiles, tz 64, XCD order (1.2x the single-step rate at 512^3; level
+    // with 32x8 there and 14% faster at 256^3, 233 against 205 GLUPS).
+    // FP64: 16x16 tiles, tz 64, XCD order, three waves per SIMD (order bit
+    // 2): 117-119 against 101 GLUPS for single steps at 512^3, 104 against
+    // 93 at 256^3, 112 against 87 at 1024^3.
+    ctx->fuse = ctx->d.bc == PPF_BC_PERIODIC ? 2 : 1;
+    ctx->tile2 = f32 ? TileCfg{32, 16} : TileCfg{16, 16};
     ctx->tz2 = ctx->d.nz < 64 ? ctx->d.nz : 64;
-    ctx->order2 = 2;
+    ctx->order2 = f32 ? 2 : 6;
 }
 
 extern "C" int ppf_create(ppf_ctx **out, const ppf_desc *desc)
@@ -915,7 +922,9 @@ extern "C" int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms)
         const TileCfg keep2 = ctx->tile2;
         const int keep_tz2 = ctx->tz2, keep_o2 = ctx->order2;
         static const int tzs2[] = {32, 64};
-        static const int orders2[] = {0, 2};
+        // order bit 2 selects the three-wave FP64 variants (no effect in FP32)
+        static const int orders2[] = {0, 2, 4, 6};
+        const int norders2 = ctx->d.precision == PPF_F64 ? 4 : 2;
         double best2 = 1e300;
         TileCfg bt2 = keep2;
         int btz2 = keep_tz2, bo2 = keep_o2;
@@ -923,7 +932,8 @@ extern "C" int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms)
             if (c.tx > 2 * ctx->d.nx && c.tx > 32)
                 continue;
             for (int tz : tzs2)
-                for (int o : orders2) {
+                for (int oi = 0; oi < norders2; oi++) {
+                    const int o = orders2[oi];
                     ctx->tile2 = c;
                     ctx->tz2 = tz > ctx->d.nz ? ctx->d.nz : tz;
                     ctx->order2 = o;
@@ -991,8 +1001,8 @@ extern "C" int ppf_set_fusion(ppf_ctx *ctx, int steps_per_launch, int tx, int ty
         return set_err(ctx, PPF_EARG, "tz must be >= 0");
     if (tz > 0)
         ctx->tz2 = tz > ctx->d.nz ? ctx->d.nz : tz;
-    if (order > 3)
-        return set_err(ctx, PPF_EARG, "order must be 0..3 (or negative to keep)");
+    if (order > 7)
+        return set_err(ctx, PPF_EARG, "order must be 0..7 (or negative to keep)");
     if (order >= 0)
         ctx->order2 = order;
     ctx->fuse = steps_per_launch;

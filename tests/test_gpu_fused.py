@@ -1,5 +1,5 @@
 """Two steps per launch (k_tiled2): bit for bit the same as single steps, for
-every compiled two-step tile and z chunk, on ragged and tiny periodic grids,
+every compiled two-step tile, z chunk and register variant, on ragged and tiny periodic grids,
 for odd step counts (a single step finishes), in FP64 and in FP32 with mass
 shifts that fall between the two steps of a pair; and against the
 reference's own outputs."""
@@ -11,7 +11,7 @@ from phasefield_coarsening_amd import solver as S
 
 pytestmark = pytest.mark.gpu
 
-TILES2 = [(64, 8), (32, 8), (32, 16), (64, 16)]
+TILES2 = [(64, 8), (32, 8), (32, 16), (64, 16), (16, 16)]
 SHAPES = [(1, 1, 1), (2, 3, 5), (7, 9, 70), (20, 16, 12), (33, 17, 65), (9, 130, 31),
           (16, 64, 64)]
 
@@ -43,9 +43,11 @@ def test_fp64_fused_equals_single_steps(shape, tile):
     st = _state(shape, 7)
     want = _run(shape, st, 7, S.F64, False)
     for tz in (1, 3, 32):
-        got = _run(shape, st, 7, S.F64, True, tile, tz, order=tz % 4)
-        for a, b in zip(got, want):
-            np.testing.assert_array_equal(a, b)
+        # order bit 2: the three-wave FP64 variant (tiles 32x8 and 16x16)
+        for w3 in (0, 4):
+            got = _run(shape, st, 7, S.F64, True, tile, tz, order=(tz % 4) | w3)
+            for a, b in zip(got, want):
+                np.testing.assert_array_equal(a, b)
 
 
 @pytest.mark.parametrize("shape", [(20, 16, 12), (33, 17, 65), (16, 64, 64)])

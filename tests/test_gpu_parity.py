@@ -239,6 +239,7 @@ def test_tilings_and_march_variants_bitwise(bc):
     c = 0.1 + 0.8 * s + rng.uniform(-0.02, 0.02, shape)
     want = O.run(s, l, c, 3, bc=bc)
     with S.Solver(shape, bc=bc) as sv:
+        sv.set_fusion(1)  # single steps only: periodic grids pair steps by default
         for tile in [(64, 8), (16, 16)]:
             for tz in (1, 2, 3, 5):
                 for order in (0, 4, 8, 10):
