@@ -184,11 +184,14 @@ int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz, int *order);
  * steps; ppf_step runs pairs and a single step for an odd remainder, and
  * never fuses across an FP32 mass shift.  On by default for periodic
  * contexts (FP64: 16x16 tiles at three waves per SIMD, ~1.17x single steps
- * at 512^3; FP32: 32x16 tiles, ~1.2x).  steps_per_launch is 1 (off) or 2;
+ * at 512^3; FP32: 32x16 tiles with the ring in waves of its own, ~1.2x).
+ * steps_per_launch is 1 (off) or 2;
  * (tx, ty) in {(64,8), (32,8), (32,16), (64,16), (16,16)} or (0,0) to
  * keep; tz > 0 or 0 to keep; order bits 0-1 as ppf_set_tiling, bit 2 (FP64,
  * tiles 32x8 and 16x16) the variant that runs three waves per SIMD with no
- * loads issued a plane early; negative to keep.
+ * loads issued a plane early, bit 3 (tiles 16x16, 32x8 and 32x16) the
+ * kernel whose ring sites are updated by waves of their own after the
+ * tile's; negative to keep.  PPF_EARG for a tile without the variant.
  * ppf_get_fusion reports 2 only while the context can fuse.  No counterpart
  * in the reference, which sweeps once per step (phasefield.c:178-215). */
 int ppf_set_fusion(ppf_ctx *ctx, int steps_per_launch, int tx, int ty, int tz, int order);
@@ -198,7 +201,7 @@ int ppf_get_fusion(ppf_ctx *ctx, int *steps_per_launch, int *tx, int *ty, int *t
  * {0,1,2,4,5,6,8,10}) on the current state, `reps` sweeps each, and keep the
  * fastest (ppf_get_tiling reads it back).  On periodic grids also time the
  * two-step kernel's tilings (ppf_set_fusion shapes x tz in {32,64} x order
- * in {0,2}, and {4,6} in FP64) and switch two steps per launch on or off, whichever is faster
+ * in {0,2,8,10}, and {4,6} in FP64) and switch two steps per launch on or off, whichever is faster
  * per step (ppf_get_fusion reads it back).  *best_ms gets the best time per
  * step.  The state is not advanced: the trial sweeps write the spare copy
  * only.  Synchronises. */

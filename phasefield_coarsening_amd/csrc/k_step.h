@@ -667,6 +667,281 @@ __device__ __forceinline__ void tile_march2(
 #undef PLANE
 }
 
+// Two steps per launch with the ring in waves of its own (periodic grids,
+// double-buffered LDS).  The same march as tile_march2<P, TX, TY, false,
+// true>, split by wave: the TX*TY tile threads (whole waves) update their
+// tile site in stage 1 and stage 2; the ring waves that follow (NRL lanes,
+// NR1 of them with a one-cell ring site) load the two-cell ring and corners
+// (the outer sites) and update the ring sites in stage 1.  The two kinds
+// of wave run separate loops over the same planes with one barrier per
+// plane in each, so a wave holds only its own column state (fewer
+// registers) and no wave does the tile's and a ring site's update in the
+// same plane.  Every site sees the same operands in the same order as in
+// tile_march2, so the results are bit for bit the same.
+template <class P, int TX, int TY>
+__device__ __forceinline__ void tile_march2s(
+    const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
+    const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
+    typename P::T *__restrict__ l_out, typename P::T *__restrict__ c_out, int nx, int ny, int nz,
+    int x0, int y0, int z0, int zend, const typename P::K &k,
+    typename P::T (*__restrict__ a1)[TY + 4][TX + 4], typename P::T (*__restrict__ a2)[TY + 2][TX + 2])
+{
+    constexpr int NT = TX * TY, NR1 = 2 * TX + 2 * TY, NO2 = 2 * TX + 2 * TY + 4;
+    constexpr int NRL = (NR1 + 63) / 64 * 64;
+    static_assert(NT % 64 == 0, "tile threads must fill whole waves");
+    static_assert(NO2 <= 2 * NRL, "two outer sites per ring lane at most");
+    using T = typename P::T;
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const size_t plane = (size_t)nx * ny;
+#define PLANE(p, zz) uniform_ptr((p) + (size_t)(zz) * plane)
+    auto colof = [&](int ax, int ay) {
+        return (unsigned)ghost_any<0>(y0 + ay, ny) * (unsigned)nx + (unsigned)ghost_any<0>(x0 + ax, nx);
+    };
+    if (wave < NT / 64) {
+        // ---- tile waves ----
+        const int tx = tid % TX, ty = tid / TX;
+        const bool active = x0 + tx < nx && y0 + ty < ny;
+        const unsigned col = colof(tx, ty);
+        const unsigned ocol = (unsigned)(y0 + ty) * (unsigned)nx + (unsigned)(x0 + tx);
+        T s_p, l_p, mu_p, M_p, s_c, l_c, c_c, h_c, mu_c, M_c;
+        {
+            const int zp = ghost_any<0>(z0 - 2, nz), zc = ghost_any<0>(z0 - 1, nz);
+            T hp;
+            s_p = PLANE(s_in, zp)[col];
+            l_p = PLANE(l_in, zp)[col];
+            P::point(s_p, l_p, PLANE(c_in, zp)[col], k, hp, mu_p, M_p);
+            s_c = PLANE(s_in, zc)[col];
+            l_c = PLANE(l_in, zc)[col];
+            c_c = PLANE(c_in, zc)[col];
+            P::point(s_c, l_c, c_c, k, h_c, mu_c, M_c);
+        }
+        T us_p = 0, ul_p = 0, umu_p = 0, uM_p = 0, us_c = 0, ul_c = 0, uc_c = 0, uh_c = 0,
+          umu_c = 0, uM_c = 0;
+        for (int zs = z0 - 1; zs <= zend; ++zs) {
+            const int zn = ghost_any<0>(zs + 1, nz);
+            const int bsel = (zs - z0 + 1) & 1;
+            T(*__restrict__ A1)[TY + 4][TX + 4] = a1 + 4 * bsel;
+            T(*__restrict__ A2)[TY + 2][TX + 2] = a2 + 4 * bsel;
+            const T(*__restrict__ A2p)[TY + 2][TX + 2] = a2 + 4 * (bsel ^ 1);
+            const T rs_n = PLANE(s_in, zn)[col];
+            const T rl_n = PLANE(l_in, zn)[col];
+            const T rc_n = PLANE(c_in, zn)[col];
+            A1[0][ty + 2][tx + 2] = s_c;
+            A1[1][ty + 2][tx + 2] = l_c;
+            A1[2][ty + 2][tx + 2] = mu_c;
+            A1[3][ty + 2][tx + 2] = M_c;
+            T h_n, mu_n, M_n;
+            P::point(rs_n, rl_n, rc_n, k, h_n, mu_n, M_n);
+            __syncthreads();
+            // stage 1: step t+1 of plane zs at the tile site
+            T s1, l1, c1, h1, mu1, M1;
+            {
+                const int ax = tx + 2, ay = ty + 2;
+                const T sn[6] = {rs_n, s_p, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
+                const T ln[6] = {rl_n, l_p, A1[1][ay + 1][ax], A1[1][ay - 1][ax], A1[1][ay][ax + 1], A1[1][ay][ax - 1]};
+                const T mun[6] = {mu_n, mu_p, A1[2][ay + 1][ax], A1[2][ay - 1][ax], A1[2][ay][ax + 1], A1[2][ay][ax - 1]};
+                const T Mn[6] = {M_n, M_p, A1[3][ay + 1][ax], A1[3][ay - 1][ax], A1[3][ay][ax + 1], A1[3][ay][ax - 1]};
+                double ds, dl, dc;
+                P::update(s_c, l_c, c_c, h_c, mu_c, M_c, sn, ln, mun, Mn, k, s1, l1, c1, ds, dl, dc);
+                P::point(s1, l1, c1, k, h1, mu1, M1);
+            }
+            A2[0][ty + 1][tx + 1] = s1;
+            A2[1][ty + 1][tx + 1] = l1;
+            A2[2][ty + 1][tx + 1] = mu1;
+            A2[3][ty + 1][tx + 1] = M1;
+            // stage 2: step t+2 of plane zs-1 at the tile site
+            if (zs >= z0 + 1 && active) {
+                const int bx = tx + 1, by = ty + 1;
+                const T sn[6] = {s1, us_p, A2p[0][by + 1][bx], A2p[0][by - 1][bx], A2p[0][by][bx + 1], A2p[0][by][bx - 1]};
+                const T ln[6] = {l1, ul_p, A2p[1][by + 1][bx], A2p[1][by - 1][bx], A2p[1][by][bx + 1], A2p[1][by][bx - 1]};
+                const T mun[6] = {mu1, umu_p, A2p[2][by + 1][bx], A2p[2][by - 1][bx], A2p[2][by][bx + 1], A2p[2][by][bx - 1]};
+                const T Mn[6] = {M1, uM_p, A2p[3][by + 1][bx], A2p[3][by - 1][bx], A2p[3][by][bx + 1], A2p[3][by][bx - 1]};
+                T so, lo, co;
+                double ds, dl, dc;
+                P::update(us_c, ul_c, uc_c, uh_c, umu_c, uM_c, sn, ln, mun, Mn, k, so, lo, co, ds, dl, dc);
+                __builtin_nontemporal_store(so, PLANE(s_out, zs - 1) + ocol);
+                __builtin_nontemporal_store(lo, PLANE(l_out, zs - 1) + ocol);
+                __builtin_nontemporal_store(co, PLANE(c_out, zs - 1) + ocol);
+            }
+            s_p = s_c;
+            l_p = l_c;
+            mu_p = mu_c;
+            M_p = M_c;
+            s_c = rs_n;
+            l_c = rl_n;
+            c_c = rc_n;
+            h_c = h_n;
+            mu_c = mu_n;
+            M_c = M_n;
+            us_p = us_c;
+            ul_p = ul_c;
+            umu_p = umu_c;
+            uM_p = uM_c;
+            us_c = s1;
+            ul_c = l1;
+            uc_c = c1;
+            uh_c = h1;
+            umu_c = mu1;
+            uM_c = M1;
+        }
+    } else {
+        // ---- ring waves ----
+        const int r = tid - NT;
+        const bool ring = r < NR1;
+        // one-cell ring site (A2 frame rx, ry; tile frame rx-1, ry-1)
+        int rx = 0, ry = 0;
+        if (r < TY) {
+            rx = 0;
+            ry = r + 1;
+        } else if (r < 2 * TY) {
+            rx = TX + 1;
+            ry = r - TY + 1;
+        } else if (r < 2 * TY + TX) {
+            rx = r - 2 * TY + 1;
+            ry = 0;
+        } else if (r < NR1) {
+            rx = r - 2 * TY - TX + 1;
+            ry = TY + 1;
+        }
+        const unsigned rcol = colof(rx - 1, ry - 1);
+        // outer sites j = r and r + NRL (A1 frame ox, oy; tile frame ox-2, oy-2)
+        int ox[2] = {0, 0}, oy[2] = {0, 0};
+        bool outer[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int j = r + u * NRL;
+            outer[u] = j < NO2;
+            if (j < TY) {
+                ox[u] = 0;
+                oy[u] = j + 2;
+            } else if (j < 2 * TY) {
+                ox[u] = TX + 3;
+                oy[u] = j - TY + 2;
+            } else if (j < 2 * TY + TX) {
+                ox[u] = j - 2 * TY + 2;
+                oy[u] = 0;
+            } else if (j < 2 * TY + 2 * TX) {
+                ox[u] = j - 2 * TY - TX + 2;
+                oy[u] = TY + 3;
+            } else if (j < NO2) {
+                const int c = j - 2 * TY - 2 * TX;
+                ox[u] = (c & 1) ? TX + 2 : 1;
+                oy[u] = (c & 2) ? TY + 2 : 1;
+            }
+        }
+        const unsigned o2col[2] = {colof(ox[0] - 2, oy[0] - 2), colof(ox[1] - 2, oy[1] - 2)};
+        T qs_p = 0, ql_p = 0, qmu_p = 0, qM_p = 0, qs_c = 0, ql_c = 0, qc_c = 0, qh_c = 0, qmu_c = 0,
+          qM_c = 0;
+        if (ring) {
+            const int zp = ghost_any<0>(z0 - 2, nz), zc = ghost_any<0>(z0 - 1, nz);
+            qs_p = PLANE(s_in, zp)[rcol];
+            ql_p = PLANE(l_in, zp)[rcol];
+            T qh;
+            P::point(qs_p, ql_p, PLANE(c_in, zp)[rcol], k, qh, qmu_p, qM_p);
+            qs_c = PLANE(s_in, zc)[rcol];
+            ql_c = PLANE(l_in, zc)[rcol];
+            qc_c = PLANE(c_in, zc)[rcol];
+            P::point(qs_c, ql_c, qc_c, k, qh_c, qmu_c, qM_c);
+        }
+        for (int zs = z0 - 1; zs <= zend; ++zs) {
+            const int zc = ghost_any<0>(zs, nz), zn = ghost_any<0>(zs + 1, nz);
+            const int bsel = (zs - z0 + 1) & 1;
+            T(*__restrict__ A1)[TY + 4][TX + 4] = a1 + 4 * bsel;
+            T(*__restrict__ A2)[TY + 2][TX + 2] = a2 + 4 * bsel;
+            T qs_n = 0, ql_n = 0, qc_n = 0;
+            if (ring) {
+                qs_n = PLANE(s_in, zn)[rcol];
+                ql_n = PLANE(l_in, zn)[rcol];
+                qc_n = PLANE(c_in, zn)[rcol];
+            }
+            T os[2], ol[2], oc[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                os[u] = ol[u] = oc[u] = 0;
+                if (outer[u]) {
+                    os[u] = PLANE(s_in, zc)[o2col[u]];
+                    ol[u] = PLANE(l_in, zc)[o2col[u]];
+                    oc[u] = PLANE(c_in, zc)[o2col[u]];
+                }
+            }
+            if (ring) {
+                A1[0][ry + 1][rx + 1] = qs_c;
+                A1[1][ry + 1][rx + 1] = ql_c;
+                A1[2][ry + 1][rx + 1] = qmu_c;
+                A1[3][ry + 1][rx + 1] = qM_c;
+            }
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                if (outer[u]) {
+                    T oh, omu, oM;
+                    P::point(os[u], ol[u], oc[u], k, oh, omu, oM);
+                    A1[0][oy[u]][ox[u]] = os[u];
+                    A1[1][oy[u]][ox[u]] = ol[u];
+                    A1[2][oy[u]][ox[u]] = omu;
+                    A1[3][oy[u]][ox[u]] = oM;
+                }
+            }
+            T qh_n = 0, qmu_n = 0, qM_n = 0;
+            if (ring)
+                P::point(qs_n, ql_n, qc_n, k, qh_n, qmu_n, qM_n);
+            __syncthreads();
+            // stage 1: step t+1 of plane zs at the ring site
+            if (ring) {
+                const int ax = rx + 1, ay = ry + 1;
+                const T sn[6] = {qs_n, qs_p, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
+                const T ln[6] = {ql_n, ql_p, A1[1][ay + 1][ax], A1[1][ay - 1][ax], A1[1][ay][ax + 1], A1[1][ay][ax - 1]};
+                const T mun[6] = {qmu_n, qmu_p, A1[2][ay + 1][ax], A1[2][ay - 1][ax], A1[2][ay][ax + 1], A1[2][ay][ax - 1]};
+                const T Mn[6] = {qM_n, qM_p, A1[3][ay + 1][ax], A1[3][ay - 1][ax], A1[3][ay][ax + 1], A1[3][ay][ax - 1]};
+                T qs1, ql1, qc1, qh1, qmu1, qM1;
+                double ds, dl, dc;
+                P::update(qs_c, ql_c, qc_c, qh_c, qmu_c, qM_c, sn, ln, mun, Mn, k, qs1, ql1, qc1, ds, dl, dc);
+                P::point(qs1, ql1, qc1, k, qh1, qmu1, qM1);
+                A2[0][ry][rx] = qs1;
+                A2[1][ry][rx] = ql1;
+                A2[2][ry][rx] = qmu1;
+                A2[3][ry][rx] = qM1;
+            }
+            qs_p = qs_c;
+            ql_p = ql_c;
+            qmu_p = qmu_c;
+            qM_p = qM_c;
+            qs_c = qs_n;
+            ql_c = ql_n;
+            qc_c = qc_n;
+            qh_c = qh_n;
+            qmu_c = qmu_n;
+            qM_c = qM_n;
+        }
+    }
+#undef PLANE
+}
+
+// Kernel of tile_march2s: TX*TY tile threads and the ring waves after them.
+template <class P, int TX, int TY, int WPE>
+__global__ __launch_bounds__(TX *TY + (2 * TX + 2 * TY + 63) / 64 * 64)
+__attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_tiled2s(
+    const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
+    const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
+    typename P::T *__restrict__ l_out, typename P::T *__restrict__ c_out, int nx, int ny, int nz,
+    int tz, int ntx, int nty, int order, typename P::K k)
+{
+    using T = typename P::T;
+    __shared__ T a1[8][TY + 4][TX + 4];
+    __shared__ T a2[8][TY + 2][TX + 2];
+    const int ntile = ntx * nty;
+    const int nblk = gridDim.x;
+    int b = blockIdx.x;
+    if ((order & 2) && (nblk & 7) == 0)
+        b = (b & 7) * (nblk >> 3) + (b >> 3);
+    const int tile = (order & 1) ? b / ((nblk + ntile - 1) / ntile) : b % ntile;
+    const int zc = (order & 1) ? b % ((nblk + ntile - 1) / ntile) : b / ntile;
+    const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY, z0 = zc * tz;
+    const int zend = min(z0 + tz, nz);
+    tile_march2s<P, TX, TY>(s_in, l_in, c_in, s_out, l_out, c_out, nx, ny, nz, x0, y0, z0, zend, k,
+                            a1, a2);
+}
+
 // WPE: the waves per SIMD the register allocation aims for (FP64 needs a
 // cap: left alone it takes ~185 VGPRs, two waves per SIMD).  PF: prefetch
 // one plane ahead, DB: double-buffered LDS, one barrier per plane

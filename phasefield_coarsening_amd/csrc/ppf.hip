@@ -208,29 +208,48 @@ using fused_fn = void (*)(const typename P::T *, const typename P::T *, const ty
 static const TileCfg k_tiles2[] = {{64, 8}, {32, 8}, {32, 16}, {64, 16}, {16, 16}};
 
 template <class P>
-static fused_fn<P> pick_fused(TileCfg t, int order)
+struct FusedPick {
+    fused_fn<P> f;
+    int threads;
+};
+
+template <class P>
+static FusedPick<P> pick_fused(TileCfg t, int order)
 {
-    // FP64 needs ~176 VGPRs: two waves per SIMD; at two waves per SIMD the
-    // loads must be issued a plane ahead (PF) to be hidden.  Order bit 2
-    // (FP64, 4-wave tiles 32x8 and 16x16): three waves per SIMD instead,
-    // without PF (164 VGPRs, no spills; 49 KB of LDS, so three workgroups
-    // share a CU).  FP32 fits five waves.  LDS double buffered (one barrier
-    // per plane) except for the FP64 64x16 tile, whose two buffers would
-    // not fit.
+    // k_tiled2: FP64 needs ~176 VGPRs: two waves per SIMD; at two waves per
+    // SIMD the loads must be issued a plane ahead (PF) to be hidden.  Order
+    // bit 2 (FP64, 4-wave tiles 32x8 and 16x16): three waves per SIMD
+    // instead, without PF (164 VGPRs, no spills; 49 KB of LDS, so three
+    // workgroups share a CU).  FP32 fits five waves.  LDS double buffered
+    // (one barrier per plane) except for the FP64 64x16 tile, whose two
+    // buffers would not fit.
+    // Order bit 3: k_tiled2s, the ring sites in waves of their own after the
+    // tile's (FP64 114 VGPRs: four waves per SIMD; FP32 55: eight).
     constexpr bool F64 = sizeof(typename P::T) == 8;
-    constexpr int W = F64 ? 2 : 5;
+    constexpr int W = F64 ? 2 : 5, WS = F64 ? 4 : 8;
+    const int nt = t.tx * t.ty;
+    if (order & 8) {
+        const int ring = (2 * t.tx + 2 * t.ty + 63) / 64 * 64;
+        if (t.tx == 16 && t.ty == 16)
+            return {k_tiled2s<P, 16, 16, WS>, nt + ring};
+        if (t.tx == 32 && t.ty == 8)
+            return {k_tiled2s<P, 32, 8, WS>, nt + ring};
+        if (t.tx == 32 && t.ty == 16)
+            return {k_tiled2s<P, 32, 16, WS>, nt + ring};
+        return {nullptr, 0};
+    }
     const bool w3 = F64 && (order & 4);
     if (t.tx == 64 && t.ty == 8)
-        return k_tiled2<P, 64, 8, W, F64, true>;
+        return {k_tiled2<P, 64, 8, W, F64, true>, nt};
     if (t.tx == 32 && t.ty == 8)
-        return w3 ? k_tiled2<P, 32, 8, 3, false, true> : k_tiled2<P, 32, 8, W, F64, true>;
+        return {w3 ? k_tiled2<P, 32, 8, 3, false, true> : k_tiled2<P, 32, 8, W, F64, true>, nt};
     if (t.tx == 16 && t.ty == 16)
-        return w3 ? k_tiled2<P, 16, 16, 3, false, true> : k_tiled2<P, 16, 16, W, F64, true>;
+        return {w3 ? k_tiled2<P, 16, 16, 3, false, true> : k_tiled2<P, 16, 16, W, F64, true>, nt};
     if (t.tx == 32 && t.ty == 16)
-        return k_tiled2<P, 32, 16, W, F64, true>;
+        return {k_tiled2<P, 32, 16, W, F64, true>, nt};
     if (t.tx == 64 && t.ty == 16)
-        return k_tiled2<P, 64, 16, W == 2 ? 1 : W, F64, !F64>;
-    return nullptr;
+        return {k_tiled2<P, 64, 16, W == 2 ? 1 : W, F64, !F64>, nt};
+    return {nullptr, 0};
 }
 
 static bool fused_available(const ppf_ctx *ctx)
@@ -246,12 +265,12 @@ static int launch_fused(ppf_ctx *ctx, const typename P::K &k)
     const int ntx = (ctx->d.nx + ctx->tile2.tx - 1) / ctx->tile2.tx;
     const int nty = (ctx->d.ny + ctx->tile2.ty - 1) / ctx->tile2.ty;
     const int nzc = (ctx->d.nz + ctx->tz2 - 1) / ctx->tz2;
-    fused_fn<P> f = pick_fused<P>(ctx->tile2, ctx->order2);
-    if (!f)
-        return set_err(ctx, PPF_EARG, "no two-step kernel for tile %dx%d", ctx->tile2.tx,
-                       ctx->tile2.ty);
-    hipLaunchKernelGGL(f, dim3((unsigned)((size_t)ntx * nty * nzc)),
-                       dim3(ctx->tile2.tx * ctx->tile2.ty), 0, ctx->stream,
+    const FusedPick<P> pk = pick_fused<P>(ctx->tile2, ctx->order2);
+    if (!pk.f)
+        return set_err(ctx, PPF_EARG, "no two-step kernel for tile %dx%d order %d", ctx->tile2.tx,
+                       ctx->tile2.ty, ctx->order2);
+    hipLaunchKernelGGL(pk.f, dim3((unsigned)((size_t)ntx * nty * nzc)), dim3(pk.threads), 0,
+                       ctx->stream,
                        (const T *)ctx->buf[cs][0], (const T *)ctx->buf[cl][1],
                        (const T *)ctx->buf[cc][2], (T *)ctx->buf[1 - cs][0],
                        (T *)ctx->buf[1 - cl][1], (T *)ctx->buf[1 - cc][2], ctx->d.nx, ctx->d.ny,
@@ -410,10 +429,20 @@ static void choose_tiles(ppf_ctx *ctx)
     // FP64: 16x16 tiles, tz 64, XCD order, three waves per SIMD (order bit
     // 2): 117-119 against 101 GLUPS for single steps at 512^3, 104 against
     // 93 at 256^3, 112 against 87 at 1024^3.
+    // FP32 runs the ring-wave kernel (order bit 3): 221 against 211 GLUPS at
+    // 256^3, level at 512^3.
     ctx->fuse = ctx->d.bc == PPF_BC_PERIODIC ? 2 : 1;
     ctx->tile2 = f32 ? TileCfg{32, 16} : TileCfg{16, 16};
-    ctx->tz2 = ctx->d.nz < 64 ? ctx->d.nz : 64;
-    ctx->order2 = f32 ? 2 : 6;
+    ctx->order2 = f32 ? 10 : 6;
+    // z chunks of 64 planes, shortened (to 16 at least) while the launch
+    // would leave CUs without a workgroup: at 128^3, 64-plane chunks of
+    // 32x16 tiles are 64 workgroups (38 GLUPS), 32-plane ones 128 (75).
+    const long tiles2 = (long)((ctx->d.nx + ctx->tile2.tx - 1) / ctx->tile2.tx) *
+                        ((ctx->d.ny + ctx->tile2.ty - 1) / ctx->tile2.ty);
+    int tz2 = 64;
+    while (tz2 > 16 && tiles2 * ((ctx->d.nz + tz2 - 1) / tz2) < 256)
+        tz2 /= 2;
+    ctx->tz2 = ctx->d.nz < tz2 ? ctx->d.nz : tz2;
 }
 
 extern "C" int ppf_create(ppf_ctx **out, const ppf_desc *desc)
@@ -922,9 +951,12 @@ extern "C" int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms)
         const TileCfg keep2 = ctx->tile2;
         const int keep_tz2 = ctx->tz2, keep_o2 = ctx->order2;
         static const int tzs2[] = {32, 64};
-        // order bit 2 selects the three-wave FP64 variants (no effect in FP32)
-        static const int orders2[] = {0, 2, 4, 6};
-        const int norders2 = ctx->d.precision == PPF_F64 ? 4 : 2;
+        // order bit 3 selects the ring-wave kernel, bit 2 the three-wave
+        // FP64 variants (no effect in FP32)
+        static const int orders2_64[] = {0, 2, 4, 6, 8, 10}, orders2_32[] = {0, 2, 8, 10};
+        const bool f64 = ctx->d.precision == PPF_F64;
+        const int *orders2 = f64 ? orders2_64 : orders2_32;
+        const int norders2 = f64 ? 6 : 4;
         double best2 = 1e300;
         TileCfg bt2 = keep2;
         int btz2 = keep_tz2, bo2 = keep_o2;
@@ -934,6 +966,8 @@ extern "C" int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms)
             for (int tz : tzs2)
                 for (int oi = 0; oi < norders2; oi++) {
                     const int o = orders2[oi];
+                    if (!pick_fused<PhysF32>(c, o).f)
+                        continue;  // no ring-wave kernel for this tile
                     ctx->tile2 = c;
                     ctx->tz2 = tz > ctx->d.nz ? ctx->d.nz : tz;
                     ctx->order2 = o;
@@ -995,16 +1029,20 @@ extern "C" int ppf_set_fusion(ppf_ctx *ctx, int steps_per_launch, int tx, int ty
             known = known || (c.tx == tx && c.ty == ty);
         if (!known)
             return set_err(ctx, PPF_EARG, "two-step tile %dx%d is not compiled", tx, ty);
-        ctx->tile2 = {tx, ty};
     }
     if (tz < 0)
         return set_err(ctx, PPF_EARG, "tz must be >= 0");
+    if (order > 15)
+        return set_err(ctx, PPF_EARG, "order must be 0..15 (or negative to keep)");
+    // the ring-wave kernel (order bit 3) exists for some tiles only
+    const TileCfg t = (tx != 0 || ty != 0) ? TileCfg{tx, ty} : ctx->tile2;
+    const int o = order >= 0 ? order : ctx->order2;
+    if (!pick_fused<PhysF32>(t, o).f)
+        return set_err(ctx, PPF_EARG, "two-step tile %dx%d has no order-%d variant", t.tx, t.ty, o);
+    ctx->tile2 = t;
     if (tz > 0)
         ctx->tz2 = tz > ctx->d.nz ? ctx->d.nz : tz;
-    if (order > 7)
-        return set_err(ctx, PPF_EARG, "order must be 0..7 (or negative to keep)");
-    if (order >= 0)
-        ctx->order2 = order;
+    ctx->order2 = o;
     ctx->fuse = steps_per_launch;
     return PPF_OK;
 }

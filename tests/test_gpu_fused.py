@@ -12,6 +12,7 @@ from phasefield_coarsening_amd import solver as S
 pytestmark = pytest.mark.gpu
 
 TILES2 = [(64, 8), (32, 8), (32, 16), (64, 16), (16, 16)]
+RING_WAVE_TILES = [(16, 16), (32, 8), (32, 16)]
 SHAPES = [(1, 1, 1), (2, 3, 5), (7, 9, 70), (20, 16, 12), (33, 17, 65), (9, 130, 31),
           (16, 64, 64)]
 
@@ -43,11 +44,14 @@ def test_fp64_fused_equals_single_steps(shape, tile):
     st = _state(shape, 7)
     want = _run(shape, st, 7, S.F64, False)
     for tz in (1, 3, 32):
-        # order bit 2: the three-wave FP64 variant (tiles 32x8 and 16x16)
-        for w3 in (0, 4):
-            got = _run(shape, st, 7, S.F64, True, tile, tz, order=(tz % 4) | w3)
+        # order bit 2: the three-wave FP64 variant (tiles 32x8 and 16x16);
+        # bit 3: the ring in waves of its own (tiles 16x16, 32x8, 32x16)
+        for var in (0, 4, 8):
+            if var == 8 and tile not in RING_WAVE_TILES:
+                continue
+            got = _run(shape, st, 7, S.F64, True, tile, tz, order=(tz % 4) | var)
             for a, b in zip(got, want):
-                np.testing.assert_array_equal(a, b)
+                np.testing.assert_array_equal(a, b, err_msg=f"tz={tz} order={(tz % 4) | var}")
 
 
 @pytest.mark.parametrize("shape", [(20, 16, 12), (33, 17, 65), (16, 64, 64)])
@@ -57,9 +61,10 @@ def test_fp32_fused_equals_single_steps_with_mass_shifts(shape, tile):
     print(f"shape={shape} tile={tile}", flush=True)
     st = _state(shape, 11)
     want = _run(shape, st, 23, S.F32, False, shift=5)
-    got = _run(shape, st, 23, S.F32, True, tile, 8, shift=5)
-    for a, b in zip(got, want):
-        np.testing.assert_array_equal(a, b)
+    for order in ((2, 8) if tile in RING_WAVE_TILES else (2,)):
+        got = _run(shape, st, 23, S.F32, True, tile, 8, order=order, shift=5)
+        for a, b in zip(got, want):
+            np.testing.assert_array_equal(a, b, err_msg=f"order={order}")
 
 
 def test_fused_matches_reference_outputs(golden):
