@@ -50,25 +50,28 @@ __global__ __launch_bounds__(256) void k_ensemble(const EnsDom *__restrict__ dom
         if (it >= nitems)
             return;
         const EnsItem w = items[it];
-        const EnsDom d = doms[w.dom];
-        const T *si = (const T *)d.f[parity][0];
-        const T *li = (const T *)d.f[parity][1];
-        const T *ci = (const T *)d.f[parity][2];
-        T *so = (T *)d.f[1 - parity][0];
-        T *lo = (T *)d.f[1 - parity][1];
-        T *co = (T *)d.f[1 - parity][2];
+        // the member's pointers read straight from the descriptor (a
+        // private copy indexed by the runtime parity would live in scratch)
+        const EnsDom *d = doms + w.dom;
+        const T *si = (const T *)d->f[parity][0];
+        const T *li = (const T *)d->f[parity][1];
+        const T *ci = (const T *)d->f[parity][2];
+        T *so = (T *)d->f[1 - parity][0];
+        T *lo = (T *)d->f[1 - parity][1];
+        T *co = (T *)d->f[1 - parity][2];
+        const int dnx = d->nx, dny = d->ny, dnz = d->nz;
         double acc[3];
         if (w.shape == ENS_64x4)
             tile_march<P, 64, 4, BC, CONLY, false, false>(
-                si, li, ci, so, lo, co, d.nx, d.ny, d.nz, w.x0, w.y0, w.z0, w.zend, k,
+                si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0, w.z0, w.zend, k,
                 reinterpret_cast<T(*)[4][6][66]>(sh), acc);
         else if (w.shape == ENS_32x8)
             tile_march<P, 32, 8, BC, CONLY, false, false>(
-                si, li, ci, so, lo, co, d.nx, d.ny, d.nz, w.x0, w.y0, w.z0, w.zend, k,
+                si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0, w.z0, w.zend, k,
                 reinterpret_cast<T(*)[4][10][34]>(sh), acc);
         else
             tile_march<P, 16, 16, BC, CONLY, false, false>(
-                si, li, ci, so, lo, co, d.nx, d.ny, d.nz, w.x0, w.y0, w.z0, w.zend, k,
+                si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0, w.z0, w.zend, k,
                 reinterpret_cast<T(*)[4][18][18]>(sh), acc);
     }
 }
@@ -142,8 +145,13 @@ enum { ENS2_32x8 = 0, ENS2_16x16 = 1, ENS2_NSHAPES = 2 };
 // FP64 keeps single buffers so that four workgroups fit a CU.
 constexpr int ENS2_LDS = 4 * 12 * 36 + 4 * 10 * 34;
 
+// Register target: FP32 members five waves per SIMD (the single-grid
+// kernel's budget, no spills); FP64 two (~200 VGPRs with both member shapes
+// in one kernel; three waves per SIMD spill).
 template <class P>
-__global__ __launch_bounds__(256) void k_ensemble2(const EnsDom *__restrict__ doms,
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(typename P::T) == 4 ? 5 : 2))) void k_ensemble2(
+    const EnsDom *__restrict__ doms,
                                                    const EnsItem *__restrict__ items, int nitems,
                                                    int *__restrict__ counter, int parity,
                                                    typename P::K k)
@@ -162,19 +170,22 @@ __global__ __launch_bounds__(256) void k_ensemble2(const EnsDom *__restrict__ do
         if (it >= nitems)
             return;
         const EnsItem w = items[it];
-        const EnsDom d = doms[w.dom];
-        const T *si = (const T *)d.f[parity][0];
-        const T *li = (const T *)d.f[parity][1];
-        const T *ci = (const T *)d.f[parity][2];
-        T *so = (T *)d.f[1 - parity][0];
-        T *lo = (T *)d.f[1 - parity][1];
-        T *co = (T *)d.f[1 - parity][2];
+        // the member's pointers read straight from the descriptor (a
+        // private copy indexed by the runtime parity would live in scratch)
+        const EnsDom *d = doms + w.dom;
+        const T *si = (const T *)d->f[parity][0];
+        const T *li = (const T *)d->f[parity][1];
+        const T *ci = (const T *)d->f[parity][2];
+        T *so = (T *)d->f[1 - parity][0];
+        T *lo = (T *)d->f[1 - parity][1];
+        T *co = (T *)d->f[1 - parity][2];
+        const int dnx = d->nx, dny = d->ny, dnz = d->nz;
         if (w.shape == ENS2_32x8)
-            tile_march2<P, 32, 8, false, DB>(si, li, ci, so, lo, co, d.nx, d.ny, d.nz, w.x0, w.y0,
+            tile_march2<P, 32, 8, false, DB>(si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0,
                                              w.z0, w.zend, k, reinterpret_cast<T(*)[12][36]>(sh),
                                              reinterpret_cast<T(*)[10][34]>(sh + NB * 4 * 12 * 36));
         else
-            tile_march2<P, 16, 16, false, DB>(si, li, ci, so, lo, co, d.nx, d.ny, d.nz, w.x0, w.y0,
+            tile_march2<P, 16, 16, false, DB>(si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0,
                                               w.z0, w.zend, k, reinterpret_cast<T(*)[20][20]>(sh),
                                               reinterpret_cast<T(*)[18][18]>(sh + NB * 4 * 20 * 20));
     }
