@@ -200,8 +200,8 @@ int ppf_get_fusion(ppf_ctx *ctx, int *steps_per_launch, int *tx, int *ty, int *t
 /* Time every compiled tiling (tile shape x tz in {16,32,64} x order in
  * {0,1,2,4,5,6,8,10}) on the current state, `reps` sweeps each, and keep the
  * fastest (ppf_get_tiling reads it back).  On periodic grids also time the
- * two-step kernel's tilings (ppf_set_fusion shapes x tz in {32,64} x order
- * in {0,2,8,10}, and {4,6} in FP64) and switch two steps per launch on or off, whichever is faster
+ * two-step kernel's tilings (the current one, then ppf_set_fusion shapes x
+ * tz in {32,64} x order in {0,2,8,10}, and {4,6} in FP64) and switch two steps per launch on or off, whichever is faster
  * per step (ppf_get_fusion reads it back).  *best_ms gets the best time per
  * step.  The state is not advanced: the trial sweeps write the spare copy
  * only.  Synchronises. */

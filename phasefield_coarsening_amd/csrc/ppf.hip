@@ -988,6 +988,17 @@ extern "C" int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms)
         double best2 = 1e300;
         TileCfg bt2 = keep2;
         int btz2 = keep_tz2, bo2 = keep_o2;
+        // the current two-step tiling (the default, e.g. an 86-plane FP64
+        // chunk outside the list below) is the first candidate
+        {
+            double ms = 0.0;
+            int rc = time_fused(ctx, 1, &ms);
+            if (rc == PPF_OK)
+                rc = time_fused(ctx, reps, &ms);
+            if (rc)
+                return rc;
+            best2 = ms;
+        }
         for (const TileCfg &c : k_tiles2) {
             if (c.tx > 2 * ctx->d.nx && c.tx > 32)
                 continue;
