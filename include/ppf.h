@@ -175,23 +175,26 @@ int ppf_spheres_generate(uint64_t seed, int n, int nz, int ny, int nx, double rm
 int ppf_set_tiling(ppf_ctx *ctx, int tx, int ty, int tz, int order);
 int ppf_get_tiling(ppf_ctx *ctx, int *tx, int *ty, int *tz, int *order);
 
-/* Two steps per launch (periodic contexts of the tiled kernels, FP64 or
- * FP32): one z march computes step t+1 on each tile plus a one-cell ring
- * and keeps it in LDS, then step t+2 on the tile, so each site's state is
- * read and written once per two steps (24 B per site update in FP64
- * instead of 48).  Every site is updated with the same operands in the same
- * order as by a single step, so results are bit for bit those of single
- * steps; ppf_step runs pairs and a single step for an odd remainder, and
- * never fuses across an FP32 mass shift.  On by default for periodic
- * contexts (FP64: 16x16 tiles at three waves per SIMD, ~1.17x single steps
- * at 512^3; FP32: 32x16 tiles with the ring in waves of its own, ~1.2x).
- * steps_per_launch is 1 (off) or 2;
+/* Two steps per launch (contexts of the tiled kernels, FP64 or FP32): one
+ * z march computes step t+1 on each tile plus a one-cell ring and keeps it
+ * in LDS, then step t+2 on the tile, so each site's state is read and
+ * written once per two steps (24 B per site update in FP64 instead of 48).
+ * Every site is updated with the same operands in the same order as by a
+ * single step, so results are bit for bit those of single steps; ppf_step
+ * runs pairs and a single step for an odd remainder, and never fuses across
+ * an FP32 mass shift or a c-only sweep.  With no-flux (mirror) boundaries a
+ * ring site or plane outside the grid is not stepped: it takes the step
+ * t+1 of the site it mirrors (phasefield.c:1151-1189), written by that
+ * site's thread.  On by default (FP64: 16x16 tiles at three waves per SIMD,
+ * ~1.17x single steps at 512^3; FP32 periodic: 32x16 tiles with the ring in
+ * waves of its own, ~1.2x).  steps_per_launch is 1 (off) or 2;
  * (tx, ty) in {(64,8), (32,8), (32,16), (64,16), (16,16)} or (0,0) to
- * keep; tz > 0 or 0 to keep; order bits 0-1 as ppf_set_tiling, bit 2 (FP64,
- * tiles 32x8 and 16x16) the variant that runs three waves per SIMD with no
- * loads issued a plane early, bit 3 (tiles 16x16, 32x8 and 32x16) the
- * kernel whose ring sites are updated by waves of their own after the
- * tile's; negative to keep.  PPF_EARG for a tile without the variant.
+ * keep (no-flux: (16,16) and (32,8), and (32,16) in FP32); tz > 0 or 0 to
+ * keep; order bits 0-1 as ppf_set_tiling, bit 2 (FP64, tiles 32x8 and
+ * 16x16) the variant that runs three waves per SIMD with no loads issued a
+ * plane early, bit 3 (periodic, tiles 16x16, 32x8 and 32x16) the kernel
+ * whose ring sites are updated by waves of their own after the tile's;
+ * negative to keep.  PPF_EARG for a tile without the variant.
  * ppf_get_fusion reports 2 only while the context can fuse.  No counterpart
  * in the reference, which sweeps once per step (phasefield.c:178-215). */
 int ppf_set_fusion(ppf_ctx *ctx, int steps_per_launch, int tx, int ty, int tz, int order);
@@ -199,12 +202,13 @@ int ppf_get_fusion(ppf_ctx *ctx, int *steps_per_launch, int *tx, int *ty, int *t
 
 /* Time every compiled tiling (tile shape x tz in {16,32,64} x order in
  * {0,1,2,4,5,6,8,10}) on the current state, `reps` sweeps each, and keep the
- * fastest (ppf_get_tiling reads it back).  On periodic grids also time the
- * two-step kernel's tilings (the current one, then ppf_set_fusion shapes x
- * tz in {32,64} x order in {0,2,8,10}, and {4,6} in FP64) and switch two steps per launch on or off, whichever is faster
- * per step (ppf_get_fusion reads it back).  *best_ms gets the best time per
- * step.  The state is not advanced: the trial sweeps write the spare copy
- * only.  Synchronises. */
+ * fastest (ppf_get_tiling reads it back).  Also time the two-step kernel's
+ * tilings (the current one, then the ppf_set_fusion shapes x tz in {32,64} x
+ * order in {0,2,8,10}, and {4,6} in FP64; {0,2} with no-flux boundaries) and
+ * switch two steps per launch on or off, whichever is faster per step
+ * (ppf_get_fusion reads it back).  *best_ms gets the best time per step.
+ * The state is not advanced: the trial sweeps write the spare copy only.
+ * Synchronises. */
 int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms);
 
 /* FP32 solute bookkeeping.  An FP32 context keeps an FP64 target for the

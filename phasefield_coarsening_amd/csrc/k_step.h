@@ -380,7 +380,7 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
 // this plane, last read by stage 2 of the previous plane (before this
 // plane's barrier).
 // ---------------------------------------------------------------------------
-template <class P, int TX, int TY, bool PF = false, bool DB = false>
+template <class P, int TX, int TY, bool PF = false, bool DB = false, int BC = 0>
 __device__ __forceinline__ void tile_march2(
     const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
     const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
@@ -398,7 +398,7 @@ __device__ __forceinline__ void tile_march2(
 #define PLANE(p, zz) uniform_ptr((p) + (size_t)(zz) * plane)
     // site (x0 + ax, y0 + ay) of the tile frame -> offset within a plane
     auto colof = [&](int ax, int ay) {
-        return (unsigned)ghost_any<0>(y0 + ay, ny) * (unsigned)nx + (unsigned)ghost_any<0>(x0 + ax, nx);
+        return (unsigned)ghost_any<BC>(y0 + ay, ny) * (unsigned)nx + (unsigned)ghost_any<BC>(x0 + ax, nx);
     };
     const unsigned col = colof(tx, ty);
     const unsigned ocol = (unsigned)(y0 + ty) * (unsigned)nx + (unsigned)(x0 + tx);
@@ -420,6 +420,10 @@ __device__ __forceinline__ void tile_march2(
     }
     const bool ring = tid < NR1;
     const unsigned rcol = colof(rx - 1, ry - 1);
+    // mirror boundaries: a ring site outside the grid is a ghost, whose step
+    // t+1 is its mirror site's; the tile thread of that site writes it
+    const bool ring_in = BC == 0 || (x0 + rx - 1 >= 0 && x0 + rx - 1 < nx && y0 + ry - 1 >= 0 &&
+                                     y0 + ry - 1 < ny);
     // outer site (A1 frame ox, oy; tile frame ox-2, oy-2)
     int ox = 0, oy = 0;
     {
@@ -452,7 +456,7 @@ __device__ __forceinline__ void tile_march2(
     T qs_p = 0, ql_p = 0, qmu_p = 0, qM_p = 0, qs_c = 0, ql_c = 0, qc_c = 0, qh_c = 0, qmu_c = 0,
       qM_c = 0;
     {
-        const int zp = ghost_any<0>(z0 - 2, nz), zc = ghost_any<0>(z0 - 1, nz);
+        const int zp = ghost_any<BC>(z0 - 2, nz), zc = ghost_any<BC>(z0 - 1, nz);
         T hp;
         s_p = PLANE(s_in, zp)[col];
         l_p = PLANE(l_in, zp)[col];
@@ -479,7 +483,7 @@ __device__ __forceinline__ void tile_march2(
     // zs (outer site); with PF the first plane's are loaded here
     T rs_n = 0, rl_n = 0, rc_n = 0, qs_n = 0, ql_n = 0, qc_n = 0, os = 0, ol = 0, oc = 0;
     if (PF) {
-        const int zn = ghost_any<0>(z0, nz), zc = ghost_any<0>(z0 - 1, nz);
+        const int zn = ghost_any<BC>(z0, nz), zc = ghost_any<BC>(z0 - 1, nz);
         rs_n = PLANE(s_in, zn)[col];
         rl_n = PLANE(l_in, zn)[col];
         rc_n = PLANE(c_in, zn)[col];
@@ -496,7 +500,7 @@ __device__ __forceinline__ void tile_march2(
     }
 
     for (int zs = z0 - 1; zs <= zend; ++zs) {
-        const int zc = ghost_any<0>(zs, nz), zn = ghost_any<0>(zs + 1, nz);
+        const int zc = ghost_any<BC>(zs, nz), zn = ghost_any<BC>(zs + 1, nz);
         // this plane's A1 and A2 buffers, and the A2 buffer of plane zs-1
         const int bsel = DB ? ((zs - z0 + 1) & 1) : 0;
         T(*__restrict__ A1)[TY + 4][TX + 4] = a1 + 4 * bsel;
@@ -521,7 +525,7 @@ __device__ __forceinline__ void tile_march2(
         // of the outer site)
         T fs = 0, fl = 0, fc = 0, fqs = 0, fql = 0, fqc = 0, fos = 0, fol = 0, foc = 0;
         if (PF && zs < zend) {
-            const int zf = ghost_any<0>(zs + 2, nz);
+            const int zf = ghost_any<BC>(zs + 2, nz);
             fs = PLANE(s_in, zf)[col];
             fl = PLANE(l_in, zf)[col];
             fc = PLANE(c_in, zf)[col];
@@ -561,9 +565,12 @@ __device__ __forceinline__ void tile_march2(
             P::point(qs_n, ql_n, qc_n, k, qh_n, qmu_n, qM_n);
         __syncthreads();
 
-        // stage 1: step t+1 of plane zs at the tile site
-        T s1, l1, c1, h1, mu1, M1;
-        {
+        // stage 1: step t+1 of plane zs at the tile site.  Mirror
+        // boundaries: a ghost plane (zs = -1 or nz) is not stepped; plane
+        // nz's step t+1 is plane nz-1's (plane -1's is set after plane 0).
+        const bool zghost = BC == 1 && (zs < 0 || zs >= nz);
+        T s1 = us_c, l1 = ul_c, c1 = uc_c, h1 = uh_c, mu1 = umu_c, M1 = uM_c;
+        if (!zghost) {
             const int ax = tx + 2, ay = ty + 2;
             const T sn[6] = {rs_n, s_p, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
             const T ln[6] = {rl_n, l_p, A1[1][ay + 1][ax], A1[1][ay - 1][ax], A1[1][ay][ax + 1], A1[1][ay][ax - 1]};
@@ -575,7 +582,7 @@ __device__ __forceinline__ void tile_march2(
         }
         // ... and at the ring site
         T qs1 = 0, ql1 = 0, qmu1 = 0, qM1 = 0;
-        if (ring) {
+        if (ring && ring_in && !zghost) {
             const int ax = rx + 1, ay = ry + 1;
             const T sn[6] = {qs_n, qs_p, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
             const T ln[6] = {ql_n, ql_p, A1[1][ay + 1][ax], A1[1][ay - 1][ax], A1[1][ay][ax + 1], A1[1][ay][ax - 1]};
@@ -588,12 +595,28 @@ __device__ __forceinline__ void tile_march2(
         }
         // A2: step t+1 of plane zs (a buffer of its own with DB; otherwise
         // after stage 2 and a barrier, below)
+        auto put_a2 = [&](int by, int bx) {
+            A2[0][by][bx] = s1;
+            A2[1][by][bx] = l1;
+            A2[2][by][bx] = mu1;
+            A2[3][by][bx] = M1;
+        };
         auto store_a2 = [&]() {
-            A2[0][ty + 1][tx + 1] = s1;
-            A2[1][ty + 1][tx + 1] = l1;
-            A2[2][ty + 1][tx + 1] = mu1;
-            A2[3][ty + 1][tx + 1] = M1;
-            if (ring) {
+            if (BC == 0) {
+                put_a2(ty + 1, tx + 1);
+            } else if (active) {
+                // the site, and the ghost slots it mirrors
+                put_a2(ty + 1, tx + 1);
+                if (x0 + tx == 0)
+                    put_a2(ty + 1, tx);
+                if (x0 + tx == nx - 1)
+                    put_a2(ty + 1, tx + 2);
+                if (y0 + ty == 0)
+                    put_a2(ty, tx + 1);
+                if (y0 + ty == ny - 1)
+                    put_a2(ty + 2, tx + 1);
+            }
+            if (ring && ring_in) {
                 A2[0][ry][rx] = qs1;
                 A2[1][ry][rx] = ql1;
                 A2[2][ry][rx] = qmu1;
@@ -652,6 +675,13 @@ __device__ __forceinline__ void tile_march2(
         uh_c = h1;
         umu_c = mu1;
         uM_c = M1;
+        if (BC == 1 && z0 == 0 && zs == 0) {
+            // mirror boundary below plane 0: ghost plane -1 holds plane 0
+            us_p = us_c;
+            ul_p = ul_c;
+            umu_p = umu_c;
+            uM_p = uM_c;
+        }
         if (PF) {
             rs_n = fs;
             rl_n = fl;
@@ -946,7 +976,7 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_tiled2s(
 // cap: left alone it takes ~185 VGPRs, two waves per SIMD).  PF: prefetch
 // one plane ahead, DB: double-buffered LDS, one barrier per plane
 // (tile_march2).
-template <class P, int TX, int TY, int WPE, bool PF = false, bool DB = false>
+template <class P, int TX, int TY, int WPE, bool PF = false, bool DB = false, int BC = 0>
 __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_tiled2(
     const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
     const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
@@ -965,6 +995,6 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE
     const int zc = (order & 1) ? b % ((nblk + ntile - 1) / ntile) : b / ntile;
     const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY, z0 = zc * tz;
     const int zend = min(z0 + tz, nz);
-    tile_march2<P, TX, TY, PF, DB>(s_in, l_in, c_in, s_out, l_out, c_out, nx, ny, nz, x0, y0, z0,
-                                   zend, k, a1, a2);
+    tile_march2<P, TX, TY, PF, DB, BC>(s_in, l_in, c_in, s_out, l_out, c_out, nx, ny, nz, x0, y0,
+                                       z0, zend, k, a1, a2);
 }

@@ -214,7 +214,7 @@ struct FusedPick {
 };
 
 template <class P>
-static FusedPick<P> pick_fused(TileCfg t, int order)
+static FusedPick<P> pick_fused(TileCfg t, int order, int bc = PPF_BC_PERIODIC)
 {
     // k_tiled2: FP64 needs ~176 VGPRs: two waves per SIMD; at two waves per
     // SIMD the loads must be issued a plane ahead (PF) to be hidden.  Order
@@ -228,6 +228,18 @@ static FusedPick<P> pick_fused(TileCfg t, int order)
     constexpr bool F64 = sizeof(typename P::T) == 8;
     constexpr int W = F64 ? 2 : 5, WS = F64 ? 4 : 8;
     const int nt = t.tx * t.ty;
+    if (bc == PPF_BC_NOFLUX) {
+        // mirror boundaries: FP64 at three waves per SIMD, FP32 at five,
+        // double-buffered LDS (order bits 2 and 3 have no effect)
+        constexpr int WN = F64 ? 3 : 5;
+        if (t.tx == 16 && t.ty == 16)
+            return {k_tiled2<P, 16, 16, WN, false, true, PPF_BC_NOFLUX>, nt};
+        if (t.tx == 32 && t.ty == 8)
+            return {k_tiled2<P, 32, 8, WN, false, true, PPF_BC_NOFLUX>, nt};
+        if (!F64 && t.tx == 32 && t.ty == 16)
+            return {k_tiled2<P, 32, 16, WN, false, true, PPF_BC_NOFLUX>, nt};
+        return {nullptr, 0};
+    }
     if (order & 8) {
         const int ring = (2 * t.tx + 2 * t.ty + 63) / 64 * 64;
         if (t.tx == 16 && t.ty == 16)
@@ -252,9 +264,17 @@ static FusedPick<P> pick_fused(TileCfg t, int order)
     return {nullptr, 0};
 }
 
+// Whether a two-step kernel is compiled for tile t and order o at the
+// context's precision and boundary condition.
+static bool fused_exists(const ppf_ctx *ctx, TileCfg t, int o)
+{
+    return ctx->d.precision == PPF_F32 ? pick_fused<PhysF32>(t, o, ctx->d.bc).f != nullptr
+                                       : pick_fused<PhysF64<true>>(t, o, ctx->d.bc).f != nullptr;
+}
+
 static bool fused_available(const ppf_ctx *ctx)
 {
-    return ctx->fuse == 2 && ctx->d.bc == PPF_BC_PERIODIC && ctx->kernel == PPF_KERNEL_TILED;
+    return ctx->fuse == 2 && ctx->kernel == PPF_KERNEL_TILED;
 }
 
 template <class P>
@@ -265,7 +285,7 @@ static int launch_fused(ppf_ctx *ctx, const typename P::K &k)
     const int ntx = (ctx->d.nx + ctx->tile2.tx - 1) / ctx->tile2.tx;
     const int nty = (ctx->d.ny + ctx->tile2.ty - 1) / ctx->tile2.ty;
     const int nzc = (ctx->d.nz + ctx->tz2 - 1) / ctx->tz2;
-    const FusedPick<P> pk = pick_fused<P>(ctx->tile2, ctx->order2);
+    const FusedPick<P> pk = pick_fused<P>(ctx->tile2, ctx->order2, ctx->d.bc);
     if (!pk.f)
         return set_err(ctx, PPF_EARG, "no two-step kernel for tile %dx%d order %d", ctx->tile2.tx,
                        ctx->tile2.ty, ctx->order2);
@@ -423,7 +443,8 @@ static void choose_tiles(ppf_ctx *ctx)
         ctx->tile = {16, 16};
     ctx->order = f32 ? 5 : 0;
     ctx->tz = ctx->d.nz < 32 ? ctx->d.nz : 32;
-    // two steps per launch: on by default for periodic grids.  FP32: 32x16
+    // two steps per launch: on by default (periodic and mirror boundaries;
+    // the mirror kernels have no ring waves and no order bits 2-3).  FP32: 32x16
     // tiles, tz 64, XCD order (1.2x the single-step rate at 512^3; level
     // with 32x8 there and 14% faster at 256^3, 233 against 205 GLUPS).
     // FP64: 16x16 tiles, tz 64, XCD order, three waves per SIMD (order bit
@@ -431,7 +452,7 @@ static void choose_tiles(ppf_ctx *ctx)
     // 93 at 256^3, 112 against 87 at 1024^3.
     // FP32 runs the ring-wave kernel (order bit 3): 221 against 211 GLUPS at
     // 256^3, level at 512^3.
-    ctx->fuse = ctx->d.bc == PPF_BC_PERIODIC ? 2 : 1;
+    ctx->fuse = 2;
     ctx->tile2 = f32 ? TileCfg{32, 16} : TileCfg{16, 16};
     ctx->order2 = f32 ? 10 : 6;
     // z chunks of 64 planes, shortened (to 16 at least) while the launch
@@ -974,8 +995,8 @@ extern "C" int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms)
     ctx->tile = bt;
     ctx->tz = btz;
     ctx->order = bo;
-    // periodic grids: the two-step kernel too, kept when faster per step
-    if (ctx->d.bc == PPF_BC_PERIODIC) {
+    // the two-step kernel too, kept when faster per step
+    {
         const TileCfg keep2 = ctx->tile2;
         const int keep_tz2 = ctx->tz2, keep_o2 = ctx->order2;
         static const int tzs2[] = {32, 64};
@@ -983,8 +1004,9 @@ extern "C" int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms)
         // FP64 variants (no effect in FP32)
         static const int orders2_64[] = {0, 2, 4, 6, 8, 10}, orders2_32[] = {0, 2, 8, 10};
         const bool f64 = ctx->d.precision == PPF_F64;
+        const bool mirror = ctx->d.bc == PPF_BC_NOFLUX;  // bits 2 and 3 have no effect there
         const int *orders2 = f64 ? orders2_64 : orders2_32;
-        const int norders2 = f64 ? 6 : 4;
+        const int norders2 = mirror ? 2 : f64 ? 6 : 4;
         double best2 = 1e300;
         TileCfg bt2 = keep2;
         int btz2 = keep_tz2, bo2 = keep_o2;
@@ -1005,8 +1027,8 @@ extern "C" int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms)
             for (int tz : tzs2)
                 for (int oi = 0; oi < norders2; oi++) {
                     const int o = orders2[oi];
-                    if (!pick_fused<PhysF32>(c, o).f)
-                        continue;  // no ring-wave kernel for this tile
+                    if (!fused_exists(ctx, c, o))
+                        continue;  // no kernel of this kind for this tile
                     ctx->tile2 = c;
                     ctx->tz2 = tz > ctx->d.nz ? ctx->d.nz : tz;
                     ctx->order2 = o;
@@ -1059,8 +1081,6 @@ extern "C" int ppf_set_fusion(ppf_ctx *ctx, int steps_per_launch, int tx, int ty
     if (steps_per_launch == 2) {
         if (ctx->kernel != PPF_KERNEL_TILED)
             return set_err(ctx, PPF_ESTATE, "two steps per launch need the tiled kernels");
-        if (ctx->d.bc != PPF_BC_PERIODIC)
-            return set_err(ctx, PPF_ESTATE, "two steps per launch need periodic boundaries");
     }
     if (tx != 0 || ty != 0) {
         bool known = false;
@@ -1073,11 +1093,13 @@ extern "C" int ppf_set_fusion(ppf_ctx *ctx, int steps_per_launch, int tx, int ty
         return set_err(ctx, PPF_EARG, "tz must be >= 0");
     if (order > 15)
         return set_err(ctx, PPF_EARG, "order must be 0..15 (or negative to keep)");
-    // the ring-wave kernel (order bit 3) exists for some tiles only
+    // the ring-wave kernel (order bit 3) and the mirror-boundary kernels
+    // exist for some tiles only
     const TileCfg t = (tx != 0 || ty != 0) ? TileCfg{tx, ty} : ctx->tile2;
     const int o = order >= 0 ? order : ctx->order2;
-    if (!pick_fused<PhysF32>(t, o).f)
-        return set_err(ctx, PPF_EARG, "two-step tile %dx%d has no order-%d variant", t.tx, t.ty, o);
+    if (!fused_exists(ctx, t, o))
+        return set_err(ctx, PPF_EARG, "two-step tile %dx%d has no order-%d variant%s", t.tx, t.ty,
+                       o, ctx->d.bc == PPF_BC_NOFLUX ? " for no-flux boundaries" : "");
     ctx->tile2 = t;
     if (tz > 0)
         ctx->tz2 = tz > ctx->d.nz ? ctx->d.nz : tz;

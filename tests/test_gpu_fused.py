@@ -1,5 +1,6 @@
 """Two steps per launch (k_tiled2): bit for bit the same as single steps, for
-every compiled two-step tile, z chunk and register variant, on ragged and tiny periodic grids,
+every compiled two-step tile, z chunk and register variant, with periodic
+and with mirror (no-flux) boundaries, on ragged and tiny periodic grids,
 for odd step counts (a single step finishes), in FP64 and in FP32 with mass
 shifts that fall between the two steps of a pair; and against the
 reference's own outputs."""
@@ -25,14 +26,17 @@ def _state(shape, seed):
     return s, l, c
 
 
-def _run(shape, state, nsteps, precision, fuse, tile=(0, 0), tz=0, order=-1, shift=0):
-    with S.Solver(shape, precision=precision) as sv:
+def _run(shape, state, nsteps, precision, fuse, tile=(0, 0), tz=0, order=-1, shift=0,
+         bc=S.BC_PERIODIC):
+    with S.Solver(shape, precision=precision, bc=bc) as sv:
         sv.upload(*state)
         if precision == S.F32:
             sv.set_mass_correction(shift)
         if fuse:
             sv.set_fusion(2, tile[0], tile[1], tz, order)
             assert sv.get_fusion()[0] == 2
+        else:
+            sv.set_fusion(1)
         sv.step(nsteps)
         return sv.download()
 
@@ -67,15 +71,48 @@ def test_fp32_fused_equals_single_steps_with_mass_shifts(shape, tile):
             np.testing.assert_array_equal(a, b, err_msg=f"order={order}")
 
 
+NOFLUX_TILES = {S.F64: [(16, 16), (32, 8)], S.F32: [(16, 16), (32, 8), (32, 16)]}
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("tile", NOFLUX_TILES[S.F64])
+def test_fp64_noflux_fused_equals_single_steps(shape, tile):
+    """Mirror boundaries: ghost ring sites and ghost planes take their
+    mirror site's step t+1; z chunks of 1 and 3 planes put chunk ends on the
+    grid's first and last planes, and (1, 1, 1) mirrors every face onto the
+    one site."""
+    print(f"noflux shape={shape} tile={tile}", flush=True)
+    st = _state(shape, 17)
+    want = _run(shape, st, 7, S.F64, False, bc=S.BC_NOFLUX)
+    for tz in (1, 3, 32):
+        for order in (0, 2, 1):
+            got = _run(shape, st, 7, S.F64, True, tile, tz, order=order, bc=S.BC_NOFLUX)
+            for a, b in zip(got, want):
+                np.testing.assert_array_equal(a, b, err_msg=f"tz={tz} order={order}")
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (20, 16, 12), (33, 17, 65), (16, 64, 64)])
+@pytest.mark.parametrize("tile", NOFLUX_TILES[S.F32])
+def test_fp32_noflux_fused_equals_single_steps_with_mass_shifts(shape, tile):
+    print(f"noflux f32 shape={shape} tile={tile}", flush=True)
+    st = _state(shape, 19)
+    want = _run(shape, st, 23, S.F32, False, shift=5, bc=S.BC_NOFLUX)
+    got = _run(shape, st, 23, S.F32, True, tile, 8, order=2, shift=5, bc=S.BC_NOFLUX)
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_fused_matches_reference_outputs(golden):
-    """The periodic golden cases (the reference program's own outputs),
-    stepped with pairs."""
+    """The golden cases (the reference program's own outputs, periodic and
+    no-flux), stepped with pairs."""
     man, arrays = golden
-    for case in ("p16_s10", "p16_s100", "p16_s1000", "pnc_s100", "p16compat_s10"):
+    for case in ("p16_s10", "p16_s100", "p16_s1000", "pnc_s100", "p16compat_s10", "n16_s10",
+                 "nnc_s100", "nnc_compat_s10"):
         m, z = man["cases"][case], arrays[case]
-        assert m["bc"] == "periodic" and m["n_iter_init"] >= m["sweeps_before_output"][-1]
+        assert m["n_iter_init"] >= m["sweeps_before_output"][-1]
         print(f"case={case}", flush=True)
-        with S.Solver(z["s0"].shape) as sv:
+        bc = S.BC_PERIODIC if m["bc"] == "periodic" else S.BC_NOFLUX
+        with S.Solver(z["s0"].shape, bc=bc) as sv:
             sv.set_fusion(2)
             sv.upload(z["s0"], z["l0"] if m["etaL"] == "read" else None, z["c0"])
             done = 0
@@ -96,6 +133,7 @@ def test_fused_512_equals_single_steps():
     with S.Solver(shape) as a, S.Solver(shape) as b:
         a.init_spheres(cen, rad)
         b.init_spheres(cen, rad)
+        a.set_fusion(1)
         b.set_fusion(2)
         a.step(4)
         b.step(4)
@@ -104,7 +142,17 @@ def test_fused_512_equals_single_steps():
 
 
 def test_fusion_refused_where_not_offered():
-    with S.Solver((8, 8, 8), bc=S.BC_NOFLUX) as sv:
+    """The naive kernel has no two-step form; no-flux grids have no 64x8 or
+    ring-wave two-step kernel.  A refused call leaves the setting as it was."""
+    with S.Solver((8, 8, 8), kernel=S.KERNEL_NAIVE) as sv:
         with pytest.raises(RuntimeError):
             sv.set_fusion(2)
         assert sv.get_fusion()[0] == 1
+    with S.Solver((8, 8, 8), bc=S.BC_NOFLUX) as sv:
+        before = sv.get_fusion()
+        assert before[0] == 2  # two steps per launch by default
+        with pytest.raises(RuntimeError):
+            sv.set_fusion(2, 64, 8, 0, -1)
+        with pytest.raises(RuntimeError):
+            sv.set_fusion(2, 32, 16, 0, -1)  # FP64: no mirror kernel for 32x16
+        assert sv.get_fusion() == before
