@@ -569,7 +569,15 @@ __device__ __forceinline__ void tile_march2(
         // boundaries: a ghost plane (zs = -1 or nz) is not stepped; plane
         // nz's step t+1 is plane nz-1's (plane -1's is set after plane 0).
         const bool zghost = BC == 1 && (zs < 0 || zs >= nz);
-        T s1 = us_c, l1 = ul_c, c1 = uc_c, h1 = uh_c, mu1 = umu_c, M1 = uM_c;
+        T s1, l1, c1, h1, mu1, M1;
+        if constexpr (BC == 1) {
+            s1 = us_c;
+            l1 = ul_c;
+            c1 = uc_c;
+            h1 = uh_c;
+            mu1 = umu_c;
+            M1 = uM_c;
+        }
         if (!zghost) {
             const int ax = tx + 2, ay = ty + 2;
             const T sn[6] = {rs_n, s_p, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
@@ -595,26 +603,30 @@ __device__ __forceinline__ void tile_march2(
         }
         // A2: step t+1 of plane zs (a buffer of its own with DB; otherwise
         // after stage 2 and a barrier, below)
-        auto put_a2 = [&](int by, int bx) {
-            A2[0][by][bx] = s1;
-            A2[1][by][bx] = l1;
-            A2[2][by][bx] = mu1;
-            A2[3][by][bx] = M1;
-        };
+        // (plain stores: routing them through a helper lambda changed the
+        // periodic kernel's register allocation and made it spill)
+#define PPF_PUT_A2(BY, BX)          \
+    do {                            \
+        A2[0][BY][BX] = s1;         \
+        A2[1][BY][BX] = l1;         \
+        A2[2][BY][BX] = mu1;        \
+        A2[3][BY][BX] = M1;         \
+    } while (0)
         auto store_a2 = [&]() {
-            if (BC == 0) {
-                put_a2(ty + 1, tx + 1);
-            } else if (active) {
-                // the site, and the ghost slots it mirrors
-                put_a2(ty + 1, tx + 1);
-                if (x0 + tx == 0)
-                    put_a2(ty + 1, tx);
-                if (x0 + tx == nx - 1)
-                    put_a2(ty + 1, tx + 2);
-                if (y0 + ty == 0)
-                    put_a2(ty, tx + 1);
-                if (y0 + ty == ny - 1)
-                    put_a2(ty + 2, tx + 1);
+            if (BC == 0 || active)
+                PPF_PUT_A2(ty + 1, tx + 1);
+            if constexpr (BC == 1) {
+                // the ghost slots this site mirrors
+                if (active) {
+                    if (x0 + tx == 0)
+                        PPF_PUT_A2(ty + 1, tx);
+                    if (x0 + tx == nx - 1)
+                        PPF_PUT_A2(ty + 1, tx + 2);
+                    if (y0 + ty == 0)
+                        PPF_PUT_A2(ty, tx + 1);
+                    if (y0 + ty == ny - 1)
+                        PPF_PUT_A2(ty + 2, tx + 1);
+                }
             }
             if (ring && ring_in) {
                 A2[0][ry][rx] = qs1;
@@ -623,6 +635,7 @@ __device__ __forceinline__ void tile_march2(
                 A2[3][ry][rx] = qM1;
             }
         };
+#undef PPF_PUT_A2
         if (DB)
             store_a2();
         // stage 2: step t+2 of plane zs-1 at the tile site (A2p holds step
@@ -675,7 +688,7 @@ __device__ __forceinline__ void tile_march2(
         uh_c = h1;
         umu_c = mu1;
         uM_c = M1;
-        if (BC == 1 && z0 == 0 && zs == 0) {
+        if (BC == 1 && z0 == 0 && zs == 0) {  // (compiled out for periodic grids)
             // mirror boundary below plane 0: ghost plane -1 holds plane 0
             us_p = us_c;
             ul_p = ul_c;
