@@ -279,7 +279,9 @@ def main():
                      "steps_per_launch": spl,
                      "hbm_bytes_per_site_update": BYTES_PER_SITE["f64"] / spl,
                      "one_step_equivalent_GBs": round(equiv, 1),
-                     "one_step_equivalent_frac": round(equiv / HBM_PEAK_GBS, 4)},
+                     "one_step_equivalent_frac": round(equiv / HBM_PEAK_GBS, 4),
+                     "kernel_limit": ("FP64 VALU latency: two steps per launch move half the "
+                                      "bytes of two single steps" if spl == 2 else "HBM")},
         "solute_drift_rel": abs(r["stats1"]["sum_c"] - r["stats0"]["sum_c"]) /
         abs(r["stats0"]["sum_c"]),
     }
