@@ -272,10 +272,10 @@ int ppf_ens_step_c_only(ppf_ens *e, int64_t nsteps);
  * threads each), tz planes per chunk. */
 int ppf_ens_set_tiling(ppf_ens *e, int dom, int shape, int tz);
 int ppf_ens_get_tiling(ppf_ens *e, int dom, int *shape, int *tz);
-/* Two steps per launch for periodic members (bit for bit two single
- * steps; pairs never straddle an FP32 mass shift): steps_per_launch 1 or 2.
- * On by default for periodic FP32 ensembles.  The two-step kernel has its
- * own member tiling: shape 0 = 32x8, 1 = 16x16, tz planes per chunk. */
+/* Two steps per launch (bit for bit two single steps, periodic or no-flux
+ * members; pairs never straddle an FP32 mass shift): steps_per_launch 1 or
+ * 2.  On by default for FP32 ensembles.  The two-step kernel has its own
+ * member tiling: shape 0 = 32x8, 1 = 16x16, tz planes per chunk. */
 int ppf_ens_set_fusion(ppf_ens *e, int steps_per_launch);
 int ppf_ens_get_fusion(ppf_ens *e, int *steps_per_launch);
 int ppf_ens_set_tiling2(ppf_ens *e, int dom, int shape, int tz);

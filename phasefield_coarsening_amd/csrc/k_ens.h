@@ -132,7 +132,7 @@ static __global__ __launch_bounds__(256) void k_ens_shift(const EnsDom *__restri
 }
 
 // ---------------------------------------------------------------------------
-// Two steps per launch (periodic members): the same persistent work list
+// Two steps per launch (periodic or, with BC = 1, no-flux members): the same persistent work list
 // walk with tile_march2 as the item body, so each member advances two steps
 // bit for bit as two single steps.  tile_march2 needs its two rings within
 // the workgroup (4 TX + 4 TY + 4 <= 256), so the shapes are 32x8 and 16x16.
@@ -148,7 +148,7 @@ constexpr int ENS2_LDS = 4 * 12 * 36 + 4 * 10 * 34;
 // Register target: FP32 members five waves per SIMD (the single-grid
 // kernel's budget, no spills); FP64 two (~200 VGPRs with both member shapes
 // in one kernel; three waves per SIMD spill).
-template <class P>
+template <class P, int BC = 0>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(typename P::T) == 4 ? 5 : 2))) void k_ensemble2(
     const EnsDom *__restrict__ doms,
@@ -181,11 +181,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(typename P::T) == 4 ? 5 : 2))) void k_
         T *co = (T *)d->f[1 - parity][2];
         const int dnx = d->nx, dny = d->ny, dnz = d->nz;
         if (w.shape == ENS2_32x8)
-            tile_march2<P, 32, 8, false, DB>(si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0,
+            tile_march2<P, 32, 8, false, DB, BC>(si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0,
                                              w.z0, w.zend, k, reinterpret_cast<T(*)[12][36]>(sh),
                                              reinterpret_cast<T(*)[10][34]>(sh + NB * 4 * 12 * 36));
         else
-            tile_march2<P, 16, 16, false, DB>(si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0,
+            tile_march2<P, 16, 16, false, DB, BC>(si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0,
                                               w.z0, w.zend, k, reinterpret_cast<T(*)[20][20]>(sh),
                                               reinterpret_cast<T(*)[18][18]>(sh + NB * 4 * 20 * 20));
     }

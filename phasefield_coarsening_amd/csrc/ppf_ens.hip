@@ -54,7 +54,7 @@ struct ppf_ens {
     EnsItem *ditems;
     size_t items_cap;
     int nitems;
-    // two steps per launch (periodic): fuse = 2 turns it on; tiling2 holds
+    // two steps per launch: fuse = 2 turns it on; tiling2 holds
     // the two-step shapes (ENS2_*), items2 their work list
     int fuse;
     std::vector<Tiling> tiling2;
@@ -197,15 +197,28 @@ static int upload_items2(ppf_ens *e, const std::vector<EnsItem> &items)
 static int launch2(ppf_ens *e)
 {
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(256), 0, e->stream, e->counter, (size_t)1, 0);
-    if (e->prec == PPF_F32)
-        hipLaunchKernelGGL(k_ensemble2<PhysF32>, dim3(e->grid), dim3(256), 0, e->stream,
-                           e->ddoms, e->ditems2, e->nitems2, e->counter, e->parity, e->k32);
-    else if (e->eqa)
-        hipLaunchKernelGGL(k_ensemble2<PhysF64<true>>, dim3(e->grid), dim3(256), 0, e->stream,
-                           e->ddoms, e->ditems2, e->nitems2, e->counter, e->parity, e->k);
-    else
-        hipLaunchKernelGGL(k_ensemble2<PhysF64<false>>, dim3(e->grid), dim3(256), 0, e->stream,
-                           e->ddoms, e->ditems2, e->nitems2, e->counter, e->parity, e->k);
+    // (no-flux members: mirror ghosts in tile_march2, BC = 1)
+#define ENS2(P, BC, K)                                                                       \
+    hipLaunchKernelGGL((k_ensemble2<P, BC>), dim3(e->grid), dim3(256), 0, e->stream, e->ddoms, \
+                       e->ditems2, e->nitems2, e->counter, e->parity, K)
+    const bool nf = e->bc == PPF_BC_NOFLUX;
+    if (e->prec == PPF_F32) {
+        if (nf)
+            ENS2(PhysF32, 1, e->k32);
+        else
+            ENS2(PhysF32, 0, e->k32);
+    } else if (e->eqa) {
+        if (nf)
+            ENS2(PhysF64<true>, 1, e->k);
+        else
+            ENS2(PhysF64<true>, 0, e->k);
+    } else {
+        if (nf)
+            ENS2(PhysF64<false>, 1, e->k);
+        else
+            ENS2(PhysF64<false>, 0, e->k);
+    }
+#undef ENS2
     EHIP(e, hipGetLastError());
     return PPF_OK;
 }
@@ -303,9 +316,9 @@ extern "C" int ppf_ens_create(ppf_ens **out, int ndom, const int *dims_zyx, int 
     e->nsite.resize(ndom);
     e->tiling.resize(ndom);
     e->tiling2.resize(ndom);
-    // two steps per launch by default for periodic FP32 members (faster
-    // there, as for single grids); FP64 steps one step per launch
-    e->fuse = (precision == PPF_F32 && bc == PPF_BC_PERIODIC) ? 2 : 1;
+    // two steps per launch by default for FP32 members (faster there, as
+    // for single grids); FP64 steps one step per launch
+    e->fuse = precision == PPF_F32 ? 2 : 1;
     e->targets.assign(ndom, 0.0);
     // pool: every array 256-byte aligned
     std::vector<size_t> off(ndom);
@@ -567,7 +580,7 @@ static int ens_steps(ppf_ens *e, int64_t nsteps, bool conly)
     if (!e || nsteps < 0)
         return e ? eset(e, PPF_EARG, "negative step count") : PPF_EARG;
     EHIP(e, hipSetDevice(e->device));
-    const bool fused = !conly && e->fuse == 2 && e->bc == PPF_BC_PERIODIC;
+    const bool fused = !conly && e->fuse == 2;
     const bool shifts = e->prec == PPF_F32 && e->shift_every > 0;
     for (int64_t i = 0; i < nsteps;) {
         // a pair in one launch unless a mass shift falls after its first step
@@ -606,8 +619,6 @@ extern "C" int ppf_ens_set_fusion(ppf_ens *e, int steps_per_launch)
 {
     if (!e || (steps_per_launch != 1 && steps_per_launch != 2))
         return e ? eset(e, PPF_EARG, "steps per launch must be 1 or 2") : PPF_EARG;
-    if (steps_per_launch == 2 && e->bc != PPF_BC_PERIODIC)
-        return eset(e, PPF_ESTATE, "two steps per launch need periodic members");
     e->fuse = steps_per_launch;
     return PPF_OK;
 }
@@ -616,7 +627,7 @@ extern "C" int ppf_ens_get_fusion(ppf_ens *e, int *steps_per_launch)
 {
     if (!e || !steps_per_launch)
         return e ? eset(e, PPF_EARG, "null argument") : PPF_EARG;
-    *steps_per_launch = (e->fuse == 2 && e->bc == PPF_BC_PERIODIC) ? 2 : 1;
+    *steps_per_launch = e->fuse == 2 ? 2 : 1;
     return PPF_OK;
 }
 
@@ -706,9 +717,9 @@ extern "C" int ppf_ens_autotune(ppf_ens *e, int reps)
             e->tiling[d] = bt;
     }
     int rc = upload_items(e, build_items(e, {}));
-    if (rc || e->bc != PPF_BC_PERIODIC)
+    if (rc)
         return rc;
-    // periodic: the two-step shapes per class too, then the faster mode
+    // the two-step shapes per class too, then the faster mode
     auto time_ms = [&](bool two, int n, float *ms) -> int {
         int r = two ? launch2(e) : launch(e, false);  // warm
         if (r)

@@ -74,18 +74,19 @@ def test_ensemble_fp32_mass_targets():
             assert st["n_nonfinite"] == 0
 
 
+@pytest.mark.parametrize("bc", [S.BC_PERIODIC, S.BC_NOFLUX])
 @pytest.mark.parametrize("prec", [S.F64, S.F32])
-def test_ensemble_two_step_equals_single_steps(prec):
-    """Two steps per launch (periodic members, both two-step shapes, several
-    z chunks, FP32 mass shifts every 5 steps so pairs split around them):
-    every member bitwise equal to the one-step-per-launch ensemble after an
-    odd number of steps."""
-    print(f"prec={prec}", flush=True)
+def test_ensemble_two_step_equals_single_steps(prec, bc):
+    """Two steps per launch (periodic or no-flux members, both two-step
+    shapes, several z chunks, FP32 mass shifts every 5 steps so pairs split
+    around them): every member bitwise equal to the one-step-per-launch
+    ensemble after an odd number of steps."""
+    print(f"prec={prec} bc={bc}", flush=True)
     dims = DIMS + [(1, 1, 1), (3, 5, 2)]
     states = [_member_state(d, 40 + i) for i, d in enumerate(dims)]
 
     def run(fuse):
-        ens = S.Ensemble(dims, precision=prec)
+        ens = S.Ensemble(dims, bc=bc, precision=prec)
         try:
             for i, st in enumerate(states):
                 ens.upload(i, *st)
