@@ -447,9 +447,10 @@ static void choose_tiles(ppf_ctx *ctx)
     // the mirror kernels have no ring waves and no order bits 2-3).  FP32: 32x16
     // tiles, tz 64, XCD order (1.2x the single-step rate at 512^3; level
     // with 32x8 there and 14% faster at 256^3, 233 against 205 GLUPS).
-    // FP64: 16x16 tiles, tz 64, XCD order, three waves per SIMD (order bit
-    // 2): 117-119 against 101 GLUPS for single steps at 512^3, 104 against
-    // 93 at 256^3, 112 against 87 at 1024^3.
+    // FP64: 16x16 tiles, XCD order, three waves per SIMD (order bit 2), the
+    // z chunk chosen below: 117-121 against 101-103 GLUPS for single steps
+    // at 512^3, 114-117 against 93-98 at 256^3, 116-118 against 87-93 at
+    // 1024^3.
     // FP32 runs the ring-wave kernel (order bit 3): 221 against 211 GLUPS at
     // 256^3, level at 512^3.
     ctx->fuse = 2;
