@@ -346,7 +346,7 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
 }
 
 // ---------------------------------------------------------------------------
-// Two steps per launch (periodic grids).  The new state of step t+2 is
+// Two steps per launch (periodic grids; BC = 1: no-flux).  The new state of step t+2 is
 // computed from step t in one z march, with step t+1 kept on chip:
 //   stage 1, plane z:   step t -> t+1 on the tile and its one-cell face ring
 //                        (the ring sites are the in-plane neighbours stage 2
@@ -357,8 +357,10 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
 //                        the tile and its ring) and registers (z neighbours).
 // Every site, ring sites included, is updated at its periodic image with the
 // same operands in the same order as a single step, so two steps here equal
-// two single steps bit for bit.  (Mirror boundaries would reorder the two x
-// or y neighbours of an image site, so they are not offered.)  Each output
+// two single steps bit for bit.  With mirror (no-flux) boundaries a ring site
+// or plane outside the grid is not updated at all (computing it at its image
+// would reorder the image site's two x or y neighbours): it takes the step
+// t+1 of the site it mirrors, which that site's thread writes.  Each output
 // plane is read and written once for two steps: 24 B per site update in
 // FP64 instead of 48.  A z chunk [z0, zend) runs stage 1 over
 // [z0-1, zend] and so reads step t over [z0-2, zend+1].

@@ -199,7 +199,7 @@ static int launch_tiled(ppf_ctx *ctx, const typename P::K &k, bool conly, double
     return PPF_OK;
 }
 
-// Two steps per launch (k_tiled2, periodic only).
+// Two steps per launch (k_tiled2, k_tiled2s).
 template <class P>
 using fused_fn = void (*)(const typename P::T *, const typename P::T *, const typename P::T *,
                           typename P::T *, typename P::T *, typename P::T *, int, int, int, int,

@@ -41,7 +41,7 @@ struct ppf_ctx {
     TileCfg tile;
     int tz;
     int order;  // workgroup order: bit 0 z-chunk fastest, bit 1 XCD-contiguous
-    // Two steps per launch (periodic tiled contexts): fuse = 2 turns it on;
+    // Two steps per launch (tiled contexts): fuse = 2 turns it on;
     // tile2 / tz2 / order2 are that kernel's tiling.
     int fuse;
     TileCfg tile2;

@@ -278,9 +278,9 @@ class Solver:
         return tuple(a.value for a in v)
 
     def set_fusion(self, steps_per_launch, tx=0, ty=0, tz=0, order=-1):
-        """Two steps per launch (periodic grids; bitwise equal to single
-        steps): steps_per_launch 1 or 2, with that kernel's tiling ((0, 0, 0,
-        -1) keeps the current one)."""
+        """Two steps per launch (bitwise equal to single steps; on by
+        default): steps_per_launch 1 or 2, with that kernel's tiling ((0, 0,
+        0, -1) keeps the current one)."""
         self._check(lib().ppf_set_fusion(self._h, int(steps_per_launch), int(tx), int(ty),
                                          int(tz), int(order)))
 
@@ -414,7 +414,7 @@ class Ensemble:
         self._check(lib().ppf_ens_autotune(self._h, int(reps)))
 
     def set_fusion(self, steps_per_launch):
-        """Two steps per launch (periodic members): 1 or 2."""
+        """Two steps per launch (bitwise equal to single steps): 1 or 2."""
         self._check(lib().ppf_ens_set_fusion(self._h, int(steps_per_launch)))
 
     def get_fusion(self):
