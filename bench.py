@@ -63,6 +63,35 @@ def copy_ceiling(nbytes=3 << 30, iters=20):
     return 2 * nbytes / (ms.value * 1e-3) / 1e9
 
 
+def stream3_rate(n, pattern, tz=32, iters=20):
+    """Measured rate (GB/s, 48 B per site) of a plain 3-in/3-out FP64 stream
+    over n^3 sites with libppfcal.so's ppfcal_stream3: pattern 0 in flat
+    memory order, 1-3 in the z-marching access pattern of the single-step
+    kernel (64x8, 128x4, 512x1 column tiles, tz-plane chunks) with no
+    stencil: the ceiling of that access pattern."""
+    import ctypes
+
+    from phasefield_coarsening_amd import _lib
+    cal = _lib.load("libppfcal.so")
+    cal.ppfcal_stream3.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_double)]
+    ms = ctypes.c_double()
+    rc = cal.ppfcal_stream3(0, n, n, n, pattern, tz, iters, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"ppfcal_stream3 failed ({rc})")
+    return 48.0 * n ** 3 / (ms.value * 1e-3) / 1e9
+
+
+def ceilings(n):
+    """Every stream pattern and chunk length: one JSON line."""
+    out = {"grid": [n, n, n], "bytes_per_site": 48, "unit": "GB/s",
+           "flat": round(stream3_rate(n, 0), 1)}
+    for pattern, tile in ((1, "64x8"), (2, "128x4"), (3, "512x1")):
+        for tz in (8, 16, 32, 64, 128):
+            if n % tz == 0:
+                out[f"march_{tile}_tz{tz}"] = round(stream3_rate(n, pattern, tz), 1)
+    return out
+
+
 def run_gpu(S, shape, precision, steps, warmup, seed=2, nsph=2000, rmin=6.0, rmax=14.0,
             fuse=None):
     cen, rad = S.spheres(seed, nsph, shape, rmin, rmax)
@@ -236,12 +265,18 @@ def main():
     ap.add_argument("--no-copy", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the ensemble and U-Net lines")
+    ap.add_argument("--ceilings", action="store_true",
+                    help="only measure the 3-in/3-out stream ceilings (flat and z-marching "
+                         "patterns, every chunk length) at --size and print them")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(multi-GPU sharding is design only)", file=sys.stderr)
         sys.exit(2)
 
+    if a.ceilings:
+        print(json.dumps(ceilings(a.size)), flush=True)
+        return
     from phasefield_coarsening_amd import solver as S
     n = a.size
     shape = (n, n, n)
@@ -329,6 +364,11 @@ def main():
         out["roofline"]["copy_ceiling_GBs"] = round(cc, 1)
         out["roofline"]["frac_of_copy_ceiling"] = round(achieved / cc, 4)
         out["roofline"]["one_step_equivalent_frac_of_copy_ceiling"] = round(equiv / cc, 4)
+        if n % 64 == 0 and n % 32 == 0:
+            # the same bytes as a plain stream: in flat order, and in the
+            # single-step kernel's z-marching pattern (64x8 tiles, 32 planes)
+            out["roofline"]["stream3_flat_GBs"] = round(stream3_rate(n, 0), 1)
+            out["roofline"]["stream3_zmarch_GBs"] = round(stream3_rate(n, 1, 32), 1)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(S, shape, a.cpu_steps)
         ref = reference_serial_baseline(S, shape)
