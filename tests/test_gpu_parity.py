@@ -249,3 +249,33 @@ def test_tilings_and_march_variants_bitwise(bc):
                     got = sv.download()
                     for g, w in zip(got, want[:3]):
                         np.testing.assert_array_equal(g, w, err_msg=f"{tile} tz={tz} order={order}")
+
+
+@pytest.mark.parametrize("bc", [S.BC_PERIODIC, S.BC_NOFLUX])
+def test_general_constants_bitwise_vs_oracle(bc):
+    """Constants with AS != AL select the general FP64 kernels (the two
+    divides by AS+(AL-AS)*h of phasefield.c:331 and :351 that the
+    reference's own constants, AS == AL, fold away): naive, single steps and
+    two steps per launch, on a ragged grid, bitwise the oracle run with the
+    same constants (AL changed, every other constant the reference's)."""
+    shape = (13, 20, 37)
+    print(f"AS != AL bc={bc}", flush=True)
+    rng = np.random.default_rng(23)
+    s = rng.uniform(0.05, 0.95, shape)
+    l = np.clip(1 - s + rng.uniform(-0.1, 0.1, shape), 0.0, 1.0)
+    c = 0.1 + 0.8 * s + rng.uniform(-0.02, 0.02, shape)
+    po = O.default_params()
+    po.AL = 1.25e9
+    want = O.run(s, l, c, 7, bc=bc, params=po)
+    pg = S.default_params()
+    for f, _ in S.Params._fields_:
+        setattr(pg, f, getattr(po, f))
+    for kernel, fuse in ((S.KERNEL_NAIVE, 1), (S.KERNEL_TILED, 1), (S.KERNEL_TILED, 2)):
+        with S.Solver(shape, bc=bc, kernel=kernel, params=pg) as sv:
+            if kernel == S.KERNEL_TILED:
+                sv.set_fusion(fuse)
+            sv.upload(s, l, c)
+            sv.step(7)
+            got = sv.download()
+        for g, w in zip(got, want[:3]):
+            np.testing.assert_array_equal(g, w, err_msg=f"kernel={kernel} fuse={fuse}")
