@@ -37,6 +37,30 @@ def test_ensemble_fp64_members_bitwise_vs_oracle(bc):
             np.testing.assert_array_equal(g, w, err_msg=f"member {i} {DIMS[i]}")
 
 
+@pytest.mark.parametrize("bc", [S.BC_PERIODIC, S.BC_NOFLUX])
+def test_ensemble_general_constants_bitwise_vs_oracle(bc):
+    """AS != AL (the general FP64 kernels): members of uneven size, one and
+    two steps per launch, bitwise the oracle run with the same constants."""
+    dims = [(20, 16, 12), (33, 17, 65), (16, 32, 48)]
+    states = [_member_state(d, 30 + i) for i, d in enumerate(dims)]
+    po = O.default_params()
+    po.AL = 1.25e9
+    pg = S.default_params()
+    for f, _ in S.Params._fields_:
+        setattr(pg, f, getattr(po, f))
+    wants = [O.run(*st, 5, bc=bc, params=po) for st in states]
+    for fuse in (1, 2):
+        print(f"bc={bc} fuse={fuse}", flush=True)
+        with S.Ensemble(dims, bc=bc, precision=S.F64, params=pg) as ens:
+            for i, st in enumerate(states):
+                ens.upload(i, *st)
+            ens.set_fusion(fuse)
+            ens.step(5)
+            for i in range(len(dims)):
+                for g, w in zip(ens.download(i), wants[i][:3]):
+                    np.testing.assert_array_equal(g, w, err_msg=f"member {i} fuse={fuse}")
+
+
 def test_ensemble_fp32_members_bitwise_vs_single_grid():
     states = [_member_state(d, 20 + i) for i, d in enumerate(DIMS)]
     with S.Ensemble(DIMS, precision=S.F32) as ens:
