@@ -11,14 +11,22 @@ synchronisations.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--size N]
                     [--cpu-steps S] [--no-cpu]
 
-Prints one JSON line.  value = FP64 GLUPS = N^3 * K / seconds / 1e9.
-roofline: algorithmic bytes per launch (48 B/site in FP64: read etaS, etaL, c
-and write them; a two-step launch, the periodic default, does so once for two
-steps) over the kernel's average launch duration, from HIP events recorded on
-the stream the kernels run on, with the one-step-equivalent rate (48 B per
-site update) beside it; f64_single_step times the one-step kernel alone.  cpu_baseline: the CPU oracle (a
-restatement of the reference algorithm, bit-exact against it) timed on this
-host's cores on a bounded sample of the same workload.
+Prints one JSON line.  value = FP64 GLUPS = N^3 * K / seconds / 1e9 with the
+fast FP64 kernel (PPF_KERNEL_FAST: fused multiply-adds, one reciprocal per
+site, within 1e-10 per field of the reference after 1000 steps); --f64-kernel
+bitwise times the kernel that reproduces the reference bit for bit instead,
+and the line always carries the other one under f64_bitwise / f64_fast.
+roofline: algorithmic bytes per launch over the kernel's average launch
+duration (HIP events recorded on the stream the kernels run on), counted as
+the roofline defines them: 48 B per site update in FP64 (read etaS, etaL, c
+and write them), 24 B in FP32, times the site updates one launch makes (two
+per site with two steps per launch, the periodic default).  The bytes a
+launch must move (one read and one write of the state per launch) are
+reported beside it as moved_GBs / moved_frac, and the HBM traffic rocprofv3
+measured per launch as traffic.  f64_single_step times the one-step kernel
+alone.  cpu_baseline: the CPU oracle (a restatement of the reference
+algorithm, bit-exact against it) timed on this host's cores on a bounded
+sample of the same workload.
 """
 import argparse
 import json
@@ -93,15 +101,16 @@ def ceilings(n):
 
 
 def run_gpu(S, shape, precision, steps, warmup, seed=2, nsph=2000, rmin=6.0, rmax=14.0,
-            fuse=None):
+            fuse=None, kernel=0):
     cen, rad = S.spheres(seed, nsph, shape, rmin, rmax)
-    sv = S.Solver(shape, precision=precision)
+    sv = S.Solver(shape, precision=precision, kernel=kernel)
     sv.init_spheres(cen, rad)
     if fuse is not None:
         sv.set_fusion(fuse)
     st0 = sv.reduce()
     sv.step(warmup)
     sv.sync()
+    nsh0 = sv.mass_target()[1] if precision == S.F32 else 0
     t0 = time.perf_counter()
     sv.timer_start()
     sv.step(steps)
@@ -112,10 +121,11 @@ def run_gpu(S, shape, precision, steps, warmup, seed=2, nsph=2000, rmin=6.0, rma
     kname = sv.kernel_name
     tiling = sv.get_tiling()
     fusion = sv.get_fusion()
-    target = sv.mass_target()[0] if precision == S.F32 else None
+    target, nsh1 = sv.mass_target() if precision == S.F32 else (None, 0)
     sv.close()
     return {"wall_s": wall, "event_ms": ev_ms, "kernel": kname, "stats0": st0, "stats1": st1,
-            "tiling": tiling, "fusion": fusion, "c_target": target}
+            "tiling": tiling, "fusion": fusion, "c_target": target, "n_shifts0": nsh0,
+            "n_shifts1": nsh1}
 
 
 def cpu_baseline(S, shape, nsteps):
@@ -189,10 +199,11 @@ def unet_line(grid=256, reps=3):
 
 
 def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20):
-    """Learned jumps against exact FP64 stepping at grid^3 (500 spheres,
-    R ~ U[6, 12]) to t_end, with the committed K = 200 weights: wall times,
-    and the errors of the mean particle radius (connected components) and
-    the solid fraction at t_end."""
+    """Learned jumps against exact stepping at grid^3 (500 spheres,
+    R ~ U[6, 12]) to t_end, with the committed K = 200 weights: wall times
+    of the surrogate, of plain FP64 stepping (the fast FP64 kernel) and of
+    plain FP32 stepping, and the errors of the mean particle radius
+    (connected components) and the solid fraction at t_end against FP64."""
     import numpy as np
 
     from phasefield_coarsening_amd import surrogate as SG
@@ -200,7 +211,7 @@ def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20):
     w = os.path.join(HERE, "phasefield_coarsening_amd", "weights", "unet_k200_b32_f16.npy")
     shape = (grid, grid, grid)
     cen, rad = S.spheres(5, 500, shape, 6.0, 12.0)
-    with S.Solver(shape) as ref:
+    with S.Solver(shape, kernel=S.KERNEL_FAST) as ref:
         ref.init_spheres(cen, rad)
         s0 = ref.download()
         ref.sync()
@@ -210,6 +221,15 @@ def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20):
         t_exact = time.perf_counter() - t0
         pr, _ = ref.particles(0.5, max_sizes=1)
         st = ref.reduce()
+    # plain FP32 stepping to the same t_end: what the network gains over the
+    # stepper it relaxes and falls back with
+    with S.Solver(shape, precision=S.F32) as f32:
+        f32.upload(*s0)
+        f32.sync()
+        t0 = time.perf_counter()
+        f32.step(t_end)
+        f32.sync()
+        t_exact32 = time.perf_counter() - t0
     with S.Solver(shape, precision=S.F32) as sv, UNet(seed=1) as net:
         net.set_params(np.load(w).astype(np.float32))
         sv.upload(*s0)
@@ -221,8 +241,11 @@ def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20):
         ctl.close()
     f_ref, f_sur = st["sum_h"] / st["n_sites"], ss["sum_h"] / ss["n_sites"]
     return {"grid": [grid] * 3, "t_end": t_end, "k": k, "relax": relax,
-            "exact_fp64_s": round(t_exact, 4), "surrogate_s": round(summ["seconds"], 4),
-            "speedup": round(t_exact / summ["seconds"], 3), "accepted": summ["accepted"],
+            "exact_fp64_s": round(t_exact, 4), "exact_fp32_s": round(t_exact32, 4),
+            "surrogate_s": round(summ["seconds"], 4),
+            "speedup_vs_exact_fp64": round(t_exact / summ["seconds"], 3),
+            "speedup_vs_exact_fp32": round(t_exact32 / summ["seconds"], 3),
+            "accepted": summ["accepted"],
             "rejected": summ["rejected"], "skipped": summ["skipped"],
             "R_err_rel": abs(ps["mean_radius"] - pr["mean_radius"]) / pr["mean_radius"],
             "frac_err_rel": abs(f_sur - f_ref) / f_ref}
@@ -244,13 +267,46 @@ def ensemble_line(S, members=256, steps=1000, seed=4):
         ens.step(steps)
         ms = ens.timer_stop()
         info = ens.info()
+        spl = ens.get_fusion()
     finally:
         ens.close()
     g = info["total_sites"] * steps / (ms * 1e-3) / 1e9
+    # 24 B per site update (FP32); a two-step launch reads and writes the
+    # state once for two updates, so it moves half of that per update
     return {"members": members, "total_sites": info["total_sites"], "steps": steps,
-            "GLUPS": round(g, 2), "roofline_achieved_GBs": round(g * 24, 1),
+            "GLUPS": round(g, 2), "steps_per_launch": spl,
+            "roofline_achieved_GBs": round(g * 24, 1),
             "roofline_frac": round(g * 24 / HBM_PEAK_GBS, 4),
+            "moved_GBs": round(g * 24 / spl, 1),
+            "moved_frac": round(g * 24 / spl / HBM_PEAK_GBS, 4),
             "work_items": info["items"], "persistent_workgroups": info["grid"]}
+
+
+def tiling_of(r):
+    f = r["fusion"]
+    if f[0] == 2:
+        return {"steps_per_launch": 2, "tx": f[1], "ty": f[2], "tz": f[3], "order": f[4]}
+    t = r["tiling"]
+    return {"steps_per_launch": 1, "tx": t[0], "ty": t[1], "tz": t[2], "order": t[3]}
+
+
+def roofline(r, sites, steps, prec, n):
+    """HBM roofline of one timed run.  achieved = B per site update (48 FP64,
+    24 FP32) x site updates per launch / average launch time; moved = the
+    bytes one launch must read and write (the state once, whatever the
+    steps per launch) / the same time; traffic = rocprofv3-measured HBM
+    bytes per launch, when profiles/traffic.json holds them."""
+    spl = r["fusion"][0]
+    b = BYTES_PER_SITE[prec]
+    launch_s = r["event_ms"] * 1e-3 / steps * spl
+    achieved = b * sites * spl / launch_s / 1e9
+    moved = b * sites / launch_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(r["kernel"], f"{n}^3-{prec}"),
+            "bytes_per_site_update": b, "steps_per_launch": spl,
+            "avg_launch_us": round(launch_s * 1e6, 2),
+            "moved_GBs": round(moved, 1), "moved_frac": round(moved / HBM_PEAK_GBS, 4)}
 
 
 def main():
@@ -261,6 +317,8 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--f64-kernel", choices=("fast", "bitwise"), default="fast",
+                    help="FP64 kernel of the headline value (the other is reported beside it)")
     ap.add_argument("--no-f32", action="store_true")
     ap.add_argument("--no-copy", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -281,16 +339,24 @@ def main():
     n = a.size
     shape = (n, n, n)
     sites = n ** 3
-    r = run_gpu(S, shape, S.F64, a.steps, a.warmup)
-    glups = sites * a.steps / r["wall_s"] / 1e9
-    # steps per launch: 2 = two steps in one z march (periodic default), so a
-    # launch reads and writes the state once for two site updates
-    spl = r["fusion"][0]
-    per_step_s = r["event_ms"] * 1e-3 / a.steps
-    avg_launch_s = per_step_s * spl
-    achieved = BYTES_PER_SITE["f64"] * sites / avg_launch_s / 1e9
-    equiv = BYTES_PER_SITE["f64"] * sites / per_step_s / 1e9
+    kernels = {"fast": S.KERNEL_FAST, "bitwise": S.KERNEL_TILED}
+    head = a.f64_kernel
+    other = "bitwise" if head == "fast" else "fast"
     workload = f"{n}^3 periodic, 2000 spheres R~U[6,14], FP64"
+    # side measurements first: they also bring the GPU to its working clocks
+    # before the timed region of the headline
+    side = {}
+    if not a.no_copy:
+        side["copy"] = copy_ceiling()
+        if n % 64 == 0 and n % 32 == 0:
+            # the same bytes as a plain stream: in flat order, and in the
+            # single-step kernel's z-marching pattern (64x8 tiles, 32 planes)
+            side["flat"] = stream3_rate(n, 0)
+            side["zmarch"] = stream3_rate(n, 1, 32)
+    r_other = run_gpu(S, shape, S.F64, a.steps, a.warmup, kernel=kernels[other])
+    r = run_gpu(S, shape, S.F64, a.steps, a.warmup, kernel=kernels[head])
+    glups = sites * a.steps / r["wall_s"] / 1e9
+    rf = roofline(r, sites, a.steps, "f64", n)
     out = {
         "metric": METRIC,
         "value": round(glups, 3),
@@ -305,70 +371,53 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded sphere initial condition generated on the device)",
         "config": {"workload": workload, "grid": [n, n, n], "bc": "periodic", "spheres": 2000,
-                   "radius": [6, 14], "kernel": r["kernel"]},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(r["kernel"], f"{n}^3-f64"),
-                     "bytes_per_site": BYTES_PER_SITE["f64"],
-                     "avg_launch_us": round(avg_launch_s * 1e6, 2),
-                     "steps_per_launch": spl,
-                     "hbm_bytes_per_site_update": BYTES_PER_SITE["f64"] / spl,
-                     "one_step_equivalent_GBs": round(equiv, 1),
-                     "one_step_equivalent_frac": round(equiv / HBM_PEAK_GBS, 4),
-                     "kernel_limit": ("FP64 VALU latency: two steps per launch move half the "
-                                      "bytes of two single steps" if spl == 2 else "HBM")},
+                   "radius": [6, 14], "kernel": r["kernel"],
+                   "f64_form": ("fast: fused multiply-adds and the collapsed AS == AL "
+                                "chemistry, within 1e-10 per field of the reference after "
+                                "1000 steps" if head == "fast" else
+                                "bitwise: the reference's operand order, bit for bit"),
+                   "tiling": tiling_of(r)},
+        "roofline": rf,
         "solute_drift_rel": abs(r["stats1"]["sum_c"] - r["stats0"]["sum_c"]) /
         abs(r["stats0"]["sum_c"]),
     }
-    if spl == 2:
-        f = r["fusion"]
-        out["config"]["tiling"] = {"steps_per_launch": 2, "tx": f[1], "ty": f[2], "tz": f[3],
-                                   "order": f[4]}
-    else:
-        out["config"]["tiling"] = {"tx": r["tiling"][0], "ty": r["tiling"][1],
-                                   "tz": r["tiling"][2], "order": r["tiling"][3]}
-    if spl == 2:
-        r1 = run_gpu(S, shape, S.F64, a.steps, a.warmup, fuse=1)
-        per1_s = r1["event_ms"] * 1e-3 / a.steps
-        ach1 = BYTES_PER_SITE["f64"] * sites / per1_s / 1e9
-        out["f64_single_step"] = {
-            "GLUPS": round(sites * a.steps / r1["wall_s"] / 1e9, 3), "kernel": r1["kernel"],
-            "tiling": list(r1["tiling"]), "avg_launch_us": round(per1_s * 1e6, 2),
-            "achieved_GBs": round(ach1, 1), "achieved_frac": round(ach1 / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(r1["kernel"], f"{n}^3-f64")}
+    g_other = sites * a.steps / r_other["wall_s"] / 1e9
+    out["f64_" + other] = {"GLUPS": round(g_other, 3), "kernel": r_other["kernel"],
+                           "tiling": tiling_of(r_other),
+                           "roofline": roofline(r_other, sites, a.steps, "f64", n),
+                           "solute_drift_rel": abs(r_other["stats1"]["sum_c"] -
+                                                   r_other["stats0"]["sum_c"]) /
+                           abs(r_other["stats0"]["sum_c"])}
+    r1 = run_gpu(S, shape, S.F64, a.steps, a.warmup, fuse=1, kernel=S.KERNEL_TILED)
+    out["f64_single_step"] = {"GLUPS": round(sites * a.steps / r1["wall_s"] / 1e9, 3),
+                              "kernel": r1["kernel"], "tiling": list(r1["tiling"]),
+                              "roofline": roofline(r1, sites, a.steps, "f64", n)}
     if not a.no_f32:
-        r32 = run_gpu(S, shape, S.F32, a.steps, a.warmup)
-        g32 = sites * a.steps / r32["wall_s"] / 1e9
-        spl32 = r32["fusion"][0]  # steps per launch: 2 = both steps in one z march
-        per_step32_s = r32["event_ms"] * 1e-3 / a.steps
-        # one-step form: 24 B per site update; two steps per launch move the
-        # same 24 B per site once for two updates
-        equiv32 = BYTES_PER_SITE["f32"] * sites / per_step32_s / 1e9
-        actual32 = equiv32 / spl32
-        out["f32"] = {"GLUPS": round(g32, 3), "ms_per_step": round(r32["wall_s"] * 1e3 / a.steps, 4),
-                      "vs_f64": round(g32 / glups, 3), "kernel": r32["kernel"],
-                      "steps_per_launch": spl32,
-                      "hbm_bytes_per_site_update": BYTES_PER_SITE["f32"] / spl32,
-                      "achieved_GBs": round(actual32, 1),
-                      "achieved_frac": round(actual32 / HBM_PEAK_GBS, 4),
-                      "one_step_equivalent_GBs": round(equiv32, 1),
-                      "one_step_equivalent_frac": round(equiv32 / HBM_PEAK_GBS, 4),
+        # at least 100 steps, so that the FP32 form's FP64 mass shifts (every
+        # 50 steps) fall inside the timed window
+        k32 = max(a.steps, 100)
+        r32 = run_gpu(S, shape, S.F32, k32, a.warmup)
+        g32 = sites * k32 / r32["wall_s"] / 1e9
+        out["f32"] = {"GLUPS": round(g32, 3), "steps": k32,
+                      "ms_per_step": round(r32["wall_s"] * 1e3 / k32, 4),
+                      "vs_f64": round(g32 / glups, 3),
+                      "vs_f64_bitwise": round(g32 / (glups if head == "bitwise" else g_other), 3),
+                      "kernel": r32["kernel"], "tiling": tiling_of(r32),
+                      "mass_shifts_in_window": r32["n_shifts1"] - r32["n_shifts0"],
+                      "roofline": roofline(r32, sites, k32, "f32", n),
                       "solute_vs_fp64_target_rel":
                           abs(r32["stats1"]["sum_c"] - r32["c_target"]) / abs(r32["c_target"])}
     if not a.no_extras:
         out["ensemble"] = ensemble_line(S)
         out["unet"] = unet_line()
         out["surrogate"] = surrogate_line(S)
-    if not a.no_copy:
-        cc = copy_ceiling()
+    if "copy" in side:
+        cc = side["copy"]
         out["roofline"]["copy_ceiling_GBs"] = round(cc, 1)
-        out["roofline"]["frac_of_copy_ceiling"] = round(achieved / cc, 4)
-        out["roofline"]["one_step_equivalent_frac_of_copy_ceiling"] = round(equiv / cc, 4)
-        if n % 64 == 0 and n % 32 == 0:
-            # the same bytes as a plain stream: in flat order, and in the
-            # single-step kernel's z-marching pattern (64x8 tiles, 32 planes)
-            out["roofline"]["stream3_flat_GBs"] = round(stream3_rate(n, 0), 1)
-            out["roofline"]["stream3_zmarch_GBs"] = round(stream3_rate(n, 1, 32), 1)
+        out["roofline"]["frac_of_copy_ceiling"] = round(rf["achieved"] / cc, 4)
+        if "flat" in side:
+            out["roofline"]["stream3_flat_GBs"] = round(side["flat"], 1)
+            out["roofline"]["stream3_zmarch_GBs"] = round(side["zmarch"], 1)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(S, shape, a.cpu_steps)
         ref = reference_serial_baseline(S, shape)

@@ -19,9 +19,10 @@ INC = os.path.join(REPO, "include")
 HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-# -ffp-contract=off: the FP64 kernels must round every product and sum on its
-# own, as the reference's x86-64 build does; FP32 kernels write their fused
-# multiply-adds explicitly.  Packed FP32 instructions are switched off.
+# -ffp-contract=off: the bitwise FP64 kernels must round every product and
+# sum on their own, as the reference's x86-64 build does; the FP32 and fast
+# FP64 kernels (PhysC in csrc/ppf_physics.h) write their fused multiply-adds
+# explicitly with fma().  Packed FP32 instructions are switched off.
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -38,9 +39,19 @@ def _run(cmd, cwd=REPO):
 
 
 def build_libppf():
+    """The three translation units are compiled in parallel (each holds its
+    own device code; no device symbol crosses them), then linked."""
+    from concurrent.futures import ThreadPoolExecutor
     out = os.path.join(PKG, "libppf.so")
-    srcs = [os.path.join(CSRC, f) for f in ("ppf.hip", "ppf_ens.hip", "unet.hip")]
-    _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{CSRC}", *srcs, "-o", out + ".tmp"])
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    srcs = ("ppf.hip", "ppf_ens.hip", "unet.hip")
+    flags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [os.path.join(objdir, os.path.splitext(f)[0] + ".o") for f in srcs]
+    with ThreadPoolExecutor(len(srcs)) as ex:
+        list(ex.map(lambda so: _run([HIPCC, *flags, f"-I{INC}", f"-I{CSRC}", "-c",
+                                     os.path.join(CSRC, so[0]), "-o", so[1]]), zip(srcs, objs)))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

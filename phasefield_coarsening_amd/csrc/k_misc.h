@@ -159,34 +159,60 @@ static __global__ __launch_bounds__(256) void k_sum3_final(const double *__restr
 
 // Sphere initial condition.  sph holds n records (cz, cy, cx, R).  The
 // profile 0.5*(1 - tanh(d/2)) falls with d = r - R, so the max over spheres
-// is the profile at the smallest d.
+// is the profile at the smallest d.  The grid is cut into blocks of
+// SPH_B^3 sites, one workgroup each (a thread per (x, y) column of the
+// block, SPH_B planes of z); list[off[b] .. off[b+1]) holds the spheres
+// whose periodic bounding box, widened by SPH_CUT, touches block b.  A
+// site farther than SPH_CUT from every sphere surface has d > SPH_CUT
+// whether or not its nearest sphere is listed, and tanh(d/2) is then 1.0
+// exactly in double precision (1 - tanh(30) ~ 2e-26), so the profile is
+// 0 either way: culling changes no bit of the result.
+constexpr int SPH_B = 16;
+constexpr double SPH_CUT = 60.0;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_init_spheres(T *__restrict__ s, T *__restrict__ l,
-                                                      T *__restrict__ c, int nx, int ny, int nz,
-                                                      const double *__restrict__ sph, int n,
-                                                      double c_liq, double c_span)
+__global__ __launch_bounds__(SPH_B *SPH_B) void k_init_spheres(
+    T *__restrict__ s, T *__restrict__ l, T *__restrict__ c, int nx, int ny, int nz,
+    const double *__restrict__ sph, const int *__restrict__ off, const int *__restrict__ list,
+    int nbx, int nby, int any, double c_liq, double c_span)
 {
-    const size_t total = (size_t)nx * ny * nz;
-    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < total;
-         q += (size_t)gridDim.x * 256) {
-        const int x = (int)(q % nx);
-        const int y = (int)((q / nx) % ny);
-        const int z = (int)(q / ((size_t)nx * ny));
-        double dmin = 1e30;
-        for (int i = 0; i < n; i++) {
-            double dz = fabs(z - sph[4 * i + 0]);
-            double dy = fabs(y - sph[4 * i + 1]);
-            double dx = fabs(x - sph[4 * i + 2]);
+    const int b = blockIdx.x;
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int x = bx * SPH_B + (int)(threadIdx.x % SPH_B);
+    const int y = by * SPH_B + (int)(threadIdx.x / SPH_B);
+    const int z0 = bz * SPH_B;
+    const int nzb = min(SPH_B, nz - z0);
+    if (x >= nx || y >= ny)
+        return;
+    double dmin[SPH_B];
+#pragma unroll
+    for (int k = 0; k < SPH_B; k++)
+        dmin[k] = 1e30;
+    for (int j = off[b]; j < off[b + 1]; j++) {
+        const double *p = sph + 4 * (size_t)list[j];
+        double dy = fabs(y - p[1]);
+        double dx = fabs(x - p[2]);
+        dy = fmin(dy, ny - dy);
+        dx = fmin(dx, nx - dx);
+        const double cz = p[0], R = p[3];
+#pragma unroll
+        for (int k = 0; k < SPH_B; k++) {
+            double dz = fabs(z0 + k - cz);
             dz = fmin(dz, nz - dz);
-            dy = fmin(dy, ny - dy);
-            dx = fmin(dx, nx - dx);
-            const double d = sqrt(dz * dz + dy * dy + dx * dx) - sph[4 * i + 3];
-            dmin = fmin(dmin, d);
+            const double d = sqrt(dz * dz + dy * dy + dx * dx) - R;
+            dmin[k] = fmin(dmin[k], d);
         }
-        const double sv = n > 0 ? 0.5 * (1.0 - tanh(dmin / 2.0)) : 0.0;
-        s[q] = (T)sv;
-        l[q] = (T)(1.0 - sv);
-        c[q] = (T)(c_liq + c_span * sv);
+    }
+    const size_t plane = (size_t)nx * ny;
+#pragma unroll
+    for (int k = 0; k < SPH_B; k++) {
+        if (k < nzb) {
+            const double sv = any ? 0.5 * (1.0 - tanh(dmin[k] / 2.0)) : 0.0;
+            const size_t q = (size_t)(z0 + k) * plane + (size_t)y * nx + x;
+            s[q] = (T)sv;
+            l[q] = (T)(1.0 - sv);
+            c[q] = (T)(c_liq + c_span * sv);
+        }
     }
 }
 

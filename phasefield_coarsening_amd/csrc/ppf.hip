@@ -17,6 +17,7 @@
 #include "k_step.h"
 #include "k_misc.h"
 #include "k_ccl.h"
+#include "init_spheres.h"
 
 #include <algorithm>
 #include <vector>
@@ -127,6 +128,27 @@ static naive_fn pick_naive(int bc, bool eqa, bool conly, bool report)
 #undef PPF_N4
 #undef PPF_N
     return nullptr;
+}
+
+// Tiled contexts run one physics policy: PhysF32 (FP32 form), PhysF64F
+// (the fast FP64 form, PPF_KERNEL_FAST), PhysF64<true> (bitwise, AS == AL)
+// or PhysF64<false> (bitwise, general constants).  fn(policy tag,
+// constants) is called with the context's.
+template <class Fn>
+static int with_policy(const ppf_ctx *ctx, Fn &&fn)
+{
+    if (ctx->d.precision == PPF_F32)
+        return fn(PhysF32{}, ctx->k32);
+    if (ctx->kernel == PPF_KERNEL_FAST)
+        return fn(PhysF64F{}, ctx->kf);
+    if (ctx->eqa)
+        return fn(PhysF64<true>{}, ctx->k);
+    return fn(PhysF64<false>{}, ctx->k);
+}
+
+static bool tiled_kernel(const ppf_ctx *ctx)
+{
+    return ctx->kernel == PPF_KERNEL_TILED || ctx->kernel == PPF_KERNEL_FAST;
 }
 
 // Column-tile shapes compiled for the tiled kernel (TX x TY).
@@ -268,13 +290,14 @@ static FusedPick<P> pick_fused(TileCfg t, int order, int bc = PPF_BC_PERIODIC)
 // context's precision and boundary condition.
 static bool fused_exists(const ppf_ctx *ctx, TileCfg t, int o)
 {
-    return ctx->d.precision == PPF_F32 ? pick_fused<PhysF32>(t, o, ctx->d.bc).f != nullptr
-                                       : pick_fused<PhysF64<true>>(t, o, ctx->d.bc).f != nullptr;
+    return with_policy(ctx, [&](auto p, const auto &) {
+               return (int)(pick_fused<decltype(p)>(t, o, ctx->d.bc).f != nullptr);
+           }) != 0;
 }
 
 static bool fused_available(const ppf_ctx *ctx)
 {
-    return ctx->fuse == 2 && ctx->kernel == PPF_KERNEL_TILED;
+    return ctx->fuse == 2 && tiled_kernel(ctx);
 }
 
 template <class P>
@@ -304,14 +327,10 @@ static int time_tiling(ppf_ctx *ctx, int reps, double *ms)
 {
     int rc = PPF_OK;
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    for (int r = 0; r < reps && rc == PPF_OK; r++) {
-        if (ctx->d.precision == PPF_F32)
-            rc = launch_tiled<PhysF32>(ctx, ctx->k32, false, nullptr);
-        else if (ctx->eqa)
-            rc = launch_tiled<PhysF64<true>>(ctx, ctx->k, false, nullptr);
-        else
-            rc = launch_tiled<PhysF64<false>>(ctx, ctx->k, false, nullptr);
-    }
+    for (int r = 0; r < reps && rc == PPF_OK; r++)
+        rc = with_policy(ctx, [&](auto p, const auto &k) {
+            return launch_tiled<decltype(p)>(ctx, k, false, nullptr);
+        });
     if (rc)
         return rc;
     HIPCHK(ctx, hipGetLastError());
@@ -331,19 +350,17 @@ static int launch_step(ppf_ctx *ctx, bool conly, bool report)
     double *part = report ? ctx->partials : nullptr;
     const int cs = ctx->cur[0], cl = ctx->cur[1], cc = ctx->cur[2];
     int rc = PPF_OK;
-    if (ctx->d.precision == PPF_F32) {
-        rc = launch_tiled<PhysF32>(ctx, ctx->k32, conly, part);
-    } else if (ctx->kernel == PPF_KERNEL_NAIVE) {
+    if (ctx->kernel == PPF_KERNEL_NAIVE) {
         naive_fn f = pick_naive(bc, ctx->eqa, conly, report);
         dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
         hipLaunchKernelGGL(f, grid, dim3(256), 0, ctx->stream, (const double *)ctx->buf[cs][0],
                            (const double *)ctx->buf[cl][1], (const double *)ctx->buf[cc][2],
                            (double *)ctx->buf[1 - cs][0], (double *)ctx->buf[1 - cl][1],
                            (double *)ctx->buf[1 - cc][2], nx, ny, nz, ctx->k, part);
-    } else if (ctx->eqa) {
-        rc = launch_tiled<PhysF64<true>>(ctx, ctx->k, conly, part);
     } else {
-        rc = launch_tiled<PhysF64<false>>(ctx, ctx->k, conly, part);
+        rc = with_policy(ctx, [&](auto p, const auto &k) {
+            return launch_tiled<decltype(p)>(ctx, k, conly, part);
+        });
     }
     if (rc)
         return rc;
@@ -360,13 +377,8 @@ static int launch_step(ppf_ctx *ctx, bool conly, bool report)
 // Two sweeps in one launch (fused_available), from copy cur to 1-cur.
 static int launch_step2(ppf_ctx *ctx)
 {
-    int rc;
-    if (ctx->d.precision == PPF_F32)
-        rc = launch_fused<PhysF32>(ctx, ctx->k32);
-    else if (ctx->eqa)
-        rc = launch_fused<PhysF64<true>>(ctx, ctx->k);
-    else
-        rc = launch_fused<PhysF64<false>>(ctx, ctx->k);
+    const int rc = with_policy(
+        ctx, [&](auto p, const auto &k) { return launch_fused<decltype(p)>(ctx, k); });
     if (rc)
         return rc;
     HIPCHK(ctx, hipGetLastError());
@@ -472,10 +484,14 @@ static void choose_tiles(ppf_ctx *ctx)
     // 256^3 and 512^3).
     if (!f32) {
         int ncu = 0, per_cu = 0;
-        const auto kf = pick_fused<PhysF64<true>>(ctx->tile2, ctx->order2).f;
+        const void *kf = ctx->kernel == PPF_KERNEL_FAST
+                             ? reinterpret_cast<const void *>(
+                                   pick_fused<PhysF64F>(ctx->tile2, ctx->order2).f)
+                             : reinterpret_cast<const void *>(
+                                   pick_fused<PhysF64<true>>(ctx->tile2, ctx->order2).f);
         if (kf && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
                                         ctx->d.device) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(kf),
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf,
                                                          ctx->tile2.tx * ctx->tile2.ty,
                                                          0) == hipSuccess &&
             ncu > 0 && per_cu > 0) {
@@ -513,8 +529,14 @@ extern "C" int ppf_create(ppf_ctx **out, const ppf_desc *desc)
         return set_err(nullptr, PPF_EARG, "the FP32 form needs ML > 0");
     if (d.precision == PPF_F32 && d.kernel == PPF_KERNEL_NAIVE)
         return set_err(nullptr, PPF_EARG, "the naive kernel is FP64 only");
-    if (d.kernel < PPF_KERNEL_AUTO || d.kernel > PPF_KERNEL_TILED)
+    if (d.kernel < PPF_KERNEL_AUTO || d.kernel > PPF_KERNEL_FAST)
         return set_err(nullptr, PPF_EARG, "bad kernel %d", d.kernel);
+    if (d.kernel == PPF_KERNEL_FAST && d.precision != PPF_F64)
+        return set_err(nullptr, PPF_EARG, "the fast kernel is an FP64 form");
+    if (d.kernel == PPF_KERNEL_FAST && d.params.AS != d.params.AL)
+        return set_err(nullptr, PPF_EARG, "the fast FP64 form needs AS == AL");
+    if (d.kernel == PPF_KERNEL_FAST && !(d.params.ML > 0.0))
+        return set_err(nullptr, PPF_EARG, "the fast FP64 form needs ML > 0");
     const size_t n = (size_t)d.nz * d.ny * d.nx;
     if ((size_t)d.ny * d.nx >= ((size_t)1 << 31))
         return set_err(nullptr, PPF_EARG, "an x-y plane of %d x %d sites is too large", d.ny,
@@ -543,12 +565,15 @@ extern "C" int ppf_create(ppf_ctx **out, const ppf_desc *desc)
     ctx->esz = d.precision == PPF_F64 ? sizeof(double) : sizeof(float);
     ctx->k = make_kconst(d.params);
     ctx->k32 = d.precision == PPF_F32 ? make_kconst32(d.params) : KConst32{};
+    ctx->kf = d.kernel == PPF_KERNEL_FAST ? make_kconstc<double>(d.params) : KConstF{};
     ctx->shift_every = 50;
     ctx->eqa = (d.params.AL == d.params.AS);
     ctx->kernel = d.kernel == PPF_KERNEL_AUTO ? PPF_KERNEL_TILED : d.kernel;
     choose_tiles(ctx);
-    ctx->kname = d.precision == PPF_F32 ? "k_tiled<PhysF32>"
-                 : ctx->kernel == PPF_KERNEL_NAIVE ? "k_naive_f64" : "k_tiled<PhysF64>";
+    ctx->kname = d.precision == PPF_F32                ? "k_tiled<PhysF32>"
+                 : ctx->kernel == PPF_KERNEL_NAIVE ? "k_naive_f64"
+                 : ctx->kernel == PPF_KERNEL_FAST  ? "k_tiled<PhysF64F>"
+                                                   : "k_tiled<PhysF64>";
     int rc = PPF_OK;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess)
@@ -841,42 +866,22 @@ extern "C" int ppf_init_spheres(ppf_ctx *ctx, int n, const double *centers_zyx,
     if (!ctx || n < 0 || (n > 0 && (!centers_zyx || !radii)))
         return ctx ? set_err(ctx, PPF_EARG, "bad sphere arguments") : PPF_EARG;
     HIPCHK(ctx, hipSetDevice(ctx->d.device));
-    double *host = (double *)malloc(sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
-    if (!host)
-        return set_err(ctx, PPF_ENOMEM, "host allocation failed");
-    for (int i = 0; i < n; i++) {
-        host[4 * i + 0] = centers_zyx[3 * i + 0];
-        host[4 * i + 1] = centers_zyx[3 * i + 1];
-        host[4 * i + 2] = centers_zyx[3 * i + 2];
-        host[4 * i + 3] = radii[i];
-    }
-    double *dev = nullptr;
-    hipError_t e = hipMalloc(&dev, sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
-    if (e == hipSuccess)
-        e = hipMemcpy(dev, host, sizeof(double) * 4 * (size_t)n, hipMemcpyHostToDevice);
-    free(host);
-    if (e != hipSuccess) {
-        if (dev)
-            hipFree(dev);
-        return set_err(ctx, PPF_EHIP, "sphere upload failed: %s", hipGetErrorString(e));
-    }
-    const unsigned nb = (unsigned)((ctx->n + 255) / 256 < 65536 ? (ctx->n + 255) / 256 : 65536);
-    if (ctx->esz == 8)
-        hipLaunchKernelGGL(k_init_spheres<double>, dim3(nb), dim3(256), 0, ctx->stream,
-                           (double *)ctx->buf[ctx->cur[0]][0], (double *)ctx->buf[ctx->cur[1]][1],
-                           (double *)ctx->buf[ctx->cur[2]][2], ctx->d.nx, ctx->d.ny, ctx->d.nz,
-                           dev, n, c_liq, c_span);
-    else
-        hipLaunchKernelGGL(k_init_spheres<float>, dim3(nb), dim3(256), 0, ctx->stream,
-                           (float *)ctx->buf[ctx->cur[0]][0], (float *)ctx->buf[ctx->cur[1]][1],
-                           (float *)ctx->buf[ctx->cur[2]][2], ctx->d.nx, ctx->d.ny, ctx->d.nz,
-                           dev, n, c_liq, c_span);
-    e = hipGetLastError();
-    hipError_t e2 = hipStreamSynchronize(ctx->stream);
-    hipFree(dev);
-    if (e != hipSuccess || e2 != hipSuccess)
-        return set_err(ctx, PPF_EHIP, "sphere kernel failed: %s",
-                       hipGetErrorString(e != hipSuccess ? e : e2));
+    const hipError_t e =
+        ctx->esz == 8
+            ? init_spheres_device<double>(ctx->stream, (double *)ctx->buf[ctx->cur[0]][0],
+                                          (double *)ctx->buf[ctx->cur[1]][1],
+                                          (double *)ctx->buf[ctx->cur[2]][2], ctx->d.nx,
+                                          ctx->d.ny, ctx->d.nz, n, centers_zyx, radii, c_liq,
+                                          c_span)
+            : init_spheres_device<float>(ctx->stream, (float *)ctx->buf[ctx->cur[0]][0],
+                                         (float *)ctx->buf[ctx->cur[1]][1],
+                                         (float *)ctx->buf[ctx->cur[2]][2], ctx->d.nx, ctx->d.ny,
+                                         ctx->d.nz, n, centers_zyx, radii, c_liq, c_span);
+    if (e != hipSuccess)
+        return set_err(ctx, e == hipErrorInvalidValue ? PPF_EARG : PPF_EHIP,
+                       "sphere initial condition failed: %s",
+                       e == hipErrorInvalidValue ? "too many sphere-block pairs"
+                                                 : hipGetErrorString(e));
     ctx->steps = 0;
     return set_target_from_device(ctx);
 }
@@ -912,7 +917,7 @@ extern "C" int ppf_set_tiling(ppf_ctx *ctx, int tx, int ty, int tz, int order)
 {
     if (!ctx)
         return PPF_EARG;
-    if (ctx->kernel != PPF_KERNEL_TILED)
+    if (!tiled_kernel(ctx))
         return set_err(ctx, PPF_ESTATE, "tiling applies to the tiled kernels only");
     bool known = false;
     for (const TileCfg &c : k_tiles)
@@ -934,14 +939,9 @@ static int time_fused(ppf_ctx *ctx, int reps, double *ms)
 {
     int rc = PPF_OK;
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    for (int r = 0; r < reps && rc == PPF_OK; r++) {
-        if (ctx->d.precision == PPF_F32)
-            rc = launch_fused<PhysF32>(ctx, ctx->k32);
-        else if (ctx->eqa)
-            rc = launch_fused<PhysF64<true>>(ctx, ctx->k);
-        else
-            rc = launch_fused<PhysF64<false>>(ctx, ctx->k);
-    }
+    for (int r = 0; r < reps && rc == PPF_OK; r++)
+        rc = with_policy(
+            ctx, [&](auto p, const auto &k) { return launch_fused<decltype(p)>(ctx, k); });
     if (rc)
         return rc;
     HIPCHK(ctx, hipGetLastError());
@@ -957,7 +957,7 @@ extern "C" int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms)
 {
     if (!ctx || reps < 1 || !best_ms)
         return ctx ? set_err(ctx, PPF_EARG, "bad autotune arguments") : PPF_EARG;
-    if (ctx->kernel != PPF_KERNEL_TILED)
+    if (!tiled_kernel(ctx))
         return set_err(ctx, PPF_ESTATE, "autotuning applies to the tiled kernels only");
     HIPCHK(ctx, hipSetDevice(ctx->d.device));
     const TileCfg keep_tile = ctx->tile;
@@ -1080,7 +1080,7 @@ extern "C" int ppf_set_fusion(ppf_ctx *ctx, int steps_per_launch, int tx, int ty
     if (steps_per_launch != 1 && steps_per_launch != 2)
         return set_err(ctx, PPF_EARG, "steps per launch must be 1 or 2");
     if (steps_per_launch == 2) {
-        if (ctx->kernel != PPF_KERNEL_TILED)
+        if (!tiled_kernel(ctx))
             return set_err(ctx, PPF_ESTATE, "two steps per launch need the tiled kernels");
     }
     if (tx != 0 || ty != 0) {
@@ -1285,7 +1285,9 @@ extern "C" const char *ppf_kernel_name(ppf_ctx *ctx)
     if (!ctx)
         return "";
     if (fused_available(ctx))
-        return ctx->d.precision == PPF_F32 ? "k_tiled2<PhysF32>" : "k_tiled2<PhysF64>";
+        return ctx->d.precision == PPF_F32         ? "k_tiled2<PhysF32>"
+               : ctx->kernel == PPF_KERNEL_FAST ? "k_tiled2<PhysF64F>"
+                                                : "k_tiled2<PhysF64>";
     return ctx->kname;
 }
 

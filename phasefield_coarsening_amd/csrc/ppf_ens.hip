@@ -22,6 +22,7 @@
 #include "ppf.h"
 #include "ppf_host.h"
 #include "k_ens.h"
+#include "init_spheres.h"
 
 namespace {
 
@@ -540,38 +541,21 @@ extern "C" int ppf_ens_init_spheres(ppf_ens *e, int dom, int n, const double *ce
     if (!good_dom(e, dom) || n < 0 || (n > 0 && (!centers_zyx || !radii)))
         return e ? eset(e, PPF_EARG, "bad sphere arguments") : PPF_EARG;
     EHIP(e, hipSetDevice(e->device));
-    std::vector<double> host(4 * (size_t)(n > 0 ? n : 1));
-    for (int i = 0; i < n; i++) {
-        host[4 * i + 0] = centers_zyx[3 * i + 0];
-        host[4 * i + 1] = centers_zyx[3 * i + 1];
-        host[4 * i + 2] = centers_zyx[3 * i + 2];
-        host[4 * i + 3] = radii[i];
-    }
-    double *dev = nullptr;
-    EHIP(e, hipMalloc(&dev, host.size() * sizeof(double)));
-    hipError_t h = hipMemcpy(dev, host.data(), host.size() * sizeof(double),
-                             hipMemcpyHostToDevice);
     const EnsDom &m = e->doms[dom];
-    const size_t ns = e->nsite[dom];
-    const unsigned g = (unsigned)std::min<size_t>((ns + 255) / 256, 65536);
-    if (h == hipSuccess) {
-        if (e->esz == 8)
-            hipLaunchKernelGGL(k_init_spheres<double>, dim3(g), dim3(256), 0, e->stream,
-                               (double *)m.f[e->parity][0], (double *)m.f[e->parity][1],
-                               (double *)m.f[e->parity][2], m.nx, m.ny, m.nz, dev, n, c_liq,
-                               c_span);
-        else
-            hipLaunchKernelGGL(k_init_spheres<float>, dim3(g), dim3(256), 0, e->stream,
-                               (float *)m.f[e->parity][0], (float *)m.f[e->parity][1],
-                               (float *)m.f[e->parity][2], m.nx, m.ny, m.nz, dev, n, c_liq,
-                               c_span);
-        h = hipGetLastError();
-    }
-    if (h == hipSuccess)
-        h = hipStreamSynchronize(e->stream);
-    hipFree(dev);
+    const hipError_t h =
+        e->esz == 8
+            ? init_spheres_device<double>(e->stream, (double *)m.f[e->parity][0],
+                                          (double *)m.f[e->parity][1],
+                                          (double *)m.f[e->parity][2], m.nx, m.ny, m.nz, n,
+                                          centers_zyx, radii, c_liq, c_span)
+            : init_spheres_device<float>(e->stream, (float *)m.f[e->parity][0],
+                                         (float *)m.f[e->parity][1], (float *)m.f[e->parity][2],
+                                         m.nx, m.ny, m.nz, n, centers_zyx, radii, c_liq, c_span);
     if (h != hipSuccess)
-        return eset(e, PPF_EHIP, "sphere kernel failed: %s", hipGetErrorString(h));
+        return eset(e, h == hipErrorInvalidValue ? PPF_EARG : PPF_EHIP,
+                    "sphere initial condition failed: %s",
+                    h == hipErrorInvalidValue ? "too many sphere-block pairs"
+                                              : hipGetErrorString(h));
     return member_target_from_device(e, dom);
 }
 

@@ -18,6 +18,7 @@ struct ppf_ctx {
     ppf_desc d;
     KConst k;
     KConst32 k32;
+    KConstF kf;  // PPF_KERNEL_FAST
     bool eqa;
     // FP32 solute bookkeeping: FP64 target of sum(c), and a mean shift of c
     // back onto it every shift_every steps (0: never).
@@ -79,22 +80,25 @@ static inline KConst make_kconst(const ppf_params &p)
     return k;
 }
 
-static inline KConst32 make_kconst32(const ppf_params &p)
+// The constants of the collapsed AS == AL form (ppf_physics.h, PhysC),
+// computed in FP64 and rounded once to F; valid for AS == AL and ML > 0,
+// which ppf_create checks for the contexts that use them.
+template <typename F>
+static inline KConstC<F> make_kconstc(const ppf_params &p)
 {
-    // the dimensionless FP32 constants (ppf_physics.h, PhysF32); valid for
-    // AS == AL, which ppf_create checks
     const double tmp1 = -1.0 * p.L * p.dt;
     const double tmp3 = p.dt * 0.5 * p.h2;
-    KConst32 k;
-    k.cL0 = (float)p.cL0;
-    k.tmp6 = (float)(p.cL0 - p.cS0);
-    k.rM = (float)((p.MS - p.ML) / p.ML);
-    k.a1 = (float)(tmp1 * p.m);
-    k.a2 = (float)(-tmp1 * p.kappa * p.h2);
-    k.g2 = (float)(2.0 * p.gamma);
-    k.b0 = (float)(tmp1 * (p.CS - p.CL));
-    k.b1 = (float)(tmp1 * p.AL * (p.cL0 - p.cS0));
-    k.d0 = (float)(tmp3 * p.ML * p.AL);
+    KConstC<F> k;
+    k.cL0 = (F)p.cL0;
+    k.tmp6 = (F)(p.cL0 - p.cS0);
+    k.rM = (F)((p.MS - p.ML) / p.ML);
+    k.a1 = (F)(tmp1 * p.m);
+    k.a2 = (F)(-tmp1 * p.kappa * p.h2);
+    k.g2 = (F)(2.0 * p.gamma);
+    k.b0 = (F)(2.0 * tmp1 * (p.CS - p.CL));
+    k.b1 = (F)(2.0 * tmp1 * p.AL * (p.cL0 - p.cS0));
+    k.d0 = (F)(tmp3 * p.ML * p.AL);
     return k;
 }
 
+static inline KConst32 make_kconst32(const ppf_params &p) { return make_kconstc<float>(p); }

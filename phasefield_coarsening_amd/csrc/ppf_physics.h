@@ -143,48 +143,76 @@ struct PhysF64 {
     }
 };
 
-// FP32 form for AS == AL.  With equal free-energy curvatures the chemistry
-// collapses (phasefield.c:351-359): cS - cL = cS0 - cL0, fS - fL = CS - CL
-// and mu = A*u with the dimensionless u = c - cL0 + (cL0 - cS0)*h.  Writing
-// M = ML*mm with mm = 1 + ((MS - ML)/ML)*h, one step is
-//   D  = 2 s l / (s^2 + l^2)^2 * (b0 + b1*u)
+// Collapsed form for AS == AL (the FP32 policy and the fast FP64 policy).
+// With equal free-energy curvatures the chemistry collapses
+// (phasefield.c:351-359): cS - cL = cS0 - cL0, fS - fL = CS - CL and
+// mu = A*u with the dimensionless u = c - cL0 + (cL0 - cS0)*h.  Writing
+// M = ML*mm with mm = 1 + ((MS - ML)/ML)*h and r = 1/(s^2 + l^2), one step is
+//   h  = s^2 r
+//   D  = s l r^2 (b0 + b1*u)
 //   ds = a1*s*(s^2 - 1 + 2 gamma l^2) + a2*Lap6(s) + l*D
 //   dl = a1*l*(l^2 - 1 + 2 gamma s^2) + a2*Lap6(l) - s*D
 //   dc = d0 * sum_nb (mm_nb + mm)*(u_nb - u)            (flux form)
-// with a1 = tmp1*m, a2 = -tmp1*kappa*h2, b0 = tmp1*(CS - CL),
-// b1 = tmp1*A*(cL0 - cS0), d0 = tmp3*ML*A, tmp1 = -L*dt, tmp3 = dt*h2/2.
+// with a1 = tmp1*m, a2 = -tmp1*kappa*h2, b0 = 2*tmp1*(CS - CL),
+// b1 = 2*tmp1*A*(cL0 - cS0), d0 = tmp3*ML*A, tmp1 = -L*dt, tmp3 = dt*h2/2
+// (the factor 2 of 2 s l/(s^2 + l^2)^2, phasefield.c:359, is in b0 and b1).
 // Every quantity is O(1) or a small increment: no 1e8-sized free energies
-// are subtracted, which single precision could not resolve.  The flux
-// (mm_i + mm_j)*(u_j - u_i) is computed identically, up to sign, at both
-// ends of a face, so the grid sum of dc differs from zero only by the
-// rounding of each site's six-term sum and of the final products.
-struct KConst32 {
-    float cL0, tmp6, rM, a1, a2, g2, b0, b1, d0;
+// are subtracted (the reference subtracts fS and fL, each ~1e8, whose
+// difference is the constant CS - CL).  The flux (mm_i + mm_j)*(u_j - u_i)
+// is computed identically, up to sign, at both ends of a face, so the grid
+// sum of dc differs from zero only by the rounding of each site's six-term
+// sum and of the final products.
+//
+// Both policies evaluate it with fused multiply-adds and one reciprocal per
+// site (shared by h and D), instead of the reference's two divides.  The
+// point() slot that carries h in PhysF64 carries r here: the update needs
+// r for D, and nothing needs h of a site once its u and mm are known.
+template <typename F>
+struct KConstC {
+    F cL0, tmp6, rM, a1, a2, g2, b0, b1, d0;
 };
+using KConst32 = KConstC<float>;
+using KConstF = KConstC<double>;
 
-struct PhysF32 {
-    using T = float;
-    using K = KConst32;
-    static __device__ __forceinline__ void point(T s, T l, T c, const K &k, T &h, T &u, T &mm)
+// 1/x: the hardware reciprocal refined by Newton-Raphson steps
+// r <- r + r (1 - x r), each squaring the relative error.  FP64: v_rcp_f64
+// is good to about 2^-26 (two steps reach the last bit); FP32: v_rcp_f32 is
+// within 1 ulp and is used as it is.  x = s^2 + l^2 is O(1) on every state
+// the scheme produces (s + l ~ 1); x = 0 (s = l = 0) gives NaN, as 0/0 does
+// in the reference's h.
+__device__ __forceinline__ double recip(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <typename F>
+struct PhysC {
+    using T = F;
+    using K = KConstC<F>;
+    static __device__ __forceinline__ void point(T s, T l, T c, const K &k, T &r, T &u, T &mm)
     {
-        const float ss = s * s;
-        h = ss / (ss + l * l);
-        u = (c - k.cL0) + k.tmp6 * h;
-        mm = 1.0f + k.rM * h;
+        const T ss = s * s;
+        r = recip(fma(l, l, ss));
+        const T h = ss * r;
+        u = fma(k.tmp6, h, c - k.cL0);
+        mm = fma(k.rM, h, (T)1);
     }
-    static __device__ __forceinline__ void update(T s, T l, T c, T h, T u, T mm, const T sn[6],
+    static __device__ __forceinline__ void update(T s, T l, T c, T r, T u, T mm, const T sn[6],
                                                   const T ln[6], const T un[6], const T mn[6],
                                                   const K &k, T &so, T &lo, T &co, double &ds_out,
                                                   double &dl_out, double &dc_out)
     {
-        (void)h;
-        const float ss = s * s, ll = l * l, den = ss + ll;
-        const float D = (2.0f * s * l) / (den * den) * (k.b0 + k.b1 * u);
-        const float lap_s = ((sn[0] + sn[1]) + (sn[2] + sn[3])) + (sn[4] + sn[5]) - 6.0f * s;
-        const float lap_l = ((ln[0] + ln[1]) + (ln[2] + ln[3])) + (ln[4] + ln[5]) - 6.0f * l;
-        const float ds = k.a1 * s * (ss - 1.0f + k.g2 * ll) + k.a2 * lap_s + l * D;
-        const float dl = k.a1 * l * (ll - 1.0f + k.g2 * ss) + k.a2 * lap_l - s * D;
-        const float dc = conly(u, mm, un, mn, k);
+        const T D = (s * l) * (r * r) * fma(k.b1, u, k.b0);
+        const T lap_s = fma((T)-6, s, ((sn[0] + sn[1]) + (sn[2] + sn[3])) + (sn[4] + sn[5]));
+        const T lap_l = fma((T)-6, l, ((ln[0] + ln[1]) + (ln[2] + ln[3])) + (ln[4] + ln[5]));
+        const T ds = fma(k.a1 * s, fma(k.g2 * l, l, fma(s, s, (T)-1)), fma(k.a2, lap_s, l * D));
+        const T dl = fma(k.a1 * l, fma(k.g2 * s, s, fma(l, l, (T)-1)), fma(k.a2, lap_l, -(s * D)));
+        const T dc = conly(u, mm, un, mn, k);
         so = s + ds;
         lo = l + dl;
         co = c + dc;
@@ -194,9 +222,19 @@ struct PhysF32 {
     }
     static __device__ __forceinline__ T conly(T u, T mm, const T un[6], const T mn[6], const K &k)
     {
-        const float f = ((mn[0] + mm) * (un[0] - u) + (mn[1] + mm) * (un[1] - u)) +
-                        ((mn[2] + mm) * (un[2] - u) + (mn[3] + mm) * (un[3] - u)) +
-                        ((mn[4] + mm) * (un[4] - u) + (mn[5] + mm) * (un[5] - u));
-        return k.d0 * f;
+        T f = (mn[0] + mm) * (un[0] - u);
+        f = fma(mn[1] + mm, un[1] - u, f);
+        T g = (mn[2] + mm) * (un[2] - u);
+        g = fma(mn[3] + mm, un[3] - u, g);
+        T q = (mn[4] + mm) * (un[4] - u);
+        q = fma(mn[5] + mm, un[5] - u, q);
+        return k.d0 * ((f + g) + q);
     }
 };
+
+// FP32 form: fields stored and stepped in single precision.
+using PhysF32 = PhysC<float>;
+// Fast FP64 form: the collapsed arithmetic in double precision.  Not bitwise
+// equal to the reference (different rounding sequence), within 1e-10 per
+// field of it after 1000 steps.
+using PhysF64F = PhysC<double>;

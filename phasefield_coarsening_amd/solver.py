@@ -23,6 +23,7 @@ F32 = 1
 KERNEL_AUTO = 0
 KERNEL_NAIVE = 1
 KERNEL_TILED = 2
+KERNEL_FAST = 3  # FP64, fused multiply-adds and the collapsed AS == AL chemistry
 
 _dp = ctypes.POINTER(ctypes.c_double)
 

@@ -1,23 +1,42 @@
-"""numpy float32 restatement of the FP32 step form (PhysF32 in
-phasefield_coarsening_amd/csrc/ppf_physics.h), for its parity tests.
+"""numpy restatement of the collapsed AS == AL step form that the FP32
+kernels and the fast FP64 kernels evaluate (PhysC in
+phasefield_coarsening_amd/csrc/ppf_physics.h), for their parity tests.
 
-Same dimensionless constants (computed in float64, rounded once to
-float32) and the same operation order as the kernel, each operation
-rounded to float32 (numpy float32 arithmetic is IEEE single), so one step
-agrees with the GPU bit for bit.  Periodic boundaries.
+Two evaluations of one step, from the kernel's own inputs and constants
+(the dimensionless constants computed in float64 and, for the FP32 kernel,
+rounded once to float32 as the kernel holds them):
+  * ``step64``: the formula in float64, the exact value the kernels
+    approximate;
+  * ``step32``: the FP32 kernel's operation sequence in float32, each
+    operation rounded to float32 (a fused multiply-add as one rounding of
+    the float64 result, exact for float32 operands up to a rare double
+    rounding), the reciprocal correctly rounded where the kernel's
+    v_rcp_f32 is within 1 ulp.  So the kernel agrees with it to a few ulps,
+    not bit for bit.
+The float32 rounding of u = c - cL0 + (cL0 - cS0) h and of the six-term flux
+sum costs the c update up to ~8 float32 ulps of c against ``step64``
+(etaS and etaL stay within one ulp): that is the FP32 form's own error.
+
+The form itself (cS - cL = cS0 - cL0, fS - fL = CS - CL, mu = A u) is pinned
+against the reference's arithmetic independently, by the oracle: the fast
+FP64 kernels stay within 1e-10 of the oracle over 1000 steps, and
+``step64`` agrees with the oracle's increments to rounding
+(tests/test_collapsed_form.py).  Periodic boundaries.
 """
 import numpy as np
 
 f32 = np.float32
 
 
-def constants(p):
+def constants(p, dtype=np.float32):
+    """The kernel's dimensionless constants, rounded to dtype."""
     tmp1 = -1.0 * p.L * p.dt
     tmp3 = p.dt * 0.5 * p.h2
-    return dict(cL0=f32(p.cL0), tmp6=f32(p.cL0 - p.cS0), rM=f32((p.MS - p.ML) / p.ML),
-                a1=f32(tmp1 * p.m), a2=f32(-tmp1 * p.kappa * p.h2), g2=f32(2.0 * p.gamma),
-                b0=f32(tmp1 * (p.CS - p.CL)), b1=f32(tmp1 * p.AL * (p.cL0 - p.cS0)),
-                d0=f32(tmp3 * p.ML * p.AL))
+    t = dtype
+    return dict(cL0=t(p.cL0), tmp6=t(p.cL0 - p.cS0), rM=t((p.MS - p.ML) / p.ML),
+                a1=t(tmp1 * p.m), a2=t(-tmp1 * p.kappa * p.h2), g2=t(2.0 * p.gamma),
+                b0=t(2.0 * tmp1 * (p.CS - p.CL)), b1=t(2.0 * tmp1 * p.AL * (p.cL0 - p.cS0)),
+                d0=t(tmp3 * p.ML * p.AL))
 
 
 def _nb(a):
@@ -26,22 +45,62 @@ def _nb(a):
             np.roll(a, -1, 2), np.roll(a, 1, 2)]
 
 
-def step(s, l, c, k):
-    s, l, c = (np.asarray(a, dtype=f32) for a in (s, l, c))
-    ss = s * s
-    h = ss / (ss + l * l)
+def increments64(s, l, c, k):
+    """(ds, dl, dc) of the collapsed form, every operation in float64."""
+    s, l, c = (np.asarray(a, dtype=np.float64) for a in (s, l, c))
+    k = {n: np.float64(v) for n, v in k.items()}
+    ss, ll = s * s, l * l
+    r = 1.0 / (ss + ll)
+    h = ss * r
     u = (c - k["cL0"]) + k["tmp6"] * h
-    mm = f32(1.0) + k["rM"] * h
+    mm = 1.0 + k["rM"] * h
     sn, ln, un, mn = _nb(s), _nb(l), _nb(u), _nb(mm)
-    ll = l * l
-    den = ss + ll
-    D = (f32(2.0) * s * l) / (den * den) * (k["b0"] + k["b1"] * u)
-    lap_s = ((sn[0] + sn[1]) + (sn[2] + sn[3])) + (sn[4] + sn[5]) - f32(6.0) * s
-    lap_l = ((ln[0] + ln[1]) + (ln[2] + ln[3])) + (ln[4] + ln[5]) - f32(6.0) * l
-    ds = k["a1"] * s * (ss - f32(1.0) + k["g2"] * ll) + k["a2"] * lap_s + l * D
-    dl = k["a1"] * l * (ll - f32(1.0) + k["g2"] * ss) + k["a2"] * lap_l - s * D
-    fl = (((mn[0] + mm) * (un[0] - u) + (mn[1] + mm) * (un[1] - u)) +
-          ((mn[2] + mm) * (un[2] - u) + (mn[3] + mm) * (un[3] - u)) +
-          ((mn[4] + mm) * (un[4] - u) + (mn[5] + mm) * (un[5] - u)))
-    dc = k["d0"] * fl
+    D = s * l * r * r * (k["b0"] + k["b1"] * u)
+    lap_s = sum(sn) - 6.0 * s
+    lap_l = sum(ln) - 6.0 * l
+    ds = k["a1"] * s * (ss - 1.0 + k["g2"] * ll) + k["a2"] * lap_s + l * D
+    dl = k["a1"] * l * (ll - 1.0 + k["g2"] * ss) + k["a2"] * lap_l - s * D
+    dc = k["d0"] * sum((m + mm) * (v - u) for m, v in zip(mn, un))
+    return ds, dl, dc
+
+
+def step64(s, l, c, k):
+    """One step of the collapsed form in float64: the new (s, l, c)."""
+    ds, dl, dc = increments64(s, l, c, k)
+    return (np.asarray(s, np.float64) + ds, np.asarray(l, np.float64) + dl,
+            np.asarray(c, np.float64) + dc)
+
+
+def _fma32(a, b, c):
+    """float32 fused multiply-add: one rounding of the (exact) float64 a*b + c."""
+    return (np.asarray(a, np.float64) * np.asarray(b, np.float64) +
+            np.asarray(c, np.float64)).astype(f32)
+
+
+def step32(s, l, c, k):
+    """One step in the FP32 kernel's operation order (PhysC<float>), float32."""
+    k = {n: f32(v) for n, v in k.items()}
+    s, l, c = (np.asarray(a, f32) for a in (s, l, c))
+    ss = s * s
+    r = (f32(1) / _fma32(l, l, ss)).astype(f32)
+    h = ss * r
+    u = _fma32(k["tmp6"], h, c - k["cL0"])
+    mm = _fma32(k["rM"], h, np.ones_like(h))
+    sn, ln, un, mn = _nb(s), _nb(l), _nb(u), _nb(mm)
+    D = (s * l) * (r * r) * _fma32(k["b1"], u, np.full_like(u, k["b0"]))
+    lap_s = _fma32(f32(-6), s, ((sn[0] + sn[1]) + (sn[2] + sn[3])) + (sn[4] + sn[5]))
+    lap_l = _fma32(f32(-6), l, ((ln[0] + ln[1]) + (ln[2] + ln[3])) + (ln[4] + ln[5]))
+    ds = _fma32(k["a1"] * s, _fma32(k["g2"] * l, l, _fma32(s, s, np.full_like(s, -1))),
+                _fma32(k["a2"], lap_s, l * D))
+    dl = _fma32(k["a1"] * l, _fma32(k["g2"] * s, s, _fma32(l, l, np.full_like(l, -1))),
+                _fma32(k["a2"], lap_l, -(s * D)))
+    f = _fma32(mn[1] + mm, un[1] - u, (mn[0] + mm) * (un[0] - u))
+    g = _fma32(mn[3] + mm, un[3] - u, (mn[2] + mm) * (un[2] - u))
+    q = _fma32(mn[5] + mm, un[5] - u, (mn[4] + mm) * (un[4] - u))
+    dc = k["d0"] * ((f + g) + q)
     return s + ds, l + dl, c + dc
+
+
+def ulp32(x):
+    """One float32 ulp at |x|."""
+    return np.spacing(np.abs(np.asarray(x, dtype=np.float32))).astype(np.float64)

@@ -42,7 +42,13 @@ def test_json_line_keys_and_roofline():
         assert key in rf, key
     assert rf["bound"] in ("hbm", "mfma") and rf["unit"] == "GB/s" and rf["peak"] == 8000.0
     assert abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-3
-    # two steps per launch: the one-step-equivalent rate is twice the
-    # per-launch rate of the same 48 B per site
-    if rf.get("steps_per_launch") == 2:
-        assert abs(rf["one_step_equivalent_GBs"] - 2 * rf["achieved"]) < 0.01 * rf["achieved"] + 1
+    # achieved counts 48 B per site update (FP64) times the updates of one
+    # launch; moved counts the state read and written once per launch, so a
+    # two-step launch moves half the algorithmic bytes of its two updates
+    spl = rf["steps_per_launch"]
+    assert rf["bytes_per_site_update"] == 48 and spl in (1, 2)
+    assert abs(rf["moved_GBs"] * spl - rf["achieved"]) < 0.01 * rf["achieved"] + 1
+    # kernel time (HIP events) is within the wall time of the same steps
+    assert rf["achieved"] >= 0.95 * out["value"] * 48
+    assert "fast" in out["config"]["f64_form"]
+    assert out["f64_bitwise"]["GLUPS"] > 0 and out["f64_single_step"]["GLUPS"] > 0

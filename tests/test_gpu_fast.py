@@ -1,0 +1,173 @@
+"""The fast FP64 kernels (PPF_KERNEL_FAST: the collapsed AS == AL chemistry,
+one reciprocal per site, fused multiply-adds) against the oracle, which is
+bitwise the reference, to the north-star bar: max|d| <= 1e-10 per field
+after 1000 steps and total solute conserved to 1e-12 relative.  Two steps
+per launch equal single steps bit for bit (both evaluate the same policy),
+and the 512^3 workload keeps its size-independent properties."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from phasefield_coarsening_amd import solver as S
+
+pytestmark = pytest.mark.gpu
+
+FAST = S.KERNEL_FAST
+
+
+def _bc(m):
+    return S.BC_PERIODIC if m["bc"] == "periodic" else S.BC_NOFLUX
+
+
+def _random_state(shape, seed):
+    rng = np.random.default_rng(seed)
+    s = rng.uniform(0.05, 0.95, shape)
+    l = np.clip(1 - s + rng.uniform(-0.1, 0.1, shape), 0.0, 1.0)
+    c = 0.1 + 0.8 * s + rng.uniform(-0.02, 0.02, shape)
+    return s, l, c
+
+
+def _spheres_state(shape, n, rmin, rmax, seed):
+    cen, rad = S.spheres(seed, n, shape, rmin, rmax)
+    with S.Solver(shape) as sv:
+        sv.init_spheres(cen, rad)
+        return sv.download()
+
+
+def test_config1_64cubed_1000_steps_within_1e10_of_the_oracle():
+    """64^3 periodic, 40 spheres R ~ U[4, 8], 1000 steps (two per launch)."""
+    shape = (64, 64, 64)
+    s0, l0, c0 = _spheres_state(shape, 40, 4.0, 8.0, 1)
+    want = O.run(s0, l0, c0, 1000)
+    with S.Solver(shape, kernel=FAST) as sv:
+        assert sv.kernel_name == "k_tiled2<PhysF64F>"
+        sv.upload(s0, l0, c0)
+        st0 = sv.reduce()
+        sv.step(1000)
+        st1 = sv.reduce()
+        got = sv.download()
+    errs = [float(np.abs(g - w).max()) for g, w in zip(got, want[:3])]
+    print(f"config 1, fast FP64 vs oracle after 1000 steps: max|d| = {errs}", flush=True)
+    assert max(errs) <= 1e-10
+    assert max(errs) > 0.0  # a different rounding sequence, not the bitwise kernel
+    assert abs(st1["sum_c"] - st0["sum_c"]) / abs(st0["sum_c"]) <= 1e-12
+    assert abs(got[2].sum() - c0.sum()) / abs(c0.sum()) <= 1e-12
+
+
+@pytest.mark.parametrize("case", ["p16_s1000", "pnc_s100", "nnc_s100", "n16_s10",
+                                  "p16compat_s10", "nnc_compat_s10", "pnc_conly"])
+def test_fast_vs_reference_outputs(golden, case):
+    """The reference program's own outputs (periodic and no-flux, etaL read
+    and as shipped, the c-only sweep), within 1e-10 per field."""
+    man, arrays = golden
+    m, z = man["cases"][case], arrays[case]
+    print(f"case={case}", flush=True)
+    with S.Solver(z["s0"].shape, bc=_bc(m), kernel=FAST) as sv:
+        sv.upload(z["s0"], z["l0"] if m["etaL"] == "read" else None, z["c0"])
+        done = 0
+        for k, nsw in enumerate(m["sweeps_before_output"], start=1):
+            while done < nsw:
+                if done < m["n_iter_init"]:
+                    n = min(nsw, m["n_iter_init"]) - done
+                    sv.step(n)
+                    done += n
+                else:
+                    sv.step_c_only(1)
+                    done += 1
+            got = sv.download()
+            for g, name in zip(got, "slc"):
+                assert np.abs(g - z[f"{name}{k}"]).max() <= 1e-10, (case, name, k)
+
+
+@pytest.mark.parametrize("bc", [S.BC_PERIODIC, S.BC_NOFLUX])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (2, 3, 5), (20, 16, 12), (33, 17, 65),
+                                   (9, 130, 31)])
+def test_fast_ragged_vs_oracle_and_two_steps_equal_single_steps(shape, bc):
+    print(f"shape={shape} bc={bc}", flush=True)
+    st = _random_state(shape, 3)
+    want = O.run(*st, 7, bc=bc)
+    runs = {}
+    for fuse, tile, order in ((1, (0, 0), -1), (2, (16, 16), 6), (2, (32, 8), 4),
+                              (2, (16, 16), 8), (2, (32, 16), 2)):
+        if bc == S.BC_NOFLUX and order in (8, 2) and tile == (32, 16):
+            continue  # no FP64 mirror kernel for this tile
+        with S.Solver(shape, bc=bc, kernel=FAST) as sv:
+            sv.upload(*st)
+            if fuse == 1:
+                sv.set_fusion(1)
+            else:
+                sv.set_fusion(2, tile[0], tile[1], 3, order)
+            sv.step(7)
+            runs[(fuse, tile, order)] = sv.download()
+    ref = runs[(1, (0, 0), -1)]
+    for g, w in zip(ref, want[:3]):
+        assert np.abs(g - w).max() <= 1e-13
+    for key, got in runs.items():
+        for g, r in zip(got, ref):
+            np.testing.assert_array_equal(g, r, err_msg=str(key))
+
+
+def test_fast_single_step_tilings_bitwise_equal():
+    """Every single-step march variant of the fast kernel gives the same
+    bits (each site's arithmetic is fixed; only the schedule changes)."""
+    shape = (13, 20, 70)
+    st = _random_state(shape, 5)
+    ref = None
+    with S.Solver(shape, kernel=FAST) as sv:
+        sv.set_fusion(1)
+        for tile in [(64, 8), (16, 16), (32, 16)]:
+            for tz in (1, 3, 5):
+                for order in (0, 4, 8, 10):
+                    sv.upload(*st)
+                    sv.set_tiling(tile[0], tile[1], tz, order)
+                    sv.step(3)
+                    got = sv.download()
+                    if ref is None:
+                        ref = got
+                    for g, r in zip(got, ref):
+                        np.testing.assert_array_equal(g, r, err_msg=f"{tile} {tz} {order}")
+
+
+@pytest.mark.parametrize("c_only", [False, True])
+def test_fast_report_sums_match_oracle(golden, c_only):
+    _, arrays = golden
+    z = arrays["pnc_s10"]
+    ds, dl, dc = O.increments(z["s0"], z["l0"], z["c0"], c_only=c_only)
+    with S.Solver(z["s0"].shape, kernel=FAST) as sv:
+        sv.upload(z["s0"], z["l0"], z["c0"])
+        got = sv.step_report(c_only=c_only)
+    np.testing.assert_allclose(got[0], (dc * dc).sum(), rtol=1e-12)
+    if not c_only:
+        np.testing.assert_allclose(got[1], (ds * ds).sum(), rtol=1e-12)
+        np.testing.assert_allclose(got[2], (dl * dl).sum(), rtol=1e-12)
+
+
+def test_fast_512cubed_properties():
+    """The 512^3 workload (2000 spheres R ~ U[6, 14]), 40 steps: within 1e-12
+    of the bitwise kernel, total solute conserved to 1e-12, and a periodic
+    shift of the input shifts the output bit for bit."""
+    shape = (512, 512, 512)
+    cen, rad = S.spheres(2, 2000, shape, 6.0, 14.0)
+    with S.Solver(shape, kernel=FAST) as a:
+        a.init_spheres(cen, rad)
+        s0, l0, c0 = a.download()
+        st0 = a.reduce()
+        a.step(40)
+        st1 = a.reduce()
+        ga = a.download()
+    assert abs(st1["sum_c"] - st0["sum_c"]) / st0["sum_c"] <= 1e-12
+    with S.Solver(shape, kernel=S.KERNEL_TILED) as b:
+        b.upload(s0, l0, c0)
+        b.step(40)
+        gb = b.download()
+    errs = [float(np.abs(x - y).max()) for x, y in zip(ga, gb)]
+    print(f"512^3 fast vs bitwise after 40 steps: {errs}", flush=True)
+    assert max(errs) <= 1e-12
+    del gb
+    shift = (37, 200, 101)
+    with S.Solver(shape, kernel=FAST) as r:
+        r.upload(*(np.roll(f, shift, axis=(0, 1, 2)) for f in (s0, l0, c0)))
+        r.step(40)
+        gr = r.download()
+    for x, y in zip(ga, gr):
+        np.testing.assert_array_equal(np.roll(x, shift, axis=(0, 1, 2)), y)

@@ -1,6 +1,9 @@
-"""The FP32 form: bit-exact against its own float32 restatement for one
-step, and within stated tolerances of the FP64 path (which is bitwise the
-reference) over many steps, with FP64 solute bookkeeping."""
+"""The FP32 form: within a few float32 ulps of its own operation sequence
+evaluated in float32 (the kernel's fused multiply-adds and hardware
+reciprocal cannot be rounded step for step in numpy), within the form's own
+rounding error of the float64 value of its formula, and within stated
+tolerances of the FP64 path (which is bitwise the reference) over many
+steps, with FP64 solute bookkeeping."""
 import numpy as np
 import pytest
 
@@ -18,17 +21,32 @@ def _state(shape, n, rmin, rmax, seed):
 
 
 @pytest.mark.parametrize("shape", [(16, 16, 16), (20, 16, 12), (33, 17, 65)])
-def test_fp32_one_step_bitwise_vs_float32_restatement(shape):
+def test_fp32_one_step_vs_its_float32_and_float64_evaluations(shape):
+    """Against the float32 emulation (step32): etaS, etaL within 2 ulps, c
+    within 4 (the reciprocal may differ by one ulp, which h, u and the flux
+    carry into c).  Against the float64 value (step64): etaS, etaL within 2
+    ulps, c within 12 (the form's own rounding of u and of the flux sum,
+    about 8 ulps at most on these states)."""
+    print(f"shape={shape}", flush=True)
     s, l, c = (a.astype(np.float32) for a in _state(shape, 4, 2.5, 5.0, 3))
     k = fp32_ref.constants(S.default_params())
-    want = fp32_ref.step(s, l, c, k)
+    want32 = fp32_ref.step32(s, l, c, k)
+    want64 = fp32_ref.step64(s, l, c, k)
     with S.Solver(shape, precision=S.F32) as sv:
         sv.set_mass_correction(0)
+        sv.set_fusion(1)
         sv.upload(s.astype(np.float64), l.astype(np.float64), c.astype(np.float64))
         sv.step(1)
         got = sv.download()
-    for g, w in zip(got, want):
-        np.testing.assert_array_equal(g.astype(np.float32), w)
+    for g, w32, w64, x0, name, b32, b64 in zip(got, want32, want64, (s, l, c), "slc",
+                                               (2, 2, 4), (2, 2, 12)):
+        e32 = np.abs(g - w32.astype(np.float64)) / fp32_ref.ulp32(w32)
+        e64 = np.abs(g - w64) / fp32_ref.ulp32(w64)
+        print(f"  {name}: max ulps vs float32 {e32.max():.2f}, vs float64 {e64.max():.2f}")
+        assert e32.max() <= b32, name
+        assert e64.max() <= b64, name
+    # the step moves the state by far more than these tolerances
+    assert max(np.abs(w - x).max() for w, x in zip(want64, (s, l, c))) > 100 * fp32_ref.ulp32(1.0)
 
 
 def test_fp32_vs_fp64_64cubed():

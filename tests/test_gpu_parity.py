@@ -107,6 +107,32 @@ def test_sphere_initial_condition_matches_numpy():
     np.testing.assert_allclose(c, 0.1 + 0.8 * want, rtol=0, atol=1e-15)
 
 
+def test_sphere_initial_condition_culled_blocks():
+    """A grid long enough in z (and x) that blocks of 16^3 sites far from
+    every sphere get no candidates: the values still match the full
+    evaluation, and sites farther than 60 cells from every surface are 0
+    exactly (tanh(d/2) rounds to 1 there)."""
+    shape = (300, 24, 170)
+    cen = np.array([[20.5, 10.0, 30.0], [150.0, 3.5, 160.0], [290.0, 20.0, 85.0]])
+    rad = np.array([5.0, 7.5, 4.0])
+    with S.Solver(shape) as sv:
+        sv.init_spheres(cen, rad)
+        s, l, c = sv.download()
+    z, y, x = np.meshgrid(*(np.arange(n) for n in shape), indexing="ij")
+    d = np.full(shape, 1e30)
+    for (cz, cy, cx), R in zip(cen, rad):
+        dz = np.abs(z - cz); dz = np.minimum(dz, shape[0] - dz)
+        dy = np.abs(y - cy); dy = np.minimum(dy, shape[1] - dy)
+        dx = np.abs(x - cx); dx = np.minimum(dx, shape[2] - dx)
+        d = np.minimum(d, np.sqrt(dz * dz + dy * dy + dx * dx) - R)
+    want = 0.5 * (1 - np.tanh(d / 2))
+    np.testing.assert_allclose(s, want, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(c, 0.1 + 0.8 * want, rtol=0, atol=1e-15)
+    far = d > 60.0
+    assert far.sum() > 0.3 * far.size
+    assert (s[far] == 0.0).all() and (l[far] == 1.0).all()
+
+
 def test_config1_64cubed_1000_steps_vs_oracle():
     """64^3 periodic, 40 spheres with R ~ U[4, 8], 1000 explicit steps: the
     FP64 kernel against the oracle, max|d| <= 1e-10 per field (observed:

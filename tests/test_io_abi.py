@@ -60,6 +60,25 @@ def test_create_rejects_bad_arguments():
     with pytest.raises(solver.PpfError) as e:
         solver.Solver((4, 4, 4), bc=7)
     assert e.value.code == -1
+    # the fast FP64 form and the FP32 form need the collapsed chemistry of
+    # AS == AL (phasefield.c:76-77) and a positive liquid mobility
+    for kw in (dict(kernel=solver.KERNEL_FAST), dict(precision=solver.F32)):
+        p = solver.default_params()
+        p.AL = 1.25e9
+        with pytest.raises(solver.PpfError) as e:
+            solver.Solver((4, 4, 4), params=p, **kw)
+        assert e.value.code == -1 and "AS == AL" in str(e.value)
+        p = solver.default_params()
+        p.ML = 0.0
+        with pytest.raises(solver.PpfError) as e:
+            solver.Solver((4, 4, 4), params=p, **kw)
+        assert e.value.code == -1 and "ML > 0" in str(e.value)
+    with pytest.raises(solver.PpfError) as e:
+        solver.Solver((4, 4, 4), precision=solver.F32, kernel=solver.KERNEL_FAST)
+    assert e.value.code == -1
+    with pytest.raises(solver.PpfError) as e:
+        solver.Solver((4, 4, 4), kernel=4)
+    assert e.value.code == -1
 
 
 def test_h5_roundtrip_and_compat(tmp_path):
