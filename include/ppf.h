@@ -255,6 +255,31 @@ const char *ppf_kernel_name(ppf_ctx *ctx);
 
 const char *ppf_last_error(ppf_ctx *ctx);
 
+/* One sweep (c_only = 0: sweep_fields, 1: sweep_fields_c) of an FP64 grid
+ * held in caller-owned DEVICE arrays, with the save-time sums of its squared
+ * increments, that allocates nothing: no hipMalloc, hipFree or allocating
+ * library routine, no stream or event creation.  No counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).
+ *   in:  s_in, l_in, c_in   device arrays [nz][ny][nx] on `device`
+ *   out: s_out, l_out, c_out the new state (c_only: only c_out is written,
+ *        s_out and l_out may be NULL); must not alias the inputs
+ *   out: *out the sums, as ppf_step_report returns them
+ * The fields equal ppf_step_report's on a context of the same grid bit for
+ * bit (same kernel and tiling), and so do the sums while the grid needs at
+ * most 65536 workgroups (1024^3 and below; larger grids take longer z
+ * chunks and the sums differ by rounding).  The workgroup partial sums live
+ * in a fixed 1.5 MiB device array of the library, one copy per GPU, held
+ * until the process ends: grids whose x-y plane needs more than 65536
+ * column tiles (nx*ny above ~33.5 M sites) are refused with PPF_EARG.  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU (two
+ * host threads, or work queued on other streams) race.  Runs on the null
+ * stream and synchronises it.  Errors are reported by ppf_last_error(NULL). */
+int ppf_sweep_report_noalloc(const double *s_in, const double *l_in, const double *c_in,
+                             double *s_out, double *l_out, double *c_out, int nz, int ny, int nx,
+                             int bc, int c_only, const ppf_params *params, int device,
+                             ppf_incsums *out);
+
 /* ------------------------------------------------------------------------
  * Ensembles: many independent grids of uneven size stepped together, one
  * persistent launch per step over a work list of (member, tile, z chunk)

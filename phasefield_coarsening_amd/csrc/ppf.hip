@@ -20,6 +20,7 @@
 #include "init_spheres.h"
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 constexpr int MAX_DIM = 1 << 16;
@@ -1292,3 +1293,101 @@ extern "C" const char *ppf_kernel_name(ppf_ctx *ctx)
 }
 
 extern "C" const char *ppf_last_error(ppf_ctx *ctx) { return ctx ? ctx->err : g_create_err; }
+
+// ----------------------------------------------------------------------------
+// ppf_sweep_report_noalloc: one sweep with its save-time sums on caller-owned
+// device arrays, allocating nothing.  The per-workgroup partial sums and the
+// three results live in `scratch`, a fixed device array of this library
+// (one per GPU, held until the process ends).  Host code never passes the
+// array's name to a kernel or a copy: it is a host-side stand-in, and the
+// device address of each GPU's copy comes from hipGetSymbolAddress, looked
+// up once per device and kept in g_scratch_ptr.
+// ----------------------------------------------------------------------------
+constexpr int NOALLOC_MAX_BLOCKS = 65536;  // partial-sum triples: 1.5 MiB
+constexpr size_t NOALLOC_SCRATCH = 3 * (size_t)NOALLOC_MAX_BLOCKS + 8;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH];
+
+constexpr int NOALLOC_MAX_DEV = 64;
+static double *g_scratch_ptr[NOALLOC_MAX_DEV];
+static std::mutex g_scratch_mu;
+
+static hipError_t scratch_on(int device, double **ptr)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        g_scratch_ptr[device] = (double *)p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+extern "C" int ppf_sweep_report_noalloc(const double *s_in, const double *l_in,
+                                        const double *c_in, double *s_out, double *l_out,
+                                        double *c_out, int nz, int ny, int nx, int bc, int c_only,
+                                        const ppf_params *params, int device, ppf_incsums *out)
+{
+    if (!s_in || !l_in || !c_in || !c_out || !params || !out || (!c_only && (!s_out || !l_out)))
+        return set_err(nullptr, PPF_EARG, "null argument");
+    if (nz < 1 || ny < 1 || nx < 1 || nz > MAX_DIM || ny > MAX_DIM || nx > MAX_DIM)
+        return set_err(nullptr, PPF_EARG, "bad grid %d x %d x %d", nz, ny, nx);
+    if ((size_t)ny * nx >= ((size_t)1 << 31))
+        return set_err(nullptr, PPF_EARG, "an x-y plane of %d x %d sites is too large", ny, nx);
+    if (bc != PPF_BC_PERIODIC && bc != PPF_BC_NOFLUX)
+        return set_err(nullptr, PPF_EARG, "bad bc %d", bc);
+    // the tiling a context of this grid uses for single steps (choose_tiles),
+    // with longer z chunks when its workgroups would outnumber the scratch
+    const TileCfg t = nx >= 64 ? TileCfg{64, 8} : nx >= 32 ? TileCfg{32, 8} : TileCfg{16, 16};
+    const long ntile = (long)((nx + t.tx - 1) / t.tx) * ((ny + t.ty - 1) / t.ty);
+    if (ntile > NOALLOC_MAX_BLOCKS)
+        return set_err(nullptr, PPF_EARG,
+                       "an x-y plane of %d x %d sites needs %ld workgroups; the scratch buffer "
+                       "holds partial sums for %d",
+                       ny, nx, ntile, NOALLOC_MAX_BLOCKS);
+    int tz = nz < 32 ? nz : 32;
+    while (ntile * ((nz + tz - 1) / tz) > NOALLOC_MAX_BLOCKS)
+        tz++;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev ||
+        device >= NOALLOC_MAX_DEV)
+        return set_err(nullptr, PPF_ENODEV, "device %d not present", device);
+    if (hipSetDevice(device) != hipSuccess)
+        return set_err(nullptr, PPF_EHIP, "hipSetDevice(%d) failed", device);
+    double *scr = nullptr;
+    hipError_t e = scratch_on(device, &scr);
+    if (e != hipSuccess)
+        return set_err(nullptr, PPF_EHIP, "hipGetSymbolAddress(scratch) failed: %s",
+                       hipGetErrorString(e));
+    const int ntx = (nx + t.tx - 1) / t.tx, nty = (ny + t.ty - 1) / t.ty;
+    const int nb = (int)(ntile * ((nz + tz - 1) / tz));
+    const KConst k = make_kconst(*params);
+    const bool eqa = params->AL == params->AS;
+    auto launch = [&](auto p) {
+        using P = decltype(p);
+        tiled_fn<P> f = pick_tiled_shape<P>(t, bc, c_only != 0, true, 0);
+        hipLaunchKernelGGL(f, dim3((unsigned)nb), dim3(t.tx * t.ty), 0, (hipStream_t)0, s_in, l_in,
+                           c_in, s_out, l_out, c_out, nx, ny, nz, tz, ntx, nty, 0, k, scr);
+    };
+    if (eqa)
+        launch(PhysF64<true>{});
+    else
+        launch(PhysF64<false>{});
+    hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(256), 0, (hipStream_t)0, (const double *)scr,
+                       nb, scr + 3 * (size_t)NOALLOC_MAX_BLOCKS);
+    e = hipGetLastError();
+    double r[3] = {0.0, 0.0, 0.0};
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(r, scr + 3 * (size_t)NOALLOC_MAX_BLOCKS, sizeof r,
+                           hipMemcpyDeviceToHost, (hipStream_t)0);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize((hipStream_t)0);
+    if (e != hipSuccess)
+        return set_err(nullptr, PPF_EHIP, "sweep failed: %s", hipGetErrorString(e));
+    out->sum_dc2 = r[0];
+    out->sum_ds2 = c_only ? 0.0 : r[1];
+    out->sum_dl2 = c_only ? 0.0 : r[2];
+    return PPF_OK;
+}

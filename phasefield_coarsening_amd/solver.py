@@ -103,6 +103,9 @@ SIGNATURES = {
     "ppf_steps_taken": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "ppf_kernel_name": (ctypes.c_char_p, [ctypes.c_void_p]),
     "ppf_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "ppf_sweep_report_noalloc": (ctypes.c_int, [ctypes.c_void_p] * 6 + [ctypes.c_int] * 5 +
+                                 [ctypes.POINTER(Params), ctypes.c_int,
+                                  ctypes.POINTER(IncSums)]),
     "ppf_ens_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.POINTER(Params)]),
@@ -458,3 +461,21 @@ class Ensemble:
         n, it, g = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
         self._check(lib().ppf_ens_info(self._h, ctypes.byref(n), ctypes.byref(it), ctypes.byref(g)))
         return {"total_sites": n.value, "items": it.value, "grid": g.value}
+
+
+def sweep_report_noalloc(s_in, l_in, c_in, s_out, l_out, c_out, shape, bc=BC_PERIODIC,
+                         c_only=False, params=None, device=0):
+    """ppf_sweep_report_noalloc on device addresses (ints, e.g. a torch
+    tensor's data_ptr()): one sweep of an FP64 grid in caller-owned device
+    memory with its save-time sums, allocating nothing.  Returns (sum_dc2,
+    sum_ds2, sum_dl2)."""
+    nz, ny, nx = (int(v) for v in shape)
+    p = params if params is not None else default_params()
+    r = IncSums()
+    L = lib()
+    rc = L.ppf_sweep_report_noalloc(s_in, l_in, c_in, s_out, l_out, c_out, nz, ny, nx, int(bc),
+                                    int(bool(c_only)), ctypes.byref(p), int(device),
+                                    ctypes.byref(r))
+    if rc:
+        raise PpfError(rc, L.ppf_last_error(None).decode(errors="replace"))
+    return r.sum_dc2, r.sum_ds2, r.sum_dl2

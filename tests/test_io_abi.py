@@ -145,3 +145,18 @@ def test_reader_dims_come_from_etaS(tmp_path):
     assert s2.shape == (2, 3, 4)
     np.testing.assert_array_equal(l2, s + 1)
     np.testing.assert_array_equal(c2, s + 2)
+
+
+def test_noalloc_sweep_refuses_before_touching_the_device():
+    """Argument and size checks of ppf_sweep_report_noalloc come first: a
+    plane that needs more column tiles than its scratch holds partial sums
+    for, a bad boundary condition or a null pointer is refused with
+    PPF_EARG (the dummy addresses below are never dereferenced)."""
+    dummy = 4096
+    for shape, bc in (((1, 8192, 8192), 0), ((4, 4, 4), 5), ((0, 4, 4), 0)):
+        with pytest.raises(solver.PpfError) as e:
+            solver.sweep_report_noalloc(dummy, dummy, dummy, dummy, dummy, dummy, shape, bc=bc)
+        assert e.value.code == -1
+    with pytest.raises(solver.PpfError) as e:
+        solver.sweep_report_noalloc(None, dummy, dummy, dummy, dummy, dummy, (4, 4, 4))
+    assert e.value.code == -1
