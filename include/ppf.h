@@ -19,7 +19,8 @@
  *
  * Every call returns 0 or a negative PPF_E* status and never prints or
  * exits; ppf_last_error(ctx) explains the last failure on that context
- * (ppf_last_error(NULL) the last failure of ppf_create).  One host thread
+ * (ppf_last_error(NULL) the calling thread's last failure of a call without
+ * a context: ppf_create, ppf_sweep_report_noalloc).  One host thread
  * per context; calls are ordered on the context's own HIP stream, and the
  * calls that hand data back to the host synchronise it.
  */
@@ -232,10 +233,23 @@ int ppf_set_mass_correction(ppf_ctx *ctx, int every);
 int ppf_mass_target(ppf_ctx *ctx, double *target, int64_t *n_shifts);
 
 /* Device-to-device copy of the current state (and step count, FP32
- * solute target) from src into dst; same grid, precision and device.
- * Checkpoints for the surrogate's exact re-run.  No counterpart in the
- * reference. */
+ * solute target) from src into dst; same grid and device.  Checkpoints for
+ * the surrogate's exact re-run.  Contexts of different precision convert
+ * (rounding to nearest, or widening); an FP32 destination of an FP64
+ * source takes the FP64 total of the source's c, summed in double-double
+ * on the device, as its solute target.  No counterpart in the reference. */
 int ppf_copy_state(ppf_ctx *dst, ppf_ctx *src);
+
+/* Per-field differences between the current states of two contexts of the
+ * same grid and device (any precisions), computed on the device in FP64:
+ * max |a - b| (NaN if any difference is NaN) and the root mean square, for
+ * etaS, etaL, c in that order.  Synchronises both.  No counterpart in the
+ * reference; it measures the FP32 form against FP64. */
+typedef struct ppf_diff {
+    double max_abs[3];
+    double rms[3];
+} ppf_diff;
+int ppf_compare(ppf_ctx *a, ppf_ctx *b, ppf_diff *out);
 
 /* Timing on the context stream: ppf_timer_start records a HIP event,
  * ppf_timer_stop records a second one, synchronises and returns the elapsed

@@ -335,3 +335,49 @@ static __global__ __launch_bounds__(256) void k_mass_shift(float *__restrict__ c
 {
     mass_shift_body(c, n, delta[0], seed, (size_t)blockIdx.x * 256, (size_t)gridDim.x * 256);
 }
+
+// Round or widen a field: dst[q] = (Td)src[q].
+template <typename Ts, typename Td>
+__global__ __launch_bounds__(256) void k_convert(const Ts *__restrict__ src, Td *__restrict__ dst,
+                                                 size_t n)
+{
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        dst[q] = (Td)src[q];
+}
+
+// Per-workgroup max |a - b| and sum (a - b)^2 of one field pair, in FP64:
+// partials[2 b] = max, partials[2 b + 1] = sum of squares.  A NaN difference
+// makes the max NaN (fmax would drop it).
+template <typename Ta, typename Tb>
+__global__ __launch_bounds__(256) void k_diff(const Ta *__restrict__ a, const Tb *__restrict__ b,
+                                              size_t n, double *__restrict__ partials)
+{
+    double mx = 0.0, ss = 0.0;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        const double d = (double)a[q] - (double)b[q];
+        const double ad = fabs(d);
+        mx = (ad > mx || ad != ad) ? ad : mx;
+        ss += d * d;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double m2 = __shfl_xor(mx, o);
+        mx = (m2 > mx || m2 != m2) ? m2 : mx;
+        ss += __shfl_xor(ss, o);
+    }
+    __shared__ double part[2][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        part[0][w] = mx;
+        part[1][w] = ss;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = part[0][0], t = part[1][0];
+        for (int j = 1; j < 4; j++) {
+            m = (part[0][j] > m || part[0][j] != part[0][j]) ? part[0][j] : m;
+            t += part[1][j];
+        }
+        partials[2 * (size_t)blockIdx.x] = m;
+        partials[2 * (size_t)blockIdx.x + 1] = t;
+    }
+}

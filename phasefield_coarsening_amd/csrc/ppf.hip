@@ -21,11 +21,14 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 constexpr int MAX_DIM = 1 << 16;
 
-static char g_create_err[512];
+// Message of the last failure without a context (ppf_create, the no-alloc
+// sweep), one per host thread.
+static thread_local char g_create_err[512];
 
 static int set_err(ppf_ctx *ctx, int code, const char *fmt, ...)
 {
@@ -37,11 +40,17 @@ static int set_err(ppf_ctx *ctx, int code, const char *fmt, ...)
     return code;
 }
 
+// A failed HIP call also leaves its error as the thread's "last error",
+// which a later, unrelated hipGetLastError would report; every error path
+// clears it (hipGetLastError resets it) so that a context or call after an
+// error (an out-of-memory create, say) starts clean.
 #define HIPCHK(ctx, call)                                                                   \
     do {                                                                                    \
         hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess)                                                               \
+        if (e_ != hipSuccess) {                                                             \
+            (void)hipGetLastError();                                                        \
             return set_err((ctx), PPF_EHIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+        }                                                                                   \
     } while (0)
 
 extern "C" int ppf_params_default(ppf_params *p)
@@ -274,6 +283,28 @@ static FusedPick<P> pick_fused(TileCfg t, int order, int bc = PPF_BC_PERIODIC)
         return {nullptr, 0};
     }
     const bool w3 = F64 && (order & 4);
+    // Order bit 4: loads issued a plane early (PF) in the variants whose
+    // registers allow it: the fast FP64 form at three waves per SIMD (160
+    // VGPRs; the bitwise form would spill) and FP32 at five.
+    constexpr bool PF_OK = !F64 || std::is_same<P, PhysF64F>::value;
+    if constexpr (PF_OK) {
+        if (order & 16) {
+            if (F64 && !w3)
+                return {nullptr, 0};
+            constexpr int WP = F64 ? 3 : 5;
+            if (t.tx == 32 && t.ty == 8)
+                return {k_tiled2<P, 32, 8, WP, true, true>, nt};
+            if (t.tx == 16 && t.ty == 16)
+                return {k_tiled2<P, 16, 16, WP, true, true>, nt};
+            if (!F64 && t.tx == 32 && t.ty == 16)
+                return {k_tiled2<P, 32, 16, WP, true, true>, nt};
+            if (!F64 && t.tx == 64 && t.ty == 8)
+                return {k_tiled2<P, 64, 8, WP, true, true>, nt};
+            return {nullptr, 0};
+        }
+    } else if (order & 16) {
+        return {nullptr, 0};
+    }
     if (t.tx == 64 && t.ty == 8)
         return {k_tiled2<P, 64, 8, W, F64, true>, nt};
     if (t.tx == 32 && t.ty == 8)
@@ -466,9 +497,17 @@ static void choose_tiles(ppf_ctx *ctx)
     // 1024^3.
     // FP32 runs the ring-wave kernel (order bit 3): 221 against 211 GLUPS at
     // 256^3, level at 512^3.
+    // The fast FP64 form: 32x8 tiles (145 against 142 GLUPS for 16x16 at
+    // 512^3, 130-134 against 119 at 256^3, level at 1024^3).  FP32 on grids
+    // of 512^3 sites and more: 64x8 tiles without ring waves (order 2;
+    // 269 against 258 GLUPS for the ring-wave kernel at 512^3, 259-261
+    // against 250 at 1024^3); below that the ring waves stay (266 against
+    // 233-252 at 256^3).
     ctx->fuse = 2;
-    ctx->tile2 = f32 ? TileCfg{32, 16} : TileCfg{16, 16};
-    ctx->order2 = f32 ? 10 : 6;
+    const bool big = ctx->n >= ((size_t)1 << 27);
+    ctx->tile2 = f32 ? (big ? TileCfg{64, 8} : TileCfg{32, 16})
+                     : ctx->kernel == PPF_KERNEL_FAST ? TileCfg{32, 8} : TileCfg{16, 16};
+    ctx->order2 = f32 ? (big ? 2 : 10) : 6;
     // z chunks of 64 planes, shortened (to 16 at least) while the launch
     // would leave CUs without a workgroup: at 128^3, 64-plane chunks of
     // 32x16 tiles are 64 workgroups (38 GLUPS), 32-plane ones 128 (75).
@@ -589,14 +628,18 @@ extern "C" int ppf_create(ppf_ctx **out, const ppf_desc *desc)
     for (int c = 0; c < 2 && rc == PPF_OK; c++)
         for (int f = 0; f < 3 && rc == PPF_OK; f++) {
             e = hipMalloc(&ctx->buf[c][f], n * ctx->esz);
-            if (e != hipSuccess)
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
                 rc = set_err(nullptr, PPF_ENOMEM, "hipMalloc of %zu bytes failed: %s",
                              n * ctx->esz, hipGetErrorString(e));
+            }
         }
     if (rc == PPF_OK) {
         e = hipMalloc(&ctx->dres, 16 * sizeof(double));
-        if (e != hipSuccess)
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
             rc = set_err(nullptr, PPF_ENOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
+        }
     }
     if (rc != PPF_OK) {
         ppf_destroy(ctx);
@@ -638,6 +681,8 @@ extern "C" int ppf_destroy(ppf_ctx *ctx)
         hipFree(ctx->ccl_labels);
     if (ctx->ccl_counts)
         hipFree(ctx->ccl_counts);
+    if (ctx->ccl_list)
+        hipFree(ctx->ccl_list);
     if (ctx->dres)
         hipFree(ctx->dres);
     if (ctx->ev0)
@@ -879,6 +924,8 @@ extern "C" int ppf_init_spheres(ppf_ctx *ctx, int n, const double *centers_zyx,
                                          (float *)ctx->buf[ctx->cur[2]][2], ctx->d.nx, ctx->d.ny,
                                          ctx->d.nz, n, centers_zyx, radii, c_liq, c_span);
     if (e != hipSuccess)
+        (void)hipGetLastError();
+    if (e != hipSuccess)
         return set_err(ctx, e == hipErrorInvalidValue ? PPF_EARG : PPF_EHIP,
                        "sphere initial condition failed: %s",
                        e == hipErrorInvalidValue ? "too many sphere-block pairs"
@@ -1004,11 +1051,13 @@ extern "C" int ppf_autotune(ppf_ctx *ctx, int reps, double *best_ms)
         static const int tzs2[] = {32, 64};
         // order bit 3 selects the ring-wave kernel, bit 2 the three-wave
         // FP64 variants (no effect in FP32)
-        static const int orders2_64[] = {0, 2, 4, 6, 8, 10}, orders2_32[] = {0, 2, 8, 10};
+        // bit 4 the variants with loads a plane early (fast FP64 and FP32)
+        static const int orders2_64[] = {0, 2, 4, 6, 8, 10, 20, 22},
+                         orders2_32[] = {0, 2, 8, 10, 16, 18};
         const bool f64 = ctx->d.precision == PPF_F64;
         const bool mirror = ctx->d.bc == PPF_BC_NOFLUX;  // bits 2 and 3 have no effect there
         const int *orders2 = f64 ? orders2_64 : orders2_32;
-        const int norders2 = mirror ? 2 : f64 ? 6 : 4;
+        const int norders2 = mirror ? 2 : f64 ? 8 : 6;
         double best2 = 1e300;
         TileCfg bt2 = keep2;
         int btz2 = keep_tz2, bo2 = keep_o2;
@@ -1093,8 +1142,8 @@ extern "C" int ppf_set_fusion(ppf_ctx *ctx, int steps_per_launch, int tx, int ty
     }
     if (tz < 0)
         return set_err(ctx, PPF_EARG, "tz must be >= 0");
-    if (order > 15)
-        return set_err(ctx, PPF_EARG, "order must be 0..15 (or negative to keep)");
+    if (order > 31)
+        return set_err(ctx, PPF_EARG, "order must be 0..31 (or negative to keep)");
     // the ring-wave kernel (order bit 3) and the mirror-boundary kernels
     // exist for some tiles only
     const TileCfg t = (tx != 0 || ty != 0) ? TileCfg{tx, ty} : ctx->tile2;
@@ -1135,6 +1184,7 @@ extern "C" int ppf_particles(ppf_ctx *ctx, double threshold, ppf_particle_stats 
         HIPCHK(ctx, hipMalloc(&ctx->ccl_labels, ctx->n * sizeof(int)));
         hipError_t e = hipMalloc(&ctx->ccl_counts, ctx->n * sizeof(int));
         if (e != hipSuccess) {
+            (void)hipGetLastError();
             hipFree(ctx->ccl_labels);
             ctx->ccl_labels = nullptr;
             ctx->ccl_counts = nullptr;
@@ -1168,8 +1218,17 @@ extern "C" int ppf_particles(ppf_ctx *ctx, double threshold, ppf_particle_stats 
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<int> host(nroots);
     if (nroots > 0) {
-        int *list = nullptr;
-        HIPCHK(ctx, hipMalloc(&list, nroots * sizeof(int)));
+        // the root list buffer is kept between calls and grows when needed
+        if (nroots > ctx->ccl_list_cap) {
+            if (ctx->ccl_list)
+                HIPCHK(ctx, hipFree(ctx->ccl_list));
+            ctx->ccl_list = nullptr;
+            ctx->ccl_list_cap = 0;
+            const size_t cap = nroots + nroots / 4 + 1024;
+            HIPCHK(ctx, hipMalloc(&ctx->ccl_list, cap * sizeof(int)));
+            ctx->ccl_list_cap = cap;
+        }
+        int *list = ctx->ccl_list;
         hipError_t e = hipMemsetAsync(dn, 0, sizeof *dn, ctx->stream);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(k_ccl_collect, dim3(g), dim3(256), 0, ctx->stream, ctx->ccl_labels,
@@ -1181,9 +1240,10 @@ extern "C" int ppf_particles(ppf_ctx *ctx, double threshold, ppf_particle_stats 
                                ctx->stream);
         if (e == hipSuccess)
             e = hipStreamSynchronize(ctx->stream);
-        hipFree(list);
-        if (e != hipSuccess)
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
             return set_err(ctx, PPF_EHIP, "particle collection failed: %s", hipGetErrorString(e));
+        }
     }
     // sizes in descending order: the order of the device list is arbitrary
     std::sort(host.begin(), host.end(), [](int a, int b) { return a > b; });
@@ -1230,18 +1290,99 @@ extern "C" int ppf_copy_state(ppf_ctx *dst, ppf_ctx *src)
 {
     if (!dst || !src)
         return dst ? set_err(dst, PPF_EARG, "null source") : PPF_EARG;
-    if (dst->n != src->n || dst->esz != src->esz || dst->d.nz != src->d.nz ||
-        dst->d.ny != src->d.ny || dst->d.device != src->d.device)
-        return set_err(dst, PPF_EARG, "contexts differ in grid, precision or device");
+    if (dst->n != src->n || dst->d.nz != src->d.nz || dst->d.ny != src->d.ny ||
+        dst->d.device != src->d.device)
+        return set_err(dst, PPF_EARG, "contexts differ in grid or device");
     HIPCHK(dst, hipSetDevice(dst->d.device));
     HIPCHK(dst, hipStreamSynchronize(src->stream));
-    for (int f = 0; f < 3; f++)
-        HIPCHK(dst, hipMemcpyAsync(dst->buf[dst->cur[f]][f], src->buf[src->cur[f]][f],
-                                   dst->n * dst->esz, hipMemcpyDeviceToDevice, dst->stream));
+    const unsigned g = (unsigned)((dst->n + 255) / 256 < 65536 ? (dst->n + 255) / 256 : 65536);
+    for (int f = 0; f < 3; f++) {
+        void *to = dst->buf[dst->cur[f]][f];
+        const void *from = src->buf[src->cur[f]][f];
+        if (dst->esz == src->esz)
+            HIPCHK(dst, hipMemcpyAsync(to, from, dst->n * dst->esz, hipMemcpyDeviceToDevice,
+                                       dst->stream));
+        else if (dst->esz == 4)
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(256), 0, dst->stream,
+                               (const double *)from, (float *)to, dst->n);
+        else
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(256), 0, dst->stream,
+                               (const float *)from, (double *)to, dst->n);
+    }
+    HIPCHK(dst, hipGetLastError());
     HIPCHK(dst, hipStreamSynchronize(dst->stream));
     dst->steps = src->steps;
-    dst->c_target = src->c_target;
     dst->since_shift = src->since_shift;
+    if (dst->d.precision == PPF_F32 && src->d.precision == PPF_F32) {
+        dst->c_target = src->c_target;
+    } else if (dst->d.precision == PPF_F32) {
+        // the FP64 solute target is the source's FP64 total, summed in
+        // double-double on the device (not the total of its FP32 rounding)
+        const int nb = 1024;
+        int rc = ensure_partials(dst, (size_t)2 * nb);
+        if (rc)
+            return rc;
+        hipLaunchKernelGGL(k_sum_dd<double>, dim3(nb), dim3(256), 0, dst->stream,
+                           (const double *)src->buf[src->cur[2]][2], src->n, dst->partials);
+        hipLaunchKernelGGL(k_mass_delta, dim3(1), dim3(256), 0, dst->stream, dst->partials, nb,
+                           0.0, 1.0, dst->dres + 8);
+        HIPCHK(dst, hipGetLastError());
+        double r[2];
+        HIPCHK(dst, hipMemcpyAsync(r, dst->dres + 8, sizeof r, hipMemcpyDeviceToHost,
+                                   dst->stream));
+        HIPCHK(dst, hipStreamSynchronize(dst->stream));
+        dst->c_target = r[1];
+        dst->since_shift = 0;
+    }
+    return PPF_OK;
+}
+
+template <typename Ta, typename Tb>
+static void launch_diff(ppf_ctx *a, ppf_ctx *b, int f, double *part, int nb)
+{
+    hipLaunchKernelGGL((k_diff<Ta, Tb>), dim3(nb), dim3(256), 0, a->stream,
+                       (const Ta *)a->buf[a->cur[f]][f], (const Tb *)b->buf[b->cur[f]][f], a->n,
+                       part);
+}
+
+extern "C" int ppf_compare(ppf_ctx *a, ppf_ctx *b, ppf_diff *out)
+{
+    if (!a || !b || !out)
+        return a ? set_err(a, PPF_EARG, "null argument") : PPF_EARG;
+    if (a->n != b->n || a->d.nz != b->d.nz || a->d.ny != b->d.ny || a->d.device != b->d.device)
+        return set_err(a, PPF_EARG, "contexts differ in grid or device");
+    HIPCHK(a, hipSetDevice(a->d.device));
+    HIPCHK(a, hipStreamSynchronize(b->stream));
+    const int nb = 1024;
+    int rc = ensure_partials(a, (size_t)6 * nb);
+    if (rc)
+        return rc;
+    for (int f = 0; f < 3; f++) {
+        double *part = a->partials + (size_t)2 * nb * f;
+        if (a->esz == 8 && b->esz == 8)
+            launch_diff<double, double>(a, b, f, part, nb);
+        else if (a->esz == 8)
+            launch_diff<double, float>(a, b, f, part, nb);
+        else if (b->esz == 8)
+            launch_diff<float, double>(a, b, f, part, nb);
+        else
+            launch_diff<float, float>(a, b, f, part, nb);
+    }
+    HIPCHK(a, hipGetLastError());
+    std::vector<double> host((size_t)6 * nb);
+    HIPCHK(a, hipMemcpyAsync(host.data(), a->partials, host.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, a->stream));
+    HIPCHK(a, hipStreamSynchronize(a->stream));
+    for (int f = 0; f < 3; f++) {
+        double m = 0.0, t = 0.0;
+        for (int j = 0; j < nb; j++) {
+            const double v = host[(size_t)2 * nb * f + 2 * j];
+            m = (v > m || v != v) ? v : m;
+            t += host[(size_t)2 * nb * f + 2 * j + 1];
+        }
+        out->max_abs[f] = m;
+        out->rms[f] = sqrt(t / (double)a->n);
+    }
     return PPF_OK;
 }
 

@@ -72,7 +72,7 @@ struct ppf_ens {
     char err[512];
 };
 
-static char g_ens_err[512];
+static thread_local char g_ens_err[512];  // last create failure, per host thread
 
 static int eset(ppf_ens *e, int code, const char *fmt, ...)
 {
@@ -84,11 +84,15 @@ static int eset(ppf_ens *e, int code, const char *fmt, ...)
     return code;
 }
 
+// (a failed call's error is also cleared from the thread's last error, so
+// later calls do not report it)
 #define EHIP(e, call)                                                                   \
     do {                                                                                \
         hipError_t e_ = (call);                                                         \
-        if (e_ != hipSuccess)                                                           \
+        if (e_ != hipSuccess) {                                                         \
+            (void)hipGetLastError();                                                    \
             return eset((e), PPF_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));  \
+        }                                                                               \
     } while (0)
 
 typedef void (*ens_fn64)(const EnsDom *, const EnsItem *, int, int *, int, KConst);
@@ -357,6 +361,7 @@ extern "C" int ppf_ens_create(ppf_ens **out, int ndom, const int *dims_zyx, int 
     if (h == hipSuccess)
         h = hipMalloc(&e->dres, 16 * sizeof(double));
     if (h != hipSuccess) {
+        (void)hipGetLastError();
         eset(nullptr, PPF_ENOMEM, "ensemble allocation (%zu bytes) failed: %s", total,
              hipGetErrorString(h));
         ppf_ens_destroy(e);
@@ -551,6 +556,8 @@ extern "C" int ppf_ens_init_spheres(ppf_ens *e, int dom, int n, const double *ce
             : init_spheres_device<float>(e->stream, (float *)m.f[e->parity][0],
                                          (float *)m.f[e->parity][1], (float *)m.f[e->parity][2],
                                          m.nx, m.ny, m.nz, n, centers_zyx, radii, c_liq, c_span);
+    if (h != hipSuccess)
+        (void)hipGetLastError();
     if (h != hipSuccess)
         return eset(e, h == hipErrorInvalidValue ? PPF_EARG : PPF_EHIP,
                     "sphere initial condition failed: %s",

@@ -28,6 +28,8 @@ struct ppf_ctx {
     int64_t n_shifts;
     int *ccl_labels;  // per-site labels and counts for ppf_particles, on first use
     int *ccl_counts;
+    int *ccl_list;        // component roots of the last ppf_particles call
+    size_t ccl_list_cap;  // in ints
     hipStream_t stream;
     hipEvent_t ev0, ev1;
     void *buf[2][3];  // [copy][field]: etaS, etaL, c

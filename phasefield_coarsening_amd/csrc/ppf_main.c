@@ -33,11 +33,24 @@
  *   --blocks=Q                [PPF_BLOCKS]       chunk = the local block of a
  *                             Q x Q x Q decomposition (Q divides each size):
  *                             the file an `mpirun -np Q^3` run writes
+ *   --kernel=bitwise|fast     [PPF_KERNEL]       default bitwise: the
+ *                             reference's arithmetic bit for bit; fast: FP64
+ *                             with fused multiply-adds and the collapsed
+ *                             AS == AL chemistry (within 1e-10 per field of
+ *                             the reference after 1000 steps)
+ *   --allow-nonfinite         [PPF_ALLOW_NONFINITE=1] step an input that holds
+ *                             NaN or Inf values instead of refusing it
  *
  * Differences from the reference, on purpose: grid sizes come from the
  * input file (the reference compiles them in, phasefield.c:47-56); a file
  * that cannot be read is an error (the reference carries on with fields of
- * 1.0); long file names do not overflow (phasefield.c:107, 1201).
+ * 1.0); /etaL is read from the file (the reference's readers read through
+ * the closed /etaS id, so its etaL stays 1.0, phasefield.c:1026-1032;
+ * --etal-compat reproduces that); an input with NaN or Inf values is
+ * refused (the reference steps it); a failed output write ends the run
+ * with status 1 (the reference ignores every HDF5 status,
+ * phasefield.c:1052-1109); long file names do not overflow
+ * (phasefield.c:107, 1201).
  */
 #define _POSIX_C_SOURCE 200809L
 #include <math.h>
@@ -58,6 +71,8 @@ typedef struct {
     int stats;
     int chunk[3];  /* output chunk dims (z, y, x); zeros: contiguous */
     int blocks;    /* > 0: chunk = dims / blocks */
+    int kernel;    /* PPF_KERNEL_AUTO (bitwise) or PPF_KERNEL_FAST */
+    int allow_nonfinite;
 } opts_t;
 
 static int parse_long(const char *s, long *out)
@@ -117,6 +132,19 @@ static int set_opt(opts_t *o, const char *key, const char *val)
         o->blocks = 0;
         return 0;
     }
+    if (!strcmp(key, "kernel")) {
+        if (!strcmp(val, "bitwise"))
+            o->kernel = PPF_KERNEL_AUTO;
+        else if (!strcmp(val, "fast"))
+            o->kernel = PPF_KERNEL_FAST;
+        else
+            return -1;
+        return 0;
+    }
+    if (!strcmp(key, "allow-nonfinite")) {
+        o->allow_nonfinite = strcmp(val, "0") != 0;
+        return 0;
+    }
     if (!strcmp(key, "blocks")) {
         if (parse_long(val, &v) || v < 1 || v > 4096)
             return -1;
@@ -136,7 +164,9 @@ static void env_opts(opts_t *o)
                                    {"PPF_DEVICE", "device"},
                                    {"PPF_STATS", "stats"},
                                    {"PPF_CHUNK", "chunk"},
-                                   {"PPF_BLOCKS", "blocks"}};
+                                   {"PPF_BLOCKS", "blocks"},
+                                   {"PPF_KERNEL", "kernel"},
+                                   {"PPF_ALLOW_NONFINITE", "allow-nonfinite"}};
     for (size_t i = 0; i < sizeof map / sizeof map[0]; i++) {
         const char *v = getenv(map[i][0]);
         if (v && set_opt(o, map[i][1], v))
@@ -165,7 +195,8 @@ static int die(const char *what, const char *msg)
 
 int main(int argc, char **argv)
 {
-    opts_t o = {PPF_BC_NOFLUX, 5000001L, 50000L, 100000000L, 0, 0, 0, {0, 0, 0}, 0};
+    opts_t o = {PPF_BC_NOFLUX, 5000001L, 50000L, 100000000L, 0, 0, 0, {0, 0, 0}, 0,
+                PPF_KERNEL_AUTO, 0};
     env_opts(&o);
     const char *input = NULL;
     for (int a = 1; a < argc; a++) {
@@ -218,6 +249,16 @@ int main(int argc, char **argv)
         return die("host memory", "allocation failed");
     if (ppfio_read(input, dims, s, l, c, o.etal_compat ? PPFIO_ETAL_COMPAT : 0))
         return die(input, ppfio_last_error());
+    if (!o.allow_nonfinite) {
+        size_t bad = 0;
+        for (size_t q = 0; q < n; q++)
+            bad += !isfinite(s[q]) || !isfinite(l[q]) || !isfinite(c[q]);
+        if (bad) {
+            fprintf(stderr, "ppf: %s: %zu of %zu sites hold NaN or Inf values "
+                    "(--allow-nonfinite steps them anyway)\n", input, bad, n);
+            return 1;
+        }
+    }
 
     ppf_desc d;
     memset(&d, 0, sizeof d);
@@ -226,7 +267,7 @@ int main(int argc, char **argv)
     d.nx = dims[2];
     d.bc = o.bc;
     d.precision = PPF_F64;
-    d.kernel = PPF_KERNEL_AUTO;
+    d.kernel = o.kernel;
     d.device = o.device;
     ppf_params_default(&d.params);
     ppf_ctx *ctx = NULL;

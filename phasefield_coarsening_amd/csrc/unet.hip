@@ -1839,7 +1839,7 @@ struct ppf_unet {
     char err[512];
 };
 
-static char g_unet_err[512];
+static thread_local char g_unet_err[512];  // last create failure, per host thread
 
 static int uerr(ppf_unet *n, int code, const char *fmt, ...)
 {
@@ -1851,11 +1851,15 @@ static int uerr(ppf_unet *n, int code, const char *fmt, ...)
     return code;
 }
 
+// (a failed call's error is also cleared from the thread's last error, so
+// later calls do not report it)
 #define UHIP(n, call)                                                                 \
     do {                                                                              \
         hipError_t e_ = (call);                                                       \
-        if (e_ != hipSuccess)                                                         \
+        if (e_ != hipSuccess) {                                                       \
+            (void)hipGetLastError();                                                  \
             return uerr((n), PPF_EHIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+        }                                                                             \
     } while (0)
 
 static void free_ws(ppf_unet *n)
@@ -1872,8 +1876,10 @@ static int ws_alloc(ppf_unet *n, T **ptr, size_t count)
 {
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
-    if (e != hipSuccess)
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
         return uerr(n, PPF_ENOMEM, "workspace allocation of %zu bytes failed", count * sizeof(T));
+    }
     n->ws.push_back(p);
     *ptr = (T *)p;
     return PPF_OK;
@@ -2508,6 +2514,7 @@ extern "C" int ppf_unet_create(ppf_unet **out, int base, int device, uint64_t se
     if (e == hipSuccess)
         e = hipMemset(n->g, 0, off * sizeof(float));
     if (e != hipSuccess) {
+        (void)hipGetLastError();
         uerr(nullptr, PPF_ENOMEM, "network allocation failed: %s", hipGetErrorString(e));
         ppf_unet_destroy(n);
         return PPF_ENOMEM;

@@ -64,6 +64,10 @@ class IncSums(ctypes.Structure):
                 ("sum_dl2", ctypes.c_double)]
 
 
+class Diff(ctypes.Structure):
+    _fields_ = [("max_abs", ctypes.c_double * 3), ("rms", ctypes.c_double * 3)]
+
+
 class PpfError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"libppf error {code}: {msg}")
@@ -97,6 +101,7 @@ SIGNATURES = {
     "ppf_mass_target": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "ppf_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
     "ppf_copy_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "ppf_compare": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Diff)]),
     "ppf_timer_start": (ctypes.c_int, [ctypes.c_void_p]),
     "ppf_timer_stop": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "ppf_sync": (ctypes.c_int, [ctypes.c_void_p]),
@@ -306,8 +311,17 @@ class Solver:
         return d, sizes[:min(d["n_particles"], int(max_sizes))]
 
     def copy_from(self, other):
-        """Device-to-device copy of other's state into this solver."""
+        """Device-to-device copy of other's state into this solver (converted
+        when the precisions differ; an FP32 solver copying an FP64 one takes
+        the FP64 total of c as its solute target)."""
         self._check(lib().ppf_copy_state(self._h, other._h))
+
+    def compare(self, other):
+        """Per-field differences to other's state, on the device:
+        {"max_abs": (etaS, etaL, c), "rms": (etaS, etaL, c)}."""
+        d = Diff()
+        self._check(lib().ppf_compare(self._h, other._h, ctypes.byref(d)))
+        return {"max_abs": tuple(d.max_abs), "rms": tuple(d.rms)}
 
     def set_mass_correction(self, every):
         """FP32 only: shift c back onto the FP64 solute target every `every`

@@ -14,6 +14,9 @@ reference's inputs and the outputs it produced.
 
 Step counting (phasefield.c:178-257): with n_iter = S and n_save = S - 1 the
 only output file, <stem>1.h5, holds the state after S sweeps.
+
+``python tests/golden/make_golden.py --config1`` adds only the config-1
+reference run (64^3, 1000 sweeps) to the manifest, as hashes.
 """
 import hashlib
 import json
@@ -83,7 +86,48 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
 
 
+def config1_reference_run(manifest):
+    """Config 1 (64^3 periodic, 40 spheres R ~ U[4, 8] from the library's
+    seeded generator, seed 1; 1000 sweeps, /etaL read): the reference
+    program's run on the machine-independent input of det_ic, stored as
+    sha256 hashes of its input and of its one output (the state after 1000
+    sweeps; 6 MB of fields are not kept as a fixture)."""
+    from phasefield_coarsening_amd import solver as S
+    from tests.golden import det_ic
+    shape = (64, 64, 64)
+    exe = os.path.join(REPO, "oracle", "_ref", "ppf_ref_p64_s1000")
+    if not os.path.exists(exe):
+        subprocess.run([os.path.join(REPO, "oracle", "build_ref.sh"), "64", "64", "64", "0",
+                        "1000", "999", "1", "ppf_ref_p64_s1000"], check=True,
+                       stdout=subprocess.DEVNULL)
+    cen, rad = S.spheres(1, 40, shape, 4.0, 8.0)
+    s0, l0, c0 = det_ic.sphere_state(shape, cen, rad)
+    with tempfile.TemporaryDirectory() as tmp:
+        h5.write(os.path.join(tmp, "in.h5"), s0, l0, c0)
+        r = subprocess.run([exe, "in.h5"], cwd=tmp, capture_output=True, text=True, check=True)
+        s1, l1, c1 = h5.read(os.path.join(tmp, "in_1.h5"))
+    stderr = [ln for ln in r.stderr.splitlines() if not ln.startswith(("HDF5-DIAG", "  #", "    "))]
+    manifest["config1_reference_run"] = {
+        "shape_zyx": list(shape), "bc": "periodic", "etaL": "read", "spheres": 40,
+        "radius": [4.0, 8.0], "sphere_seed": 1, "initial_condition": "tests/golden/det_ic.py",
+        "n_iter": 1000, "n_save": 999, "output": "in_1.h5", "sweeps": 1000,
+        "stdout": r.stdout.splitlines(), "stderr": stderr,
+        "sha256_input": {"s0": det_ic.sha256(s0), "l0": det_ic.sha256(l0), "c0": det_ic.sha256(c0)},
+        "sha256_output": {"s1": det_ic.sha256(s1), "l1": det_ic.sha256(l1), "c1": det_ic.sha256(c1)},
+        "sum_c_input": float(np.sum(c0)), "sum_c_output": float(np.sum(c1)),
+    }
+    print("config1", stderr[-1])
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--config1":
+        # add (or redo) only the config-1 reference run
+        path = os.path.join(HERE, "manifest.json")
+        manifest = json.load(open(path))
+        config1_reference_run(manifest)
+        with open(path, "w") as f:
+            json.dump(manifest, f, indent=1)
+        return
     manifest = {"generator": "tests/golden/make_golden.py",
                 "reference": "jijn/phasefield_coarsening phasefield.c, serial path "
                              "(USE_MPI 0), gcc -O2, HDF5 1.10.6",
@@ -116,6 +160,7 @@ def main():
             "sha256": {k: sha(v) for k, v in arrays.items()},
         }
         print(name, outs, stderr[-1])
+    config1_reference_run(manifest)
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1)
 

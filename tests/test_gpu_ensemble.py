@@ -129,3 +129,32 @@ def test_ensemble_two_step_equals_single_steps(prec, bc):
     for i in range(len(dims)):
         for x, y in zip(a[i], b[i]):
             np.testing.assert_array_equal(x, y)
+
+
+def test_config4_member_list_edges_bitwise_vs_single_grids():
+    """The configured ensemble itself (256 members, edges drawn from {48, 64,
+    96, 128}, solid fraction U[0.1, 0.4], radii U[3, 10], FP32, seed 4):
+    after 6 steps (two per launch), one member of each edge equals a
+    single-grid FP32 context stepped from the same initial condition, bit
+    for bit."""
+    from phasefield_coarsening_amd import surrogate as SG
+    mem = SG.ensemble_members(256, 4)
+    first = {}
+    for i, (dims, _, _) in enumerate(mem):
+        first.setdefault(dims[0], i)
+    assert sorted(first) == [48, 64, 96, 128]
+    with S.Ensemble([m[0] for m in mem], precision=S.F32) as ens:
+        for i, (_, cen, rad) in enumerate(mem):
+            ens.init_spheres(i, cen, rad)
+        ens.step(6)
+        assert ens.get_fusion() == 2
+        got = {e: ens.download(i) for e, i in first.items()}
+    for e, i in first.items():
+        dims, cen, rad = mem[i]
+        print(f"edge {e}: member {i}, {len(rad)} spheres", flush=True)
+        with S.Solver(dims, precision=S.F32) as sv:
+            sv.init_spheres(cen, rad)
+            sv.step(6)
+            want = sv.download()
+        for g, w in zip(got[e], want):
+            np.testing.assert_array_equal(g, w, err_msg=f"edge {e}")

@@ -87,8 +87,10 @@ def test_fast_ragged_vs_oracle_and_two_steps_equal_single_steps(shape, bc):
     st = _random_state(shape, 3)
     want = O.run(*st, 7, bc=bc)
     runs = {}
+    # order bits: 2 three waves per SIMD, 3 ring waves, 4 loads a plane early
     for fuse, tile, order in ((1, (0, 0), -1), (2, (16, 16), 6), (2, (32, 8), 4),
-                              (2, (16, 16), 8), (2, (32, 16), 2)):
+                              (2, (16, 16), 8), (2, (32, 16), 2), (2, (32, 8), 22),
+                              (2, (16, 16), 20)):
         if bc == S.BC_NOFLUX and order in (8, 2) and tile == (32, 16):
             continue  # no FP64 mirror kernel for this tile
         with S.Solver(shape, bc=bc, kernel=FAST) as sv:

@@ -65,7 +65,10 @@ def test_fp32_fused_equals_single_steps_with_mass_shifts(shape, tile):
     print(f"shape={shape} tile={tile}", flush=True)
     st = _state(shape, 11)
     want = _run(shape, st, 23, S.F32, False, shift=5)
-    for order in ((2, 8) if tile in RING_WAVE_TILES else (2,)):
+    # order bit 3: ring waves; bit 4: loads issued a plane early
+    orders = [2] + ([8] if tile in RING_WAVE_TILES else []) + \
+        ([18] if tile in [(32, 8), (16, 16), (32, 16), (64, 8)] else [])
+    for order in orders:
         got = _run(shape, st, 23, S.F32, True, tile, 8, order=order, shift=5)
         for a, b in zip(got, want):
             np.testing.assert_array_equal(a, b, err_msg=f"order={order}")

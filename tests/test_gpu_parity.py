@@ -281,9 +281,11 @@ def test_tilings_and_march_variants_bitwise(bc):
 def test_general_constants_bitwise_vs_oracle(bc):
     """Constants with AS != AL select the general FP64 kernels (the two
     divides by AS+(AL-AS)*h of phasefield.c:331 and :351 that the
-    reference's own constants, AS == AL, fold away): naive, single steps and
-    two steps per launch, on a ragged grid, bitwise the oracle run with the
-    same constants (AL changed, every other constant the reference's)."""
+    reference's own constants, AS == AL, fold away): naive, single steps
+    under every compiled march variant and two steps per launch under every
+    two-step variant, then the c-only sweep and the reported sums, on a
+    ragged grid, bitwise the oracle run with the same constants (AL changed,
+    every other constant the reference's)."""
     shape = (13, 20, 37)
     print(f"AS != AL bc={bc}", flush=True)
     rng = np.random.default_rng(23)
@@ -296,12 +298,88 @@ def test_general_constants_bitwise_vs_oracle(bc):
     pg = S.default_params()
     for f, _ in S.Params._fields_:
         setattr(pg, f, getattr(po, f))
-    for kernel, fuse in ((S.KERNEL_NAIVE, 1), (S.KERNEL_TILED, 1), (S.KERNEL_TILED, 2)):
+    runs = [(S.KERNEL_NAIVE, 1, None, None)]
+    for tile in [(64, 8), (16, 16), (32, 16)]:
+        for tz, order in ((1, 0), (3, 4), (5, 8), (32, 10)):
+            runs.append((S.KERNEL_TILED, 1, (tile[0], tile[1], tz, order), None))
+    tiles2 = [(16, 16), (32, 8)] if bc == S.BC_NOFLUX else [(16, 16), (32, 8), (64, 8), (32, 16)]
+    for tile in tiles2:
+        for order in (0, 2, 4, 6, 8, 10):
+            runs.append((S.KERNEL_TILED, 2, None, (tile[0], tile[1], 3, order)))
+    for kernel, fuse, tiling, tiling2 in runs:
         with S.Solver(shape, bc=bc, kernel=kernel, params=pg) as sv:
             if kernel == S.KERNEL_TILED:
-                sv.set_fusion(fuse)
+                if fuse == 1:
+                    sv.set_fusion(1)
+                    sv.set_tiling(*tiling)
+                else:
+                    try:
+                        sv.set_fusion(2, *tiling2)
+                    except S.PpfError:
+                        continue  # no variant of this kind for this tile
             sv.upload(s, l, c)
             sv.step(7)
             got = sv.download()
         for g, w in zip(got, want[:3]):
-            np.testing.assert_array_equal(g, w, err_msg=f"kernel={kernel} fuse={fuse}")
+            np.testing.assert_array_equal(g, w, err_msg=f"{kernel} {fuse} {tiling} {tiling2}")
+    # c-only sweeps after four full ones (phasefield.c:214-218), and the sums
+    # of one reported sweep of each kind
+    want_c = O.run(s, l, c, 7, bc=bc, n_iter_init=4, params=po)
+    want_r = O.run(s, l, c, 1, bc=bc, params=po)[3]
+    want_rc = O.run(s, l, c, 1, bc=bc, n_iter_init=0, params=po)[3]
+    for kernel in (S.KERNEL_NAIVE, S.KERNEL_TILED):
+        with S.Solver(shape, bc=bc, kernel=kernel, params=pg) as sv:
+            sv.upload(s, l, c)
+            sv.step(4)
+            sv.step_c_only(3)
+            got = sv.download()
+            for g, w in zip(got, want_c[:3]):
+                np.testing.assert_array_equal(g, w, err_msg=f"c-only kernel={kernel}")
+            sv.upload(s, l, c)
+            r = sv.step_report()
+            np.testing.assert_allclose(r, want_r, rtol=1e-13)
+            sv.upload(s, l, c)
+            rc = sv.step_report(c_only=True)
+            np.testing.assert_allclose(rc[0], want_rc[0], rtol=1e-13)
+
+
+def test_oversize_grid_is_out_of_memory():
+    """A grid whose first field alone exceeds the device's 288 GB (3400^3
+    FP64 = 314 GB): ppf_create returns PPF_ENOMEM with a message, and the
+    device stays usable."""
+    with pytest.raises(S.PpfError) as e:
+        S.Solver((3400, 3400, 3400))
+    assert e.value.code == -2 and "hipMalloc" in str(e.value)
+    with S.Solver((8, 8, 8)) as sv:
+        sv.upload(*(np.full((8, 8, 8), v) for v in (0.5, 0.5, 0.4)))
+        sv.step(2)
+        assert np.isfinite(sv.download()[0]).all()
+
+
+def test_compare_and_converting_copy():
+    """ppf_compare against numpy, and ppf_copy_state between precisions:
+    FP64 -> FP32 rounds to nearest and takes the FP64 total of c as the
+    solute target; FP32 -> FP64 widens exactly."""
+    import math
+    shape = (20, 16, 12)
+    rng = np.random.default_rng(4)
+    s = rng.uniform(0.0, 1.0, shape)
+    l = 1.0 - s
+    c = rng.uniform(0.1, 0.9, shape)
+    with S.Solver(shape) as a, S.Solver(shape, precision=S.F32) as b, S.Solver(shape) as w:
+        a.upload(s, l, c)
+        b.copy_from(a)
+        got = b.download()
+        for g, x in zip(got, (s, l, c)):
+            np.testing.assert_array_equal(g, x.astype(np.float32).astype(np.float64))
+        target, _ = b.mass_target()
+        assert abs(target - math.fsum(c.ravel())) <= 1e-15 * abs(target)
+        d = a.compare(b)
+        for i, x in enumerate((s, l, c)):
+            e = x - x.astype(np.float32).astype(np.float64)
+            assert d["max_abs"][i] == np.abs(e).max()
+            np.testing.assert_allclose(d["rms"][i], np.sqrt(np.mean(e * e)), rtol=1e-12)
+        w.copy_from(b)
+        for g, x in zip(w.download(), got):
+            np.testing.assert_array_equal(g, x)
+        assert w.compare(b)["max_abs"] == (0.0, 0.0, 0.0)

@@ -264,3 +264,40 @@ def test_apply_equals_the_host_forward(kind):
             # one uniform shift (FP32: stochastically rounded, within an ulp)
             tol = 0.0 if prec == S.F64 else 2.0 * np.spacing(np.float32(np.abs(c).max()))
             assert np.ptp(shift) <= tol + 1e-15
+
+
+def test_adam_steps_match_the_closed_form():
+    """Two train_step calls against Adam written out in float32 numpy
+    (beta1 0.9, beta2 0.999, eps 1e-8, bias corrections 1 - beta^t):
+    m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2,
+    p -= lr (m / c1) / (sqrt(v / c2) + eps), with g from loss_grad at the
+    parameters each step starts from.  The first step moves every parameter
+    with |g| >> eps by lr in the direction of -g (m/c1 = g, v/c2 = g^2)."""
+    x, y = _pairs((16, 16, 16), (21,), 10)
+    f = np.float32
+    b1, b2, eps, lr = f(0.9), f(0.999), f(1e-8), f(1e-3)
+    with UNet(seed=3) as net:
+        p = net.get_params()
+        m = np.zeros_like(p)
+        v = np.zeros_like(p)
+        for t in (1, 2):
+            _, g = net.loss_grad(x, y)
+            net.train_step(x, y, float(lr))
+            got = net.get_params()
+            m = b1 * m + (f(1) - b1) * g
+            v = b2 * v + (f(1) - b2) * g * g
+            c1 = f(1) - f(np.power(b1, f(t), dtype=np.float32))
+            c2 = f(1) - f(np.power(b2, f(t), dtype=np.float32))
+            want = p - lr * (m / c1) / (np.sqrt(v / c2) + eps)
+            err = np.abs(got - want)
+            ulps = err / np.spacing(np.abs(want))
+            print(f"Adam step {t}: max {float(ulps.max()):.2f} ulps from the closed form",
+                  flush=True)
+            assert (ulps <= 2).all(), (t, float(ulps.max()))
+            assert (np.abs(got - p) <= lr * 1.001).all()
+            if t == 1:
+                # |g| >> eps: the step is lr in the direction of -g
+                steady = np.abs(g) > f(1e-6)
+                assert steady.sum() > 20
+                np.testing.assert_allclose((p - got)[steady], lr * np.sign(g[steady]), rtol=1e-2)
+            p = got

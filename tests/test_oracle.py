@@ -176,3 +176,22 @@ def test_solute_conserved_periodic(golden):
     c0 = z["c0"].sum()
     _, _, c, _ = O.run(z["s0"], z["l0"], z["c0"], 300)
     assert abs(c.sum() - c0) / c0 < 1e-12
+
+
+def test_oracle_reproduces_the_config1_reference_run(golden):
+    """64^3 periodic, 40 spheres, 1000 sweeps: the oracle's output hashes
+    equal those of the reference program's own run on the same input
+    (tests/golden/manifest.json, config1_reference_run; phasefield.c:178-258),
+    and the input regenerated here hashes as the one the reference read."""
+    from phasefield_coarsening_amd import solver as S
+    from tests.golden import det_ic
+    man, _ = golden
+    m = man["config1_reference_run"]
+    shape = tuple(m["shape_zyx"])
+    cen, rad = S.spheres(m["sphere_seed"], m["spheres"], shape, *m["radius"])
+    s0, l0, c0 = det_ic.sphere_state(shape, cen, rad)
+    assert {"s0": det_ic.sha256(s0), "l0": det_ic.sha256(l0),
+            "c0": det_ic.sha256(c0)} == m["sha256_input"]
+    s1, l1, c1, _ = O.run(s0, l0, c0, m["sweeps"])
+    assert {"s1": det_ic.sha256(s1), "l1": det_ic.sha256(l1),
+            "c1": det_ic.sha256(c1)} == m["sha256_output"]

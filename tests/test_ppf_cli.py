@@ -34,3 +34,30 @@ def test_bad_option_is_rejected(tmp_path):
     r = subprocess.run([PPF, "--bc=sideways", "x.h5"], cwd=tmp_path, capture_output=True,
                        text=True)
     assert r.returncode == 255 and "bad option" in r.stderr
+
+
+def test_nonfinite_input_is_refused(tmp_path):
+    """A NaN or Inf in any field ends the run before a device is opened
+    (the reference would step it: it checks no value and no status,
+    phasefield.c:1007-1046); --allow-nonfinite lets it through to the device."""
+    s = np.full((3, 4, 5), 0.5)
+    l = 1.0 - s
+    c = np.full_like(s, 0.4)
+    s[1, 2, 3] = np.nan
+    c[0, 0, 0] = np.inf
+    h5.write(tmp_path / "bad.h5", s, l, c)
+    r = subprocess.run([PPF, "bad.h5"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 1
+    assert "2 of 60 sites hold NaN or Inf values" in r.stderr
+    assert not (tmp_path / "bad_1.h5").exists()
+    # the check covers etaL too
+    l[2, 3, 4] = -np.inf
+    s[1, 2, 3] = 0.5
+    c[0, 0, 0] = 0.4
+    h5.write(tmp_path / "badl.h5", s, l, c)
+    r = subprocess.run([PPF, "badl.h5"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 1 and "1 of 60 sites" in r.stderr
+    # --etal-compat never reads /etaL, so that file is clean then
+    r = subprocess.run([PPF, "--etal-compat", "--allow-nonfinite", "--kernel=warp", "badl.h5"],
+                       cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 255 and "bad option --kernel=warp" in r.stderr
