@@ -198,21 +198,27 @@ def unet_line(grid=256, reps=3):
             "mfma_bf16_peak_TFLOPs": 2500.0, "mfma_frac": round(tf / 2500.0, 4)}
 
 
-def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20):
+def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium"):
     """Learned jumps against exact stepping at grid^3 (500 spheres,
-    R ~ U[6, 12]) to t_end, with the committed K = 200 weights: wall times
-    of the surrogate, of plain FP64 stepping (the fast FP64 kernel) and of
-    plain FP32 stepping, and the errors of the mean particle radius
-    (connected components) and the solid fraction at t_end against FP64."""
+    R ~ U[6, 12]) to t_end: wall times of the surrogate (K = 200 jumps, 20
+    FP32 relax steps, judge, exact re-run on rejection), of plain FP64
+    stepping (the fast FP64 kernel) and of plain FP32 stepping, and the
+    errors of the mean particle radius (connected components, over all
+    particles) and of the solid fraction at t_end against FP64.  c_init
+    "equilibrium" (c = 0.999 - 0.523 etaS, phasefield.c:82-83): the
+    particles coarsen, with the weights trained on that regime; "config"
+    (c = 0.1 + 0.8 etaS): the box solidifies into one cluster, with the
+    weights trained there."""
     import numpy as np
 
     from phasefield_coarsening_amd import surrogate as SG
     from phasefield_coarsening_amd.unet import UNet
-    w = os.path.join(HERE, "phasefield_coarsening_amd", "weights", "unet_k200_b32_f16.npy")
+    wname = "unet_k200_b32_eq_f16.npy" if c_init == "equilibrium" else "unet_k200_b32_f16.npy"
+    w = os.path.join(HERE, "phasefield_coarsening_amd", "weights", wname)
     shape = (grid, grid, grid)
     cen, rad = S.spheres(5, 500, shape, 6.0, 12.0)
     with S.Solver(shape, kernel=S.KERNEL_FAST) as ref:
-        ref.init_spheres(cen, rad)
+        ref.init_spheres(cen, rad, *SG.COMPOSITIONS[c_init])
         s0 = ref.download()
         ref.sync()
         t0 = time.perf_counter()
@@ -240,13 +246,15 @@ def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20):
         ss = sv.reduce()
         ctl.close()
     f_ref, f_sur = st["sum_h"] / st["n_sites"], ss["sum_h"] / ss["n_sites"]
-    return {"grid": [grid] * 3, "t_end": t_end, "k": k, "relax": relax,
+    return {"grid": [grid] * 3, "t_end": t_end, "k": k, "relax": relax, "c_init": c_init,
+            "weights": wname, "particles_fp64": pr["n_particles"],
+            "particles_surrogate": ps["n_particles"],
             "exact_fp64_s": round(t_exact, 4), "exact_fp32_s": round(t_exact32, 4),
             "surrogate_s": round(summ["seconds"], 4),
             "speedup_vs_exact_fp64": round(t_exact / summ["seconds"], 3),
             "speedup_vs_exact_fp32": round(t_exact32 / summ["seconds"], 3),
-            "accepted": summ["accepted"],
-            "rejected": summ["rejected"], "skipped": summ["skipped"],
+            "accepted": summ["accepted"], "rejected": summ["rejected"],
+            "skipped": summ["skipped"],
             "R_err_rel": abs(ps["mean_radius"] - pr["mean_radius"]) / pr["mean_radius"],
             "frac_err_rel": abs(f_sur - f_ref) / f_ref}
 
@@ -411,6 +419,7 @@ def main():
         out["ensemble"] = ensemble_line(S)
         out["unet"] = unet_line()
         out["surrogate"] = surrogate_line(S)
+        out["surrogate_solidifying"] = surrogate_line(S, c_init="config")
     if "copy" in side:
         cc = side["copy"]
         out["roofline"]["copy_ceiling_GBs"] = round(cc, 1)

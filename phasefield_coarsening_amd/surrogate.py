@@ -41,15 +41,24 @@ def ensemble_members(n, seed, edges=(48, 64, 96, 128), phi=(0.1, 0.4), radius=(3
     return out
 
 
-def trajectories(members, steps, every, precision=S.F32, autotune=True, keep=None):
+# Initial compositions c = c_liq + c_span etaS: the configured construction,
+# and the reference's two equilibrium values (liquid cSeq = 0.999, solid
+# cLeq = 0.476, phasefield.c:82-83), with which the particles coarsen
+# instead of the box solidifying.
+COMPOSITIONS = {"config": (0.1, 0.8), "equilibrium": (0.999, -0.523)}
+
+
+def trajectories(members, steps, every, precision=S.F32, autotune=True, keep=None,
+                 c_init="config"):
     """Step all members together; return (snapshots, stats).  snapshots[i] is
     a float32 array [T, D, H, W, 3] (T = steps // every + 1) for the members
     in ``keep`` (all when None), and stats the ensemble throughput."""
     keep = range(len(members)) if keep is None else keep
+    c_liq, c_span = COMPOSITIONS[c_init]
     ens = S.Ensemble([m[0] for m in members], precision=precision)
     try:
         for i, (_, cen, rad) in enumerate(members):
-            ens.init_spheres(i, cen, rad)
+            ens.init_spheres(i, cen, rad, c_liq, c_span)
         if autotune:
             ens.autotune(reps=2)
         snaps = {i: [np.stack(ens.download(i, f32=True), axis=-1)] for i in keep}

@@ -12,12 +12,15 @@
 3. Held-out evaluation: mean squared error of the network's jump against
    "no change" (persistence) on the held-out pairs.
 4. Surrogate at ``--grid``^3 (``--spheres`` spheres, R ~ U[6, 12]): the
-   exact FP64 run to ``--t-end`` steps against the controller (jump K, 20
-   FP32 relax steps, judge ``--judge``, exact re-run on rejection), compared
-   on mean particle radius (connected components) and solid fraction at
-   t_end; wall times give the speedup.
+   exact FP64 run (fast FP64 kernel) to ``--t-end`` steps against the
+   controller (jump K, 20 FP32 relax steps, judge ``--judge``, exact re-run
+   on rejection), compared on mean particle radius (connected components)
+   and solid fraction at t_end; wall times give the speedup against plain
+   FP64 stepping and against plain FP32 stepping (what the network gains).
+``--c-init`` picks the initial composition of the ensemble and of the
+surrogate grid (surrogate.COMPOSITIONS).
 
-Weights go to ``--save`` (float32 .npy) when given; ``--load`` skips
+Weights go to ``--save`` (float16 .npy) when given; ``--load`` skips
 training.  One JSON line per stage is printed.
 """
 import argparse
@@ -54,6 +57,9 @@ def main(argv=None):
                     help="synchronise around each controller phase and report its seconds "
                          "(adds a little time)")
     ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--c-init", default="config", choices=sorted(SG.COMPOSITIONS),
+                    help="initial composition of the ensemble and of the surrogate grid: "
+                         "config (c = 0.1 + 0.8 etaS) or equilibrium (c = 0.999 - 0.523 etaS)")
     ap.add_argument("--save", default="")
     ap.add_argument("--load", default="",
                     help="weights .npy (float16 or float32); skips the ensemble and training")
@@ -66,7 +72,7 @@ def main(argv=None):
         edges = tuple(int(e) for e in a.edges.split(","))
         members = SG.ensemble_members(a.members, a.seed, edges=edges)
         keep = [i for i, m in enumerate(members) if m[0][0] == a.train_edge][:a.max_train_members]
-        snaps, ens = SG.trajectories(members, a.ens_steps, a.k, keep=keep)
+        snaps, ens = SG.trajectories(members, a.ens_steps, a.k, keep=keep, c_init=a.c_init)
         print(json.dumps({"stage": "ensemble", **ens, "train_members": len(keep)}), flush=True)
         held = {i: v for j, (i, v) in enumerate(sorted(snaps.items())) if j % 8 == 7}
         train_snaps = {i: v for i, v in snaps.items() if i not in held}
@@ -82,14 +88,14 @@ def main(argv=None):
         print(json.dumps({"stage": "heldout", "mse_net": ln, "mse_persistence": lp,
                           "gain": lp / ln if ln > 0 else None}), flush=True)
         if a.save:
-            np.save(a.save, net.get_params())
+            np.save(a.save, net.get_params().astype(np.float16))
 
     # surrogate vs exact FP64 at grid^3
     n = a.grid
     shape = (n, n, n)
     cen, rad = S.spheres(5, a.spheres, shape, 6.0, 12.0)
-    with S.Solver(shape) as ref:
-        ref.init_spheres(cen, rad)
+    with S.Solver(shape, kernel=S.KERNEL_FAST) as ref:
+        ref.init_spheres(cen, rad, *SG.COMPOSITIONS[a.c_init])
         s0 = ref.download()
         ref.sync()
         t0 = time.perf_counter()
@@ -99,6 +105,14 @@ def main(argv=None):
         pr, _ = ref.particles(0.5, max_sizes=1)
         st = ref.reduce()
         r_exact, f_exact = pr["mean_radius"], st["sum_h"] / st["n_sites"]
+        n_exact = pr["n_particles"]
+    with S.Solver(shape, precision=S.F32) as f32:
+        f32.upload(*s0)
+        f32.sync()
+        t0 = time.perf_counter()
+        f32.step(a.t_end)
+        f32.sync()
+        t_exact32 = time.perf_counter() - t0
     with S.Solver(shape, precision=S.F32) as sv:
         sv.upload(*s0)
         del s0
@@ -110,9 +124,12 @@ def main(argv=None):
         ms, fl = net.last_forward()
         ctl.close()
     print(json.dumps({"stage": "surrogate", "grid": n, "t_end": a.t_end, "k": a.k,
+                      "c_init": a.c_init, "particles_exact": n_exact,
                       "relax": a.relax, "exact_fp64_seconds": round(t_exact, 4),
+                      "exact_fp32_seconds": round(t_exact32, 4),
                       "surrogate_seconds": round(summ["seconds"], 4),
-                      "speedup": round(t_exact / summ["seconds"], 3),
+                      "speedup_vs_exact_fp64": round(t_exact / summ["seconds"], 3),
+                      "speedup_vs_exact_fp32": round(t_exact32 / summ["seconds"], 3),
                       "judge": a.judge, "accepted": summ["accepted"],
                       "rejected": summ["rejected"], "skipped": summ["skipped"],
                       "phase_seconds": summ.get("phase_seconds"),

@@ -1,5 +1,8 @@
-"""The trained K = 200 jump: what it gains over the method without it, and
-the controller's accuracy against exact FP64 stepping."""
+"""The trained K = 200 jumps: what they gain over the method without them,
+and the controller's accuracy against exact FP64 stepping, in the
+solidifying regime (the configured composition c = 0.1 + 0.8 etaS: the box
+turns ~97% solid) and in the coarsening regime (the equilibrium composition
+c = 0.999 - 0.523 etaS, phasefield.c:82-83: hundreds of particles coarsen)."""
 import os
 
 import numpy as np
@@ -12,11 +15,12 @@ from phasefield_coarsening_amd.unet import UNet
 
 pytestmark = pytest.mark.gpu
 WEIGHTS = os.path.join(_lib.PKG_DIR, "weights", "unet_k200_b32_f16.npy")
+WEIGHTS_EQ = os.path.join(_lib.PKG_DIR, "weights", "unet_k200_b32_eq_f16.npy")
 
 
-def _net():
+def _net(path=WEIGHTS):
     net = UNet(seed=1)
-    net.set_params(np.load(WEIGHTS).astype(np.float32))
+    net.set_params(np.load(path).astype(np.float32))
     return net
 
 
@@ -80,3 +84,52 @@ def test_rate_gate_holds_jumps_back_in_a_transient():
     print(summ, ctl.log)
     assert summ["accepted"] + summ["rejected"] == 0 and summ["skipped"] == 4
     assert summ["exact_steps"] == 880
+
+
+def test_coarsening_regime_controller_vs_exact_fp64():
+    """The re-aimed surrogate workload: 256^3, 500 spheres R ~ U[6, 12] at the
+    equilibrium composition, t_end = 2200, with the weights trained on that
+    regime.  At t_end the exact run holds hundreds of particles; the
+    surrogate's particle count is within 2% of it, its mean particle radius
+    (over all particles) and solid fraction within 2%, at least five jumps
+    are accepted, and the solute stays on its FP64 target."""
+    shape = (256, 256, 256)
+    cen, rad = S.spheres(5, 500, shape, 6.0, 12.0)
+    c_liq, c_span = SG.COMPOSITIONS["equilibrium"]
+    with S.Solver(shape, kernel=S.KERNEL_FAST) as ref:
+        ref.init_spheres(cen, rad, c_liq, c_span)
+        s0 = ref.download()
+        ref.step(2200)
+        pr, _ = ref.particles(0.5, 1)
+        st = ref.reduce()
+    with S.Solver(shape, precision=S.F32) as sv, _net(WEIGHTS_EQ) as net:
+        sv.upload(*s0)
+        del s0
+        target = sv.mass_target()[0]
+        ctl = SG.Controller(sv, net)
+        summ = ctl.advance(2200)
+        ps, _ = sv.particles(0.5, 1)
+        ss = sv.reduce()
+        ctl.close()
+    print(summ, pr, ps)
+    assert pr["n_particles"] > 200
+    assert abs(ps["n_particles"] - pr["n_particles"]) <= 0.02 * pr["n_particles"]
+    assert summ["accepted"] >= 5
+    assert abs(ps["mean_radius"] - pr["mean_radius"]) / pr["mean_radius"] < 0.02
+    f_ref, f_sur = st["sum_h"] / st["n_sites"], ss["sum_h"] / ss["n_sites"]
+    assert abs(f_sur - f_ref) / f_ref < 0.02
+    assert abs(ss["sum_c"] - target) / target < 1e-8
+
+
+def test_coarsening_weights_beat_no_jump_on_fresh_states():
+    """Fresh 64^3 members at the equilibrium composition (seeds not used in
+    training): the jump is closer to the exact FP32 trajectory than leaving
+    the state unchanged.  In this regime 200 steps change little, so the
+    margin is small (1.1x on the held-out training members)."""
+    members = SG.ensemble_members(6, seed=987, edges=(64,))
+    snaps, _ = SG.trajectories(members, 600, 200, autotune=False, c_init="equilibrium")
+    data = SG.pairs(snaps)
+    with _net(WEIGHTS_EQ) as net:
+        mse_net, mse_keep = SG.eval_loss(net, data)
+    print("mse net %.3e, persistence %.3e, gain %.2f" % (mse_net, mse_keep, mse_keep / mse_net))
+    assert mse_net < mse_keep

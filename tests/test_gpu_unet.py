@@ -294,8 +294,9 @@ def test_adam_steps_match_the_closed_form():
             print(f"Adam step {t}: max {float(ulps.max()):.2f} ulps from the closed form",
                   flush=True)
             assert (ulps <= 2).all(), (t, float(ulps.max()))
-            assert (np.abs(got - p) <= lr * 1.001).all()
             if t == 1:
+                # m/c1 = g and v/c2 = g^2: no parameter moves by more than lr
+                assert (np.abs(got - p) <= lr * 1.001).all()
                 # |g| >> eps: the step is lr in the direction of -g
                 steady = np.abs(g) > f(1e-6)
                 assert steady.sum() > 20
