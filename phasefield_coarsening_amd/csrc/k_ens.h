@@ -180,13 +180,18 @@ __attribute__((amdgpu_waves_per_eu(sizeof(typename P::T) == 4 ? 5 : 2))) void k_
         T *lo = (T *)d->f[1 - parity][1];
         T *co = (T *)d->f[1 - parity][2];
         const int dnx = d->nx, dny = d->ny, dnz = d->nz;
+        // register rotation in the march: periodic members only (the no-flux
+        // FP32 kernel would spill with it)
+        constexpr bool ROT = P::kRotate && BC == 0;
         if (w.shape == ENS2_32x8)
-            tile_march2<P, 32, 8, false, DB, BC>(si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0,
-                                             w.z0, w.zend, k, reinterpret_cast<T(*)[12][36]>(sh),
-                                             reinterpret_cast<T(*)[10][34]>(sh + NB * 4 * 12 * 36));
+            tile_march2<P, 32, 8, false, DB, BC, ROT>(
+                si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0, w.z0, w.zend, k,
+                reinterpret_cast<T(*)[12][36]>(sh),
+                reinterpret_cast<T(*)[10][34]>(sh + NB * 4 * 12 * 36));
         else
-            tile_march2<P, 16, 16, false, DB, BC>(si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0,
-                                              w.z0, w.zend, k, reinterpret_cast<T(*)[20][20]>(sh),
-                                              reinterpret_cast<T(*)[18][18]>(sh + NB * 4 * 20 * 20));
+            tile_march2<P, 16, 16, false, DB, BC, ROT>(
+                si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0, w.z0, w.zend, k,
+                reinterpret_cast<T(*)[20][20]>(sh),
+                reinterpret_cast<T(*)[18][18]>(sh + NB * 4 * 20 * 20));
     }
 }

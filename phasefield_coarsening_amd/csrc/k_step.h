@@ -35,6 +35,38 @@ __device__ __forceinline__ __attribute__((address_space(1))) T *uniform_ptr(T *p
     return (__attribute__((address_space(1))) T *)(((unsigned long long)hi << 32) | lo);
 }
 
+// Element i of a plane whose base is uniform: the byte offset i*sizeof(T) is
+// formed in 32 bits (contexts keep a plane under 4 GiB), so the access is
+// one global instruction with the base in scalar registers and a 32-bit
+// lane offset, with no 64-bit address arithmetic per lane.
+template <typename T>
+__device__ __forceinline__ T ld_plane(const __attribute__((address_space(1))) T *base, unsigned i)
+{
+    using C = const __attribute__((address_space(1))) char;
+    return *(const __attribute__((address_space(1))) T *)((C *)base + i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *at_plane(
+    __attribute__((address_space(1))) T *base, unsigned i)
+{
+    using C = __attribute__((address_space(1))) char;
+    return (__attribute__((address_space(1))) T *)((C *)base + i * (unsigned)sizeof(T));
+}
+
+// The step-t (or t+1) state of one site column at one plane, as the z march
+// keeps it in registers: the raw fields and pass A's per-site values.
+template <typename T>
+struct Col {
+    T s, l, c, h, mu, M;
+};
+
+// Marks lane offsets as redefined at the top of each plane of a z march (an
+// empty asm, no instruction): the compiler then forms each plane's
+// addresses from the 32-bit offsets inside the loop body, as scalar base +
+// 32-bit vector offset, instead of hoisting their 64-bit zero extensions
+// out of the loop and adding 64-bit register pairs per access.
+#define PPF_FRESH(...) asm volatile("" : __VA_ARGS__)
+
 // ---------------------------------------------------------------------------
 // Naive form: one thread per site, every neighbour read from global memory and
 // its h, mu, M recomputed (7x the pointwise work).  Kept as an independent
@@ -135,8 +167,8 @@ __device__ __forceinline__ void tile_march(
     // scalar registers; each lane keeps only 32-bit offsets within a plane
     // (nx*ny < 2^31 is checked at context creation).
 #define PLANE(p, zz) uniform_ptr((p) + (size_t)(zz) * plane)
-    const unsigned col = (unsigned)ghost_any<BC>(y, ny) * (unsigned)nx + (unsigned)ghost_any<BC>(x, nx);
-    const unsigned ocol = (unsigned)y * (unsigned)nx + (unsigned)x;
+    unsigned col = (unsigned)ghost_any<BC>(y, ny) * (unsigned)nx + (unsigned)ghost_any<BC>(x, nx);
+    unsigned ocol = (unsigned)y * (unsigned)nx + (unsigned)x;
 
     // This thread's ring site, if any: left, right, bottom, top faces.
     int rx = 0, ry = 0;
@@ -154,7 +186,7 @@ __device__ __forceinline__ void tile_march(
         ry = TY + 1;
     }
     const bool ring = tid < NRING;
-    const unsigned rcol = (unsigned)ghost_any<BC>(y0 + ry - 1, ny) * (unsigned)nx +
+    unsigned rcol = (unsigned)ghost_any<BC>(y0 + ry - 1, ny) * (unsigned)nx +
                           (unsigned)ghost_any<BC>(x0 + rx - 1, nx);
 
     // Prologue: planes z0-1 and z0 computed, plane z0+1 and the ring of z0
@@ -167,28 +199,28 @@ __device__ __forceinline__ void tile_march(
     T qs[NQ], ql[NQ], qc[NQ];          // raw column values 2 .. CD-1 planes ahead
     {
         const int zp = ghost<BC>(z0 - 1, nz), zn = ghost_any<BC>(z0 + 1, nz);
-        s_p = PLANE(s_in, zp)[col];
-        l_p = PLANE(l_in, zp)[col];
-        const T c_p = PLANE(c_in, zp)[col];
-        s_c = PLANE(s_in, z0)[col];
-        l_c = PLANE(l_in, z0)[col];
-        c_c = PLANE(c_in, z0)[col];
-        rs_n = PLANE(s_in, zn)[col];
-        rl_n = PLANE(l_in, zn)[col];
-        rc_n = PLANE(c_in, zn)[col];
+        s_p = ld_plane(PLANE(s_in, zp), col);
+        l_p = ld_plane(PLANE(l_in, zp), col);
+        const T c_p = ld_plane(PLANE(c_in, zp), col);
+        s_c = ld_plane(PLANE(s_in, z0), col);
+        l_c = ld_plane(PLANE(l_in, z0), col);
+        c_c = ld_plane(PLANE(c_in, z0), col);
+        rs_n = ld_plane(PLANE(s_in, zn), col);
+        rl_n = ld_plane(PLANE(l_in, zn), col);
+        rc_n = ld_plane(PLANE(c_in, zn), col);
         if (ring) {
-            rs_r = PLANE(s_in, z0)[rcol];
-            rl_r = PLANE(l_in, z0)[rcol];
-            rc_r = PLANE(c_in, z0)[rcol];
+            rs_r = ld_plane(PLANE(s_in, z0), rcol);
+            rl_r = ld_plane(PLANE(l_in, z0), rcol);
+            rc_r = ld_plane(PLANE(c_in, z0), rcol);
         }
 #pragma unroll
         for (int k = 0; k < NQ; k++) {
             qs[k] = ql[k] = qc[k] = 0;
             if (PIPE && CD > 2 && z0 + 1 + k < zend) {
                 const int z2 = ghost_any<BC>(z0 + 2 + k, nz);
-                qs[k] = PLANE(s_in, z2)[col];
-                ql[k] = PLANE(l_in, z2)[col];
-                qc[k] = PLANE(c_in, z2)[col];
+                qs[k] = ld_plane(PLANE(s_in, z2), col);
+                ql[k] = ld_plane(PLANE(l_in, z2), col);
+                qc[k] = ld_plane(PLANE(c_in, z2), col);
             }
         }
         T hp;
@@ -197,33 +229,34 @@ __device__ __forceinline__ void tile_march(
     }
 
     for (int z = z0; z < zend; ++z) {
+        PPF_FRESH("+v"(col), "+v"(ocol), "+v"(rcol));
         const int buf = (z - z0) & 1;
         const bool more = z + 1 < zend;
         if (!PIPE && z > z0) {
             // unpipelined form: this plane's ring and the next plane now
             const int zn = ghost_any<BC>(z + 1, nz);
-            rs_n = PLANE(s_in, zn)[col];
-            rl_n = PLANE(l_in, zn)[col];
-            rc_n = PLANE(c_in, zn)[col];
+            rs_n = ld_plane(PLANE(s_in, zn), col);
+            rl_n = ld_plane(PLANE(l_in, zn), col);
+            rc_n = ld_plane(PLANE(c_in, zn), col);
             if (ring) {
-                rs_r = PLANE(s_in, z)[rcol];
-                rl_r = PLANE(l_in, z)[rcol];
-                rc_r = PLANE(c_in, z)[rcol];
+                rs_r = ld_plane(PLANE(s_in, z), rcol);
+                rl_r = ld_plane(PLANE(l_in, z), rcol);
+                rc_r = ld_plane(PLANE(c_in, z), rcol);
             }
         }
         // prefetch: column CD planes ahead, ring one plane ahead
         T ps = 0, pl = 0, pc = 0, prs = 0, prl = 0, prc = 0;
         if (PIPE && z + CD - 1 < zend) {
             const int zf = ghost_any<BC>(z + CD, nz);
-            ps = PLANE(s_in, zf)[col];
-            pl = PLANE(l_in, zf)[col];
-            pc = PLANE(c_in, zf)[col];
+            ps = ld_plane(PLANE(s_in, zf), col);
+            pl = ld_plane(PLANE(l_in, zf), col);
+            pc = ld_plane(PLANE(c_in, zf), col);
         }
         if (PIPE && more) {
             if (ring) {
-                prs = PLANE(s_in, z + 1)[rcol];
-                prl = PLANE(l_in, z + 1)[rcol];
-                prc = PLANE(c_in, z + 1)[rcol];
+                prs = ld_plane(PLANE(s_in, z + 1), rcol);
+                prl = ld_plane(PLANE(l_in, z + 1), rcol);
+                prc = ld_plane(PLANE(c_in, z + 1), rcol);
             }
         }
 
@@ -254,7 +287,7 @@ __device__ __forceinline__ void tile_march(
                                   sh[buf][3][ty + 1][tx + 2], sh[buf][3][ty + 1][tx]};
             if (CONLY) {
                 const T dc = P::conly(mu_c, M_c, mun, Mn, k);
-                __builtin_nontemporal_store((T)(c_c + dc), PLANE(c_out, z) + ocol);
+                __builtin_nontemporal_store((T)(c_c + dc), at_plane(PLANE(c_out, z), ocol));
                 if (REPORT)
                     acc[0] += (double)dc * (double)dc;
             } else {
@@ -267,9 +300,9 @@ __device__ __forceinline__ void tile_march(
                 P::update(s_c, l_c, c_c, h_c, mu_c, M_c, sn, ln, mun, Mn, k, so, lo, co, ds, dl,
                           dc);
                 // streaming stores: the new state is not read again this step
-                __builtin_nontemporal_store(so, PLANE(s_out, z) + ocol);
-                __builtin_nontemporal_store(lo, PLANE(l_out, z) + ocol);
-                __builtin_nontemporal_store(co, PLANE(c_out, z) + ocol);
+                __builtin_nontemporal_store(so, at_plane(PLANE(s_out, z), ocol));
+                __builtin_nontemporal_store(lo, at_plane(PLANE(l_out, z), ocol));
+                __builtin_nontemporal_store(co, at_plane(PLANE(c_out, z), ocol));
                 if (REPORT) {
                     acc[0] += dc * dc;
                     acc[1] += ds * ds;
@@ -382,7 +415,11 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
 // this plane, last read by stage 2 of the previous plane (before this
 // plane's barrier).
 // ---------------------------------------------------------------------------
-template <class P, int TX, int TY, bool PF = false, bool DB = false, int BC = 0>
+// ROT: the march without early loads rotates three register sets per
+// column instead of copying (see below); callers with a tighter register
+// budget (the no-flux FP32 ensemble kernel) switch it off.
+template <class P, int TX, int TY, bool PF = false, bool DB = false, int BC = 0,
+          bool ROT = P::kRotate>
 __device__ __forceinline__ void tile_march2(
     const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
     const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
@@ -402,8 +439,8 @@ __device__ __forceinline__ void tile_march2(
     auto colof = [&](int ax, int ay) {
         return (unsigned)ghost_any<BC>(y0 + ay, ny) * (unsigned)nx + (unsigned)ghost_any<BC>(x0 + ax, nx);
     };
-    const unsigned col = colof(tx, ty);
-    const unsigned ocol = (unsigned)(y0 + ty) * (unsigned)nx + (unsigned)(x0 + tx);
+    unsigned col = colof(tx, ty);
+    unsigned ocol = (unsigned)(y0 + ty) * (unsigned)nx + (unsigned)(x0 + tx);
 
     // one-cell ring site (A2 frame rx, ry; tile frame rx-1, ry-1)
     int rx = 0, ry = 0;
@@ -421,7 +458,7 @@ __device__ __forceinline__ void tile_march2(
         ry = TY + 1;
     }
     const bool ring = tid < NR1;
-    const unsigned rcol = colof(rx - 1, ry - 1);
+    unsigned rcol = colof(rx - 1, ry - 1);
     // mirror boundaries: a ring site outside the grid is a ghost, whose step
     // t+1 is its mirror site's; the tile thread of that site writes it
     const bool ring_in = BC == 0 || (x0 + rx - 1 >= 0 && x0 + rx - 1 < nx && y0 + ry - 1 >= 0 &&
@@ -450,7 +487,7 @@ __device__ __forceinline__ void tile_march2(
         }
     }
     const bool outer = tid >= NR1 && tid < NR1 + NO2;
-    const unsigned o2col = colof(ox - 2, oy - 2);
+    unsigned o2col = colof(ox - 2, oy - 2);
 
     // step t columns of the tile site (p: z-1, c: z, n: z+1 raw) and of the
     // ring site (q*)
@@ -460,21 +497,21 @@ __device__ __forceinline__ void tile_march2(
     {
         const int zp = ghost_any<BC>(z0 - 2, nz), zc = ghost_any<BC>(z0 - 1, nz);
         T hp;
-        s_p = PLANE(s_in, zp)[col];
-        l_p = PLANE(l_in, zp)[col];
-        P::point(s_p, l_p, PLANE(c_in, zp)[col], k, hp, mu_p, M_p);
-        s_c = PLANE(s_in, zc)[col];
-        l_c = PLANE(l_in, zc)[col];
-        c_c = PLANE(c_in, zc)[col];
+        s_p = ld_plane(PLANE(s_in, zp), col);
+        l_p = ld_plane(PLANE(l_in, zp), col);
+        P::point(s_p, l_p, ld_plane(PLANE(c_in, zp), col), k, hp, mu_p, M_p);
+        s_c = ld_plane(PLANE(s_in, zc), col);
+        l_c = ld_plane(PLANE(l_in, zc), col);
+        c_c = ld_plane(PLANE(c_in, zc), col);
         P::point(s_c, l_c, c_c, k, h_c, mu_c, M_c);
         if (ring) {
-            qs_p = PLANE(s_in, zp)[rcol];
-            ql_p = PLANE(l_in, zp)[rcol];
+            qs_p = ld_plane(PLANE(s_in, zp), rcol);
+            ql_p = ld_plane(PLANE(l_in, zp), rcol);
             T qh;
-            P::point(qs_p, ql_p, PLANE(c_in, zp)[rcol], k, qh, qmu_p, qM_p);
-            qs_c = PLANE(s_in, zc)[rcol];
-            ql_c = PLANE(l_in, zc)[rcol];
-            qc_c = PLANE(c_in, zc)[rcol];
+            P::point(qs_p, ql_p, ld_plane(PLANE(c_in, zp), rcol), k, qh, qmu_p, qM_p);
+            qs_c = ld_plane(PLANE(s_in, zc), rcol);
+            ql_c = ld_plane(PLANE(l_in, zc), rcol);
+            qc_c = ld_plane(PLANE(c_in, zc), rcol);
             P::point(qs_c, ql_c, qc_c, k, qh_c, qmu_c, qM_c);
         }
     }
@@ -486,22 +523,190 @@ __device__ __forceinline__ void tile_march2(
     T rs_n = 0, rl_n = 0, rc_n = 0, qs_n = 0, ql_n = 0, qc_n = 0, os = 0, ol = 0, oc = 0;
     if (PF) {
         const int zn = ghost_any<BC>(z0, nz), zc = ghost_any<BC>(z0 - 1, nz);
-        rs_n = PLANE(s_in, zn)[col];
-        rl_n = PLANE(l_in, zn)[col];
-        rc_n = PLANE(c_in, zn)[col];
+        rs_n = ld_plane(PLANE(s_in, zn), col);
+        rl_n = ld_plane(PLANE(l_in, zn), col);
+        rc_n = ld_plane(PLANE(c_in, zn), col);
         if (ring) {
-            qs_n = PLANE(s_in, zn)[rcol];
-            ql_n = PLANE(l_in, zn)[rcol];
-            qc_n = PLANE(c_in, zn)[rcol];
+            qs_n = ld_plane(PLANE(s_in, zn), rcol);
+            ql_n = ld_plane(PLANE(l_in, zn), rcol);
+            qc_n = ld_plane(PLANE(c_in, zn), rcol);
         }
         if (outer) {
-            os = PLANE(s_in, zc)[o2col];
-            ol = PLANE(l_in, zc)[o2col];
-            oc = PLANE(c_in, zc)[o2col];
+            os = ld_plane(PLANE(s_in, zc), o2col);
+            ol = ld_plane(PLANE(l_in, zc), o2col);
+            oc = ld_plane(PLANE(c_in, zc), o2col);
         }
     }
 
+    if constexpr (!PF && ROT) {
+        // Without the early loads the state of three consecutive planes
+        // rotates through three register sets (tile column, ring column and
+        // step t+1 column: z-1, z, z+1), and the march is unrolled by three
+        // with the sets passed in rotated order, so no plane's values are
+        // copied from one register to another.
+        Col<T> ta{s_p, l_p, 0, 0, mu_p, M_p}, tb{s_c, l_c, c_c, h_c, mu_c, M_c}, tn{};
+        Col<T> qa{qs_p, ql_p, 0, 0, qmu_p, qM_p}, qb{qs_c, ql_c, qc_c, qh_c, qmu_c, qM_c}, qn{};
+        Col<T> ua{}, ub{}, un{};
+        auto plane_step = [&](const int zs, const Col<T> &tp, const Col<T> &tc, Col<T> &tnx,
+                              const Col<T> &qp, const Col<T> &qc, Col<T> &qnx, const Col<T> &up,
+                              Col<T> &uc, Col<T> &unx) {
+            PPF_FRESH("+v"(col), "+v"(ocol), "+v"(rcol), "+v"(o2col));
+            const int zc = ghost_any<BC>(zs, nz), zn = ghost_any<BC>(zs + 1, nz);
+            const int bsel = DB ? ((zs - z0 + 1) & 1) : 0;
+            T(*__restrict__ A1)[TY + 4][TX + 4] = a1 + 4 * bsel;
+            T(*__restrict__ A2)[TY + 2][TX + 2] = a2 + 4 * bsel;
+            const T(*__restrict__ A2p)[TY + 2][TX + 2] = a2 + 4 * (DB ? bsel ^ 1 : 0);
+            tnx.s = ld_plane(PLANE(s_in, zn), col);
+            tnx.l = ld_plane(PLANE(l_in, zn), col);
+            tnx.c = ld_plane(PLANE(c_in, zn), col);
+            if (ring) {
+                qnx.s = ld_plane(PLANE(s_in, zn), rcol);
+                qnx.l = ld_plane(PLANE(l_in, zn), rcol);
+                qnx.c = ld_plane(PLANE(c_in, zn), rcol);
+            }
+            T os = 0, ol = 0, oc = 0;
+            if (outer) {
+                os = ld_plane(PLANE(s_in, zc), o2col);
+                ol = ld_plane(PLANE(l_in, zc), o2col);
+                oc = ld_plane(PLANE(c_in, zc), o2col);
+            }
+            // A1: step t of plane zs
+            A1[0][ty + 2][tx + 2] = tc.s;
+            A1[1][ty + 2][tx + 2] = tc.l;
+            A1[2][ty + 2][tx + 2] = tc.mu;
+            A1[3][ty + 2][tx + 2] = tc.M;
+            if (ring) {
+                A1[0][ry + 1][rx + 1] = qc.s;
+                A1[1][ry + 1][rx + 1] = qc.l;
+                A1[2][ry + 1][rx + 1] = qc.mu;
+                A1[3][ry + 1][rx + 1] = qc.M;
+            }
+            if (outer) {
+                T oh, omu, oM;
+                P::point(os, ol, oc, k, oh, omu, oM);
+                A1[0][oy][ox] = os;
+                A1[1][oy][ox] = ol;
+                A1[2][oy][ox] = omu;
+                A1[3][oy][ox] = oM;
+            }
+            P::point(tnx.s, tnx.l, tnx.c, k, tnx.h, tnx.mu, tnx.M);
+            if (ring)
+                P::point(qnx.s, qnx.l, qnx.c, k, qnx.h, qnx.mu, qnx.M);
+            __syncthreads();
+
+            // stage 1: step t+1 of plane zs at the tile site (mirror
+            // boundaries: a ghost plane is not stepped, it keeps the step
+            // t+1 of the plane before it)
+            const bool zghost = BC == 1 && (zs < 0 || zs >= nz);
+            if constexpr (BC == 1)
+                unx = uc;
+            if (!zghost) {
+                const int ax = tx + 2, ay = ty + 2;
+                const T sn[6] = {tnx.s, tp.s, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
+                const T ln[6] = {tnx.l, tp.l, A1[1][ay + 1][ax], A1[1][ay - 1][ax], A1[1][ay][ax + 1], A1[1][ay][ax - 1]};
+                const T mun[6] = {tnx.mu, tp.mu, A1[2][ay + 1][ax], A1[2][ay - 1][ax], A1[2][ay][ax + 1], A1[2][ay][ax - 1]};
+                const T Mn[6] = {tnx.M, tp.M, A1[3][ay + 1][ax], A1[3][ay - 1][ax], A1[3][ay][ax + 1], A1[3][ay][ax - 1]};
+                double ds, dl, dc;
+                P::update(tc.s, tc.l, tc.c, tc.h, tc.mu, tc.M, sn, ln, mun, Mn, k, unx.s, unx.l,
+                          unx.c, ds, dl, dc);
+                P::point(unx.s, unx.l, unx.c, k, unx.h, unx.mu, unx.M);
+            }
+            // ... and at the ring site
+            T qs1 = 0, ql1 = 0, qmu1 = 0, qM1 = 0;
+            if (ring && ring_in && !zghost) {
+                const int ax = rx + 1, ay = ry + 1;
+                const T sn[6] = {qnx.s, qp.s, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
+                const T ln[6] = {qnx.l, qp.l, A1[1][ay + 1][ax], A1[1][ay - 1][ax], A1[1][ay][ax + 1], A1[1][ay][ax - 1]};
+                const T mun[6] = {qnx.mu, qp.mu, A1[2][ay + 1][ax], A1[2][ay - 1][ax], A1[2][ay][ax + 1], A1[2][ay][ax - 1]};
+                const T Mn[6] = {qnx.M, qp.M, A1[3][ay + 1][ax], A1[3][ay - 1][ax], A1[3][ay][ax + 1], A1[3][ay][ax - 1]};
+                T qc1, qh1;
+                double ds, dl, dc;
+                P::update(qc.s, qc.l, qc.c, qc.h, qc.mu, qc.M, sn, ln, mun, Mn, k, qs1, ql1, qc1, ds,
+                          dl, dc);
+                P::point(qs1, ql1, qc1, k, qh1, qmu1, qM1);
+            }
+            // A2: step t+1 of plane zs (plain stores, see tile_march2's loop)
+#define PPF_PUT_A2(BY, BX)          \
+    do {                            \
+        A2[0][BY][BX] = unx.s;      \
+        A2[1][BY][BX] = unx.l;      \
+        A2[2][BY][BX] = unx.mu;     \
+        A2[3][BY][BX] = unx.M;      \
+    } while (0)
+            auto store_a2 = [&]() {
+                if (BC == 0 || active)
+                    PPF_PUT_A2(ty + 1, tx + 1);
+                if constexpr (BC == 1) {
+                    if (active) {
+                        if (x0 + tx == 0)
+                            PPF_PUT_A2(ty + 1, tx);
+                        if (x0 + tx == nx - 1)
+                            PPF_PUT_A2(ty + 1, tx + 2);
+                        if (y0 + ty == 0)
+                            PPF_PUT_A2(ty, tx + 1);
+                        if (y0 + ty == ny - 1)
+                            PPF_PUT_A2(ty + 2, tx + 1);
+                    }
+                }
+                if (ring && ring_in) {
+                    A2[0][ry][rx] = qs1;
+                    A2[1][ry][rx] = ql1;
+                    A2[2][ry][rx] = qmu1;
+                    A2[3][ry][rx] = qM1;
+                }
+            };
+#undef PPF_PUT_A2
+            if (DB)
+                store_a2();
+            // stage 2: step t+2 of plane zs-1 at the tile site
+            if (zs >= z0 + 1 && active) {
+                const int bx = tx + 1, by = ty + 1;
+                const T sn[6] = {unx.s, up.s, A2p[0][by + 1][bx], A2p[0][by - 1][bx], A2p[0][by][bx + 1], A2p[0][by][bx - 1]};
+                const T ln[6] = {unx.l, up.l, A2p[1][by + 1][bx], A2p[1][by - 1][bx], A2p[1][by][bx + 1], A2p[1][by][bx - 1]};
+                const T mun[6] = {unx.mu, up.mu, A2p[2][by + 1][bx], A2p[2][by - 1][bx], A2p[2][by][bx + 1], A2p[2][by][bx - 1]};
+                const T Mn[6] = {unx.M, up.M, A2p[3][by + 1][bx], A2p[3][by - 1][bx], A2p[3][by][bx + 1], A2p[3][by][bx - 1]};
+                T so, lo, co;
+                double ds, dl, dc;
+                P::update(uc.s, uc.l, uc.c, uc.h, uc.mu, uc.M, sn, ln, mun, Mn, k, so, lo, co, ds, dl,
+                          dc);
+                __builtin_nontemporal_store(so, at_plane(PLANE(s_out, zs - 1), ocol));
+                __builtin_nontemporal_store(lo, at_plane(PLANE(l_out, zs - 1), ocol));
+                __builtin_nontemporal_store(co, at_plane(PLANE(c_out, zs - 1), ocol));
+            }
+            if (!DB) {
+                __syncthreads();
+                store_a2();
+            }
+            // mirror boundary below plane 0: ghost plane -1 holds plane 0's
+            // step t+1 (uc becomes the next plane's z-1 set)
+            if (BC == 1 && z0 == 0 && zs == 0) {
+                uc.s = unx.s;
+                uc.l = unx.l;
+                uc.mu = unx.mu;
+                uc.M = unx.M;
+            }
+        };
+        int zs = z0 - 1;
+        for (; zs + 2 <= zend; zs += 3) {
+            plane_step(zs, ta, tb, tn, qa, qb, qn, ua, ub, un);
+            plane_step(zs + 1, tb, tn, ta, qb, qn, qa, ub, un, ua);
+            plane_step(zs + 2, tn, ta, tb, qn, qa, qb, un, ua, ub);
+        }
+        // the last one or two planes, rotating by copies.  (Policies whose
+        // registers would not hold the third set at three waves per SIMD,
+        // the bitwise FP64 one, take the loop below instead.)
+        for (; zs <= zend; ++zs) {
+            plane_step(zs, ta, tb, tn, qa, qb, qn, ua, ub, un);
+            ta = tb;
+            tb = tn;
+            qa = qb;
+            qb = qn;
+            ua = ub;
+            ub = un;
+        }
+    } else {
     for (int zs = z0 - 1; zs <= zend; ++zs) {
+        PPF_FRESH("+v"(col), "+v"(ocol), "+v"(rcol), "+v"(o2col));
         const int zc = ghost_any<BC>(zs, nz), zn = ghost_any<BC>(zs + 1, nz);
         // this plane's A1 and A2 buffers, and the A2 buffer of plane zs-1
         const int bsel = DB ? ((zs - z0 + 1) & 1) : 0;
@@ -509,18 +714,18 @@ __device__ __forceinline__ void tile_march2(
         T(*__restrict__ A2)[TY + 2][TX + 2] = a2 + 4 * bsel;
         const T(*__restrict__ A2p)[TY + 2][TX + 2] = a2 + 4 * (DB ? bsel ^ 1 : 0);
         if (!PF) {
-            rs_n = PLANE(s_in, zn)[col];
-            rl_n = PLANE(l_in, zn)[col];
-            rc_n = PLANE(c_in, zn)[col];
+            rs_n = ld_plane(PLANE(s_in, zn), col);
+            rl_n = ld_plane(PLANE(l_in, zn), col);
+            rc_n = ld_plane(PLANE(c_in, zn), col);
             if (ring) {
-                qs_n = PLANE(s_in, zn)[rcol];
-                ql_n = PLANE(l_in, zn)[rcol];
-                qc_n = PLANE(c_in, zn)[rcol];
+                qs_n = ld_plane(PLANE(s_in, zn), rcol);
+                ql_n = ld_plane(PLANE(l_in, zn), rcol);
+                qc_n = ld_plane(PLANE(c_in, zn), rcol);
             }
             if (outer) {
-                os = PLANE(s_in, zc)[o2col];
-                ol = PLANE(l_in, zc)[o2col];
-                oc = PLANE(c_in, zc)[o2col];
+                os = ld_plane(PLANE(s_in, zc), o2col);
+                ol = ld_plane(PLANE(l_in, zc), o2col);
+                oc = ld_plane(PLANE(c_in, zc), o2col);
             }
         }
         // PF: the next iteration's values (plane zs+2 of the columns, zs+1
@@ -528,18 +733,18 @@ __device__ __forceinline__ void tile_march2(
         T fs = 0, fl = 0, fc = 0, fqs = 0, fql = 0, fqc = 0, fos = 0, fol = 0, foc = 0;
         if (PF && zs < zend) {
             const int zf = ghost_any<BC>(zs + 2, nz);
-            fs = PLANE(s_in, zf)[col];
-            fl = PLANE(l_in, zf)[col];
-            fc = PLANE(c_in, zf)[col];
+            fs = ld_plane(PLANE(s_in, zf), col);
+            fl = ld_plane(PLANE(l_in, zf), col);
+            fc = ld_plane(PLANE(c_in, zf), col);
             if (ring) {
-                fqs = PLANE(s_in, zf)[rcol];
-                fql = PLANE(l_in, zf)[rcol];
-                fqc = PLANE(c_in, zf)[rcol];
+                fqs = ld_plane(PLANE(s_in, zf), rcol);
+                fql = ld_plane(PLANE(l_in, zf), rcol);
+                fqc = ld_plane(PLANE(c_in, zf), rcol);
             }
             if (outer) {
-                fos = PLANE(s_in, zn)[o2col];
-                fol = PLANE(l_in, zn)[o2col];
-                foc = PLANE(c_in, zn)[o2col];
+                fos = ld_plane(PLANE(s_in, zn), o2col);
+                fol = ld_plane(PLANE(l_in, zn), o2col);
+                foc = ld_plane(PLANE(c_in, zn), o2col);
             }
         }
         // A1: step t of plane zs
@@ -651,9 +856,9 @@ __device__ __forceinline__ void tile_march2(
             T so, lo, co;
             double ds, dl, dc;
             P::update(us_c, ul_c, uc_c, uh_c, umu_c, uM_c, sn, ln, mun, Mn, k, so, lo, co, ds, dl, dc);
-            __builtin_nontemporal_store(so, PLANE(s_out, zs - 1) + ocol);
-            __builtin_nontemporal_store(lo, PLANE(l_out, zs - 1) + ocol);
-            __builtin_nontemporal_store(co, PLANE(c_out, zs - 1) + ocol);
+            __builtin_nontemporal_store(so, at_plane(PLANE(s_out, zs - 1), ocol));
+            __builtin_nontemporal_store(lo, at_plane(PLANE(l_out, zs - 1), ocol));
+            __builtin_nontemporal_store(co, at_plane(PLANE(c_out, zs - 1), ocol));
         }
         if (!DB) {
             __syncthreads();
@@ -709,6 +914,7 @@ __device__ __forceinline__ void tile_march2(
             oc = foc;
         }
     }
+    }
 #undef PLANE
 }
 
@@ -747,31 +953,32 @@ __device__ __forceinline__ void tile_march2s(
         // ---- tile waves ----
         const int tx = tid % TX, ty = tid / TX;
         const bool active = x0 + tx < nx && y0 + ty < ny;
-        const unsigned col = colof(tx, ty);
-        const unsigned ocol = (unsigned)(y0 + ty) * (unsigned)nx + (unsigned)(x0 + tx);
+        unsigned col = colof(tx, ty);
+        unsigned ocol = (unsigned)(y0 + ty) * (unsigned)nx + (unsigned)(x0 + tx);
         T s_p, l_p, mu_p, M_p, s_c, l_c, c_c, h_c, mu_c, M_c;
         {
             const int zp = ghost_any<0>(z0 - 2, nz), zc = ghost_any<0>(z0 - 1, nz);
             T hp;
-            s_p = PLANE(s_in, zp)[col];
-            l_p = PLANE(l_in, zp)[col];
-            P::point(s_p, l_p, PLANE(c_in, zp)[col], k, hp, mu_p, M_p);
-            s_c = PLANE(s_in, zc)[col];
-            l_c = PLANE(l_in, zc)[col];
-            c_c = PLANE(c_in, zc)[col];
+            s_p = ld_plane(PLANE(s_in, zp), col);
+            l_p = ld_plane(PLANE(l_in, zp), col);
+            P::point(s_p, l_p, ld_plane(PLANE(c_in, zp), col), k, hp, mu_p, M_p);
+            s_c = ld_plane(PLANE(s_in, zc), col);
+            l_c = ld_plane(PLANE(l_in, zc), col);
+            c_c = ld_plane(PLANE(c_in, zc), col);
             P::point(s_c, l_c, c_c, k, h_c, mu_c, M_c);
         }
         T us_p = 0, ul_p = 0, umu_p = 0, uM_p = 0, us_c = 0, ul_c = 0, uc_c = 0, uh_c = 0,
           umu_c = 0, uM_c = 0;
         for (int zs = z0 - 1; zs <= zend; ++zs) {
+            PPF_FRESH("+v"(col), "+v"(ocol));
             const int zn = ghost_any<0>(zs + 1, nz);
             const int bsel = (zs - z0 + 1) & 1;
             T(*__restrict__ A1)[TY + 4][TX + 4] = a1 + 4 * bsel;
             T(*__restrict__ A2)[TY + 2][TX + 2] = a2 + 4 * bsel;
             const T(*__restrict__ A2p)[TY + 2][TX + 2] = a2 + 4 * (bsel ^ 1);
-            const T rs_n = PLANE(s_in, zn)[col];
-            const T rl_n = PLANE(l_in, zn)[col];
-            const T rc_n = PLANE(c_in, zn)[col];
+            const T rs_n = ld_plane(PLANE(s_in, zn), col);
+            const T rl_n = ld_plane(PLANE(l_in, zn), col);
+            const T rc_n = ld_plane(PLANE(c_in, zn), col);
             A1[0][ty + 2][tx + 2] = s_c;
             A1[1][ty + 2][tx + 2] = l_c;
             A1[2][ty + 2][tx + 2] = mu_c;
@@ -805,9 +1012,9 @@ __device__ __forceinline__ void tile_march2s(
                 T so, lo, co;
                 double ds, dl, dc;
                 P::update(us_c, ul_c, uc_c, uh_c, umu_c, uM_c, sn, ln, mun, Mn, k, so, lo, co, ds, dl, dc);
-                __builtin_nontemporal_store(so, PLANE(s_out, zs - 1) + ocol);
-                __builtin_nontemporal_store(lo, PLANE(l_out, zs - 1) + ocol);
-                __builtin_nontemporal_store(co, PLANE(c_out, zs - 1) + ocol);
+                __builtin_nontemporal_store(so, at_plane(PLANE(s_out, zs - 1), ocol));
+                __builtin_nontemporal_store(lo, at_plane(PLANE(l_out, zs - 1), ocol));
+                __builtin_nontemporal_store(co, at_plane(PLANE(c_out, zs - 1), ocol));
             }
             s_p = s_c;
             l_p = l_c;
@@ -849,7 +1056,7 @@ __device__ __forceinline__ void tile_march2s(
             rx = r - 2 * TY - TX + 1;
             ry = TY + 1;
         }
-        const unsigned rcol = colof(rx - 1, ry - 1);
+        unsigned rcol = colof(rx - 1, ry - 1);
         // outer sites j = r and r + NRL (A1 frame ox, oy; tile frame ox-2, oy-2)
         int ox[2] = {0, 0}, oy[2] = {0, 0};
         bool outer[2];
@@ -875,39 +1082,40 @@ __device__ __forceinline__ void tile_march2s(
                 oy[u] = (c & 2) ? TY + 2 : 1;
             }
         }
-        const unsigned o2col[2] = {colof(ox[0] - 2, oy[0] - 2), colof(ox[1] - 2, oy[1] - 2)};
+        unsigned o2col[2] = {colof(ox[0] - 2, oy[0] - 2), colof(ox[1] - 2, oy[1] - 2)};
         T qs_p = 0, ql_p = 0, qmu_p = 0, qM_p = 0, qs_c = 0, ql_c = 0, qc_c = 0, qh_c = 0, qmu_c = 0,
           qM_c = 0;
         if (ring) {
             const int zp = ghost_any<0>(z0 - 2, nz), zc = ghost_any<0>(z0 - 1, nz);
-            qs_p = PLANE(s_in, zp)[rcol];
-            ql_p = PLANE(l_in, zp)[rcol];
+            qs_p = ld_plane(PLANE(s_in, zp), rcol);
+            ql_p = ld_plane(PLANE(l_in, zp), rcol);
             T qh;
-            P::point(qs_p, ql_p, PLANE(c_in, zp)[rcol], k, qh, qmu_p, qM_p);
-            qs_c = PLANE(s_in, zc)[rcol];
-            ql_c = PLANE(l_in, zc)[rcol];
-            qc_c = PLANE(c_in, zc)[rcol];
+            P::point(qs_p, ql_p, ld_plane(PLANE(c_in, zp), rcol), k, qh, qmu_p, qM_p);
+            qs_c = ld_plane(PLANE(s_in, zc), rcol);
+            ql_c = ld_plane(PLANE(l_in, zc), rcol);
+            qc_c = ld_plane(PLANE(c_in, zc), rcol);
             P::point(qs_c, ql_c, qc_c, k, qh_c, qmu_c, qM_c);
         }
         for (int zs = z0 - 1; zs <= zend; ++zs) {
+            PPF_FRESH("+v"(rcol), "+v"(o2col[0]), "+v"(o2col[1]));
             const int zc = ghost_any<0>(zs, nz), zn = ghost_any<0>(zs + 1, nz);
             const int bsel = (zs - z0 + 1) & 1;
             T(*__restrict__ A1)[TY + 4][TX + 4] = a1 + 4 * bsel;
             T(*__restrict__ A2)[TY + 2][TX + 2] = a2 + 4 * bsel;
             T qs_n = 0, ql_n = 0, qc_n = 0;
             if (ring) {
-                qs_n = PLANE(s_in, zn)[rcol];
-                ql_n = PLANE(l_in, zn)[rcol];
-                qc_n = PLANE(c_in, zn)[rcol];
+                qs_n = ld_plane(PLANE(s_in, zn), rcol);
+                ql_n = ld_plane(PLANE(l_in, zn), rcol);
+                qc_n = ld_plane(PLANE(c_in, zn), rcol);
             }
             T os[2], ol[2], oc[2];
 #pragma unroll
             for (int u = 0; u < 2; u++) {
                 os[u] = ol[u] = oc[u] = 0;
                 if (outer[u]) {
-                    os[u] = PLANE(s_in, zc)[o2col[u]];
-                    ol[u] = PLANE(l_in, zc)[o2col[u]];
-                    oc[u] = PLANE(c_in, zc)[o2col[u]];
+                    os[u] = ld_plane(PLANE(s_in, zc), o2col[u]);
+                    ol[u] = ld_plane(PLANE(l_in, zc), o2col[u]);
+                    oc[u] = ld_plane(PLANE(c_in, zc), o2col[u]);
                 }
             }
             if (ring) {

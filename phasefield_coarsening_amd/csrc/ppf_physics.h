@@ -125,6 +125,9 @@ template <bool EQA>
 struct PhysF64 {
     using T = double;
     using K = KConst;
+    // the two-step march keeps two planes of state per column (a third set
+    // of registers would make the three-wave kernel spill)
+    static constexpr bool kRotate = false;
     static __device__ __forceinline__ void point(T s, T l, T c, const K &k, T &h, T &mu, T &M)
     {
         pointwise<EQA>(s, l, c, k, h, mu, M);
@@ -194,6 +197,8 @@ template <typename F>
 struct PhysC {
     using T = F;
     using K = KConstC<F>;
+    // the two-step march rotates three register sets per column (no copies)
+    static constexpr bool kRotate = true;
     static __device__ __forceinline__ void point(T s, T l, T c, const K &k, T &r, T &u, T &mm)
     {
         const T ss = s * s;
