@@ -578,9 +578,12 @@ extern "C" int ppf_create(ppf_ctx **out, const ppf_desc *desc)
     if (d.kernel == PPF_KERNEL_FAST && !(d.params.ML > 0.0))
         return set_err(nullptr, PPF_EARG, "the fast FP64 form needs ML > 0");
     const size_t n = (size_t)d.nz * d.ny * d.nx;
-    if ((size_t)d.ny * d.nx >= ((size_t)1 << 31))
-        return set_err(nullptr, PPF_EARG, "an x-y plane of %d x %d sites is too large", d.ny,
-                       d.nx);
+    // the step kernels address a plane with 32-bit byte offsets
+    const size_t esz = d.precision == PPF_F64 ? sizeof(double) : sizeof(float);
+    if ((size_t)d.ny * d.nx * esz >= ((size_t)1 << 32))
+        return set_err(nullptr, PPF_EARG,
+                       "an x-y plane of %d x %d sites is too large (4 GiB per plane at most)",
+                       d.ny, d.nx);
     if ((n >> 36) != 0)
         return set_err(nullptr, PPF_EARG, "grid too large");
     int ndev = 0;
@@ -1475,8 +1478,10 @@ extern "C" int ppf_sweep_report_noalloc(const double *s_in, const double *l_in,
         return set_err(nullptr, PPF_EARG, "null argument");
     if (nz < 1 || ny < 1 || nx < 1 || nz > MAX_DIM || ny > MAX_DIM || nx > MAX_DIM)
         return set_err(nullptr, PPF_EARG, "bad grid %d x %d x %d", nz, ny, nx);
-    if ((size_t)ny * nx >= ((size_t)1 << 31))
-        return set_err(nullptr, PPF_EARG, "an x-y plane of %d x %d sites is too large", ny, nx);
+    if ((size_t)ny * nx * sizeof(double) >= ((size_t)1 << 32))
+        return set_err(nullptr, PPF_EARG,
+                       "an x-y plane of %d x %d sites is too large (4 GiB per plane at most)",
+                       ny, nx);
     if (bc != PPF_BC_PERIODIC && bc != PPF_BC_NOFLUX)
         return set_err(nullptr, PPF_EARG, "bad bc %d", bc);
     // the tiling a context of this grid uses for single steps (choose_tiles),

@@ -79,6 +79,11 @@ def test_create_rejects_bad_arguments():
     with pytest.raises(solver.PpfError) as e:
         solver.Solver((4, 4, 4), kernel=4)
     assert e.value.code == -1
+    # the step kernels address a plane with 32-bit byte offsets: an FP64
+    # plane of 23200^2 sites (4.3 GB) is refused before any device call
+    with pytest.raises(solver.PpfError) as e:
+        solver.Solver((1, 23200, 23200))
+    assert e.value.code == -1 and "4 GiB" in str(e.value)
 
 
 def test_h5_roundtrip_and_compat(tmp_path):
