@@ -93,9 +93,12 @@ int ppf_unet_last_forward(ppf_unet *net, double *ms, double *flop);
  *     are rounded to BF16 once, so kind 6 agrees with kind 2 to BF16 weight
  *     rounding, not bitwise; training runs as kind 2; the last decoder
  *     convolution of the base-32 network runs z-marching, with its fine and
- *     coarse inputs staged in LDS;
+ *     coarse inputs staged in LDS and its K dimension split over two waves
+ *     per SIMD;
  *   7 kind 6 with the row-blocked polyphase kernel for both decoder
- *     convolutions (the same sums as kind 6, for cross-checks);
+ *     convolutions (kind 6's products; in the last decoder convolution
+ *     summed in one instead of two partial sums, so the two agree to FP32
+ *     rounding before the BF16 output rounding);
  *   1 LDS-tiled: a 16 x 8 brick's halo staged in LDS;
  *   0 direct: every operand straight from global memory.
  * Kinds 2-5 add the same (tap, 32-channel) products in the same order:

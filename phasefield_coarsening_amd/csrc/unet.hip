@@ -60,6 +60,13 @@ __device__ __forceinline__ bf16x4 leaky4(f32x4 v)
 
 constexpr int NCONV = 10;
 
+// An empty asm that reads v: the compiler must wait for the load of v here.
+// Before a loop this keeps a wait for loop-invariant operands (weights, a
+// bias) out of the loop body, where vmcnt, counting in issue order, would
+// also wait for every load and store issued after them.
+__device__ __forceinline__ void settle(f32x4 v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void settle(bf16x8 v) { settle(__builtin_bit_cast(f32x4, v)); }
+
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // Any coordinate to [0, n): a staged brick can be larger than the grid (the
@@ -561,7 +568,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 #pragma unroll
     for (int cr = 0; cr < RC; cr++)
         roc[cr] = wrapm(y0 / 2 - 1 + cr, Hc) * rowlen_c;
-    // weights: Wt[tap][n0+16j+xl][kc+8q] (skip), Wp[par][grp][n0+16j+xl][kc+8q]
+    // weights: Wt[tap][n0+16j+xl][kc+8q] (skip), Wp[par][grp][kc/32][n0+16j+xl][8q]
     // (upsampled, par = 4 pz + 2 py + px, grp = 4 oz + 2 oy + ox)
     const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(Wt + (size_t)n0 * Cin), (short)0, (27 * CoutW - n0) * Cin * 2, 0x00020000);
@@ -569,7 +576,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     const int tapstride = CoutW * Cin, jstride = 16 * Cin;
     const __amdgpu_buffer_rsrc_t wpr = __builtin_amdgcn_make_buffer_rsrc(
         (void *)Wp, (short)0, 64 * CoutW * cup * 2, 0x00020000);
-    const int wpo = (xl * cup + 8 * q) * 2;
+    const int wpo = (xl * 32 + 8 * q) * 2;
     // accumulators start at the bias of the lane's four output channels
     f32x4 acc[R][NJ];
 #pragma unroll
@@ -601,8 +608,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
                             bf16x8,
                             __builtin_amdgcn_raw_buffer_load_b128(
                                 wpr, wpo,
-                                ((((4 * pz + 2 * py + px) * 8 + 4 * oz + 2 * oy + ox) * CoutW + n0 +
-                                  16 * j) * cup + kc) * 2,
+                                (((((4 * pz + 2 * py + px) * 8 + 4 * oz + 2 * oy + ox) * (cup / 32) +
+                                   kc / 32) * CoutW + n0 + 16 * j) * 32) * 2,
                                 0));
         };
         // oy outermost: each accumulator still takes oy = 0 then 1, and R*NJ
@@ -870,40 +877,56 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 4,
 }
 
 // ----------------------------------------------------------------------------
-// k_conv_poly's sums (upsampled channels in polyphase form, then the 27 taps
-// of the skip channels, in the same order: bitwise the same output), for
-// Cout = 32, 32 skip channels and 64 upsampled channels (L8 of the base-32
-// network), z-marching with both sources staged in LDS.  A workgroup owns a
-// fine 32 (x) x 8 (y) tile and marches a chunk of z planes; wave w computes
-// x parity px = w & 1 (sites x0 + 2 l + px) and output channels 16 (w >> 1)
-// .. +15 of all 8 rows, with its 27 skip-tap weights in registers and the
-// summed weights read through the cache.  LDS holds three fine planes of the
-// skip channels (tile + one-site halo, 10 rows x 34 sites, stored by x
-// parity so that a wave's lanes read consecutive sites; 32 channels padded
-// to 48) and three coarse planes of the upsampled source (6 rows x 18
-// sites, 64 channels padded to 80).  A fine plane z reads coarse planes
-// (z >> 1) + (z & 1) - 1 and the next one, so a new coarse plane is staged
-// every second fine plane.  D, H, W even; chunks start at even z.
+// L8 of the base-32 network (Cout 32, 32 skip and 64 upsampled channels):
+// k_conv_poly's products (upsampled channels in polyphase form, the 27 taps
+// of the skip channels), z-marching with both sources staged in LDS.  A
+// workgroup owns a fine 32 (x) x 8 (y) tile and marches a chunk of z planes.
+// LDS holds three fine planes of the skip channels (tile + one-site halo,
+// 10 rows x 34 sites, stored by x parity so that a wave's lanes read
+// consecutive sites; 32 channels padded to 48) and two coarse planes of the
+// upsampled source (6 rows x 18 sites, 64 channels padded to 72).  A fine
+// plane z reads coarse planes (z >> 1) + (z & 1) - 1 and the next one, so a
+// new coarse plane is staged every second fine plane, over the one that the
+// plane being computed no longer reads.  D, H, W even; chunks start at even
+// z.
+// Each output (x parity px: sites x0 + 2 l + px; 16 channels, nt) is shared
+// by two waves that split the K dimension, so that every SIMD runs two waves
+// of eight accumulator chains (one wave per SIMD with eight chains issues
+// MFMAs at about a third of the peak rate; two waves, above four fifths):
+//   waves 0-3 ("up"): the 8 upsampled-channel groups ((chunk, oz, ox), 16
+//     MFMAs each, summed weights streamed from the cache four groups ahead,
+//     one contiguous kilobyte per fragment), then the skip groups (dz, dx) =
+//     (2, 0), (2, 1): 176 MFMAs per plane; the partial sum (from zero) goes
+//     to LDS;
+//   waves 4-7 ("skip"): the skip groups (0, 0..2), (1, 0..2), (2, 2), weights
+//     in registers: 168 MFMAs per plane, from the bias; after the next
+//     plane's first barrier they add the partial and store.
+// The sum is k_conv_poly's products in another association (two partial
+// sums), so the outputs agree with it to FP32 rounding, not bitwise.  LDS:
+// 97,920 + 31,104 bytes of planes and 32,768 of partial sums.
 // ----------------------------------------------------------------------------
 constexpr int ZP_TX = 32, ZP_TY = 8;
 constexpr int ZP_FX = 17, ZP_FROWS = ZP_TY + 2, ZP_FPAD = LDS_PAD32;
 constexpr int ZP_FSITES = ZP_FROWS * 2 * ZP_FX;  // 340 per fine plane
-constexpr int ZP_CX = ZP_TX / 2 + 2, ZP_CROWS = ZP_TY / 2 + 2, ZP_CPAD = LDS_PAD64;
+constexpr int ZP_CX = ZP_TX / 2 + 2, ZP_CROWS = ZP_TY / 2 + 2;
 constexpr int ZP_CSITES = ZP_CROWS * ZP_CX;      // 108 per coarse plane
-constexpr int ZP_FCH = ZP_FSITES * 4, ZP_FPT = (ZP_FCH + 255) / 256;  // 16-byte chunks
-constexpr int ZP_CCH = ZP_CSITES * 8, ZP_CPT = (ZP_CCH + 255) / 256;
+constexpr int ZP_FCH = ZP_FSITES * 4;            // 16-byte chunks per plane
+constexpr int ZP_CCH = ZP_CSITES * 8;
+constexpr int ZQ_CPAD = 72;
+constexpr int ZQ_FPT = (ZP_FCH + 511) / 512, ZQ_CPT = (ZP_CCH + 511) / 512;
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_conv_zpoly(
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_zpoly8(
     const bf16 *__restrict__ X, int ldx, int xoff, const bf16 *__restrict__ Wt,
     const bf16 *__restrict__ Wp, const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy,
     int B, int D, int H, int Wd, const bf16 *__restrict__ Xc, int ldc, int zc, int ntx, int nty)
 {
     __shared__ __attribute__((aligned(16))) bf16 fsh[3 * ZP_FSITES * ZP_FPAD];  // 97,920 bytes
-    __shared__ __attribute__((aligned(16))) bf16 csh[3 * ZP_CSITES * ZP_CPAD];  // 51,840 bytes
+    __shared__ __attribute__((aligned(16))) bf16 csh[2 * ZP_CSITES * ZQ_CPAD];  // 31,104 bytes
+    __shared__ f32x4 red[4 * ZP_TY * 64];                                       // 32,768 bytes
     constexpr int CIN = 96, CUP = 64, COUT = 32;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int px = wave & 1, nt = wave >> 1;
+    const int px = wave & 1, nt = (wave >> 1) & 1, role = wave >> 2;
     int r = blockIdx.x;
     const int tx = r % ntx;
     r /= ntx;
@@ -914,66 +937,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const int x0 = tx * ZP_TX, y0 = ty * ZP_TY, z0 = zci * zc, zend = min(z0 + zc, D);
     const int Dc = D / 2, Hc = H / 2, Wc = Wd / 2, Z0 = z0 / 2;
     const int xl = lane & 15, q = lane >> 4;
-    // skip-tap weights of this wave's channels: ws[tap] = Wt[tap][16 nt + xl][64 + 8q ..]
-    bf16x8 ws[27];
-#pragma unroll
-    for (int tap = 0; tap < 27; tap++)
-        ws[tap] = *(const bf16x8 *)(Wt + ((size_t)tap * COUT + 16 * nt + xl) * CIN + CUP + 8 * q);
-    const __amdgpu_buffer_rsrc_t wpr = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)Wp, (short)0, 64 * COUT * CUP * 2, 0x00020000);
-    const int wpo = ((16 * nt + xl) * CUP + 8 * q) * 2;
-    const f32x4 bias0 = bias4(bias, 16 * nt + 4 * q);
-    // staging: fine chunk c -> site s = c / 4 (LDS order: row, parity, index),
+    // staging: fine chunk c -> site c / 4 (LDS order: row, parity, index),
     // part c % 4; coarse chunk c -> site c / 8, part c % 8.  Every thread
     // issues the same loads (chunks past the end re-read the last one and
-    // are not stored), so the compiler can count them in its waits.
-    bf16x8 pf[ZP_FPT], pc[ZP_CPT];
-    auto load_fine = [&](int zz) {
+    // are not stored), so the compiler can count them in its waits; each
+    // thread's in-plane element offsets (planes of under 2^31 elements) are
+    // computed once
+    bf16x8 pf[ZQ_FPT], pc[ZQ_CPT];
+    unsigned fo[ZQ_FPT], co[ZQ_CPT];
 #pragma unroll
-        for (int i = 0; i < ZP_FPT; i++) {
-            const int c = min(tid + 256 * i, ZP_FCH - 1);
-            {
-                const int s = c >> 2, part = c & 3;
-                const int fr = s / (2 * ZP_FX), rem = s - fr * 2 * ZP_FX;
-                const int par = rem / ZP_FX, idx = rem - par * ZP_FX;
-                const int lx = 2 * idx + par;
-                const size_t v = (((size_t)b * D + wrapm(zz, D)) * H + wrapm(y0 + fr - 1, H)) * Wd +
-                                 wrapm(x0 + lx - 1, Wd);
-                pf[i] = *(const bf16x8 *)(X + v * ldx + xoff + 8 * part);
-            }
-        }
+    for (int i = 0; i < ZQ_FPT; i++) {
+        const int c = min(tid + 512 * i, ZP_FCH - 1);
+        const int s = c >> 2, part = c & 3;
+        const int fr = s / (2 * ZP_FX), rem = s - fr * 2 * ZP_FX;
+        const int par = rem / ZP_FX, idx = rem - par * ZP_FX;
+        const int lx = 2 * idx + par;
+        fo[i] = (unsigned)((wrapm(y0 + fr - 1, H) * Wd + wrapm(x0 + lx - 1, Wd)) * ldx + xoff + 8 * part);
+    }
+#pragma unroll
+    for (int i = 0; i < ZQ_CPT; i++) {
+        const int c = min(tid + 512 * i, ZP_CCH - 1);
+        const int s = c >> 3, part = c & 7;
+        const int cr = s / ZP_CX, cx = s - cr * ZP_CX;
+        co[i] = (unsigned)((wrapm(y0 / 2 + cr - 1, Hc) * Wc + wrapm(x0 / 2 + cx - 1, Wc)) * ldc + 8 * part);
+    }
+    auto load_fine = [&](int zz) {
+        const bf16 *pl = X + ((size_t)b * D + wrapm(zz, D)) * H * (size_t)Wd * ldx;
+#pragma unroll
+        for (int i = 0; i < ZQ_FPT; i++)
+            pf[i] = *(const bf16x8 *)(pl + fo[i]);
     };
     auto store_fine = [&](int slot) {
 #pragma unroll
-        for (int i = 0; i < ZP_FPT; i++) {
-            const int c = tid + 256 * i;
+        for (int i = 0; i < ZQ_FPT; i++) {
+            const int c = tid + 512 * i;
             if (c < ZP_FCH)
                 *(bf16x8 *)(fsh + ((size_t)slot * ZP_FSITES + (c >> 2)) * ZP_FPAD + 8 * (c & 3)) = pf[i];
         }
     };
     auto load_coarse = [&](int zz) {
+        const bf16 *pl = Xc + ((size_t)b * Dc + wrapm(zz, Dc)) * Hc * (size_t)Wc * ldc;
 #pragma unroll
-        for (int i = 0; i < ZP_CPT; i++) {
-            const int c = min(tid + 256 * i, ZP_CCH - 1);
-            {
-                const int s = c >> 3, part = c & 7;
-                const int cr = s / ZP_CX, cx = s - cr * ZP_CX;
-                const size_t v = (((size_t)b * Dc + wrapm(zz, Dc)) * Hc + wrapm(y0 / 2 + cr - 1, Hc)) *
-                                     Wc + wrapm(x0 / 2 + cx - 1, Wc);
-                pc[i] = *(const bf16x8 *)(Xc + v * ldc + 8 * part);
-            }
-        }
+        for (int i = 0; i < ZQ_CPT; i++)
+            pc[i] = *(const bf16x8 *)(pl + co[i]);
     };
     auto store_coarse = [&](int slot) {
 #pragma unroll
-        for (int i = 0; i < ZP_CPT; i++) {
-            const int c = tid + 256 * i;
+        for (int i = 0; i < ZQ_CPT; i++) {
+            const int c = tid + 512 * i;
             if (c < ZP_CCH)
-                *(bf16x8 *)(csh + ((size_t)slot * ZP_CSITES + (c >> 3)) * ZP_CPAD + 8 * (c & 7)) = pc[i];
+                *(bf16x8 *)(csh + ((size_t)slot * ZP_CSITES + (c >> 3)) * ZQ_CPAD + 8 * (c & 7)) = pc[i];
         }
     };
-    // fine planes z0-1, z0 staged and z0+1 in registers; coarse planes
-    // Z0-1, Z0 staged (coarse plane P lives in slot (P - Z0 + 1) % 3)
+    // fine plane P in slot (P - z0 + 1) % 3, coarse plane P in (P - Z0 + 1) & 1
     load_fine(z0 - 1);
     store_fine(0);
     load_fine(z0);
@@ -983,121 +999,215 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     load_coarse(Z0);
     store_coarse(1);
     load_fine(z0 + 1);
-    // a plane's summed weights (32 fragments, parity pz), loaded one plane
-    // ahead, after that plane's output stores: vmcnt completes in issue
-    // order, so loads issued after the plane prefetch would make their
-    // first use wait for the prefetch as well
-    bf16x8 wpl[2][4][2][2];
-    auto load_wpl = [&](int pzz) {
-#pragma unroll
-        for (int k = 0; k < 2; k++)
-#pragma unroll
-            for (int grp = 0; grp < 4; grp++)
-#pragma unroll
-                for (int py = 0; py < 2; py++)
-#pragma unroll
-                    for (int oy = 0; oy < 2; oy++)
-                        wpl[k][grp][py][oy] = __builtin_bit_cast(
-                            bf16x8,
-                            __builtin_amdgcn_raw_buffer_load_b128(
-                                wpr, wpo,
-                                (((4 * pzz + 2 * py + px) * 8 + 4 * (grp >> 1) + 2 * oy + (grp & 1)) *
-                                     COUT * CUP + 32 * k) * 2,
-                                0));
+    // skip-channel weight fragment of tap t: Wt[t][16 nt + xl][64 + 8q ..]
+    auto wskip = [&](int t) {
+        return *(const bf16x8 *)(Wt + ((size_t)t * COUT + 16 * nt + xl) * CIN + CUP + 8 * q);
     };
-    load_wpl(z0 & 1);
-    for (int z = z0; z < zend; z++) {
-        const int pz = z & 1;
-        __syncthreads();  // every read of the previous plane is done
-        store_fine((z - z0 + 2) % 3);  // fine plane z+1
-        if (pz)
-            store_coarse(((z >> 1) + 1 - Z0 + 1) % 3);  // coarse plane (z >> 1) + 1
-        __syncthreads();
-        // fine plane z+2, and the coarse plane the next odd plane stores
-        // ((z >> 1) + 1 for even z; odd z loads the plane the following even
-        // one loads again); past the chunk's end these are wrapped planes,
-        // read and not used
-        load_fine(z + 2);
-        load_coarse(((z + 1) >> 1) + 1);
-        // keep the prefetch ahead of the MFMAs (the scheduler would sink the
-        // loads to their stores, and the next plane would wait for them)
-        __builtin_amdgcn_sched_barrier(0);
-        f32x4 acc[ZP_TY];
+    // fragment reads of one group: fine rows of plane slot fs at offset dx,
+    // or coarse rows of slot cs at offset ox, chunk kc
+    auto ld_fine = [&](bf16x8(&buf)[ZP_FROWS], int fs, int dx) {
+        const int par = (px + dx) & 1, idx = xl + ((px + dx) >> 1);
+        const bf16 *fb = fsh + ((size_t)fs * ZP_FSITES + par * ZP_FX + idx) * ZP_FPAD + 8 * q;
 #pragma unroll
-        for (int rr = 0; rr < ZP_TY; rr++)
-            acc[rr] = bias0;
-        // 17 groups in k_conv_poly's order: 8 of the upsampled channels
-        // ((chunk, oz, ox): oy, rows), then 9 of the skip channels ((dz, dx):
-        // dy, rows).  Two register buffers: each group's LDS reads are issued
-        // before the previous group's MFMAs (scheduling fences keep them there).
-        const int fsl[3] = {(z - z0) % 3, (z - z0 + 1) % 3, (z - z0 + 2) % 3};
-        auto ldg = [&](bf16x8(&buf)[ZP_FROWS], int g) {
-            if (g < 8) {
-                const int kc = 32 * (g >> 2), oz = (g >> 1) & 1, ox = g & 1;
-                const int cs = ((z >> 1) + pz - 1 + oz - Z0 + 1) % 3;
-                const bf16 *cb = csh + ((size_t)cs * ZP_CSITES + xl + px + ox) * ZP_CPAD + kc + 8 * q;
+        for (int rr = 0; rr < ZP_FROWS; rr++)
+            buf[rr] = *(const bf16x8 *)(fb + rr * 2 * ZP_FX * ZP_FPAD);
+    };
+    auto ld_coarse = [&](bf16x8(&buf)[ZP_FROWS], int cs, int ox, int kc) {
+        const bf16 *cb = csh + ((size_t)cs * ZP_CSITES + xl + px + ox) * ZQ_CPAD + kc + 8 * q;
 #pragma unroll
-                for (int cr = 0; cr < ZP_CROWS; cr++)
-                    buf[cr] = *(const bf16x8 *)(cb + cr * ZP_CX * ZP_CPAD);
-            } else {
-                const int dz = (g - 8) / 3, dx = (g - 8) % 3;
-                // lane's site: local x 2 xl + px + dx, stored at parity
-                // (px + dx) & 1, index xl + ((px + dx) >> 1)
-                const int par = (px + dx) & 1, idx = xl + ((px + dx) >> 1);
-                const bf16 *fb = fsh + ((size_t)fsl[dz] * ZP_FSITES + par * ZP_FX + idx) * ZP_FPAD + 8 * q;
+        for (int cr = 0; cr < ZP_CROWS; cr++)
+            buf[cr] = *(const bf16x8 *)(cb + cr * ZP_CX * ZQ_CPAD);
+    };
+    f32x4 *const redp = red + (px + 2 * nt) * ZP_TY * 64 + lane;
+    f32x4 acc[ZP_TY];
+    if (role == 0) {
+        // ---- up waves ----
+        bf16x8 ws[6];  // taps (dz, dy, dx) = (2, dy, 0), (2, dy, 1)
 #pragma unroll
-                for (int rr = 0; rr < ZP_FROWS; rr++)
-                    buf[rr] = *(const bf16x8 *)(fb + rr * 2 * ZP_FX * ZP_FPAD);
-            }
-        };
-        auto mmg = [&](const bf16x8(&buf)[ZP_FROWS], int g) {
-            if (g < 8) {
-                const int k = g >> 2, grp = g & 3;
+        for (int dx = 0; dx < 2; dx++)
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++)
+                ws[3 * dx + dy] = wskip(18 + 3 * dy + dx);
+        const __amdgpu_buffer_rsrc_t wpr = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)Wp, (short)0, 64 * COUT * CUP * 2, 0x00020000);
+        const int wpo = ((16 * nt + xl) * 32 + 8 * q) * 2;
+        // summed weights of group u (chunk u >> 2, oz = (u >> 1) & 1, ox = u & 1)
+        // for plane parity pzz, ring slot u & 3 (layout: k_prep_wpoly)
+        bf16x8 wq[4][2][2];
+        auto load_wq = [&](int pzz, int u) {
+#pragma unroll
+            for (int py = 0; py < 2; py++)
 #pragma unroll
                 for (int oy = 0; oy < 2; oy++)
+                    wq[u & 3][py][oy] = __builtin_bit_cast(
+                        bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
+                                    wpr, wpo,
+                                    ((((4 * pzz + 2 * py + px) * 8 + 4 * ((u >> 1) & 1) + 2 * oy + (u & 1)) *
+                                          (CUP / 32) + (u >> 2)) * COUT * 32) * 2,
+                                    0));
+        };
 #pragma unroll
-                    for (int rr = 0; rr < ZP_TY; rr++)
-                        acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            wpl[k][grp][rr & 1][oy], buf[(rr >> 1) + (rr & 1) + oy], acc[rr], 0, 0, 0);
-            } else {
-                const int dz = (g - 8) / 3, dx = (g - 8) % 3;
+        for (int u = 0; u < 4; u++)
+            load_wq(z0 & 1, u);
 #pragma unroll
-                for (int dy = 0; dy < 3; dy++)
+        for (int i = 0; i < 6; i++)
+            settle(ws[i]);
+        for (int z = z0; z < zend; z++) {
+            const int pz = z & 1;
+            __syncthreads();  // every read of the previous plane is done
+            store_fine((z - z0 + 2) % 3);
+            if (pz)
+                store_coarse(((z >> 1) + 1 - Z0 + 1) & 1);
+            __syncthreads();
+            // (this wave's share of the next planes is loaded after its last
+            // weight load: a use waits for every load issued before it)
 #pragma unroll
-                    for (int rr = 0; rr < ZP_TY; rr++)
-                        acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            ws[dz * 9 + dy * 3 + dx], buf[rr + dy], acc[rr], 0, 0, 0);
+            for (int rr = 0; rr < ZP_TY; rr++)
+                acc[rr] = f32x4{0.f, 0.f, 0.f, 0.f};
+            auto ldg = [&](bf16x8(&buf)[ZP_FROWS], int g) {
+                if (g < 8) {
+                    const int oz = (g >> 1) & 1;
+                    ld_coarse(buf, ((z >> 1) + pz - 1 + oz - Z0 + 1) & 1, g & 1, 32 * (g >> 2));
+                } else {
+                    ld_fine(buf, (z - z0 + 2) % 3, g - 8);
+                }
+            };
+            auto mmg = [&](const bf16x8(&buf)[ZP_FROWS], int g) {
+                if (g < 8) {
+#pragma unroll
+                    for (int oy = 0; oy < 2; oy++)
+#pragma unroll
+                        for (int rr = 0; rr < ZP_TY; rr++)
+                            acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                wq[g & 3][rr & 1][oy], buf[(rr >> 1) + (rr & 1) + oy], acc[rr], 0, 0, 0);
+                } else {
+#pragma unroll
+                    for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+                        for (int rr = 0; rr < ZP_TY; rr++)
+                            acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                ws[3 * (g - 8) + dy], buf[rr + dy], acc[rr], 0, 0, 0);
+                }
+            };
+            bf16x8 b0[ZP_FROWS], b1[ZP_FROWS];
+            ldg(b0, 0);
+#pragma unroll
+            for (int g = 0; g < 10; g++) {
+                if (g % 2 == 0) {
+                    if (g + 1 < 10)
+                        ldg(b1, g + 1);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mmg(b0, g);
+                } else {
+                    if (g + 1 < 10)
+                        ldg(b0, g + 1);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mmg(b1, g);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                if (g < 4)
+                    load_wq(pz, g + 4);  // into the slot group g has just read
+                if (g == 3) {
+                    load_fine(z + 2);
+                    load_coarse(((z + 1) >> 1) + 1);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int rr = 0; rr < ZP_TY; rr++)
+                redp[rr * 64] = acc[rr];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                load_wq(pz ^ 1, u);  // the next plane's (read and not used after the last)
+        }
+        __syncthreads();
+    } else {
+        // ---- skip waves ----
+        bf16x8 ws[21];  // groups (dz, dx): 0..5 = (dz 0..1, dx 0..2), 6 = (2, 2)
+#pragma unroll
+        for (int g = 0; g < 7; g++)
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++)
+                ws[3 * g + dy] = wskip(g < 6 ? (g / 3) * 9 + 3 * dy + g % 3 : 18 + 3 * dy + 2);
+        const f32x4 bias0 = bias4(bias, 16 * nt + 4 * q);
+        // lane holds channels 16 nt + 4q .. +3 of site (x0 + 2 xl + px, y0 + rr)
+        auto add_partial = [&]() {
+#pragma unroll
+            for (int rr = 0; rr < ZP_TY; rr++)
+                acc[rr] += redp[rr * 64];
+        };
+        auto store_out = [&](int zz) {
+            const int x = x0 + 2 * xl + px;
+            if (x < Wd) {
+#pragma unroll
+                for (int rr = 0; rr < ZP_TY; rr++) {
+                    const int y = y0 + rr;
+                    if (y >= H)
+                        break;
+                    const size_t o = ((((size_t)b * D + zz) * H + y) * Wd + x) * ldy + 16 * nt + 4 * q;
+                    *(bf16x4 *)(Y + o) = leaky4(acc[rr]);
+                }
             }
         };
-        bf16x8 b0[ZP_FROWS], b1[ZP_FROWS];
-        ldg(b0, 0);
+        // the weights and bias have arrived before the march (settle)
 #pragma unroll
-        for (int g = 0; g < 17; g++) {
-            if (g % 2 == 0) {
-                if (g + 1 < 17)
-                    ldg(b1, g + 1);
-                __builtin_amdgcn_sched_barrier(0);
-                mmg(b0, g);
-            } else {
-                ldg(b0, g + 1);
-                __builtin_amdgcn_sched_barrier(0);
-                mmg(b1, g);
+        for (int i = 0; i < 21; i++)
+            settle(ws[i]);
+        settle(bias0);
+        for (int z = z0; z < zend; z++) {
+            const int pz = z & 1;
+            __syncthreads();  // every read of the previous plane is done, its partial written
+            if (z > z0) {
+                add_partial();
+                store_out(z - 1);
             }
+            store_fine((z - z0 + 2) % 3);
+            if (pz)
+                store_coarse(((z >> 1) + 1 - Z0 + 1) & 1);
+            __syncthreads();
+            load_fine(z + 2);
+            load_coarse(((z + 1) >> 1) + 1);
             __builtin_amdgcn_sched_barrier(0);
-        }
-        // lane holds channels 16 nt + 4q .. +3 of site (x0 + 2 xl + px, y0 + rr)
-        const int x = x0 + 2 * xl + px;
-        if (x < Wd) {
 #pragma unroll
-            for (int rr = 0; rr < ZP_TY; rr++) {
-                const int y = y0 + rr;
-                if (y >= H)
-                    break;
-                const size_t o = ((((size_t)b * D + z) * H + y) * Wd + x) * ldy + 16 * nt + 4 * q;
-                *(bf16x4 *)(Y + o) = leaky4(acc[rr]);
+            for (int rr = 0; rr < ZP_TY; rr++)
+                acc[rr] = bias0;
+            // one row buffer, walked by anti-diagonals: step i runs the MFMAs
+            // that read row i (rows rr = i - dy), after which the row is
+            // reloaded with the next group's row i (a group ahead)
+            auto row_ptr = [&](int g) {
+                const int dz = g < 6 ? g / 3 : 2, dx = g < 6 ? g % 3 : 2;
+                const int par = (px + dx) & 1, idx = xl + ((px + dx) >> 1);
+                return fsh + ((size_t)((z - z0 + dz) % 3) * ZP_FSITES + par * ZP_FX + idx) * ZP_FPAD + 8 * q;
+            };
+            bf16x8 buf[ZP_FROWS];
+            {
+                const bf16 *fb = row_ptr(0);
+#pragma unroll
+                for (int i = 0; i < ZP_FROWS; i++)
+                    buf[i] = *(const bf16x8 *)(fb + i * 2 * ZP_FX * ZP_FPAD);
+            }
+#pragma unroll
+            for (int g = 0; g < 7; g++) {
+                const bf16 *fn = row_ptr(g + 1 < 7 ? g + 1 : g);
+#pragma unroll
+                for (int i = 0; i < ZP_FROWS; i++) {
+#pragma unroll
+                    for (int dy = 0; dy < 3; dy++) {
+                        const int rr = i - dy;
+                        if (rr >= 0 && rr < ZP_TY)
+                            acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ws[3 * g + dy], buf[i],
+                                                                              acc[rr], 0, 0, 0);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (g + 1 < 7)
+                        buf[i] = *(const bf16x8 *)(fn + i * 2 * ZP_FX * ZP_FPAD);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
             }
         }
-        load_wpl(pz ^ 1);  // the next plane's (read and not used after the last)
+        __syncthreads();
+        add_partial();
+        store_out(zend - 1);
     }
 }
 
@@ -1344,17 +1454,19 @@ __global__ __launch_bounds__(256) void k_prep_w0(const float *__restrict__ w, in
 }
 
 // k_conv_poly's weights for the first cup (upsampled) input channels of a
-// layer: Wp[par][grp][co][ci], par = 4 pz + 2 py + px, grp = 4 oz + 2 oy +
-// ox, is the sum of the FP32 weights w[tap][co][ci] over the taps (dz, dy,
-// dx) with floor((p + d) / 2) = p - 1 + o in every dimension, rounded to
-// BF16 once.
+// layer: Wp[par][grp][ci / 32][co][ci % 32], par = 4 pz + 2 py + px, grp =
+// 4 oz + 2 oy + ox, is the sum of the FP32 weights w[tap][co][ci] over the
+// taps (dz, dy, dx) with floor((p + d) / 2) = p - 1 + o in every dimension,
+// rounded to BF16 once.  An MFMA fragment (16 output channels x 32 input
+// channels) is one contiguous kilobyte: its load touches eight whole cache
+// lines instead of halves of sixteen.
 __global__ __launch_bounds__(256) void k_prep_wpoly(const float *__restrict__ w, int Cout,
                                                     int CinLog, int cup, bf16 *__restrict__ Wp)
 {
     const size_t n = (size_t)64 * Cout * cup;
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        const int ci = (int)(i % cup);
-        const int co = (int)((i / cup) % Cout);
+        const int ci = (int)((i / ((size_t)32 * Cout)) % (cup / 32)) * 32 + (int)(i % 32);
+        const int co = (int)((i / 32) % Cout);
         const int grp = (int)((i / ((size_t)cup * Cout)) % 8);
         const int par = (int)(i / ((size_t)8 * cup * Cout));
         const int p[3] = {(par >> 2) & 1, (par >> 1) & 1, par & 1};
@@ -1820,7 +1932,7 @@ struct ppf_unet {
     float *p, *g, *m, *v;   // FP32 parameters, gradient, Adam moments
     bf16 *wb, *wt;          // BF16 weights and their flipped transposes
     bf16 *w0;               // first layer's weights for k_conv0, [5][Cout][32]
-    bf16 *wpoly;            // k_conv_poly's weights of L6 and L8, [8][8][Cout][cup] each
+    bf16 *wpoly;            // k_conv_poly's weights of L6 and L8, [8][8][cup/32][Cout][32] each
     size_t wp_off6, wp_off8;
     int64_t t;              // Adam steps
     // workspace for one (batch, grid) shape
@@ -1987,12 +2099,13 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
     const size_t total = (size_t)B * D * H * W;
     if (n->conv_kind == 6 && Xc && Wp && mode == 0 && xoff == 0 && cin == 96 && ldx == 96 &&
         cup == 64 && ldc == 64 && coutw == 32 && cout == 32 && ldy % 4 == 0 && yoff == 0 &&
-        D % 2 == 0 && H % 2 == 0 && W % 2 == 0) {
-        // L8 of the base-32 network: the z-marching polyphase kernel (the
-        // same sums as k_conv_poly), z chunks of 16 planes (even)
+        D % 2 == 0 && H % 2 == 0 && W % 2 == 0 && (size_t)H * W * ldx < (1u << 31)) {
+        // L8 of the base-32 network: the eight-wave z-marching polyphase
+        // kernel (k_conv_poly's products; in-plane offsets in 32 bits), z
+        // chunks of 16 planes (even)
         const int zc = D < 16 ? D : 16, ntx = (W + ZP_TX - 1) / ZP_TX, nty = (H + ZP_TY - 1) / ZP_TY;
         const unsigned nblk = (unsigned)((size_t)B * ((D + zc - 1) / zc) * nty * ntx);
-        hipLaunchKernelGGL(k_conv_zpoly, dim3(nblk), dim3(256), 0, n->stream, X, ldx, cup, Wt, Wp,
+        hipLaunchKernelGGL(k_conv_zpoly8, dim3(nblk), dim3(512), 0, n->stream, X, ldx, cup, Wt, Wp,
                            bias, Yb, ldy, B, D, H, W, Xc, ldc, zc, ntx, nty);
         return;
     }

@@ -74,12 +74,20 @@ def test_fused_upsampling_is_bitwise_the_materialized_one(shape):
 
 
 @pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 24, 12, 40), (1, 8, 36, 20), (1, 36, 20, 68)])
-def test_zmarching_polyphase_is_bitwise_the_row_blocked_one(shape):
-    """Kind 6 runs the last decoder convolution (L8) with the z-marching
-    polyphase kernel (both sources staged in LDS), kind 7 with the
-    row-blocked polyphase kernel: same products, same order, bitwise equal.
-    The shapes cover tiles larger than the grid, ragged x and y (not
-    multiples of the 32 x 8 tile) and z chunks shorter than 16 planes."""
+def test_zmarching_polyphase_matches_the_row_blocked_one(shape):
+    """Kind 6 runs the last decoder convolution (L8) with the eight-wave
+    z-marching polyphase kernel (both sources staged in LDS, the K dimension
+    split over two waves and their partial sums added), kind 7 with the
+    row-blocked polyphase kernel: the same products, summed in another
+    association, so L8's outputs differ at most by FP32 rounding before
+    their BF16 rounding: now and then a BF16 output lands one unit apart,
+    which L9 and the head spread over their neighbourhoods.  The network
+    outputs agree to 5e-3 of the head's scale at worst and 5e-5 on average,
+    and most of them exactly (a wrong or missing product group would move
+    every output by a sizeable fraction of the scale; agreement with the
+    float64 reference is checked for every kind below).  The shapes cover tiles larger
+    than the grid, ragged x and y (not multiples of the 32 x 8 tile) and z
+    chunks shorter than 16 planes."""
     p = _params(11)
     x, _ = _data(12, shape)
     with UNet(seed=3) as net:
@@ -88,7 +96,14 @@ def test_zmarching_polyphase_is_bitwise_the_row_blocked_one(shape):
         a = net.forward(x)
         net.set_conv_kernel(7)
         b = net.forward(x)
-    np.testing.assert_array_equal(a, b)
+    scale = np.abs(b - x).max()
+    d = np.abs(a - b)
+    same = float(np.mean(d == 0))
+    print("kind 6 vs 7: max |d| %.3e, mean %.3e, scale %.3e, exactly equal %.5f"
+          % (d.max(), d.mean(), scale, same))
+    assert d.max() <= 5e-3 * scale
+    assert d.mean() <= 5e-5 * scale
+    assert same >= 0.9
 
 
 @pytest.mark.parametrize("base", [32, 64])
