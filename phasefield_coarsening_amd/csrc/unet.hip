@@ -304,7 +304,9 @@ __global__ __launch_bounds__(256) void k_conv_lds(const bf16 *__restrict__ X, in
 // UP: the first `cup` input channels are the nearest x2 upsampling of Xc
 // ([B][D/2][H/2][W/2][ldc]), read at the coarse site (z/2, y/2, x/2) of each
 // (wrapped) fine neighbour instead of from a materialized copy in X; the
-// values, and so the sums, are the same as with the copy.
+// values, and so the sums, are the same as with the copy.  Input channel
+// c >= cup is then X's element xoff + c - cup of a site (the skip channels
+// alone, or after room left for the upsampled ones).
 template <int R, int NJ, int MODE, int WPE, bool UP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_conv_rows(
     const bf16 *__restrict__ X, int ldx, int xoff, int Cin, const bf16 *__restrict__ Wt, int CoutW,
@@ -370,12 +372,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
         for (int r = 0; r < R + 2; r++)
             roc[r] = (wrapm(y0 + r - 1, H) >> 1) * rowlen_c;
     }
-    // lane's weight row in N tile j: Wt[tap][n0+16j+xl][kc+8q]; CoutW is a
-    // multiple of NJ*16, so every N tile exists
+    // lane's weight row in N tile j: Wt[tap][n0+16j+xl][kc+8q]; MODE 0 (the
+    // forward) reads the fragment-contiguous copy Wt[tap][kc/32][n0+16j+xl][8q]
+    // (k_prep_w).  CoutW is a multiple of NJ*16, so every N tile exists.
+    constexpr bool WCH = MODE == 0;
     const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(Wt + (size_t)n0 * Cin), (short)0, (27 * CoutW - n0) * Cin * 2, 0x00020000);
-    const int wo = (xl * Cin + 8 * q) * 2;
-    const int tapstride = CoutW * Cin, jstride = 16 * Cin;
+        (void *)(Wt + (size_t)n0 * (WCH ? 32 : Cin)), (short)0, (27 * CoutW * Cin - n0 * (WCH ? 32 : Cin)) * 2,
+        0x00020000);
+    const int wo = (xl * (WCH ? 32 : Cin) + 8 * q) * 2;
+    const int tapstride = CoutW * Cin, jstride = 16 * (WCH ? 32 : Cin);
+    // element offset of chunk kc's fragments of a tap (N tile 0)
+    auto wk = [&](int tap, int kc) { return WCH ? tap * tapstride + kc * CoutW : tap * tapstride + kc; };
     // MODE 0: the accumulators start at the bias of the lane's four output
     // channels
     f32x4 acc[R][NJ];
@@ -401,7 +408,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
             for (int r = 0; r < R + 2; r++)
                 a[r] = __builtin_bit_cast(
                     bf16x8,
-                    __builtin_amdgcn_raw_buffer_load_b128(pl[dz], xo[dx], (ro[r] + kc) * 2, 0));
+                    __builtin_amdgcn_raw_buffer_load_b128(pl[dz], xo[dx], (ro[r] + kc - (UP ? cup : 0)) * 2, 0));
         }
 #pragma unroll
         for (int dy = 0; dy < 3; dy++)
@@ -409,7 +416,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
             for (int j = 0; j < NJ; j++)
                 w[dy][j] = __builtin_bit_cast(
                     bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
-                                wr, wo, ((dz * 9 + dy * 3 + dx) * tapstride + j * jstride + kc) * 2, 0));
+                                wr, wo, (wk(dz * 9 + dy * 3 + dx, kc) + j * jstride) * 2, 0));
     };
     auto mm = [&](const bf16x8(&a)[R + 2], const bf16x8(&w)[3][NJ]) {
 #pragma unroll
@@ -497,7 +504,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 // channels is a 2x2x2 convolution of the coarse tensor, whose weights Wp
 // sum the fine taps that land on the same coarse offset (k_prep_wpoly, from
 // the FP32 weights, rounded to BF16 once): 8 taps instead of 27.  The skip
-// channels (cup .. Cin-1 of X) take the ordinary 27 taps.
+// channels (input channels cup .. Cin-1: X's elements xoff .. xoff + Cin -
+// cup - 1 of a site) take the ordinary 27 taps.
 // A wave owns the 16 sites x = 32 xt + 2 l + px (l = lane % 16) of one
 // parity px, R rows (R even, y0 even: row r has py = r & 1) of one z plane,
 // and NJ*16 output channels.  Per 32 upsampled channels and (oz, ox) coarse
@@ -507,7 +515,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 // ----------------------------------------------------------------------------
 template <int R, int NJ, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_conv_poly(
-    const bf16 *__restrict__ X, int ldx, int Cin, const bf16 *__restrict__ Wt, int CoutW,
+    const bf16 *__restrict__ X, int ldx, int xoff, int Cin, const bf16 *__restrict__ Wt, int CoutW,
     const bf16 *__restrict__ Wp, const float *__restrict__ bias, int Cout, bf16 *__restrict__ Yb,
     int ldy, int B, int D, int H, int Wd, int nnb, int nxt2, int nyt, const bf16 *__restrict__ Xc,
     int ldc, int cup)
@@ -568,12 +576,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 #pragma unroll
     for (int cr = 0; cr < RC; cr++)
         roc[cr] = wrapm(y0 / 2 - 1 + cr, Hc) * rowlen_c;
-    // weights: Wt[tap][n0+16j+xl][kc+8q] (skip), Wp[par][grp][kc/32][n0+16j+xl][8q]
-    // (upsampled, par = 4 pz + 2 py + px, grp = 4 oz + 2 oy + ox)
+    // weights, fragment-contiguous: Wt[tap][kc/32][n0+16j+xl][8q] (skip,
+    // k_prep_w), Wp[par][grp][kc/32][n0+16j+xl][8q] (upsampled, par = 4 pz +
+    // 2 py + px, grp = 4 oz + 2 oy + ox)
     const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(Wt + (size_t)n0 * Cin), (short)0, (27 * CoutW - n0) * Cin * 2, 0x00020000);
-    const int wo = (xl * Cin + 8 * q) * 2;
-    const int tapstride = CoutW * Cin, jstride = 16 * Cin;
+        (void *)(Wt + (size_t)n0 * 32), (short)0, (27 * CoutW * Cin - n0 * 32) * 2, 0x00020000);
+    const int wo = (xl * 32 + 8 * q) * 2;
+    const int tapstride = CoutW * Cin, jstride = 16 * 32;
     const __amdgpu_buffer_rsrc_t wpr = __builtin_amdgcn_make_buffer_rsrc(
         (void *)Wp, (short)0, 64 * CoutW * cup * 2, 0x00020000);
     const int wpo = (xl * 32 + 8 * q) * 2;
@@ -657,14 +666,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
             for (int r = 0; r < R + 2; r++)
                 a[r] = __builtin_bit_cast(
                     bf16x8,
-                    __builtin_amdgcn_raw_buffer_load_b128(pl[dz], xo[dx], (ro[r] + kc) * 2, 0));
+                    __builtin_amdgcn_raw_buffer_load_b128(pl[dz], xo[dx], (ro[r] + xoff + kc - cup) * 2, 0));
 #pragma unroll
             for (int dy = 0; dy < 3; dy++)
 #pragma unroll
                 for (int j = 0; j < NJ; j++)
                     w[dy][j] = __builtin_bit_cast(
                         bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
-                                    wr, wo, ((dz * 9 + dy * 3 + dx) * tapstride + j * jstride + kc) * 2,
+                                    wr, wo,
+                                    ((dz * 9 + dy * 3 + dx) * tapstride + kc * CoutW + j * jstride) * 2,
                                     0));
         };
         auto mm = [&](const bf16x8(&a)[R + 2], const bf16x8(&w)[3][NJ]) {
@@ -1860,7 +1870,7 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float
 // channel-transposed copy for the input gradient, [27][CinPad][Cout].
 __global__ __launch_bounds__(256) void k_prep_w(const float *__restrict__ w, int Cout, int CinLog,
                                                 int CinPad, bf16 *__restrict__ wb,
-                                                bf16 *__restrict__ wt)
+                                                bf16 *__restrict__ wt, bf16 *__restrict__ wc)
 {
     const size_t n = (size_t)27 * Cout * CinPad;
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
@@ -1870,6 +1880,7 @@ __global__ __launch_bounds__(256) void k_prep_w(const float *__restrict__ w, int
         const float val = ci < CinLog ? w[((size_t)tap * Cout + co) * CinLog + ci] : 0.f;
         wb[i] = (bf16)val;
         wt[((size_t)(26 - tap) * CinPad + ci) * Cout + co] = (bf16)val;
+        wc[(((size_t)tap * (CinPad / 32) + ci / 32) * Cout + co) * 32 + ci % 32] = (bf16)val;
     }
 }
 
@@ -1931,6 +1942,7 @@ struct ppf_unet {
     size_t nwb;
     float *p, *g, *m, *v;   // FP32 parameters, gradient, Adam moments
     bf16 *wb, *wt;          // BF16 weights and their flipped transposes
+    bf16 *wc;               // wb in fragment-contiguous order (k_prep_w)
     bf16 *w0;               // first layer's weights for k_conv0, [5][Cout][32]
     bf16 *wpoly;            // k_conv_poly's weights of L6 and L8, [8][8][cup/32][Cout][32] each
     size_t wp_off6, wp_off8;
@@ -2072,7 +2084,7 @@ static int prep_weights(ppf_unet *n)
         const Layer &l = n->L[i];
         hipLaunchKernelGGL(k_prep_w, dim3(grid_for((size_t)27 * l.cout * l.cin)), dim3(256), 0,
                            n->stream, n->p + l.w_off, l.cout, l.cin_log, l.cin, n->wb + l.wb_off,
-                           n->wt + l.wb_off);
+                           n->wt + l.wb_off, n->wc + l.wb_off);
     }
     hipLaunchKernelGGL(k_prep_w0, dim3(grid_for((size_t)C0_CH * n->L[0].cout * 32)), dim3(256), 0,
                        n->stream, n->p + n->L[0].w_off, n->L[0].cout, n->w0);
@@ -2090,14 +2102,21 @@ static int prep_weights(ppf_unet *n)
 }
 
 // Launch one 3x3x3 convolution of layer geometry (cin, cout) on the grid
-// [B][D][H][W].
+// [B][D][H][W].  Input channel c is X's element xoff + c of a site (rows of
+// ldx); with Xc (inference over upsampled channels) channels c < cup come
+// from the coarse tensor Xc and channel c >= cup is X's element
+// xoff + c - cup.
 static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int cin,
                  const bf16 *Wt, int coutw, const float *bias, int cout, bf16 *Yb, float *Yf,
                  int ldy, int yoff, int B, int D, int H, int W, const bf16 *Xc = nullptr,
                  int ldc = 0, int cup = 0, const bf16 *Wp = nullptr)
 {
     const size_t total = (size_t)B * D * H * W;
-    if (n->conv_kind == 6 && Xc && Wp && mode == 0 && xoff == 0 && cin == 96 && ldx == 96 &&
+    // the row-blocked and polyphase kernels read the forward weights in
+    // fragment-contiguous order (mode 0 passes wb's layers)
+    const bf16 *Wr = mode == 0 ? n->wc + (Wt - n->wb) : Wt;
+    if (n->conv_kind == 6 && Xc && Wp && mode == 0 && cin == 96 &&
+        ((ldx == 96 && xoff == 64) || (ldx == 32 && xoff == 0)) &&
         cup == 64 && ldc == 64 && coutw == 32 && cout == 32 && ldy % 4 == 0 && yoff == 0 &&
         D % 2 == 0 && H % 2 == 0 && W % 2 == 0 && (size_t)H * W * ldx < (1u << 31)) {
         // L8 of the base-32 network: the eight-wave z-marching polyphase
@@ -2105,11 +2124,11 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
         // chunks of 16 planes (even)
         const int zc = D < 16 ? D : 16, ntx = (W + ZP_TX - 1) / ZP_TX, nty = (H + ZP_TY - 1) / ZP_TY;
         const unsigned nblk = (unsigned)((size_t)B * ((D + zc - 1) / zc) * nty * ntx);
-        hipLaunchKernelGGL(k_conv_zpoly8, dim3(nblk), dim3(512), 0, n->stream, X, ldx, cup, Wt, Wp,
+        hipLaunchKernelGGL(k_conv_zpoly8, dim3(nblk), dim3(512), 0, n->stream, X, ldx, xoff, Wt, Wp,
                            bias, Yb, ldy, B, D, H, W, Xc, ldc, zc, ntx, nty);
         return;
     }
-    if ((n->conv_kind == 6 || n->conv_kind == 7) && Xc && Wp && mode == 0 && xoff == 0 && coutw == cout &&
+    if ((n->conv_kind == 6 || n->conv_kind == 7) && Xc && Wp && mode == 0 && coutw == cout &&
         coutw % 32 == 0 && ldy % 4 == 0 && yoff == 0 && cup % 32 == 0 && cin % 32 == 0) {
         // decoder layer over an upsampled tensor: polyphase form (8 x 16
         // sites x (32 or 64) channels per wave)
@@ -2119,11 +2138,11 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
         const size_t waves = (size_t)B * D * nyt * nxt2 * nnb;
         const dim3 g3((unsigned)((waves + 3) / 4));
         if (wide)
-            hipLaunchKernelGGL((k_conv_poly<8, 4, 1>), g3, dim3(256), 0, n->stream, X, ldx, cin, Wt,
+            hipLaunchKernelGGL((k_conv_poly<8, 4, 1>), g3, dim3(256), 0, n->stream, X, ldx, xoff, cin, Wr,
                                coutw, Wp, bias, cout, Yb, ldy, B, D, H, W, nnb, nxt2, nyt, Xc, ldc,
                                cup);
         else
-            hipLaunchKernelGGL((k_conv_poly<8, 2, 2>), g3, dim3(256), 0, n->stream, X, ldx, cin, Wt,
+            hipLaunchKernelGGL((k_conv_poly<8, 2, 2>), g3, dim3(256), 0, n->stream, X, ldx, xoff, cin, Wr,
                                coutw, Wp, bias, cout, Yb, ldy, B, D, H, W, nnb, nxt2, nyt, Xc, ldc,
                                cup);
         return;
@@ -2170,11 +2189,11 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
         const dim3 g3((unsigned)((waves + 3) / 4));
 #define ROWS(RR, JJ, MM)                                                                      \
     hipLaunchKernelGGL((k_conv_rows<RR, JJ, MM, (RR * JJ > 16 ? 1 : 2), false>), g3, dim3(256),   \
-                       0, n->stream, X, ldx, xoff, cin, Wt, coutw, bias, cout, Yb, Yf, ldy, yoff, \
+                       0, n->stream, X, ldx, xoff, cin, Wr, coutw, bias, cout, Yb, Yf, ldy, yoff, \
                        B, D, H, W, nnb, nxt, nyt, (const bf16 *)nullptr, 0, 0)
 #define ROWS_UP(RR, JJ)                                                                        \
     hipLaunchKernelGGL((k_conv_rows<RR, JJ, 0, (RR * JJ > 16 ? 1 : 2), true>), g3, dim3(256), 0,  \
-                       n->stream, X, ldx, xoff, cin, Wt, coutw, bias, cout, Yb, Yf, ldy, yoff, B, \
+                       n->stream, X, ldx, xoff, cin, Wr, coutw, bias, cout, Yb, Yf, ldy, yoff, B, \
                        D, H, W, nnb, nxt, nyt, Xc, ldc, cup)
         if (Xc) {  // forward only
             if (R == 8 && NJ == 2)
@@ -2294,29 +2313,32 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
         conv(n, 0, n->X0, 32, 0, L[0].cin, wb(0), L[0].cout, bias(0), L[0].cout, n->E1a, nullptr,
              C, 0, B, D, H, W);
     }
-    conv(n, 0, n->E1a, C, 0, C, wb(1), C, bias(1), C, n->U1, nullptr, 3 * C, 2 * C, B, D, H, W);
+    // inference with the row-blocked kernels reads the upsampled channels
+    // straight from the coarse tensor and the skip channels from their own
+    // contiguous tensor (in U1's and U2's memory); training materializes the
+    // concatenation [upsampled | skip] (the weight gradients read it)
+    const bool fuse_up = !need_x0 && (n->conv_kind >= 2 && n->conv_kind != 5);
+    const int ld1 = fuse_up ? C : 3 * C, off1 = fuse_up ? 0 : 2 * C;
+    const int ld2 = fuse_up ? 2 * C : 6 * C, off2 = fuse_up ? 0 : 4 * C;
+    conv(n, 0, n->E1a, C, 0, C, wb(1), C, bias(1), C, n->U1, nullptr, ld1, off1, B, D, H, W);
     hipLaunchKernelGGL(k_pool_fwd, dim3(row_blocks(B * D2 * H2, W2 * C / 8)), dim3(256), 0,
-                       n->stream, n->U1, 3 * C, 2 * C, C, n->P1, B, D, H, W);
+                       n->stream, n->U1, ld1, off1, C, n->P1, B, D, H, W);
     // level 2
     conv(n, 0, n->P1, C, 0, C, wb(2), 2 * C, bias(2), 2 * C, n->E2a, nullptr, 2 * C, 0, B, D2, H2,
          W2);
-    conv(n, 0, n->E2a, 2 * C, 0, 2 * C, wb(3), 2 * C, bias(3), 2 * C, n->U2, nullptr, 6 * C, 4 * C,
-         B, D2, H2, W2);
+    conv(n, 0, n->E2a, 2 * C, 0, 2 * C, wb(3), 2 * C, bias(3), 2 * C, n->U2, nullptr, ld2, off2, B,
+         D2, H2, W2);
     hipLaunchKernelGGL(k_pool_fwd, dim3(row_blocks(B * D3 * H3, W3 * 2 * C / 8)), dim3(256), 0,
-                       n->stream, n->U2, 6 * C, 4 * C, 2 * C, n->P2, B, D2, H2, W2);
+                       n->stream, n->U2, ld2, off2, 2 * C, n->P2, B, D2, H2, W2);
     // level 3
     conv(n, 0, n->P2, 2 * C, 0, 2 * C, wb(4), 4 * C, bias(4), 4 * C, n->E3a, nullptr, 4 * C, 0, B,
          D3, H3, W3);
     conv(n, 0, n->E3a, 4 * C, 0, 4 * C, wb(5), 4 * C, bias(5), 4 * C, n->E3b, nullptr, 4 * C, 0, B,
          D3, H3, W3);
     // decoder
-    // inference with the row-blocked kernels reads the upsampled channels
-    // straight from the coarse tensor; training materializes them (the
-    // weight gradients read the concatenated input)
-    const bool fuse_up = !need_x0 && (n->conv_kind >= 2 && n->conv_kind != 5);
     const bf16 *wp6 = n->wpoly + n->wp_off6, *wp8 = n->wpoly + n->wp_off8;
     if (fuse_up) {
-        conv(n, 0, n->U2, 6 * C, 0, 6 * C, wb(6), 2 * C, bias(6), 2 * C, n->D2a, nullptr, 2 * C, 0,
+        conv(n, 0, n->U2, ld2, off2, 6 * C, wb(6), 2 * C, bias(6), 2 * C, n->D2a, nullptr, 2 * C, 0,
              B, D2, H2, W2, n->E3b, 4 * C, 4 * C, wp6);
     } else {
         hipLaunchKernelGGL(k_up_fwd, dim3(row_blocks(B * D2 * H2, W2 * 4 * C / 8)), dim3(256), 0,
@@ -2327,7 +2349,7 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
     conv(n, 0, n->D2a, 2 * C, 0, 2 * C, wb(7), 2 * C, bias(7), 2 * C, n->D2b, nullptr, 2 * C, 0, B,
          D2, H2, W2);
     if (fuse_up) {
-        conv(n, 0, n->U1, 3 * C, 0, 3 * C, wb(8), C, bias(8), C, n->D1a, nullptr, C, 0, B, D, H, W,
+        conv(n, 0, n->U1, ld1, off1, 3 * C, wb(8), C, bias(8), C, n->D1a, nullptr, C, 0, B, D, H, W,
              n->D2b, 2 * C, 2 * C, wp8);
     } else {
         hipLaunchKernelGGL(k_up_fwd, dim3(row_blocks(B * D * H, W * 2 * C / 8)), dim3(256), 0,
@@ -2507,7 +2529,7 @@ extern "C" int ppf_unet_destroy(ppf_unet *n)
     if (n->stream)
         hipStreamSynchronize(n->stream);
     free_ws(n);
-    void *bufs[] = {n->p, n->g, n->m, n->v, n->wb, n->wt, n->w0, n->wpoly, n->tmp, n->part};
+    void *bufs[] = {n->p, n->g, n->m, n->v, n->wb, n->wt, n->wc, n->w0, n->wpoly, n->tmp, n->part};
     for (void *b : bufs)
         if (b)
             hipFree(b);
@@ -2608,6 +2630,8 @@ extern "C" int ppf_unet_create(ppf_unet **out, int base, int device, uint64_t se
         e = hipMalloc(&n->wb, woff * sizeof(bf16));
     if (e == hipSuccess)
         e = hipMalloc(&n->wt, woff * sizeof(bf16));
+    if (e == hipSuccess)
+        e = hipMalloc(&n->wc, woff * sizeof(bf16));
     if (e == hipSuccess)
         e = hipMalloc(&n->w0, (size_t)C0_CH * C * 32 * sizeof(bf16));
     if (e == hipSuccess)
