@@ -1339,17 +1339,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 
 // ----------------------------------------------------------------------------
 // First layer (3 input channels) without padding each tap to 32 channels.
-// K runs over the 9 (dz, dy) groups x 16 slots; slots 0..8 are (dx, c) of
-// the 3 taps dx and 3 channels (contiguous in an x row of the staged input),
-// slots 9..15 zero: K = 144 in 5 chunks of 32 (the last half empty), 5 MFMA
-// chunks per site row instead of 27.  A workgroup owns 64 x 8 sites of one
-// z plane (one 16-site column of 8 rows per wave); it stages the 3 x 10 x 66
-// input halo in LDS as BF16 (the same rounding as the padded input), and
-// each lane gathers its 8 K values per chunk from there.  Weights (5 chunks x
-// Cout x 32, prepared by k_prep_w0) stay in registers.  Output as k_conv
-// MODE 0: bf16(leaky(acc + bias)).
+// K runs over the 9 (dz, dy) groups x 16 slots; slot s = 4 dx' + c (dx' =
+// dx + 1 in 0..2, c in 0..2) holds channel c of tap dx, slots with dx' = 3
+// or c = 3 are zero: K = 144 in 5 chunks of 32 (the last half empty), 5 MFMA
+// chunks per site row instead of 27.  A workgroup owns 64 x 8 sites and
+// marches a chunk of z planes (wave w: the 16-site column 16 w .. +15, all 8
+// rows); each input plane (tile + one-site halo, 10 x 66 sites) is staged
+// once in LDS as BF16, four channels per site (s, l, c, 0: 8 bytes, the same
+// rounding as the padded input), for the three output planes that read it.
+// A lane's 8 K values of a chunk are then two adjacent sites (slots 0..7:
+// dx' = 0, 1) or one site and four zeros (slots 8..15: dx' = 2): one or two
+// 8-byte LDS reads.  Weights (5 chunks x Cout x 32, k_prep_w0) stay in
+// registers.  Output bf16(leaky(acc + bias)).
 // ----------------------------------------------------------------------------
 constexpr int C0_TX = 64, C0_R = 8, C0_HX = C0_TX + 2, C0_HY = C0_R + 2, C0_CH = 5;
+constexpr int C0_SITES = C0_HY * C0_HX;                   // 660 per plane
+constexpr int C0_PT = (C0_SITES + 255) / 256;             // staged sites per thread
+constexpr int C0_ZC = 16;                                 // planes per workgroup
 
 // The network's input: packed FP32 [V][3] (InPacked), or a stepping
 // context's three field arrays of type T (InFields; the value is rounded to
@@ -1373,80 +1379,126 @@ __global__ __launch_bounds__(256) void k_conv0(const Src src, const bf16 *__rest
                                                bf16 *__restrict__ Y, int ldy, int B, int D, int H,
                                                int Wd)
 {
-    __shared__ bf16 sx[3 * C0_HY * C0_HX * 3];
+    __shared__ __attribute__((aligned(16))) bf16 sx[3 * C0_SITES * 4];  // 15,840 bytes
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nbx = (Wd + C0_TX - 1) / C0_TX, nby = (H + C0_R - 1) / C0_R;
+    const int nzc = (D + C0_ZC - 1) / C0_ZC;
     int r = blockIdx.x;
     const int bx = r % nbx;
     r /= nbx;
     const int by = r % nby;
     r /= nby;
-    const int z = r % D, b = r / D;
-    const int x0 = bx * C0_TX, y0 = by * C0_R;
-    for (int i = tid; i < 3 * C0_HY * C0_HX; i += 256) {
-        const int hx = i % C0_HX, u = i / C0_HX, hy = u % C0_HY, hz = u / C0_HY;
-        const size_t v = (((size_t)b * D + wrapm(z + hz - 1, D)) * H + wrapm(y0 + hy - 1, H)) * Wd +
-                         wrapm(x0 + hx - 1, Wd);
-        sx[i * 3 + 0] = (bf16)src.get(v, 0);
-        sx[i * 3 + 1] = (bf16)src.get(v, 1);
-        sx[i * 3 + 2] = (bf16)src.get(v, 2);
+    const int zci = r % nzc, b = r / nzc;
+    const int x0 = bx * C0_TX, y0 = by * C0_R, z0 = zci * C0_ZC, zend = min(z0 + C0_ZC, D);
+    // staging: site i = tid + 256 k (row i / C0_HX, x i % C0_HX); threads
+    // past the plane re-read its last site and do not store
+    unsigned so[C0_PT];
+#pragma unroll
+    for (int k = 0; k < C0_PT; k++) {
+        const int i = min(tid + 256 * k, C0_SITES - 1);
+        const int hy = i / C0_HX, hx = i - hy * C0_HX;
+        so[k] = (unsigned)(wrapm(y0 + hy - 1, H) * Wd + wrapm(x0 + hx - 1, Wd));
     }
+    const size_t plane = (size_t)H * Wd;
+    float pv[C0_PT][3];
+    auto load = [&](int zz) {
+        const size_t base = ((size_t)b * D + wrapm(zz, D)) * plane;
+#pragma unroll
+        for (int k = 0; k < C0_PT; k++)
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++)
+                pv[k][ch] = src.get(base + so[k], ch);
+    };
+    auto store = [&](int slot) {
+#pragma unroll
+        for (int k = 0; k < C0_PT; k++) {
+            const int i = tid + 256 * k;
+            if (i < C0_SITES) {
+                bf16x4 v;
+                v[0] = (bf16)pv[k][0];
+                v[1] = (bf16)pv[k][1];
+                v[2] = (bf16)pv[k][2];
+                v[3] = (bf16)0.0f;
+                *(bf16x4 *)(sx + ((size_t)slot * C0_SITES + i) * 4) = v;
+            }
+        }
+    };
     const int m = lane & 15, q = lane >> 4;
     bf16x8 w[C0_CH][NJ];
+#pragma unroll
     for (int c = 0; c < C0_CH; c++)
+#pragma unroll
         for (int j = 0; j < NJ; j++)
             w[c][j] = *(const bf16x8 *)(W0 + ((size_t)c * Cout + 16 * j + m) * 32 + 8 * q);
-    __syncthreads();
-    const int sxl = wave * 16 + m;  // staged x of the site's dx = -1 neighbour
-    // the accumulators start at the bias of the lane's channels 16j+4q .. +3
-    f32x4 acc[C0_R][NJ];
+    f32x4 bj[NJ];
+#pragma unroll
     for (int j = 0; j < NJ; j++) {
         const int c0 = 16 * j + 4 * q;
-        const f32x4 bj = c0 < Cout ? bias4(bias, c0) : f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int row = 0; row < C0_R; row++)
-            acc[row][j] = bj;
+        bj[j] = c0 < Cout ? bias4(bias, c0) : f32x4{0.f, 0.f, 0.f, 0.f};
     }
-    for (int row = 0; row < C0_R; row++) {
+    // plane P in slot (P - z0 + 1) % 3
+    load(z0 - 1);
+    store(0);
+    load(z0);
+    store(1);
+    load(z0 + 1);
+    const int sxl = wave * 16 + m;  // staged x of the site's dx = -1 neighbour
+    const int xx = x0 + sxl;
+    for (int z = z0; z < zend; z++) {
+        __syncthreads();  // every read of the previous plane is done
+        store((z - z0 + 2) % 3);
+        __syncthreads();
+        load(z + 2);  // the next plane's (past the chunk: read and not used)
+        f32x4 acc[C0_R][NJ];
+#pragma unroll
+        for (int row = 0; row < C0_R; row++)
+#pragma unroll
+            for (int j = 0; j < NJ; j++)
+                acc[row][j] = bj[j];
 #pragma unroll
         for (int c = 0; c < C0_CH; c++) {
             const int g = 2 * c + (q >> 1);  // (dz, dy) group of this lane's 8 K values
-            bf16x8 a;
-            for (int e = 0; e < 8; e++)
-                a[e] = (bf16)0.0f;
-            if (g < 9) {
-                const bf16 *pp = sx + (((g / 3) * C0_HY + row + g % 3) * C0_HX + sxl) * 3;
-                if ((q & 1) == 0) {
-                    for (int e = 0; e < 8; e++)
-                        a[e] = pp[e];
-                } else {
-                    a[0] = pp[8];
+            const int gg = min(g, 8);
+            const int dz = gg / 3, dy = gg - 3 * (gg / 3);
+            const bf16 *pp = sx + (((size_t)((z - z0 + dz) % 3) * C0_HY + dy) * C0_HX + sxl + 2 * (q & 1)) * 4;
+#pragma unroll
+            for (int row = 0; row < C0_R; row++) {
+                const bf16 *pr = pp + row * C0_HX * 4;
+                // (odd q reads its one site twice: the site after it may lie
+                // past the staged planes)
+                bf16x4 lo = *(const bf16x4 *)pr, hi = *(const bf16x4 *)(pr + 4 * (1 - (q & 1)));
+                if (q & 1)
+                    hi = bf16x4{(bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f};
+                if (g >= 9)
+                    lo = hi = bf16x4{(bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f};
+                const bf16x8 a = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+#pragma unroll
+                for (int j = 0; j < NJ; j++)
+                    acc[row][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[c][j], a, acc[row][j], 0, 0, 0);
+            }
+        }
+        // lane holds channels 16j+4q .. +3 of site (x0+16w+m, y0+row) in acc[row][j]
+        if (xx < Wd) {
+#pragma unroll
+            for (int row = 0; row < C0_R; row++) {
+                const int y = y0 + row;
+                if (y >= H)
+                    break;
+                const size_t vox = (((size_t)b * D + z) * H + y) * Wd + xx;
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    const int c0 = 16 * j + 4 * q;
+                    if (c0 < Cout)
+                        *(bf16x4 *)(Y + vox * ldy + c0) = leaky4(acc[row][j]);
                 }
             }
-#pragma unroll
-            for (int j = 0; j < NJ; j++)
-                acc[row][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[c][j], a, acc[row][j], 0, 0, 0);
-        }
-    }
-    // lane holds channels 16j+4q .. +3 of site (x0+16w+m, y0+row) in acc[row][j]
-    const int xx = x0 + wave * 16 + m;
-    if (xx >= Wd)
-        return;
-    for (int row = 0; row < C0_R; row++) {
-        const int y = y0 + row;
-        if (y >= H)
-            break;
-        const size_t vox = (((size_t)b * D + z) * H + y) * Wd + xx;
-        for (int j = 0; j < NJ; j++) {
-            const int c0 = 16 * j + 4 * q;
-            if (c0 < Cout)
-                *(bf16x4 *)(Y + vox * ldy + c0) = leaky4(acc[row][j]);
         }
     }
 }
 
 // k_conv0's weights: W0[c][co][kk] = w[tap][co][ci] for K index 32c + kk =
-// 16 g + s with g = (dz+1)*3 + (dy+1) < 9 and s = (dx+1)*3 + ci < 9, else 0;
-// w is the first layer's flat [27][Cout][3].
+// 16 g + s with g = (dz+1)*3 + (dy+1) < 9 and s = 4 (dx+1) + ci, dx+1 < 3,
+// ci < 3; else 0.  w is the first layer's flat [27][Cout][3].
 __global__ __launch_bounds__(256) void k_prep_w0(const float *__restrict__ w, int Cout,
                                                  bf16 *__restrict__ W0)
 {
@@ -1455,8 +1507,8 @@ __global__ __launch_bounds__(256) void k_prep_w0(const float *__restrict__ w, in
         const int kk = i % 32, co = (i / 32) % Cout, c = i / (32 * Cout);
         const int k = 32 * c + kk, g = k / 16, s = k % 16;
         float v = 0.f;
-        if (g < 9 && s < 9) {
-            const int tap = g * 3 + s / 3, ci = s % 3;
+        if (g < 9 && s / 4 < 3 && s % 4 < 3) {
+            const int tap = g * 3 + s / 4, ci = s % 4;
             v = w[((size_t)tap * Cout + co) * 3 + ci];
         }
         W0[i] = (bf16)v;
@@ -2277,8 +2329,8 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
     auto bias = [&](int i) { return n->p + L[i].b_off; };
     // level 1
     if (n->conv_kind >= 2) {
-        const unsigned nb0 = (unsigned)((size_t)B * D * ((H + C0_R - 1) / C0_R) *
-                                        ((W + C0_TX - 1) / C0_TX));
+        const unsigned nb0 = (unsigned)((size_t)B * ((D + C0_ZC - 1) / C0_ZC) *
+                                        ((H + C0_R - 1) / C0_R) * ((W + C0_TX - 1) / C0_TX));
 #define CONV0(NJ, SRC)                                                                        \
     hipLaunchKernelGGL((k_conv0<NJ, decltype(SRC)>), dim3(nb0), dim3(256), 0, n->stream, SRC,     \
                        n->w0, bias(0), C, n->E1a, C, B, D, H, W)
