@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "ppf_host.h"
@@ -883,6 +884,293 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 4,
                     *(bf16x4 *)(Y + o + 16 * j) = leaky4(acc[rr][j]);
             }
         }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Input-plane-stationary z march for Cin = Cout = 32 (the level-1 layers),
+// forward (MODE 0).  A workgroup owns a 16 (x) x 4R (y) tile and marches a
+// chunk of z planes; wave w computes rows R (w & 3) .. +R-1 and the channel
+// block w >> 2, with its 27 weight fragments in registers (two waves per
+// SIMD).  Each staged input plane p is read from LDS once and feeds the three
+// output planes that need it: p + 1 (taps dz = -1), p (dz = 0) and p - 1
+// (dz = +1).  Their accumulators (three sets of R rows) rotate with the
+// plane; output p - 1 is complete after plane p.  Per input plane a wave runs
+// 3 (dx) x (R + 2) LDS row reads and 27 R MFMAs, 9 R / (R + 2) MFMAs per read
+// where k_conv_z32 runs 3 R / (R + 2).  Each accumulator still receives its
+// products in the order (dz, dx, dy) from the bias, so the sums are bitwise
+// those of k_conv_z32 and k_conv_rows.  Two LDS plane buffers, sites padded
+// to 48 bf16 (96 bytes: conflict-free fragment reads), one barrier per plane.
+// The host picks z chunks of 3k + 1 planes (k >= 1) that tile D: the zc + 2
+// input planes of a chunk are whole triples, and the unrolled triple rotates
+// the accumulator sets without copies.  In-plane offsets in 32 bits (H W ldx
+// < 2^31).
+//
+// The epilogue policy EPI either stores bf16(leaky(acc)) (EpiStore), or
+// (inference of the last layer, FULL tiles) applies the network's 1x1x1 head
+// in place of k_head_fwd / k_head_ctx: the output plane's bf16 activations go
+// to an LDS plane (double-buffered), and after the next plane's barrier 256
+// threads, one per site, run the head in k_head_fwd's operation order, so the
+// outputs are bitwise k_head_fwd's, without the activation tensor's round
+// trip through HBM.
+// ----------------------------------------------------------------------------
+constexpr int ZS_PAD = LDS_PAD32, ZS_TX = 16, ZS_HX = ZS_TX + 2, ZS_R = 4;
+
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+// y = x + head(F) into a packed [V][3] output (k_head_fwd).  get() reads a
+// site's input (a plane ahead of its use), put() writes its output.
+struct HeadPacked {
+    typedef float T;
+    const float *hw, *hb, *x;
+    float *out;
+    __device__ void get(size_t v, T (&in)[3]) const
+    {
+        in[0] = x[v * 3 + 0];
+        in[1] = x[v * 3 + 1];
+        in[2] = x[v * 3 + 2];
+    }
+    __device__ void put(size_t v, const T (&in)[3], float a0, float a1, float a2) const
+    {
+        out[v * 3 + 0] = in[0] + a0;
+        out[v * 3 + 1] = in[1] + a1;
+        out[v * 3 + 2] = in[2] + a2;
+    }
+};
+// a context's fields += head(F) (k_head_ctx)
+template <typename TF>
+struct HeadCtx {
+    typedef TF T;
+    const float *hw, *hb;
+    T *s, *l, *c;
+    __device__ void get(size_t v, T (&in)[3]) const
+    {
+        in[0] = s[v];
+        in[1] = l[v];
+        in[2] = c[v];
+    }
+    __device__ void put(size_t v, const T (&in)[3], float a0, float a1, float a2) const
+    {
+        s[v] = (T)((float)in[0] + a0);
+        l[v] = (T)((float)in[1] + a1);
+        c[v] = (T)((float)in[2] + a2);
+    }
+};
+struct EpiStore {
+    typedef float T;
+    const float *hw, *hb;  // unused
+};
+
+template <int R, bool FULL, class EPI>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_zs32(
+    const bf16 *__restrict__ X, int ldx, int xoff, const bf16 *__restrict__ Wt,
+    const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy, int yoff, int B, int D, int H,
+    int Wd, int zc, int ntx, int nty, const EPI epi)
+{
+    constexpr bool HEAD = !std::is_same<EPI, EpiStore>::value;
+    constexpr int TY = 4 * R, HY = TY + 2, SITES = ZS_HX * HY, CHUNKS = 4 * SITES;
+    constexpr int PT = (CHUNKS + 511) / 512;  // staged 16-byte chunks per thread
+    constexpr int HSITES = HEAD ? ZS_TX * TY : 1;
+    __shared__ __attribute__((aligned(16))) bf16 zsh[2 * SITES * ZS_PAD];  // 62,208 bytes at R = 4
+    __shared__ __attribute__((aligned(16))) bf16 hsh[2 * HSITES * ZS_PAD];  // 49,152 bytes (head)
+    static_assert(!HEAD || (FULL && ZS_TX * TY <= 512), "the head epilogue runs on full tiles");
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int rg = wave & 3, jb = wave >> 2;
+    int r = blockIdx.x;
+    const int tx = r % ntx;
+    r /= ntx;
+    const int ty = r % nty;
+    r /= nty;
+    const int nzc = D / zc;
+    const int zci = r % nzc, b = r / nzc;
+    const int x0 = tx * ZS_TX, y0 = ty * TY, z0 = zci * zc, zend = z0 + zc;
+    const int xl = lane & 15, q = lane >> 4;
+    // weights: w[tap] = Wt[tap][16 jb + xl][8q .. 8q+7]
+    bf16x8 w[27];
+#pragma unroll
+    for (int tap = 0; tap < 27; tap++)
+        w[tap] = *(const bf16x8 *)(Wt + ((size_t)tap * 32 + 16 * jb + xl) * 32 + 8 * q);
+    const f32x4 bias0 = bias4(bias, 16 * jb + 4 * q);
+    // staging: chunk c -> site c / 4 (row-major HY x 18), part c % 4; every
+    // thread issues the same loads (chunks past the end re-read the last one)
+    unsigned fo[PT];
+#pragma unroll
+    for (int i = 0; i < PT; i++) {
+        const int c = min(tid + 512 * i, CHUNKS - 1);
+        const int s = c >> 2, part = c & 3;
+        const int hy = s / ZS_HX, hx = s - hy * ZS_HX;
+        fo[i] = (unsigned)((wrapm(y0 + hy - 1, H) * Wd + wrapm(x0 + hx - 1, Wd)) * ldx + xoff + 8 * part);
+    }
+    bf16x8 pf[PT];
+    auto load_plane = [&](int zz) {
+        const bf16 *pl = X + ((size_t)b * D + wrapm(zz, D)) * H * (size_t)Wd * ldx;
+#pragma unroll
+        for (int i = 0; i < PT; i++)
+            pf[i] = *(const bf16x8 *)(pl + fo[i]);
+    };
+    // (the chunks past the end rewrite the last one with its own value: no
+    // branch, so that the compiler can count every thread's memory
+    // operations in its waits)
+    auto store_plane = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < PT; i++) {
+            const int c = min(tid + 512 * i, CHUNKS - 1);
+            *(bf16x8 *)(zsh + ((size_t)buf * SITES + (c >> 2)) * ZS_PAD + 8 * (c & 3)) = pf[i];
+        }
+    };
+    // lane holds channels 16 jb + 4q .. +3 of site (x0 + xl, y0 + R rg + rr).
+    // FULL (the tiles cover the plane exactly): every store is issued, so
+    // the waits before the next plane's staging leave them in flight; vmcnt
+    // counts stores too, and a store the compiler cannot count makes it wait
+    // for all of them.
+    auto store_out = [&](const f32x4(&S)[R], int zz, int k) {
+        if (HEAD) {
+            bf16 *hb = hsh + (size_t)(k & 1) * HSITES * ZS_PAD + 16 * jb + 4 * q;
+#pragma unroll
+            for (int rr = 0; rr < R; rr++)
+                *(bf16x4 *)(hb + ((R * rg + rr) * ZS_TX + xl) * ZS_PAD) = leaky4(S[rr]);
+            return;
+        }
+        const int x = x0 + xl;
+        if (FULL || x < Wd) {
+#pragma unroll
+            for (int rr = 0; rr < R; rr++) {
+                const int y = y0 + R * rg + rr;
+                if (!FULL && y >= H)
+                    break;
+                const size_t o = ((((size_t)b * D + zz) * H + y) * Wd + x) * ldy + yoff + 16 * jb + 4 * q;
+                *(bf16x4 *)(Y + o) = leaky4(S[rr]);
+            }
+        }
+    };
+    // the head of output plane zz from LDS plane buffer hb_k (threads 0 ..
+    // HSITES-1, one site each; k_head_fwd's order: bias, then the 32
+    // channels, products and sums rounded separately)
+    typename EPI::T hin[3];
+    auto head_site = [&](int zz) {
+        return (((size_t)b * D + zz) * H + y0 + tid / ZS_TX) * Wd + x0 + tid % ZS_TX;
+    };
+    auto head_load = [&](int zz) {
+        if constexpr (HEAD) {
+            if (tid < HSITES)
+                epi.get(head_site(zz), hin);
+        }
+    };
+    auto head_pass = [&](int zz, int hb_k) {
+        if constexpr (HEAD) {
+            if (tid >= HSITES)
+                return;
+            const bf16 *hp = hsh + ((size_t)(hb_k & 1) * HSITES + tid) * ZS_PAD;
+            float a0 = epi.hb[0], a1 = epi.hb[1], a2 = epi.hb[2];
+#pragma unroll
+            for (int c8 = 0; c8 < 32; c8 += 8) {
+                const bf16x8 f8 = *(const bf16x8 *)(hp + c8);
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const float f = (float)f8[e];
+                    a0 += epi.hw[c8 + e] * f;
+                    a1 += epi.hw[32 + c8 + e] * f;
+                    a2 += epi.hw[64 + c8 + e] * f;
+                }
+            }
+            epi.put(head_site(zz), hin, a0, a1, a2);
+        }
+    };
+    const int lofs = ((R * rg) * ZS_HX + xl) * ZS_PAD + 8 * q;
+    // One input plane (k: its index in the chunk, buffer k & 1).  MSK bit 2:
+    // output p + 1 (Sp), bit 1: output p (S0), bit 0: output p - 1 (Sm).
+    // The 3 (R + 2) row reads (dx, row) run PD ahead of their MFMAs in a
+    // ring of PD + 1 (one ahead with the head epilogue, whose registers it
+    // needs; the distance made no measurable difference).
+    constexpr int PD = HEAD ? 1 : 2;
+    auto plane = [&](auto MSK, f32x4(&Sp)[R], f32x4(&S0)[R], f32x4(&Sm)[R], int k) {
+        constexpr int M = decltype(MSK)::value, NR = 3 * (R + 2);
+        const bf16 *pb = zsh + (size_t)(k & 1) * SITES * ZS_PAD + lofs;
+        auto rd = [&](int idx) {
+            const int dx = idx / (R + 2), rr = idx % (R + 2);
+            return *(const bf16x8 *)(pb + (rr * ZS_HX + dx) * ZS_PAD);
+        };
+        bf16x8 a[PD + 1];
+#pragma unroll
+        for (int i = 0; i < PD; i++)
+            a[i] = rd(i);
+#pragma unroll
+        for (int idx = 0; idx < NR; idx++) {
+            if (idx + PD < NR)
+                a[(idx + PD) % (PD + 1)] = rd(idx + PD);
+            __builtin_amdgcn_sched_barrier(0);
+            const int dx = idx / (R + 2), rr = idx % (R + 2);
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++) {
+                const int ro = rr - dy;
+                if (ro >= 0 && ro < R) {
+                    const bf16x8 av = a[idx % (PD + 1)];
+                    if (M & 4)
+                        Sp[ro] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[dy * 3 + dx], av, Sp[ro], 0, 0, 0);
+                    if (M & 2)
+                        S0[ro] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[9 + dy * 3 + dx], av, S0[ro], 0, 0, 0);
+                    if (M & 1)
+                        Sm[ro] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[18 + dy * 3 + dx], av, Sm[ro], 0, 0,
+                                                                         0);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+    // Plane k of the chunk (p = z0 - 1 + k): stage plane p + 1 (STG), run
+    // the head of output p - 2 (HP) and load the inputs of output p - 1's,
+    // start loading plane p + 2 (LDG), run plane p, finish output p - 1
+    // (MSK bit 0) and restart its accumulators at the bias for output p + 2.
+    auto step = [&](auto MSK, auto STG, auto LDG, auto HP, f32x4(&Sp)[R], f32x4(&S0)[R],
+                    f32x4(&Sm)[R], int k) {
+        const int p = z0 - 1 + k;
+        __syncthreads();  // plane p is staged; plane p - 1's buffer is no longer read
+        if (decltype(STG)::value)
+            store_plane((k + 1) & 1);
+        if (decltype(HP)::value)  // (between the staging and the next loads: pf is free)
+            head_pass(p - 2, k - 1);
+        if (decltype(HP)::value || k == 2)
+            head_load(p - 1);  // the next head pass's inputs, a plane ahead
+        if (decltype(LDG)::value)
+            load_plane(p + 2);
+        plane(MSK, Sp, S0, Sm, k);
+        if (decltype(MSK)::value & 1)
+            store_out(Sm, p - 1, k);
+#pragma unroll
+        for (int rr = 0; rr < R; rr++)
+            Sm[rr] = bias0;
+    };
+    f32x4 A0[R], A1[R], A2[R];
+#pragma unroll
+    for (int rr = 0; rr < R; rr++)
+        A0[rr] = A1[rr] = A2[rr] = bias0;
+#pragma unroll
+    for (int i = 0; i < 27; i++)
+        settle(w[i]);
+    settle(bias0);
+    load_plane(z0 - 1);
+    store_plane(0);
+    load_plane(z0);
+    // triple t: planes k = 3t, 3t + 1, 3t + 2 with (Sp, S0, Sm) = (A0, A2, A1),
+    // (A1, A0, A2), (A2, A1, A0); the first triple has no output z0 - 1 or
+    // z0 - 2, the last none past zend - 1
+    const int T = (zc + 2) / 3;
+    step(IC<4>{}, IC<1>{}, IC<1>{}, IC<0>{}, A0, A2, A1, 0);
+    step(IC<6>{}, IC<1>{}, IC<1>{}, IC<0>{}, A1, A0, A2, 1);
+    step(IC<7>{}, IC<1>{}, IC<1>{}, IC<0>{}, A2, A1, A0, 2);
+    for (int t = 1; t < T - 1; t++) {
+        step(IC<7>{}, IC<1>{}, IC<1>{}, IC<HEAD>{}, A0, A2, A1, 3 * t);
+        step(IC<7>{}, IC<1>{}, IC<1>{}, IC<HEAD>{}, A1, A0, A2, 3 * t + 1);
+        step(IC<7>{}, IC<1>{}, IC<1>{}, IC<HEAD>{}, A2, A1, A0, 3 * t + 2);
+    }
+    step(IC<7>{}, IC<1>{}, IC<1>{}, IC<HEAD>{}, A0, A2, A1, 3 * (T - 1));
+    step(IC<3>{}, IC<1>{}, IC<0>{}, IC<HEAD>{}, A1, A0, A2, 3 * (T - 1) + 1);
+    step(IC<1>{}, IC<0>{}, IC<0>{}, IC<HEAD>{}, A2, A1, A0, 3 * (T - 1) + 2);
+    if (HEAD) {
+        __syncthreads();
+        head_pass(zend - 1, 3 * (T - 1) + 2);
     }
 }
 
@@ -2158,6 +2446,46 @@ static int prep_weights(ppf_unet *n)
 // ldx); with Xc (inference over upsampled channels) channels c < cup come
 // from the coarse tensor Xc and channel c >= cup is X's element
 // xoff + c - cup.
+// z chunk of the input-plane-stationary kernels: 3k + 1 planes (k >= 1)
+// that tile D.  64 planes (4% faster than 16 at 256^3: fewer chunk edges)
+// while the launch keeps at least 1024 workgroups (four rounds of one per
+// CU), else 16, else any such divisor; 0 if there is none.
+static int zs_chunk(int D, size_t tiles_per_plane)
+{
+    if (D % 64 == 0 && tiles_per_plane * (D / 64) >= 1024)
+        return 64;
+    if (D % 16 == 0)
+        return 16;
+    for (int zc = std::min(D, 64); zc >= 4; zc--)
+        if (zc % 3 == 1 && D % zc == 0)
+            return zc;
+    return 0;
+}
+
+// The input-plane-stationary level-1 kernel with epilogue epi, where D has a
+// chunk of 3k + 1 planes and the in-plane offsets fit 32 bits; the head
+// epilogues need full tiles.  false: not launched.
+template <class EPI>
+static bool launch_zs32(ppf_unet *n, const bf16 *X, int ldx, int xoff, const bf16 *Wt,
+                        const float *bias, bf16 *Yb, int ldy, int yoff, int B, int D, int H, int W,
+                        const EPI &epi)
+{
+    constexpr int R = ZS_R;
+    const int ntx = (W + ZS_TX - 1) / ZS_TX, nty = (H + 4 * R - 1) / (4 * R);
+    const int zs = zs_chunk(D, (size_t)B * ntx * nty);
+    const bool full = W % ZS_TX == 0 && H % (4 * R) == 0;
+    if (!zs || (size_t)H * W * ldx >= (1u << 31) || (!std::is_same<EPI, EpiStore>::value && !full))
+        return false;
+    const unsigned nblk = (unsigned)((size_t)B * (D / zs) * nty * ntx);
+    if (full)
+        hipLaunchKernelGGL((k_conv_zs32<R, true, EPI>), dim3(nblk), dim3(512), 0, n->stream, X, ldx,
+                           xoff, Wt, bias, Yb, ldy, yoff, B, D, H, W, zs, ntx, nty, epi);
+    else if constexpr (std::is_same<EPI, EpiStore>::value)
+        hipLaunchKernelGGL((k_conv_zs32<R, false, EPI>), dim3(nblk), dim3(512), 0, n->stream, X, ldx,
+                           xoff, Wt, bias, Yb, ldy, yoff, B, D, H, W, zs, ntx, nty, epi);
+    return true;
+}
+
 static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int cin,
                  const bf16 *Wt, int coutw, const float *bias, int cout, bf16 *Yb, float *Yf,
                  int ldy, int yoff, int B, int D, int H, int W, const bf16 *Xc = nullptr,
@@ -2217,11 +2545,15 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
     }
     if ((n->conv_kind == 2 || n->conv_kind >= 5) && mode == 0 && !Xc && cin == 32 &&
         coutw == 32 && cout == 32) {
-        // level-1 layers: the z-marching kernel (same sums as k_conv_rows)
-        const int zc = 16, ntx = (W + ZM_TX - 1) / ZM_TX, nty = (H + ZM_TY - 1) / ZM_TY;
-        const unsigned nblk = (unsigned)((size_t)B * ((D + zc - 1) / zc) * nty * ntx);
+        // level-1 layers: the z-marching kernels (same sums as k_conv_rows);
+        // the input-plane-stationary one where D has a chunk of 3k + 1
+        // planes (k >= 1)
+        if (launch_zs32(n, X, ldx, xoff, Wt, bias, Yb, ldy, yoff, B, D, H, W, EpiStore{}))
+            return;
+        const int zc = 16, mtx = (W + ZM_TX - 1) / ZM_TX, mty = (H + ZM_TY - 1) / ZM_TY;
+        const unsigned nblk = (unsigned)((size_t)B * ((D + zc - 1) / zc) * mty * mtx);
         hipLaunchKernelGGL(k_conv_z32<8>, dim3(nblk), dim3(512), 0, n->stream, X, ldx, xoff, Wt,
-                           bias, Yb, ldy, yoff, B, D, H, W, zc, ntx, nty);
+                           bias, Yb, ldy, yoff, B, D, H, W, zc, mtx, mty);
         return;
     }
     // (the row-blocked kernels store 4 channels at once: aligned when the
@@ -2408,6 +2740,25 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
                            n->stream, n->D2b, 2 * C, n->U1, 3 * C, 0, B, D, H, W);
         conv(n, 0, n->U1, 3 * C, 0, 3 * C, wb(8), C, bias(8), C, n->D1a, nullptr, C, 0, B, D, H,
              W);
+    }
+    // the last layer with the head in its epilogue (inference, base 32, the
+    // kinds that run the level-1 layers z-marching), else L9 and the head
+    const float *hw = n->p + n->head_w, *hb = n->p + n->head_b;
+    if (!need_x0 && C == 32 && (n->conv_kind == 2 || n->conv_kind >= 5)) {
+        bool done;
+        if (io && io->f64)
+            done = launch_zs32(n, n->D1a, C, 0, wb(9), bias(9), nullptr, C, 0, B, D, H, W,
+                               HeadCtx<double>{hw, hb, (double *)io->s, (double *)io->l, (double *)io->c});
+        else if (io)
+            done = launch_zs32(n, n->D1a, C, 0, wb(9), bias(9), nullptr, C, 0, B, D, H, W,
+                               HeadCtx<float>{hw, hb, (float *)io->s, (float *)io->l, (float *)io->c});
+        else
+            done = launch_zs32(n, n->D1a, C, 0, wb(9), bias(9), nullptr, C, 0, B, D, H, W,
+                               HeadPacked{hw, hb, n->xin, n->yout});
+        if (done) {
+            UHIP(n, hipGetLastError());
+            return PPF_OK;
+        }
     }
     conv(n, 0, n->D1a, C, 0, C, wb(9), C, bias(9), C, n->D1b, nullptr, C, 0, B, D, H, W);
     if (io && io->f64)
