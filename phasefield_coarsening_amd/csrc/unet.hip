@@ -755,6 +755,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 constexpr int LDS_PAD32 = 48, LDS_PAD64 = 80;
 
 // ----------------------------------------------------------------------------
+// The 1x1x1 head (CB = 32 NK channels -> 3) of 16 sites on the matrix cores:
+// C^T = Hw F^T, the FP32 head weights split into BF16 hi + lo parts (hi =
+// bf16(w), lo = bf16(w - hi): w to 2^-17 relative, the products exact in the
+// FP32 accumulator).  Whd (k_prep_head): [NK][2][16][32] BF16, rows 3..15
+// zero.  f[k]: lane (l % 16, q = l / 16) holds channels 32 k + 8 q .. +7 of
+// site l % 16.  The accumulator starts at the bias; lane (site, q = 0) ends
+// with the site's three outputs in [0..2].  Every head (k_head_fwd,
+// k_head_ctx and the fused epilogue of k_conv_zs32) runs this routine, so
+// their outputs are bitwise equal.
+// ----------------------------------------------------------------------------
+template <int NK>
+__device__ __forceinline__ f32x4 head_mfma(const bf16 *__restrict__ Whd, const float *__restrict__ hb,
+                                           const bf16x8 (&f)[NK], int lane)
+{
+    const int xl = lane & 15, q = lane >> 4;
+    const float b0 = hb[0], b1 = hb[1], b2 = hb[2];
+    f32x4 c = q == 0 ? f32x4{b0, b1, b2, 0.f} : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < NK; k++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                *(const bf16x8 *)(Whd + ((size_t)(2 * k + h) * 16 + xl) * 32 + 8 * q), f[k], c, 0, 0, 0);
+    return c;
+}
+
+// Whd from the FP32 head weights hw [3][CB].
+__global__ __launch_bounds__(256) void k_prep_head(const float *__restrict__ hw, int CB,
+                                                   bf16 *__restrict__ Whd)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;  // (k, h, row, c)
+    if (i >= CB / 32 * 2 * 16 * 32)
+        return;
+    const int c = i & 31, row = (i >> 5) & 15, h = (i >> 9) & 1, k = i >> 10;
+    const float w = row < 3 ? hw[row * CB + 32 * k + c] : 0.f;
+    const bf16 hi = (bf16)w;
+    Whd[i] = h == 0 ? hi : (bf16)(w - (float)hi);
+}
+
+// ----------------------------------------------------------------------------
 // z-marching convolution for Cin = Cout = 32 (the level-1 layers), forward
 // (MODE 0).  A workgroup owns a 16 (x) x 32 (y) tile and marches a chunk of
 // z planes; wave w computes rows 8w .. 8w+7.  Each input plane is staged
@@ -906,13 +946,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 4,
 // the accumulator sets without copies.  In-plane offsets in 32 bits (H W ldx
 // < 2^31).
 //
-// The epilogue policy EPI either stores bf16(leaky(acc)) (EpiStore), or
+// The epilogue policy EPI stores bf16(leaky(acc)) (EpiStore), stores it and
+// its 2x2x2 average pooling (EpiPool, FULL tiles; k_pool_fwd's sums), or
 // (inference of the last layer, FULL tiles) applies the network's 1x1x1 head
 // in place of k_head_fwd / k_head_ctx: the output plane's bf16 activations go
-// to an LDS plane (double-buffered), and after the next plane's barrier 256
-// threads, one per site, run the head in k_head_fwd's operation order, so the
-// outputs are bitwise k_head_fwd's, without the activation tensor's round
-// trip through HBM.
+// to an LDS plane (double-buffered), and after the next plane's barrier each
+// wave runs head_mfma on two of its rows, so the outputs are bitwise those
+// of k_head_fwd, without the activation tensor's round trip through HBM.
 // ----------------------------------------------------------------------------
 constexpr int ZS_PAD = LDS_PAD32, ZS_TX = 16, ZS_HX = ZS_TX + 2, ZS_R = 4;
 
@@ -923,7 +963,8 @@ using IC = std::integral_constant<int, V>;
 // site's input (a plane ahead of its use), put() writes its output.
 struct HeadPacked {
     typedef float T;
-    const float *hw, *hb, *x;
+    const bf16 *whd;
+    const float *hb, *x;
     float *out;
     __device__ void get(size_t v, T (&in)[3]) const
     {
@@ -942,7 +983,8 @@ struct HeadPacked {
 template <typename TF>
 struct HeadCtx {
     typedef TF T;
-    const float *hw, *hb;
+    const bf16 *whd;
+    const float *hb;
     T *s, *l, *c;
     __device__ void get(size_t v, T (&in)[3]) const
     {
@@ -959,7 +1001,12 @@ struct HeadCtx {
 };
 struct EpiStore {
     typedef float T;
-    const float *hw, *hb;  // unused
+};
+// store the output and its 2x2x2 average pooling P [B][D/2][H/2][W/2][32]
+// (k_pool_fwd)
+struct EpiPool {
+    typedef float T;
+    bf16 *P;
 };
 
 template <int R, bool FULL, class EPI>
@@ -968,13 +1015,17 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy, int yoff, int B, int D, int H,
     int Wd, int zc, int ntx, int nty, const EPI epi)
 {
-    constexpr bool HEAD = !std::is_same<EPI, EpiStore>::value;
+    constexpr bool POOL = std::is_same<EPI, EpiPool>::value;
+    constexpr bool HEAD = !std::is_same<EPI, EpiStore>::value && !POOL;
+    static_assert(!POOL || (FULL && R % 2 == 0), "pooling runs on full tiles of row pairs");
     constexpr int TY = 4 * R, HY = TY + 2, SITES = ZS_HX * HY, CHUNKS = 4 * SITES;
     constexpr int PT = (CHUNKS + 511) / 512;  // staged 16-byte chunks per thread
     constexpr int HSITES = HEAD ? ZS_TX * TY : 1;
     __shared__ __attribute__((aligned(16))) bf16 zsh[2 * SITES * ZS_PAD];  // 62,208 bytes at R = 4
     __shared__ __attribute__((aligned(16))) bf16 hsh[2 * HSITES * ZS_PAD];  // 49,152 bytes (head)
-    static_assert(!HEAD || (FULL && ZS_TX * TY <= 512), "the head epilogue runs on full tiles");
+    __shared__ __attribute__((aligned(16))) bf16 whs[HEAD ? 2 * 16 * 32 : 8];  // head weights (Whd)
+    __shared__ float hbs[4];                                                     // head bias
+    static_assert(!HEAD || FULL, "the head epilogue runs on full tiles");
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int rg = wave & 3, jb = wave >> 2;
@@ -1025,6 +1076,48 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     // the waits before the next plane's staging leave them in flight; vmcnt
     // counts stores too, and a store the compiler cannot count makes it wait
     // for all of them.
+    // pooling (EpiPool): the even lane of an x pair sums, for row pair j and
+    // its four channels, the pair's values in k_pool_fwd's order (from 0:
+    // (dz, dy, dx) = 000, 001, 010, 011 on the even plane, kept in pp, then
+    // 100 .. 111 on the odd plane) and stores the average
+    float pp[R / 2][4];
+    auto pool_out = [&](const bf16x4(&o)[R], int zz) {
+        bf16x4 nb[R];  // the x + 1 neighbour's values (lane xl ^ 1)
+#pragma unroll
+        for (int rr = 0; rr < R; rr++) {
+            const uint2 u = __builtin_bit_cast(uint2, o[rr]);
+            uint2 v;
+            v.x = __builtin_amdgcn_mov_dpp(u.x, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+            v.y = __builtin_amdgcn_mov_dpp(u.y, 0xB1, 0xF, 0xF, false);
+            nb[rr] = __builtin_bit_cast(bf16x4, v);
+        }
+        const bool odd = zz & 1;
+#pragma unroll
+        for (int j = 0; j < R / 2; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                float t = odd ? pp[j][e] : 0.f;
+                t += (float)o[2 * j][e];
+                t += (float)nb[2 * j][e];
+                t += (float)o[2 * j + 1][e];
+                t += (float)nb[2 * j + 1][e];
+                pp[j][e] = t;
+            }
+        if constexpr (POOL)
+            if (odd && !(xl & 1)) {
+            const int H2 = H >> 1, W2 = Wd >> 1;
+#pragma unroll
+            for (int j = 0; j < R / 2; j++) {
+                bf16x4 r4;
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    r4[e] = (bf16)(pp[j][e] * 0.125f);
+                const size_t pv = (((size_t)b * (D >> 1) + (zz >> 1)) * H2 + ((y0 + R * rg) >> 1) + j) * W2 +
+                                  ((x0 + xl) >> 1);
+                *(bf16x4 *)(epi.P + pv * 32 + 16 * jb + 4 * q) = r4;
+            }
+        }
+    };
     auto store_out = [&](const f32x4(&S)[R], int zz, int k) {
         if (HEAD) {
             bf16 *hb = hsh + (size_t)(k & 1) * HSITES * ZS_PAD + 16 * jb + 4 * q;
@@ -1034,7 +1127,17 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             return;
         }
         const int x = x0 + xl;
-        if (FULL || x < Wd) {
+        if constexpr (POOL) {
+            bf16x4 o4[R];
+#pragma unroll
+            for (int rr = 0; rr < R; rr++) {
+                const int y = y0 + R * rg + rr;
+                const size_t o = ((((size_t)b * D + zz) * H + y) * Wd + x) * ldy + yoff + 16 * jb + 4 * q;
+                o4[rr] = leaky4(S[rr]);
+                *(bf16x4 *)(Y + o) = o4[rr];
+            }
+            pool_out(o4, zz);
+        } else if (FULL || x < Wd) {
 #pragma unroll
             for (int rr = 0; rr < R; rr++) {
                 const int y = y0 + R * rg + rr;
@@ -1045,37 +1148,36 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             }
         }
     };
-    // the head of output plane zz from LDS plane buffer hb_k (threads 0 ..
-    // HSITES-1, one site each; k_head_fwd's order: bias, then the 32
-    // channels, products and sums rounded separately)
-    typename EPI::T hin[3];
-    auto head_site = [&](int zz) {
-        return (((size_t)b * D + zz) * H + y0 + tid / ZS_TX) * Wd + x0 + tid % ZS_TX;
+    // head pass (HEAD): the output plane's 16 rows of 16 sites, two rows per
+    // wave (wave w: rows 2w, 2w + 1), each by head_mfma on the LDS plane; the
+    // lanes of q = 0 read their two sites' inputs a plane ahead (head_load)
+    // and write the outputs
+    static_assert(!HEAD || TY == 16, "the head pass covers 16 rows with 8 waves");
+    typename EPI::T hin[2][3];
+    auto head_site = [&](int zz, int t) {
+        return (((size_t)b * D + zz) * H + y0 + 2 * wave + t) * Wd + x0 + xl;
     };
     auto head_load = [&](int zz) {
         if constexpr (HEAD) {
-            if (tid < HSITES)
-                epi.get(head_site(zz), hin);
+            if (q == 0)
+#pragma unroll
+                for (int t = 0; t < 2; t++)
+                    epi.get(head_site(zz, t), hin[t]);
         }
     };
     auto head_pass = [&](int zz, int hb_k) {
         if constexpr (HEAD) {
-            if (tid >= HSITES)
-                return;
-            const bf16 *hp = hsh + ((size_t)(hb_k & 1) * HSITES + tid) * ZS_PAD;
-            float a0 = epi.hb[0], a1 = epi.hb[1], a2 = epi.hb[2];
+            const bf16 *hp = hsh + ((size_t)(hb_k & 1) * HSITES + 2 * wave * ZS_TX + xl) * ZS_PAD + 8 * q;
+            f32x4 o[2];
 #pragma unroll
-            for (int c8 = 0; c8 < 32; c8 += 8) {
-                const bf16x8 f8 = *(const bf16x8 *)(hp + c8);
-#pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    const float f = (float)f8[e];
-                    a0 += epi.hw[c8 + e] * f;
-                    a1 += epi.hw[32 + c8 + e] * f;
-                    a2 += epi.hw[64 + c8 + e] * f;
-                }
+            for (int t = 0; t < 2; t++) {
+                const bf16x8 f[1] = {*(const bf16x8 *)(hp + t * ZS_TX * ZS_PAD)};
+                o[t] = head_mfma<1>(whs, hbs, f, lane);
             }
-            epi.put(head_site(zz), hin, a0, a1, a2);
+            if (q == 0)
+#pragma unroll
+                for (int t = 0; t < 2; t++)
+                    epi.put(head_site(zz, t), hin[t], o[t][0], o[t][1], o[t][2]);
         }
     };
     const int lofs = ((R * rg) * ZS_HX + xl) * ZS_PAD + 8 * q;
@@ -1146,6 +1248,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
     for (int rr = 0; rr < R; rr++)
         A0[rr] = A1[rr] = A2[rr] = bias0;
+    if constexpr (HEAD) {
+        // the head weights and bias go to LDS once (published by the first
+        // plane's barrier): a global load in the head pass would wait for the
+        // plane loads and stores in flight (vmcnt counts them all)
+        if (tid < 2 * 16 * 32 / 8)
+            *(bf16x8 *)(whs + 8 * tid) = *(const bf16x8 *)(epi.whd + 8 * tid);
+        if (tid < 3)
+            hbs[tid] = epi.hb[tid];
+    }
 #pragma unroll
     for (int i = 0; i < 27; i++)
         settle(w[i]);
@@ -2008,32 +2119,30 @@ __global__ __launch_bounds__(256) void k_up_fwd(const bf16 *__restrict__ S, int 
 }
 
 // Head straight into a stepping context: the residual input is the
-// context's own fields, and out = x + head replaces them in place (each
-// thread reads and writes its own site only).  The sums are k_head_fwd's.
-template <typename T>
-__global__ __launch_bounds__(256) void k_head_ctx(const bf16 *__restrict__ F, int CB,
-                                                  const float *__restrict__ hw,
+// context's own fields, and out = x + head replaces them in place (each site
+// is read and written by its own lane only).  head_mfma, as k_head_fwd.
+template <typename T, int NK>
+__global__ __launch_bounds__(256) void k_head_ctx(const bf16 *__restrict__ F,
+                                                  const bf16 *__restrict__ Whd,
                                                   const float *__restrict__ hb, size_t total,
                                                   T *__restrict__ s, T *__restrict__ l,
                                                   T *__restrict__ c)
 {
-    const size_t v = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (v >= total)
-        return;
-    float a0 = hb[0], a1 = hb[1], a2 = hb[2];
-    for (int c8 = 0; c8 < CB; c8 += 8) {
-        const bf16x8 f8 = *(const bf16x8 *)(F + v * CB + c8);
-        for (int e = 0; e < 8; e++) {
-            const int ci = c8 + e;
-            const float f = (float)f8[e];
-            a0 += hw[ci] * f;
-            a1 += hw[CB + ci] * f;
-            a2 += hw[2 * CB + ci] * f;
-        }
+    const int lane = threadIdx.x & 63, xl = lane & 15, q = lane >> 4;
+    const size_t v0 = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    if (v0 >= total)
+        return;  // whole wave
+    const size_t v = v0 + xl, vr = v < total ? v : total - 1;
+    bf16x8 f[NK];
+#pragma unroll
+    for (int k = 0; k < NK; k++)
+        f[k] = *(const bf16x8 *)(F + vr * (32 * NK) + 32 * k + 8 * q);
+    const f32x4 o = head_mfma<NK>(Whd, hb, f, lane);
+    if (q == 0 && v < total) {
+        s[v] = (T)((float)s[v] + o[0]);
+        l[v] = (T)((float)l[v] + o[1]);
+        c[v] = (T)((float)c[v] + o[2]);
     }
-    s[v] = (T)((float)s[v] + a0);
-    l[v] = (T)((float)l[v] + a1);
-    c[v] = (T)((float)c[v] + a2);
 }
 
 // k_mass_delta with the target read from the device (the pre-jump sum).
@@ -2063,30 +2172,30 @@ __global__ __launch_bounds__(256) void k_pack_in(const float *__restrict__ x, si
     *(bf16x8 *)(X0 + i * 8) = r;
 }
 
-// Head: out = x + hb + hw F (1x1x1, CB -> 3 channels), summed over ci in order.
-__global__ __launch_bounds__(256) void k_head_fwd(const bf16 *__restrict__ F, int CB,
-                                                  const float *__restrict__ hw,
+// Head: out = x + hb + hw F (1x1x1, 32 NK -> 3 channels) by head_mfma, a
+// wave per 16 sites.
+template <int NK>
+__global__ __launch_bounds__(256) void k_head_fwd(const bf16 *__restrict__ F,
+                                                  const bf16 *__restrict__ Whd,
                                                   const float *__restrict__ hb,
                                                   const float *__restrict__ x, size_t total,
                                                   float *__restrict__ out)
 {
-    const size_t v = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (v >= total)
-        return;
-    float a0 = hb[0], a1 = hb[1], a2 = hb[2];
-    for (int c8 = 0; c8 < CB; c8 += 8) {
-        const bf16x8 f8 = *(const bf16x8 *)(F + v * CB + c8);
-        for (int e = 0; e < 8; e++) {
-            const int ci = c8 + e;
-            const float f = (float)f8[e];
-            a0 += hw[ci] * f;
-            a1 += hw[CB + ci] * f;
-            a2 += hw[2 * CB + ci] * f;
-        }
+    const int lane = threadIdx.x & 63, xl = lane & 15, q = lane >> 4;
+    const size_t v0 = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    if (v0 >= total)
+        return;  // whole wave
+    const size_t v = v0 + xl, vr = v < total ? v : total - 1;
+    bf16x8 f[NK];
+#pragma unroll
+    for (int k = 0; k < NK; k++)
+        f[k] = *(const bf16x8 *)(F + vr * (32 * NK) + 32 * k + 8 * q);
+    const f32x4 o = head_mfma<NK>(Whd, hb, f, lane);
+    if (q == 0 && v < total) {
+        out[v * 3 + 0] = x[v * 3 + 0] + o[0];
+        out[v * 3 + 1] = x[v * 3 + 1] + o[1];
+        out[v * 3 + 2] = x[v * 3 + 2] + o[2];
     }
-    out[v * 3 + 0] = x[v * 3 + 0] + a0;
-    out[v * 3 + 1] = x[v * 3 + 1] + a1;
-    out[v * 3 + 2] = x[v * 3 + 2] + a2;
 }
 
 // gS[child][c] += gP[parent][c] / 8
@@ -2285,6 +2394,7 @@ struct ppf_unet {
     bf16 *wc;               // wb in fragment-contiguous order (k_prep_w)
     bf16 *w0;               // first layer's weights for k_conv0, [5][Cout][32]
     bf16 *wpoly;            // k_conv_poly's weights of L6 and L8, [8][8][cup/32][Cout][32] each
+    bf16 *whd;              // head weights split into BF16 hi + lo (k_prep_head)
     size_t wp_off6, wp_off8;
     int64_t t;              // Adam steps
     // workspace for one (batch, grid) shape
@@ -2428,6 +2538,8 @@ static int prep_weights(ppf_unet *n)
     }
     hipLaunchKernelGGL(k_prep_w0, dim3(grid_for((size_t)C0_CH * n->L[0].cout * 32)), dim3(256), 0,
                        n->stream, n->p + n->L[0].w_off, n->L[0].cout, n->w0);
+    hipLaunchKernelGGL(k_prep_head, dim3(grid_for((size_t)n->base / 32 * 1024)), dim3(256), 0,
+                       n->stream, n->p + n->head_w, n->base, n->whd);
     const int C = n->base;
     const int lay[2] = {6, 8}, cups[2] = {4 * C, 2 * C};
     const size_t offs[2] = {n->wp_off6, n->wp_off8};
@@ -2475,6 +2587,8 @@ static bool launch_zs32(ppf_unet *n, const bf16 *X, int ldx, int xoff, const bf1
     const int zs = zs_chunk(D, (size_t)B * ntx * nty);
     const bool full = W % ZS_TX == 0 && H % (4 * R) == 0;
     if (!zs || (size_t)H * W * ldx >= (1u << 31) || (!std::is_same<EPI, EpiStore>::value && !full))
+        return false;
+    if (std::is_same<EPI, EpiPool>::value && (zs % 2 || D % 2))
         return false;
     const unsigned nblk = (unsigned)((size_t)B * (D / zs) * nty * ntx);
     if (full)
@@ -2704,9 +2818,15 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
     const bool fuse_up = !need_x0 && (n->conv_kind >= 2 && n->conv_kind != 5);
     const int ld1 = fuse_up ? C : 3 * C, off1 = fuse_up ? 0 : 2 * C;
     const int ld2 = fuse_up ? 2 * C : 6 * C, off2 = fuse_up ? 0 : 4 * C;
-    conv(n, 0, n->E1a, C, 0, C, wb(1), C, bias(1), C, n->U1, nullptr, ld1, off1, B, D, H, W);
-    hipLaunchKernelGGL(k_pool_fwd, dim3(row_blocks(B * D2 * H2, W2 * C / 8)), dim3(256), 0,
-                       n->stream, n->U1, ld1, off1, C, n->P1, B, D, H, W);
+    // L1 with the pooling in its epilogue where the level-1 kernel runs (base
+    // 32), else L1 and k_pool_fwd
+    if (!(C == 32 && (n->conv_kind == 2 || n->conv_kind >= 5) &&
+          launch_zs32(n, n->E1a, C, 0, wb(1), bias(1), n->U1, ld1, off1, B, D, H, W,
+                      EpiPool{n->P1}))) {
+        conv(n, 0, n->E1a, C, 0, C, wb(1), C, bias(1), C, n->U1, nullptr, ld1, off1, B, D, H, W);
+        hipLaunchKernelGGL(k_pool_fwd, dim3(row_blocks(B * D2 * H2, W2 * C / 8)), dim3(256), 0,
+                           n->stream, n->U1, ld1, off1, C, n->P1, B, D, H, W);
+    }
     // level 2
     conv(n, 0, n->P1, C, 0, C, wb(2), 2 * C, bias(2), 2 * C, n->E2a, nullptr, 2 * C, 0, B, D2, H2,
          W2);
@@ -2743,35 +2863,43 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
     }
     // the last layer with the head in its epilogue (inference, base 32, the
     // kinds that run the level-1 layers z-marching), else L9 and the head
-    const float *hw = n->p + n->head_w, *hb = n->p + n->head_b;
+    const float *hb = n->p + n->head_b;
     if (!need_x0 && C == 32 && (n->conv_kind == 2 || n->conv_kind >= 5)) {
         bool done;
         if (io && io->f64)
             done = launch_zs32(n, n->D1a, C, 0, wb(9), bias(9), nullptr, C, 0, B, D, H, W,
-                               HeadCtx<double>{hw, hb, (double *)io->s, (double *)io->l, (double *)io->c});
+                               HeadCtx<double>{n->whd, hb, (double *)io->s, (double *)io->l,
+                                               (double *)io->c});
         else if (io)
             done = launch_zs32(n, n->D1a, C, 0, wb(9), bias(9), nullptr, C, 0, B, D, H, W,
-                               HeadCtx<float>{hw, hb, (float *)io->s, (float *)io->l, (float *)io->c});
+                               HeadCtx<float>{n->whd, hb, (float *)io->s, (float *)io->l, (float *)io->c});
         else
             done = launch_zs32(n, n->D1a, C, 0, wb(9), bias(9), nullptr, C, 0, B, D, H, W,
-                               HeadPacked{hw, hb, n->xin, n->yout});
+                               HeadPacked{n->whd, hb, n->xin, n->yout});
         if (done) {
             UHIP(n, hipGetLastError());
             return PPF_OK;
         }
     }
     conv(n, 0, n->D1a, C, 0, C, wb(9), C, bias(9), C, n->D1b, nullptr, C, 0, B, D, H, W);
-    if (io && io->f64)
-        hipLaunchKernelGGL(k_head_ctx<double>, dim3(blocks_for(V1)), dim3(256), 0, n->stream,
-                           n->D1b, C, n->p + n->head_w, n->p + n->head_b, V1, (double *)io->s,
-                           (double *)io->l, (double *)io->c);
-    else if (io)
-        hipLaunchKernelGGL(k_head_ctx<float>, dim3(blocks_for(V1)), dim3(256), 0, n->stream,
-                           n->D1b, C, n->p + n->head_w, n->p + n->head_b, V1, (float *)io->s,
-                           (float *)io->l, (float *)io->c);
+    const dim3 gh((unsigned)((V1 + 63) / 64));
+#define HEADK(K)                                                                                  \
+    do {                                                                                          \
+        if (io && io->f64)                                                                        \
+            hipLaunchKernelGGL((k_head_ctx<double, K>), gh, dim3(256), 0, n->stream, n->D1b, n->whd, \
+                               hb, V1, (double *)io->s, (double *)io->l, (double *)io->c);         \
+        else if (io)                                                                              \
+            hipLaunchKernelGGL((k_head_ctx<float, K>), gh, dim3(256), 0, n->stream, n->D1b, n->whd, \
+                               hb, V1, (float *)io->s, (float *)io->l, (float *)io->c);            \
+        else                                                                                      \
+            hipLaunchKernelGGL((k_head_fwd<K>), gh, dim3(256), 0, n->stream, n->D1b, n->whd, hb,   \
+                               n->xin, V1, n->yout);                                              \
+    } while (0)
+    if (C == 32)
+        HEADK(1);
     else
-        hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(V1)), dim3(256), 0, n->stream, n->D1b, C,
-                           n->p + n->head_w, n->p + n->head_b, n->xin, V1, n->yout);
+        HEADK(2);
+#undef HEADK
     UHIP(n, hipGetLastError());
     return PPF_OK;
 }
@@ -2932,7 +3060,7 @@ extern "C" int ppf_unet_destroy(ppf_unet *n)
     if (n->stream)
         hipStreamSynchronize(n->stream);
     free_ws(n);
-    void *bufs[] = {n->p, n->g, n->m, n->v, n->wb, n->wt, n->wc, n->w0, n->wpoly, n->tmp, n->part};
+    void *bufs[] = {n->p, n->g, n->m, n->v, n->wb, n->wt, n->wc, n->w0, n->wpoly, n->whd, n->tmp, n->part};
     for (void *b : bufs)
         if (b)
             hipFree(b);
@@ -3039,6 +3167,8 @@ extern "C" int ppf_unet_create(ppf_unet **out, int base, int device, uint64_t se
         e = hipMalloc(&n->w0, (size_t)C0_CH * C * 32 * sizeof(bf16));
     if (e == hipSuccess)
         e = hipMalloc(&n->wpoly, nwp * sizeof(bf16));
+    if (e == hipSuccess)
+        e = hipMalloc(&n->whd, (size_t)C / 32 * 2 * 16 * 32 * sizeof(bf16));
     if (e == hipSuccess)
         e = hipMalloc(&n->tmp, (2 * 1024 + 8) * sizeof(double));
     if (e == hipSuccess)
