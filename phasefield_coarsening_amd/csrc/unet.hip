@@ -1737,6 +1737,191 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 }
 
 // ----------------------------------------------------------------------------
+// Input-plane-stationary z march for Cin = 32 CK (CK = 1, 2) and Cout = 64,
+// forward (MODE 0): the 128^3-level layers of the base-32 network (L2, L3,
+// L7).  As k_conv_zs32: each staged input plane is read from LDS once for the
+// three output planes that need it (three rotating accumulator sets of 8
+// rows), two plane buffers, one barrier per plane, z chunks of 3k + 1
+// planes, FULL tiles only.  Eight waves, two per SIMD: wave w owns output
+// channels 16 (w & 3) .. +15 with 27 weight fragments in registers, and
+// (R = 4 rows per wave; 8 spill)
+//   CK = 1: rows R (w >> 2) .. +R-1 of a 16 x 2R tile; the sums are bitwise
+//     those of k_conv_z64 and k_conv_rows;
+//   CK = 2: input channels 32 (w >> 2) .. +31 of all R rows of a 16 x R
+//     tile.  The two waves of an output block sum the two halves of K from
+//     the bias and from zero; when an output plane is complete both write
+//     their sums to LDS (double-buffered) and, after the next plane's
+//     barrier, each adds them for half of the rows (bias + first half, plus
+//     second half), applies the activation and stores.  The association differs from k_conv_rows'
+//     (one running sum over both halves) by FP32 rounding.
+// ----------------------------------------------------------------------------
+constexpr int ZS64_R = 4;  // rows per wave (8 spill)
+
+template <int CK>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_zs64(
+    const bf16 *__restrict__ X, int ldx, int xoff, const bf16 *__restrict__ Wt,
+    const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy, int yoff, int B, int D, int H,
+    int Wd, int zc, int ntx, int nty)
+{
+    constexpr int R = ZS64_R, TX = 16, HX = TX + 2, TY = CK == 1 ? 2 * R : R, HY = TY + 2;
+    constexpr int CIN = 32 * CK, PAD = CK == 1 ? LDS_PAD32 : LDS_PAD64, PARTS = CIN / 8;
+    constexpr int SITES = HX * HY, CHUNKS = SITES * PARTS, PT = (CHUNKS + 511) / 512;
+    __shared__ __attribute__((aligned(16))) bf16 zsh[2 * SITES * PAD];  // 62,208 / 57,600 bytes
+    __shared__ f32x4 red[CK == 2 ? 2 * 2 * 4 * R * 64 : 1];              // 65,536 bytes at CK 2
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int jb = wave & 3, g = wave >> 2;
+    const int kc = CK == 2 ? g : 0, rg = CK == 1 ? g : 0;  // K half, row group
+    int r = blockIdx.x;
+    const int tx = r % ntx;
+    r /= ntx;
+    const int ty = r % nty;
+    r /= nty;
+    const int nzc = D / zc;
+    const int zci = r % nzc, b = r / nzc;
+    const int x0 = tx * TX, y0 = ty * TY, z0 = zci * zc;
+    const int xl = lane & 15, q = lane >> 4;
+    bf16x8 w[27];
+#pragma unroll
+    for (int tap = 0; tap < 27; tap++)
+        w[tap] = *(const bf16x8 *)(Wt + ((size_t)tap * 64 + 16 * jb + xl) * CIN + 32 * kc + 8 * q);
+    const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
+    const f32x4 bias0 = bias4(bias, 16 * jb + 4 * q);
+    const f32x4 init = kc == 0 ? bias0 : zero4;
+    unsigned fo[PT];
+#pragma unroll
+    for (int i = 0; i < PT; i++) {
+        const int c = min(tid + 512 * i, CHUNKS - 1);
+        const int s = c / PARTS, part = c % PARTS;
+        const int hy = s / HX, hx = s - hy * HX;
+        fo[i] = (unsigned)((wrapm(y0 + hy - 1, H) * Wd + wrapm(x0 + hx - 1, Wd)) * ldx + xoff + 8 * part);
+    }
+    bf16x8 pf[PT];
+    auto load_plane = [&](int zz) {
+        const bf16 *pl = X + ((size_t)b * D + wrapm(zz, D)) * H * (size_t)Wd * ldx;
+#pragma unroll
+        for (int i = 0; i < PT; i++)
+            pf[i] = *(const bf16x8 *)(pl + fo[i]);
+    };
+    auto store_plane = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < PT; i++) {
+            const int c = min(tid + 512 * i, CHUNKS - 1);
+            *(bf16x8 *)(zsh + ((size_t)buf * SITES + c / PARTS) * PAD + 8 * (c % PARTS)) = pf[i];
+        }
+    };
+    auto out_at = [&](int zz, int row) {
+        return ((((size_t)b * D + zz) * H + y0 + row) * Wd + x0 + xl) * ldy + yoff + 16 * jb + 4 * q;
+    };
+    // output plane zz (accumulators S, plane index k): CK = 1 stores its
+    // rows; CK = 2 writes its half of K to LDS buffer k & 1, and after the
+    // next plane's barrier combine() adds both halves of half of the rows
+    auto finish = [&](const f32x4(&S)[R], int zz, int k) {
+        if constexpr (CK == 1) {
+#pragma unroll
+            for (int rr = 0; rr < R; rr++)
+                *(bf16x4 *)(Y + out_at(zz, R * rg + rr)) = leaky4(S[rr]);
+        } else {
+            f32x4 *const mine = red + (((k & 1) * 8 + g * 4 + jb) * R) * 64 + lane;
+#pragma unroll
+            for (int rr = 0; rr < R; rr++)
+                mine[rr * 64] = S[rr];
+        }
+    };
+    auto combine = [&](int zz, int k) {
+        if constexpr (CK == 2) {
+            const f32x4 *const h0 = red + (((k & 1) * 8 + jb) * R) * 64 + lane;
+            const f32x4 *const h1 = red + (((k & 1) * 8 + 4 + jb) * R) * 64 + lane;
+#pragma unroll
+            for (int i = 0; i < R / 2; i++) {
+                const int rr = (R / 2) * g + i;
+                *(bf16x4 *)(Y + out_at(zz, rr)) = leaky4(h0[rr * 64] + h1[rr * 64]);
+            }
+        }
+    };
+    const int lofs = ((R * rg) * HX + xl) * PAD + 32 * kc + 8 * q;
+    auto plane = [&](auto MSK, f32x4(&Sp)[R], f32x4(&S0)[R], f32x4(&Sm)[R], int k) {
+        constexpr int M = decltype(MSK)::value, NR = 3 * (R + 2);
+        const bf16 *pb = zsh + (size_t)(k & 1) * SITES * PAD + lofs;
+        auto rd = [&](int idx) {
+            const int dx = idx / (R + 2), rr = idx % (R + 2);
+            return *(const bf16x8 *)(pb + (rr * HX + dx) * PAD);
+        };
+        bf16x8 a[3];
+        a[0] = rd(0);
+        a[1] = rd(1);
+#pragma unroll
+        for (int idx = 0; idx < NR; idx++) {
+            if (idx + 2 < NR)
+                a[(idx + 2) % 3] = rd(idx + 2);
+            __builtin_amdgcn_sched_barrier(0);
+            const int dx = idx / (R + 2), rr = idx % (R + 2);
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++) {
+                const int ro = rr - dy;
+                if (ro >= 0 && ro < R) {
+                    const bf16x8 av = a[idx % 3];
+                    if (M & 4)
+                        Sp[ro] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[dy * 3 + dx], av, Sp[ro], 0, 0, 0);
+                    if (M & 2)
+                        S0[ro] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[9 + dy * 3 + dx], av, S0[ro], 0, 0, 0);
+                    if (M & 1)
+                        Sm[ro] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[18 + dy * 3 + dx], av, Sm[ro], 0, 0,
+                                                                         0);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+    // (CB: combine output p - 2, finished by the previous plane)
+    auto step = [&](auto MSK, auto STG, auto LDG, auto CB, f32x4(&Sp)[R], f32x4(&S0)[R], f32x4(&Sm)[R],
+                    int k) {
+        const int p = z0 - 1 + k;
+        __syncthreads();  // plane p is staged; plane p - 1's buffer and the older sums are no longer read
+        if (decltype(CB)::value)
+            combine(p - 2, k - 1);
+        if (decltype(STG)::value)
+            store_plane((k + 1) & 1);
+        if (decltype(LDG)::value)
+            load_plane(p + 2);
+        plane(MSK, Sp, S0, Sm, k);
+        if (decltype(MSK)::value & 1)
+            finish(Sm, p - 1, k);
+#pragma unroll
+        for (int rr = 0; rr < R; rr++)
+            Sm[rr] = init;
+    };
+    f32x4 A0[R], A1[R], A2[R];
+#pragma unroll
+    for (int rr = 0; rr < R; rr++)
+        A0[rr] = A1[rr] = A2[rr] = init;
+#pragma unroll
+    for (int i = 0; i < 27; i++)
+        settle(w[i]);
+    settle(init);
+    load_plane(z0 - 1);
+    store_plane(0);
+    load_plane(z0);
+    constexpr int SPLIT = CK == 2;
+    const int T = (zc + 2) / 3;
+    step(IC<4>{}, IC<1>{}, IC<1>{}, IC<0>{}, A0, A2, A1, 0);
+    step(IC<6>{}, IC<1>{}, IC<1>{}, IC<0>{}, A1, A0, A2, 1);
+    step(IC<7>{}, IC<1>{}, IC<1>{}, IC<0>{}, A2, A1, A0, 2);
+    for (int t = 1; t < T - 1; t++) {
+        step(IC<7>{}, IC<1>{}, IC<1>{}, IC<SPLIT>{}, A0, A2, A1, 3 * t);
+        step(IC<7>{}, IC<1>{}, IC<1>{}, IC<SPLIT>{}, A1, A0, A2, 3 * t + 1);
+        step(IC<7>{}, IC<1>{}, IC<1>{}, IC<SPLIT>{}, A2, A1, A0, 3 * t + 2);
+    }
+    step(IC<7>{}, IC<1>{}, IC<1>{}, IC<SPLIT>{}, A0, A2, A1, 3 * (T - 1));
+    step(IC<3>{}, IC<1>{}, IC<0>{}, IC<SPLIT>{}, A1, A0, A2, 3 * (T - 1) + 1);
+    step(IC<1>{}, IC<0>{}, IC<0>{}, IC<SPLIT>{}, A2, A1, A0, 3 * (T - 1) + 2);
+    if (SPLIT) {
+        __syncthreads();
+        combine(z0 + zc - 1, 3 * (T - 1) + 2);
+    }
+}
+
+// ----------------------------------------------------------------------------
 // First layer (3 input channels) without padding each tap to 32 channels.
 // K runs over the 9 (dz, dy) groups x 16 slots; slot s = 4 dx' + c (dx' =
 // dx + 1 in 0..2, c in 0..2) holds channel c of tap dx, slots with dx' = 3
@@ -2646,6 +2831,24 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
         // Cout 64: the 16 x 16 z-marching kernel (same sums as k_conv_rows);
         // z chunks of 8 planes at most, so that a launch has at least ~4
         // workgroups per CU
+        // the input-plane-stationary kernel: CK = 1 (same sums) where the
+        // level-1 kernel runs, CK = 2 (K split over two waves) with kind 6
+        const int ty = cin == 32 ? 2 * ZS64_R : ZS64_R;
+        if ((cin == 32 || n->conv_kind == 6) && W % 16 == 0 && H % ty == 0 &&
+            (size_t)H * W * ldx < (1u << 31)) {
+            const int sx = W / 16, sy = H / ty;
+            const int zs = zs_chunk(D, (size_t)B * sx * sy);
+            if (zs) {
+                const unsigned nb = (unsigned)((size_t)B * (D / zs) * sy * sx);
+                if (cin == 32)
+                    hipLaunchKernelGGL(k_conv_zs64<1>, dim3(nb), dim3(512), 0, n->stream, X, ldx, xoff, Wt,
+                                       bias, Yb, ldy, yoff, B, D, H, W, zs, sx, sy);
+                else
+                    hipLaunchKernelGGL(k_conv_zs64<2>, dim3(nb), dim3(512), 0, n->stream, X, ldx, xoff, Wt,
+                                       bias, Yb, ldy, yoff, B, D, H, W, zs, sx, sy);
+                return;
+            }
+        }
         const int ntx = (W + Z64_T - 1) / Z64_T, nty = (H + Z64_T - 1) / Z64_T;
         const int zc = D < 8 ? D : 8;
         const unsigned nblk = (unsigned)((size_t)B * ((D + zc - 1) / zc) * nty * ntx);

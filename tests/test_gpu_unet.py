@@ -39,12 +39,16 @@ def _data(seed, shape=(2, 16, 16, 16)):
 KINDS = [2, 3, 4, 5, 6, 7, 1, 0]
 
 
-@pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 24, 12, 40), (1, 8, 36, 20)])
+@pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 24, 12, 40), (1, 8, 36, 20), (1, 32, 32, 32)])
 def test_row_blocked_kinds_agree_bitwise(shape):
     """Kinds 2-5 use different tiles (8 or 4 rows, 32 or 64 channels per
     wave), the z-marching level-1 kernel (2, 5) or not (3, 4), and fused or
     materialized upsampling, but every output sums the same (chunk, dz, dx,
-    dy) products in the same order: their forwards are bitwise equal."""
+    dy) products in the same order: their forwards are bitwise equal.  At
+    32^3 kinds 2 and 5 run the input-plane-stationary kernels (level-1
+    layers, and the 32 -> 64 layer), with the pooling and the head in their
+    epilogues; kinds 3 and 4 run the row-blocked kernels, k_pool_fwd and
+    k_head_fwd."""
     p = _params(5)
     x, _ = _data(6, shape)
     out = {}
@@ -151,6 +155,36 @@ def test_forward_matches_reference(kind, shape):
     assert np.abs(got - unet_ref.predict(p, x, bf16=False)).mean() < 2e-2 * scale
 
 
+def test_forward_32cubed_matches_reference():
+    """At 32^3 every layer of kinds 2, 6 and 7 runs a z-marching kernel where
+    one exists (input-plane-stationary ones at levels 1 and 2, the pooling
+    and the head fused into the level-1 layers).  Kind 6 also runs the 64 ->
+    64 layers L3 and L7 with the K dimension split over two waves (two
+    partial sums), kind 7 with one running sum: a different FP32 association
+    flips a BF16 rounding of L3's output now and then, and the deeper layers
+    spread each flip over their neighbourhoods, so kinds 6 and 7 differ
+    slightly almost everywhere (mean 3.7e-4 of the head's scale 0.42
+    observed).  The test holds each kind to the float64 reference and kind 6
+    to no more than 1.25 times the larger mean error of kinds 2 and 7: a
+    wrong product group in L3 or L7 would move the outputs by a sizeable
+    fraction of the scale."""
+    p = _params(1)
+    x, _ = _data(2, (1, 32, 32, 32))
+    want = unet_ref.predict(p, x, bf16=True)
+    scale = np.abs(want - x).max()
+    err = {}
+    with UNet(seed=3) as net:
+        net.set_params(p)
+        for kind in (2, 6, 7):
+            net.set_conv_kernel(kind)
+            d = np.abs(net.forward(x) - want)
+            err[kind] = d.mean()
+            print("kind %d 32^3: max |d| %.3e, mean |d| %.3e, head scale %.3e"
+                  % (kind, d.max(), d.mean(), scale))
+            assert d.max() < 2e-2 * scale and d.mean() < 3e-3 * scale
+    assert err[6] <= 1.25 * max(err[2], err[7])
+
+
 @pytest.mark.parametrize("kind", [2, 6, 0])
 def test_forward_base64_matches_reference(kind):
     """The 64-channel network (first-layer kernel with four N tiles, 64-wide
@@ -254,13 +288,15 @@ def test_apply_jumps_state_and_conserves_solute():
             assert ms > 0 and flop > 0
 
 
+@pytest.mark.parametrize("shape", [(16, 24, 32), (32, 32, 32)])
 @pytest.mark.parametrize("kind", [2, 6, 0])
-def test_apply_equals_the_host_forward(kind):
+def test_apply_equals_the_host_forward(kind, shape):
     """apply reads and writes the context's fields (row-blocked kernels) or
     goes through packed FP32 copies (the others); either way etaS and etaL
     are bitwise the network's output on the same state (forward from host
-    arrays), and c is that output shifted uniformly onto the old total."""
-    shape = (16, 24, 32)
+    arrays), and c is that output shifted uniformly onto the old total.  At
+    32^3 the head runs in the last layer's epilogue (kinds 2 and 6), on the
+    context's fields or on the packed output."""
     cen, rad = S.spheres(4, 6, shape, 3.0, 6.0)
     for prec in (S.F64, S.F32):
         with S.Solver(shape, precision=prec) as sv, UNet(seed=8) as net:
