@@ -198,7 +198,7 @@ def unet_line(grid=256, reps=3):
             "mfma_bf16_peak_TFLOPs": 2500.0, "mfma_frac": round(tf / 2500.0, 4)}
 
 
-def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium"):
+def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium", no_op=True):
     """Learned jumps against exact stepping at grid^3 (500 spheres,
     R ~ U[6, 12]) to t_end: wall times of the surrogate (K = 200 jumps, 20
     FP32 relax steps, judge, exact re-run on rejection), of plain FP64
@@ -206,15 +206,18 @@ def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium
     errors of the mean particle radius (connected components, over all
     particles) and of the solid fraction at t_end against FP64.  c_init
     "equilibrium" (c = 0.999 - 0.523 etaS, phasefield.c:82-83): the
-    particles coarsen, with the weights trained on that regime; "config"
-    (c = 0.1 + 0.8 etaS): the box solidifies into one cluster, with the
-    weights trained there."""
+    particles' interfaces relax and then settle, with the weights trained on
+    that regime; "config" (c = 0.1 + 0.8 etaS): the box solidifies into one
+    cluster, with the weights trained there.  no_op: also run the same
+    controller with jumps that change nothing (the network's head zeroed) and
+    report its errors: what the trained jumps gain in accuracy at the same
+    cost."""
     import numpy as np
 
     from phasefield_coarsening_amd import surrogate as SG
-    from phasefield_coarsening_amd.unet import UNet
+    from phasefield_coarsening_amd.unet import UNet, layer_shapes
     wname = "unet_k200_b32_eq_f16.npy" if c_init == "equilibrium" else "unet_k200_b32_f16.npy"
-    w = os.path.join(HERE, "phasefield_coarsening_amd", "weights", wname)
+    w = np.load(os.path.join(HERE, "phasefield_coarsening_amd", "weights", wname)).astype(np.float32)
     shape = (grid, grid, grid)
     cen, rad = S.spheres(5, 500, shape, 6.0, 12.0)
     with S.Solver(shape, kernel=S.KERNEL_FAST) as ref:
@@ -236,27 +239,37 @@ def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium
         f32.step(t_end)
         f32.sync()
         t_exact32 = time.perf_counter() - t0
-    with S.Solver(shape, precision=S.F32) as sv, UNet(seed=1) as net:
-        net.set_params(np.load(w).astype(np.float32))
-        sv.upload(*s0)
-        del s0
-        ctl = SG.Controller(sv, net, k=k, relax=relax)
-        summ = ctl.advance(t_end)
-        ps, _ = sv.particles(0.5, max_sizes=1)
-        ss = sv.reduce()
-        ctl.close()
-    f_ref, f_sur = st["sum_h"] / st["n_sites"], ss["sum_h"] / ss["n_sites"]
-    return {"grid": [grid] * 3, "t_end": t_end, "k": k, "relax": relax, "c_init": c_init,
-            "weights": wname, "particles_fp64": pr["n_particles"],
-            "particles_surrogate": ps["n_particles"],
-            "exact_fp64_s": round(t_exact, 4), "exact_fp32_s": round(t_exact32, 4),
-            "surrogate_s": round(summ["seconds"], 4),
-            "speedup_vs_exact_fp64": round(t_exact / summ["seconds"], 3),
-            "speedup_vs_exact_fp32": round(t_exact32 / summ["seconds"], 3),
-            "accepted": summ["accepted"], "rejected": summ["rejected"],
-            "skipped": summ["skipped"],
-            "R_err_rel": abs(ps["mean_radius"] - pr["mean_radius"]) / pr["mean_radius"],
-            "frac_err_rel": abs(f_sur - f_ref) / f_ref}
+    f_ref = st["sum_h"] / st["n_sites"]
+
+    def controlled(params):
+        with S.Solver(shape, precision=S.F32) as sv, UNet(seed=1) as net:
+            net.set_params(params)
+            sv.upload(*s0)
+            ctl = SG.Controller(sv, net, k=k, relax=relax)
+            summ = ctl.advance(t_end)
+            ps, _ = sv.particles(0.5, max_sizes=1)
+            ss = sv.reduce()
+            ctl.close()
+        return (summ, ps, abs(ps["mean_radius"] - pr["mean_radius"]) / pr["mean_radius"],
+                abs(ss["sum_h"] / ss["n_sites"] - f_ref) / f_ref)
+
+    summ, ps, r_err, f_err = controlled(w)
+    out = {"grid": [grid] * 3, "t_end": t_end, "k": k, "relax": relax, "c_init": c_init,
+           "weights": wname, "particles_fp64": pr["n_particles"],
+           "particles_surrogate": ps["n_particles"],
+           "exact_fp64_s": round(t_exact, 4), "exact_fp32_s": round(t_exact32, 4),
+           "surrogate_s": round(summ["seconds"], 4),
+           "speedup_vs_exact_fp64": round(t_exact / summ["seconds"], 3),
+           "speedup_vs_exact_fp32": round(t_exact32 / summ["seconds"], 3),
+           "accepted": summ["accepted"], "rejected": summ["rejected"],
+           "skipped": summ["skipped"], "R_err_rel": r_err, "frac_err_rel": f_err}
+    if no_op:
+        wz = w.copy()
+        nh = sum(int(np.prod(shp)) for name, shp in layer_shapes(32) if name.startswith("head"))
+        wz[-nh:] = 0.0
+        _, _, r0, f0 = controlled(wz)
+        out["no_op_jumps"] = {"R_err_rel": r0, "frac_err_rel": f0}
+    return out
 
 
 def ensemble_line(S, members=256, steps=1000, seed=4):
@@ -419,7 +432,8 @@ def main():
         out["ensemble"] = ensemble_line(S)
         out["unet"] = unet_line()
         out["surrogate"] = surrogate_line(S)
-        out["surrogate_solidifying"] = surrogate_line(S, c_init="config")
+        out["surrogate_long"] = surrogate_line(S, t_end=19800, no_op=False)
+        out["surrogate_solidifying"] = surrogate_line(S, c_init="config", no_op=False)
     if "copy" in side:
         cc = side["copy"]
         out["roofline"]["copy_ceiling_GBs"] = round(cc, 1)

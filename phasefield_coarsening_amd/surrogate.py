@@ -83,14 +83,37 @@ def trajectories(members, steps, every, precision=S.F32, autotune=True, keep=Non
         ens.close()
 
 
-def pairs(snaps, k_snap=1):
-    """Training pairs (x, y) k_snap snapshots apart, grouped by grid shape."""
+class PairSet:
+    """Training pairs of one grid shape, k_snap snapshots apart, indexed into
+    the trajectories without copying them (a 20000-step trajectory of a 64^3
+    member with snapshots every 200 steps holds 318 MB)."""
+
+    def __init__(self, trajs, k_snap, early=(0, 1)):
+        self.trajs = trajs
+        n_early, rep = early
+        self.index = [(m, t) for m, tr in enumerate(trajs) for t in range(len(tr) - k_snap)
+                      for _ in range(rep if t < n_early else 1)]
+        self.k = k_snap
+
+    def __len__(self):
+        return len(self.index)
+
+    def batch(self, idx):
+        """(x, y) arrays [len(idx), D, H, W, 3] of the pairs idx."""
+        sel = [self.index[i] for i in idx]
+        return (np.stack([self.trajs[m][t] for m, t in sel]),
+                np.stack([self.trajs[m][t + self.k] for m, t in sel]))
+
+
+def pairs(snaps, k_snap=1, early=(0, 1)):
+    """Training pairs k_snap snapshots apart, grouped by grid shape:
+    {(D, H, W): PairSet}.  early = (n, rep): the pairs starting at the first
+    n snapshots of each trajectory count rep times (balances a trajectory's
+    transient against its long near-steady tail)."""
     by_shape = {}
     for traj in snaps.values():
-        for t in range(len(traj) - k_snap):
-            by_shape.setdefault(traj.shape[1:4], []).append((traj[t], traj[t + k_snap]))
-    return {s: (np.array([p[0] for p in v]), np.array([p[1] for p in v]))
-            for s, v in by_shape.items()}
+        by_shape.setdefault(traj.shape[1:4], []).append(traj)
+    return {s: PairSet(v, k_snap, early) for s, v in by_shape.items()}
 
 
 def train(net, data, steps, lr=1e-3, batch=4, seed=0, log_every=0):
@@ -99,12 +122,12 @@ def train(net, data, steps, lr=1e-3, batch=4, seed=0, log_every=0):
     shapes = sorted(data)
     losses = []
     for it in range(steps):
-        s = shapes[it % len(shapes)]
-        x, y = data[s]
-        idx = rng.choice(len(x), size=min(batch, len(x)), replace=False)
+        ps = data[shapes[it % len(shapes)]]
+        idx = rng.choice(len(ps), size=min(batch, len(ps)), replace=False)
+        x, y = ps.batch(idx)
         # a cosine decay of the step size over the run
         lr_t = lr * 0.5 * (1.0 + np.cos(np.pi * it / max(steps, 1)))
-        losses.append(net.train_step(x[idx], y[idx], lr_t))
+        losses.append(net.train_step(x, y, lr_t))
         if log_every and (it % log_every == 0 or it == steps - 1):
             print(f"train step {it}: loss {losses[-1]:.3e}", flush=True)
     return losses
@@ -113,9 +136,9 @@ def train(net, data, steps, lr=1e-3, batch=4, seed=0, log_every=0):
 def eval_loss(net, data, batch=4):
     """Mean loss of the network and of 'no change' (persistence) over data."""
     ln, lp, n = 0.0, 0.0, 0
-    for x, y in data.values():
-        for i in range(0, len(x), batch):
-            xb, yb = x[i:i + batch], y[i:i + batch]
+    for ps in data.values():
+        for i in range(0, len(ps), batch):
+            xb, yb = ps.batch(range(i, min(i + batch, len(ps))))
             pred = net.forward(xb)
             ln += float(np.mean((pred - yb) ** 2)) * len(xb)
             lp += float(np.mean((xb - yb) ** 2)) * len(xb)
@@ -133,6 +156,9 @@ class Controller:
     - ``"particles"``: the mean particle radius from connected components
       (``ppf_particles``) and the solid fraction, two labelling passes per
       jump.
+    A jump is also rejected when it moves the radius or the fraction by more
+    than ``max_cycle_change`` itself (a jump that wrecks the state can leave
+    one that relaxation no longer moves).
     Two gates keep network passes away from fast transients, where jumps
     fail the judge:
     - rate: the radius and fraction are measured at every cycle start (one
@@ -214,8 +240,13 @@ class Controller:
             r_j, f_j, bad_j = self._timed("judge", self._measure)
             self._timed("relax", self.sv.step, self.relax)
             r_r, f_r, bad_r = self._timed("judge", self._measure)
+            # the relaxation must not move the jumped state by more than the
+            # tolerances, and the jump itself by no more than the rate gate
+            # lets a whole cycle move (a wrecked state can be stable under
+            # relaxation)
             ok = (bad_j == 0 and bad_r == 0 and r_j > 0 and
-                  abs(r_r - r_j) <= self.tol_r * r_j and abs(f_r - f_j) <= self.tol_f * max(f_j, 1e-12))
+                  abs(r_r - r_j) <= self.tol_r * r_j and abs(f_r - f_j) <= self.tol_f * max(f_j, 1e-12) and
+                  abs(r_j - r0) <= self.max_change * r0 and abs(f_j - f0) <= self.max_change * max(f0, 1e-12))
             if ok:
                 acc += 1
                 wait = 1

@@ -56,6 +56,9 @@ def main(argv=None):
     ap.add_argument("--timing", action="store_true",
                     help="synchronise around each controller phase and report its seconds "
                          "(adds a little time)")
+    ap.add_argument("--early", default="0,1",
+                    help="n,rep: training pairs from the first n snapshots of a trajectory "
+                         "count rep times")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--c-init", default="config", choices=sorted(SG.COMPOSITIONS),
                     help="initial composition of the ensemble and of the surrogate grid: "
@@ -76,14 +79,15 @@ def main(argv=None):
         print(json.dumps({"stage": "ensemble", **ens, "train_members": len(keep)}), flush=True)
         held = {i: v for j, (i, v) in enumerate(sorted(snaps.items())) if j % 8 == 7}
         train_snaps = {i: v for i, v in snaps.items() if i not in held}
-        data, test = SG.pairs(train_snaps), SG.pairs(held)
+        early = tuple(int(v) for v in a.early.split(","))
+        data, test = SG.pairs(train_snaps, early=early), SG.pairs(held)
         t0 = time.perf_counter()
         losses = SG.train(net, data, a.train_steps, lr=a.lr, batch=a.batch, seed=a.seed,
                           log_every=max(1, a.train_steps // 10))
         dt = time.perf_counter() - t0
         print(json.dumps({"stage": "train", "steps": a.train_steps, "seconds": round(dt, 1),
                           "loss_first": losses[0], "loss_last": float(np.mean(losses[-20:])),
-                          "pairs": sum(len(v[0]) for v in data.values())}), flush=True)
+                          "pairs": sum(len(v) for v in data.values())}), flush=True)
         ln, lp = SG.eval_loss(net, test, a.batch)
         print(json.dumps({"stage": "heldout", "mse_net": ln, "mse_persistence": lp,
                           "gain": lp / ln if ln > 0 else None}), flush=True)

@@ -18,10 +18,41 @@ WEIGHTS = os.path.join(_lib.PKG_DIR, "weights", "unet_k200_b32_f16.npy")
 WEIGHTS_EQ = os.path.join(_lib.PKG_DIR, "weights", "unet_k200_b32_eq_f16.npy")
 
 
-def _net(path=WEIGHTS):
+def _net(path=WEIGHTS, no_op=False):
+    """The trained network; no_op: with its head zeroed, so that a jump
+    leaves the state unchanged (the controller then relaxes and judges as
+    usual: the method without the network)."""
     net = UNet(seed=1)
-    net.set_params(np.load(path).astype(np.float32))
+    p = np.load(path).astype(np.float32)
+    if no_op:
+        p[-(3 * 32 + 3):] = 0.0  # head_w [3][32], head_b [3]: the last parameters
+    net.set_params(p)
     return net
+
+
+def _coarsening_run(t_end, no_op=False):
+    """(exact FP64 particles, exact stats, surrogate summary, surrogate
+    particles, surrogate stats, FP64 solute target) of the coarsening
+    workload: 256^3, 500 spheres R ~ U[6, 12], equilibrium composition."""
+    shape = (256, 256, 256)
+    cen, rad = S.spheres(5, 500, shape, 6.0, 12.0)
+    c_liq, c_span = SG.COMPOSITIONS["equilibrium"]
+    with S.Solver(shape, kernel=S.KERNEL_FAST) as ref:
+        ref.init_spheres(cen, rad, c_liq, c_span)
+        s0 = ref.download()
+        ref.step(t_end)
+        pr, _ = ref.particles(0.5, 1)
+        st = ref.reduce()
+    with S.Solver(shape, precision=S.F32) as sv, _net(WEIGHTS_EQ, no_op) as net:
+        sv.upload(*s0)
+        del s0
+        target = sv.mass_target()[0]
+        ctl = SG.Controller(sv, net)
+        summ = ctl.advance(t_end)
+        ps, _ = sv.particles(0.5, 1)
+        ss = sv.reduce()
+        ctl.close()
+    return pr, st, summ, ps, ss, target
 
 
 def test_trained_jump_beats_no_jump_on_fresh_states():
@@ -92,44 +123,56 @@ def test_coarsening_regime_controller_vs_exact_fp64():
     regime.  At t_end the exact run holds hundreds of particles; the
     surrogate's particle count is within 2% of it, its mean particle radius
     (over all particles) and solid fraction within 2%, at least five jumps
-    are accepted, and the solute stays on its FP64 target."""
-    shape = (256, 256, 256)
-    cen, rad = S.spheres(5, 500, shape, 6.0, 12.0)
-    c_liq, c_span = SG.COMPOSITIONS["equilibrium"]
-    with S.Solver(shape, kernel=S.KERNEL_FAST) as ref:
-        ref.init_spheres(cen, rad, c_liq, c_span)
-        s0 = ref.download()
-        ref.step(2200)
-        pr, _ = ref.particles(0.5, 1)
-        st = ref.reduce()
-    with S.Solver(shape, precision=S.F32) as sv, _net(WEIGHTS_EQ) as net:
-        sv.upload(*s0)
-        del s0
-        target = sv.mass_target()[0]
-        ctl = SG.Controller(sv, net)
-        summ = ctl.advance(2200)
-        ps, _ = sv.particles(0.5, 1)
-        ss = sv.reduce()
-        ctl.close()
+    are accepted, and the solute stays on its FP64 target.  The same
+    controller with jumps that change nothing (the network's head zeroed)
+    misses the radius by more: over the first 2200 steps the particles'
+    interfaces relax (the mean radius grows 2.4%), which the jumps carry and
+    20 relaxation steps per cycle do not (observed 0.36% against 1.17%)."""
+    pr, st, summ, ps, ss, target = _coarsening_run(2200)
     print(summ, pr, ps)
     assert pr["n_particles"] > 200
     assert abs(ps["n_particles"] - pr["n_particles"]) <= 0.02 * pr["n_particles"]
     assert summ["accepted"] >= 5
+    r_err = abs(ps["mean_radius"] - pr["mean_radius"]) / pr["mean_radius"]
+    assert r_err < 0.02
+    f_ref, f_sur = st["sum_h"] / st["n_sites"], ss["sum_h"] / ss["n_sites"]
+    assert abs(f_sur - f_ref) / f_ref < 0.02
+    assert abs(ss["sum_c"] - target) / target < 1e-8
+    _, _, summ0, ps0, _, _ = _coarsening_run(2200, no_op=True)
+    r_err0 = abs(ps0["mean_radius"] - pr["mean_radius"]) / pr["mean_radius"]
+    print("mean-radius error: jumps %.4f, no-op jumps %.4f" % (r_err, r_err0))
+    assert summ0["accepted"] >= 5
+    assert r_err < 0.5 * r_err0
+
+
+def test_coarsening_regime_long_horizon():
+    """The coarsening workload to t_end = 19800 (90 cycles): the exact run
+    settles (its mean radius moves 0.9% after step 2200), and the
+    surrogate must not drift away from it: mean radius and solid fraction
+    within 2%, particle count within 3% (observed 0.15%, 0.62%, 1.2%).
+    Weights trained on 4000-step trajectories drifted here (+2.8% radius,
+    +6.9% fraction), which the relaxation judge does not see; the weights
+    trained on the full 20000-step trajectories, their first 4000 steps
+    weighted up, do not."""
+    pr, st, summ, ps, ss, target = _coarsening_run(19800)
+    print(summ, pr, ps)
+    assert summ["accepted"] >= 80
     assert abs(ps["mean_radius"] - pr["mean_radius"]) / pr["mean_radius"] < 0.02
     f_ref, f_sur = st["sum_h"] / st["n_sites"], ss["sum_h"] / ss["n_sites"]
     assert abs(f_sur - f_ref) / f_ref < 0.02
+    assert abs(ps["n_particles"] - pr["n_particles"]) <= 0.03 * pr["n_particles"]
     assert abs(ss["sum_c"] - target) / target < 1e-8
 
 
 def test_coarsening_weights_beat_no_jump_on_fresh_states():
     """Fresh 64^3 members at the equilibrium composition (seeds not used in
-    training): the jump is closer to the exact FP32 trajectory than leaving
-    the state unchanged.  In this regime 200 steps change little, so the
-    margin is small (1.1x on the held-out training members)."""
+    training), their first 600 steps: the jump is much closer to the exact
+    FP32 trajectory than leaving the state unchanged (15.6x lower mean
+    squared error on the held-out training members' whole trajectories)."""
     members = SG.ensemble_members(6, seed=987, edges=(64,))
     snaps, _ = SG.trajectories(members, 600, 200, autotune=False, c_init="equilibrium")
     data = SG.pairs(snaps)
     with _net(WEIGHTS_EQ) as net:
         mse_net, mse_keep = SG.eval_loss(net, data)
     print("mse net %.3e, persistence %.3e, gain %.2f" % (mse_net, mse_keep, mse_keep / mse_net))
-    assert mse_net < mse_keep
+    assert mse_net < mse_keep / 3
