@@ -181,9 +181,14 @@ def reference_serial_baseline(S, shape, nsteps=3):
 
 def unet_line(grid=256, reps=3):
     """U-Net forward (BF16 MFMA) at grid^3: time from HIP events and the
-    MFMA work 2*27*Cin*Cout per site of each 3x3x3 conv."""
+    MFMA work 2*27*Cin*Cout per site of each 3x3x3 conv.  forward_ms: the
+    host-side forward (random input in [0, 1], random weights); apply_ms:
+    ppf_unet_apply on an FP32 context of the surrogate's workload (500
+    spheres, equilibrium composition, trained weights), the surrogate's
+    path."""
     import numpy as np
 
+    from phasefield_coarsening_amd import solver as S
     from phasefield_coarsening_amd.unet import UNet
     with UNet(seed=1) as net:
         x = np.random.default_rng(0).uniform(0, 1, (1, grid, grid, grid, 3)).astype(np.float32)
@@ -193,9 +198,22 @@ def unet_line(grid=256, reps=3):
             net.forward(x)
             ms, flop = net.last_forward()
             best = ms if best is None else min(best, ms)
+    shape = (grid,) * 3
+    cen, rad = S.spheres(5, 500, shape, 6.0, 12.0)
+    w = os.path.join(HERE, "phasefield_coarsening_amd", "weights", "unet_k200_b32_eq_f16.npy")
+    with S.Solver(shape, precision=S.F32) as sv, UNet(seed=1) as net:
+        net.set_params(np.load(w).astype(np.float32))
+        sv.init_spheres(cen, rad, 0.999, -0.523)
+        best_a = None
+        for _ in range(reps + 1):
+            net.apply(sv)
+            sv.sync()
+            ms, _ = net.last_forward()
+            best_a = ms if best_a is None else min(best_a, ms)
     tf = flop / best / 1e9
     return {"grid": [grid] * 3, "forward_ms": round(best, 3), "TFLOPs": round(tf, 1),
-            "mfma_bf16_peak_TFLOPs": 2500.0, "mfma_frac": round(tf / 2500.0, 4)}
+            "mfma_bf16_peak_TFLOPs": 2500.0, "mfma_frac": round(tf / 2500.0, 4),
+            "apply_ms": round(best_a, 3), "apply_TFLOPs": round(flop / best_a / 1e9, 1)}
 
 
 def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium", no_op=True):
