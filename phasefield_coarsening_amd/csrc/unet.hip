@@ -514,12 +514,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 // (r >> 1) + py + oy (oy = 0, 1) with the weights of (py, oy).  Output as
 // k_conv_rows MODE 0.  D, H, W even.
 // ----------------------------------------------------------------------------
-template <int R, int NJ, int WPE>
+// UP_ONLY: the upsampled channels only (bias + their sum), stored as FP32
+// partial sums in Yf [site][ldy] for k_conv_zs64's partial-input mode.
+template <int R, int NJ, int WPE, bool UP_ONLY = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_conv_poly(
     const bf16 *__restrict__ X, int ldx, int xoff, int Cin, const bf16 *__restrict__ Wt, int CoutW,
     const bf16 *__restrict__ Wp, const float *__restrict__ bias, int Cout, bf16 *__restrict__ Yb,
     int ldy, int B, int D, int H, int Wd, int nnb, int nxt2, int nyt, const bf16 *__restrict__ Xc,
-    int ldc, int cup)
+    int ldc, int cup, float *__restrict__ Yf = nullptr)
 {
     static_assert(R % 2 == 0, "rows come in parity pairs");
     constexpr int RC = R / 2 + 2;  // coarse rows per wave
@@ -659,7 +661,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     }
     // skip channels: the 27 taps, as k_conv_rows (9 (dz, dx) groups per
     // chunk, R + 2 rows serve the three dy taps)
-    if (cup < Cin) {
+    if (!UP_ONLY && cup < Cin) {
         bf16x8 a0[R + 2], w0[3][NJ], a1[R + 2], w1[3][NJ];
         auto ld = [&](bf16x8(&a)[R + 2], bf16x8(&w)[3][NJ], int kc, int grp) {
             const int dz = grp / 3, dx = grp % 3;
@@ -735,8 +737,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             const int c0 = n0 + 16 * j + 4 * q;
-            if (c0 < Cout)
-                *(bf16x4 *)(Yb + vox * ldy + c0) = leaky4(acc[r][j]);
+            if (c0 < Cout) {
+                if (UP_ONLY)
+                    *(f32x4 *)(Yf + vox * ldy + c0) = acc[r][j];
+                else
+                    *(bf16x4 *)(Yb + vox * ldy + c0) = leaky4(acc[r][j]);
+            }
         }
     }
 }
@@ -1757,12 +1763,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 // ----------------------------------------------------------------------------
 constexpr int ZS64_R = 4;  // rows per wave (8 spill)
 
-template <int CK>
+// PIN (CK = 2): the outputs start from FP32 partial sums Pin [site][64]
+// (bias and the upsampled channels of L6, k_conv_poly<UP_ONLY>) instead of
+// the bias: both K halves sum from zero, and the combine adds (Pin + first
+// half) + second half, Pin read a plane ahead.  Weights: row stride wld,
+// first input channel woff (L6's skip channels: 192, 128).
+template <int CK, bool PIN = false>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_zs64(
     const bf16 *__restrict__ X, int ldx, int xoff, const bf16 *__restrict__ Wt,
     const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy, int yoff, int B, int D, int H,
-    int Wd, int zc, int ntx, int nty)
+    int Wd, int zc, int ntx, int nty, int wld, int woff, const float *__restrict__ Pin)
 {
+    static_assert(!PIN || CK == 2, "partial input with the K split");
     constexpr int R = ZS64_R, TX = 16, HX = TX + 2, TY = CK == 1 ? 2 * R : R, HY = TY + 2;
     constexpr int CIN = 32 * CK, PAD = CK == 1 ? LDS_PAD32 : LDS_PAD64, PARTS = CIN / 8;
     constexpr int SITES = HX * HY, CHUNKS = SITES * PARTS, PT = (CHUNKS + 511) / 512;
@@ -1784,9 +1796,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     bf16x8 w[27];
 #pragma unroll
     for (int tap = 0; tap < 27; tap++)
-        w[tap] = *(const bf16x8 *)(Wt + ((size_t)tap * 64 + 16 * jb + xl) * CIN + 32 * kc + 8 * q);
+        w[tap] = *(const bf16x8 *)(Wt + ((size_t)tap * 64 + 16 * jb + xl) * wld + woff + 32 * kc + 8 * q);
     const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
-    const f32x4 bias0 = bias4(bias, 16 * jb + 4 * q);
+    const f32x4 bias0 = PIN ? zero4 : bias4(bias, 16 * jb + 4 * q);
     const f32x4 init = kc == 0 ? bias0 : zero4;
     unsigned fo[PT];
 #pragma unroll
@@ -1828,6 +1840,19 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 mine[rr * 64] = S[rr];
         }
     };
+    // (PIN: the partial sums of this wave's combine rows of output zz, a
+    // plane ahead)
+    f32x4 pin[R / 2];
+    auto pin_load = [&](int zz) {
+        if constexpr (PIN) {
+#pragma unroll
+            for (int i = 0; i < R / 2; i++) {
+                const int rr = (R / 2) * g + i;
+                pin[i] = *(const f32x4 *)(Pin + ((((size_t)b * D + zz) * H + y0 + rr) * Wd + x0 + xl) * 64 +
+                                          16 * jb + 4 * q);
+            }
+        }
+    };
     auto combine = [&](int zz, int k) {
         if constexpr (CK == 2) {
             const f32x4 *const h0 = red + (((k & 1) * 8 + jb) * R) * 64 + lane;
@@ -1835,7 +1860,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
             for (int i = 0; i < R / 2; i++) {
                 const int rr = (R / 2) * g + i;
-                *(bf16x4 *)(Y + out_at(zz, rr)) = leaky4(h0[rr * 64] + h1[rr * 64]);
+                if (PIN)
+                    *(bf16x4 *)(Y + out_at(zz, rr)) = leaky4((pin[i] + h0[rr * 64]) + h1[rr * 64]);
+                else
+                    *(bf16x4 *)(Y + out_at(zz, rr)) = leaky4(h0[rr * 64] + h1[rr * 64]);
             }
         }
     };
@@ -1882,6 +1910,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             combine(p - 2, k - 1);
         if (decltype(STG)::value)
             store_plane((k + 1) & 1);
+        if (decltype(MSK)::value & 1)
+            pin_load(p - 1);  // for the next combine (PIN)
         if (decltype(LDG)::value)
             load_plane(p + 2);
         plane(MSK, Sp, S0, Sm, k);
@@ -2841,11 +2871,13 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
             if (zs) {
                 const unsigned nb = (unsigned)((size_t)B * (D / zs) * sy * sx);
                 if (cin == 32)
-                    hipLaunchKernelGGL(k_conv_zs64<1>, dim3(nb), dim3(512), 0, n->stream, X, ldx, xoff, Wt,
-                                       bias, Yb, ldy, yoff, B, D, H, W, zs, sx, sy);
+                    hipLaunchKernelGGL((k_conv_zs64<1>), dim3(nb), dim3(512), 0, n->stream, X, ldx, xoff, Wt,
+                                       bias, Yb, ldy, yoff, B, D, H, W, zs, sx, sy, 32, 0,
+                                       (const float *)nullptr);
                 else
-                    hipLaunchKernelGGL(k_conv_zs64<2>, dim3(nb), dim3(512), 0, n->stream, X, ldx, xoff, Wt,
-                                       bias, Yb, ldy, yoff, B, D, H, W, zs, sx, sy);
+                    hipLaunchKernelGGL((k_conv_zs64<2>), dim3(nb), dim3(512), 0, n->stream, X, ldx, xoff, Wt,
+                                       bias, Yb, ldy, yoff, B, D, H, W, zs, sx, sy, 64, 0,
+                                       (const float *)nullptr);
                 return;
             }
         }
@@ -3044,7 +3076,26 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
          D3, H3, W3);
     // decoder
     const bf16 *wp6 = n->wpoly + n->wp_off6, *wp8 = n->wpoly + n->wp_off8;
-    if (fuse_up) {
+    // L6 of the base-32 network at inference, kind 6, split in two: the
+    // upsampled channels (bias + their polyphase sum, 16 rows per wave) into
+    // FP32 partial sums, then the skip channels by the input-plane-stationary
+    // kernel from those sums (K split over two waves)
+    const int sx6 = W2 / 16, sy6 = H2 / ZS64_R;
+    const int zs6 = (W2 % 16 == 0 && H2 % ZS64_R == 0) ? zs_chunk(D2, (size_t)B * sx6 * sy6) : 0;
+    if (fuse_up && n->conv_kind == 6 && C == 32 && zs6 && (size_t)H2 * W2 * 2 * C < (1u << 31)) {
+        int rc = ensure_part(n, (size_t)B * D2 * H2 * W2 * 2 * C);
+        if (rc)
+            return rc;
+        const int nxt2 = 2 * ((W2 + 31) / 32), nyt = (H2 + 15) / 16;
+        const dim3 gu((unsigned)(((size_t)B * D2 * nyt * nxt2 + 3) / 4));
+        hipLaunchKernelGGL((k_conv_poly<16, 4, 1, true>), gu, dim3(256), 0, n->stream, n->U2, ld2, off2,
+                           6 * C, n->wc + L[6].wb_off, 2 * C, wp6, bias(6), 2 * C, (bf16 *)nullptr, 2 * C,
+                           B, D2, H2, W2, 1, nxt2, nyt, n->E3b, 4 * C, 4 * C, n->part);
+        const unsigned nb = (unsigned)((size_t)B * (D2 / zs6) * sy6 * sx6);
+        hipLaunchKernelGGL((k_conv_zs64<2, true>), dim3(nb), dim3(512), 0, n->stream, n->U2, ld2, off2,
+                           wb(6), bias(6), n->D2a, 2 * C, 0, B, D2, H2, W2, zs6, sx6, sy6, 6 * C, 4 * C,
+                           (const float *)n->part);
+    } else if (fuse_up) {
         conv(n, 0, n->U2, ld2, off2, 6 * C, wb(6), 2 * C, bias(6), 2 * C, n->D2a, nullptr, 2 * C, 0,
              B, D2, H2, W2, n->E3b, 4 * C, 4 * C, wp6);
     } else {
