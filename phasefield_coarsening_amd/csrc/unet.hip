@@ -1763,18 +1763,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 // ----------------------------------------------------------------------------
 constexpr int ZS64_R = 4;  // rows per wave (8 spill)
 
-// PIN (CK = 2): the outputs start from FP32 partial sums Pin [site][64]
-// (bias and the upsampled channels of L6, k_conv_poly<UP_ONLY>) instead of
-// the bias: both K halves sum from zero, and the combine adds (Pin + first
-// half) + second half, Pin read a plane ahead.  Weights: row stride wld,
-// first input channel woff (L6's skip channels: 192, 128).
-template <int CK, bool PIN = false>
+// A launch computes the 64 output channels co0 .. co0 + 63 of a layer whose
+// weights are Wt [27][wcout][wld], its input channels starting at woff (L6's
+// skip channels: wld 192, woff 128; the 64^3 layers: wcout 128, co0 0 or
+// 64, woff 0 or 64).  Bias and outputs are those of channels co0 ..; the
+// output element of a site is yoff + its channel offset from co0.
+// MODE (CK = 2):
+//   0: from the bias, bf16(leaky(sum)) into Y;
+//   1 (partial input): from FP32 partial sums P [site][pld] (columns co0
+//     ..): both K halves sum from zero and the combine adds (P + first half)
+//     + second half, P read a plane ahead; bf16(leaky) into Y;
+//   2 (partial output): from the bias, the plain FP32 sum into P.
+template <int CK, int MODE = 0>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_zs64(
     const bf16 *__restrict__ X, int ldx, int xoff, const bf16 *__restrict__ Wt,
     const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy, int yoff, int B, int D, int H,
-    int Wd, int zc, int ntx, int nty, int wld, int woff, const float *__restrict__ Pin)
+    int Wd, int zc, int ntx, int nty, int wcout, int co0, int wld, int woff, float *__restrict__ P,
+    int pld)
 {
-    static_assert(!PIN || CK == 2, "partial input with the K split");
+    constexpr bool PIN = MODE == 1, POUT = MODE == 2;
+    static_assert(MODE == 0 || CK == 2, "partial sums with the K split");
     constexpr int R = ZS64_R, TX = 16, HX = TX + 2, TY = CK == 1 ? 2 * R : R, HY = TY + 2;
     constexpr int CIN = 32 * CK, PAD = CK == 1 ? LDS_PAD32 : LDS_PAD64, PARTS = CIN / 8;
     constexpr int SITES = HX * HY, CHUNKS = SITES * PARTS, PT = (CHUNKS + 511) / 512;
@@ -1796,9 +1804,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     bf16x8 w[27];
 #pragma unroll
     for (int tap = 0; tap < 27; tap++)
-        w[tap] = *(const bf16x8 *)(Wt + ((size_t)tap * 64 + 16 * jb + xl) * wld + woff + 32 * kc + 8 * q);
+        w[tap] = *(const bf16x8 *)(Wt + ((size_t)tap * wcout + co0 + 16 * jb + xl) * wld + woff + 32 * kc + 8 * q);
     const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
-    const f32x4 bias0 = PIN ? zero4 : bias4(bias, 16 * jb + 4 * q);
+    const f32x4 bias0 = PIN ? zero4 : bias4(bias, co0 + 16 * jb + 4 * q);
     const f32x4 init = kc == 0 ? bias0 : zero4;
     unsigned fo[PT];
 #pragma unroll
@@ -1848,8 +1856,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
             for (int i = 0; i < R / 2; i++) {
                 const int rr = (R / 2) * g + i;
-                pin[i] = *(const f32x4 *)(Pin + ((((size_t)b * D + zz) * H + y0 + rr) * Wd + x0 + xl) * 64 +
-                                          16 * jb + 4 * q);
+                pin[i] = *(const f32x4 *)(P + ((((size_t)b * D + zz) * H + y0 + rr) * Wd + x0 + xl) * pld +
+                                          co0 + 16 * jb + 4 * q);
             }
         }
     };
@@ -1862,6 +1870,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 const int rr = (R / 2) * g + i;
                 if (PIN)
                     *(bf16x4 *)(Y + out_at(zz, rr)) = leaky4((pin[i] + h0[rr * 64]) + h1[rr * 64]);
+                else if (POUT)
+                    *(f32x4 *)(P + ((((size_t)b * D + zz) * H + y0 + rr) * Wd + x0 + xl) * pld + co0 + 16 * jb +
+                               4 * q) = h0[rr * 64] + h1[rr * 64];
                 else
                     *(bf16x4 *)(Y + out_at(zz, rr)) = leaky4(h0[rr * 64] + h1[rr * 64]);
             }
@@ -2872,12 +2883,12 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
                 const unsigned nb = (unsigned)((size_t)B * (D / zs) * sy * sx);
                 if (cin == 32)
                     hipLaunchKernelGGL((k_conv_zs64<1>), dim3(nb), dim3(512), 0, n->stream, X, ldx, xoff, Wt,
-                                       bias, Yb, ldy, yoff, B, D, H, W, zs, sx, sy, 32, 0,
-                                       (const float *)nullptr);
+                                       bias, Yb, ldy, yoff, B, D, H, W, zs, sx, sy, 64, 0, 32, 0,
+                                       (float *)nullptr, 0);
                 else
                     hipLaunchKernelGGL((k_conv_zs64<2>), dim3(nb), dim3(512), 0, n->stream, X, ldx, xoff, Wt,
-                                       bias, Yb, ldy, yoff, B, D, H, W, zs, sx, sy, 64, 0,
-                                       (const float *)nullptr);
+                                       bias, Yb, ldy, yoff, B, D, H, W, zs, sx, sy, 64, 0, 64, 0,
+                                       (float *)nullptr, 0);
                 return;
             }
         }
@@ -3070,10 +3081,35 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
     hipLaunchKernelGGL(k_pool_fwd, dim3(row_blocks(B * D3 * H3, W3 * 2 * C / 8)), dim3(256), 0,
                        n->stream, n->U2, ld2, off2, 2 * C, n->P2, B, D2, H2, W2);
     // level 3
-    conv(n, 0, n->P2, 2 * C, 0, 2 * C, wb(4), 4 * C, bias(4), 4 * C, n->E3a, nullptr, 4 * C, 0, B,
-         D3, H3, W3);
-    conv(n, 0, n->E3a, 4 * C, 0, 4 * C, wb(5), 4 * C, bias(5), 4 * C, n->E3b, nullptr, 4 * C, 0, B,
-         D3, H3, W3);
+    // the level-3 layers of the base-32 network (kind 6): the 64-channel
+    // input-plane-stationary kernel per half of the 128 output channels;
+    // L5 (128 input channels) in two passes over 64 of them each, through
+    // FP32 partial sums
+    const int sx3 = W3 / 16, sy3 = H3 / ZS64_R;
+    const int zs3 = (W3 % 16 == 0 && H3 % ZS64_R == 0) ? zs_chunk(D3, (size_t)B * sx3 * sy3 * 2) : 0;
+    if (n->conv_kind == 6 && C == 32 && zs3 && (size_t)H3 * W3 * 4 * C < (1u << 31)) {
+        int rc = ensure_part(n, (size_t)B * D3 * H3 * W3 * 4 * C);
+        if (rc)
+            return rc;
+        const unsigned nb = (unsigned)((size_t)B * (D3 / zs3) * sy3 * sx3);
+        for (int co0 = 0; co0 < 4 * C; co0 += 64)
+            hipLaunchKernelGGL((k_conv_zs64<2, 0>), dim3(nb), dim3(512), 0, n->stream, n->P2, 2 * C, 0, wb(4),
+                               bias(4), n->E3a, 4 * C, co0, B, D3, H3, W3, zs3, sx3, sy3, 4 * C, co0, 2 * C, 0,
+                               (float *)nullptr, 0);
+        for (int co0 = 0; co0 < 4 * C; co0 += 64) {
+            hipLaunchKernelGGL((k_conv_zs64<2, 2>), dim3(nb), dim3(512), 0, n->stream, n->E3a, 4 * C, 0, wb(5),
+                               bias(5), (bf16 *)nullptr, 4 * C, co0, B, D3, H3, W3, zs3, sx3, sy3, 4 * C, co0,
+                               4 * C, 0, n->part, 4 * C);
+            hipLaunchKernelGGL((k_conv_zs64<2, 1>), dim3(nb), dim3(512), 0, n->stream, n->E3a, 4 * C, 64,
+                               wb(5), bias(5), n->E3b, 4 * C, co0, B, D3, H3, W3, zs3, sx3, sy3, 4 * C, co0,
+                               4 * C, 64, n->part, 4 * C);
+        }
+    } else {
+        conv(n, 0, n->P2, 2 * C, 0, 2 * C, wb(4), 4 * C, bias(4), 4 * C, n->E3a, nullptr, 4 * C, 0, B,
+             D3, H3, W3);
+        conv(n, 0, n->E3a, 4 * C, 0, 4 * C, wb(5), 4 * C, bias(5), 4 * C, n->E3b, nullptr, 4 * C, 0, B,
+             D3, H3, W3);
+    }
     // decoder
     const bf16 *wp6 = n->wpoly + n->wp_off6, *wp8 = n->wpoly + n->wp_off8;
     // L6 of the base-32 network at inference, kind 6, split in two: the
@@ -3092,9 +3128,9 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
                            6 * C, n->wc + L[6].wb_off, 2 * C, wp6, bias(6), 2 * C, (bf16 *)nullptr, 2 * C,
                            B, D2, H2, W2, 1, nxt2, nyt, n->E3b, 4 * C, 4 * C, n->part);
         const unsigned nb = (unsigned)((size_t)B * (D2 / zs6) * sy6 * sx6);
-        hipLaunchKernelGGL((k_conv_zs64<2, true>), dim3(nb), dim3(512), 0, n->stream, n->U2, ld2, off2,
-                           wb(6), bias(6), n->D2a, 2 * C, 0, B, D2, H2, W2, zs6, sx6, sy6, 6 * C, 4 * C,
-                           (const float *)n->part);
+        hipLaunchKernelGGL((k_conv_zs64<2, 1>), dim3(nb), dim3(512), 0, n->stream, n->U2, ld2, off2,
+                           wb(6), bias(6), n->D2a, 2 * C, 0, B, D2, H2, W2, zs6, sx6, sy6, 2 * C, 0, 6 * C,
+                           4 * C, n->part, 2 * C);
     } else if (fuse_up) {
         conv(n, 0, n->U2, ld2, off2, 6 * C, wb(6), 2 * C, bias(6), 2 * C, n->D2a, nullptr, 2 * C, 0,
              B, D2, H2, W2, n->E3b, 4 * C, 4 * C, wp6);

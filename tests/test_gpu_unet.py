@@ -155,21 +155,25 @@ def test_forward_matches_reference(kind, shape):
     assert np.abs(got - unet_ref.predict(p, x, bf16=False)).mean() < 2e-2 * scale
 
 
-def test_forward_32cubed_matches_reference():
-    """At 32^3 every layer of kinds 2, 6 and 7 runs a z-marching kernel where
-    one exists (input-plane-stationary ones at levels 1 and 2, the pooling
-    and the head fused into the level-1 layers).  Kind 6 also runs the 64 ->
-    64 layers L3 and L7 with the K dimension split over two waves (two
-    partial sums), kind 7 with one running sum: a different FP32 association
-    flips a BF16 rounding of L3's output now and then, and the deeper layers
-    spread each flip over their neighbourhoods, so kinds 6 and 7 differ
-    slightly almost everywhere (mean 3.7e-4 of the head's scale 0.42
-    observed).  The test holds each kind to the float64 reference and kind 6
-    to no more than 1.25 times the larger mean error of kinds 2 and 7: a
-    wrong product group in L3 or L7 would move the outputs by a sizeable
-    fraction of the scale."""
+@pytest.mark.parametrize("n", [32, 64])
+def test_forward_32cubed_matches_reference(n):
+    """At 32^3 and 64^3 every layer of kinds 2, 6 and 7 runs a z-marching
+    kernel where one exists (input-plane-stationary ones at levels 1 and 2,
+    the pooling and the head fused into the level-1 layers).  Kind 6 also
+    runs the 64 -> 64 layers L3 and L7 with the K dimension split over two
+    waves (two partial sums), L6 as a polyphase pass over the upsampled
+    channels into FP32 partial sums followed by the skip channels, and (at
+    64^3, level 3 of 16^3) L4 and L5 by the same kernel per half of the
+    output channels, L5 in two passes through FP32 partial sums.  Kind 7
+    keeps one running sum per output: a different FP32 association flips a
+    BF16 rounding now and then, and the deeper layers spread each flip over
+    their neighbourhoods, so kinds 6 and 7 differ slightly almost everywhere
+    (mean 3.7e-4 of the head's scale 0.42 observed at 32^3).  The test holds
+    each kind to the float64 reference and kind 6 to no more than 1.25 times
+    the larger mean error of kinds 2 and 7: a wrong product group would move
+    the outputs by a sizeable fraction of the scale."""
     p = _params(1)
-    x, _ = _data(2, (1, 32, 32, 32))
+    x, _ = _data(2, (1, n, n, n))
     want = unet_ref.predict(p, x, bf16=True)
     scale = np.abs(want - x).max()
     err = {}
@@ -179,8 +183,8 @@ def test_forward_32cubed_matches_reference():
             net.set_conv_kernel(kind)
             d = np.abs(net.forward(x) - want)
             err[kind] = d.mean()
-            print("kind %d 32^3: max |d| %.3e, mean |d| %.3e, head scale %.3e"
-                  % (kind, d.max(), d.mean(), scale))
+            print("kind %d %d^3: max |d| %.3e, mean |d| %.3e, head scale %.3e"
+                  % (kind, n, d.max(), d.mean(), scale))
             assert d.max() < 2e-2 * scale and d.mean() < 3e-3 * scale
     assert err[6] <= 1.25 * max(err[2], err[7])
 
