@@ -221,8 +221,10 @@ class Controller:
         skip, wait = 0, 1
         cyc = self.k + self.relax
         prev = None  # (radius, fraction) at the previous cycle start
+        last = None  # the measure after an accepted jump's relaxation: this cycle's start
         while nsteps - done >= cyc:
-            r0, f0, _ = self._timed("judge", self._measure)
+            r0, f0, _ = last if last is not None else self._timed("judge", self._measure)
+            last = None
             fast = prev is None or r0 <= 0 or prev[0] <= 0 or \
                 abs(r0 - prev[0]) > self.max_change * prev[0] or \
                 abs(f0 - prev[1]) > self.max_change * max(prev[1], 1e-12)
@@ -250,6 +252,7 @@ class Controller:
             if ok:
                 acc += 1
                 wait = 1
+                last = (r_r, f_r, bad_r)
             else:
                 rej += 1
                 self.sv.copy_from(self.ckpt)
