@@ -1998,8 +1998,15 @@ struct InFields {
     }
 };
 
-template <int NJ, class Src>
-__global__ __launch_bounds__(256) void k_conv0(const Src src, const bf16 *__restrict__ W0,
+// FULL (the tiles cover the grid, Cout = 16 NJ): every LDS and output store
+// is issued unconditionally, so that the compiler can count them in its
+// waits (vmcnt counts stores too: a store it cannot count makes the next
+// plane's wait for the staging loads a wait for every store).
+// (two waves per SIMD for NJ = 2: with the loads fenced a plane ahead the
+// compiler would otherwise take 260 registers and one wave; NJ = 4 needs them)
+template <int NJ, class Src, bool FULL = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NJ == 2 ? 2 : 1, NJ == 2 ? 2 : 1))) void k_conv0(
+    const Src src, const bf16 *__restrict__ W0,
                                                const float *__restrict__ bias, int Cout,
                                                bf16 *__restrict__ Y, int ldy, int B, int D, int H,
                                                int Wd)
@@ -2037,8 +2044,10 @@ __global__ __launch_bounds__(256) void k_conv0(const Src src, const bf16 *__rest
     auto store = [&](int slot) {
 #pragma unroll
         for (int k = 0; k < C0_PT; k++) {
-            const int i = tid + 256 * k;
-            if (i < C0_SITES) {
+            // (FULL: the threads past the plane rewrite its last site with
+            // its own value)
+            const int i = FULL ? min(tid + 256 * k, C0_SITES - 1) : tid + 256 * k;
+            if (FULL || i < C0_SITES) {
                 bf16x4 v;
                 v[0] = (bf16)pv[k][0];
                 v[1] = (bf16)pv[k][1];
@@ -2074,6 +2083,9 @@ __global__ __launch_bounds__(256) void k_conv0(const Src src, const bf16 *__rest
         store((z - z0 + 2) % 3);
         __syncthreads();
         load(z + 2);  // the next plane's (past the chunk: read and not used)
+        // (the fence keeps the loads here, a whole plane ahead of their use;
+        // left alone the compiler issues them after most of the MFMAs)
+        __builtin_amdgcn_sched_barrier(0);
         f32x4 acc[C0_R][NJ];
 #pragma unroll
         for (int row = 0; row < C0_R; row++)
@@ -2103,17 +2115,17 @@ __global__ __launch_bounds__(256) void k_conv0(const Src src, const bf16 *__rest
             }
         }
         // lane holds channels 16j+4q .. +3 of site (x0+16w+m, y0+row) in acc[row][j]
-        if (xx < Wd) {
+        if (FULL || xx < Wd) {
 #pragma unroll
             for (int row = 0; row < C0_R; row++) {
                 const int y = y0 + row;
-                if (y >= H)
+                if (!FULL && y >= H)
                     break;
                 const size_t vox = (((size_t)b * D + z) * H + y) * Wd + xx;
 #pragma unroll
                 for (int j = 0; j < NJ; j++) {
                     const int c0 = 16 * j + 4 * q;
-                    if (c0 < Cout)
+                    if (FULL || c0 < Cout)
                         *(bf16x4 *)(Y + vox * ldy + c0) = leaky4(acc[row][j]);
                 }
             }
@@ -3023,9 +3035,16 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
     if (n->conv_kind >= 2) {
         const unsigned nb0 = (unsigned)((size_t)B * ((D + C0_ZC - 1) / C0_ZC) *
                                         ((H + C0_R - 1) / C0_R) * ((W + C0_TX - 1) / C0_TX));
-#define CONV0(NJ, SRC)                                                                        \
-    hipLaunchKernelGGL((k_conv0<NJ, decltype(SRC)>), dim3(nb0), dim3(256), 0, n->stream, SRC,     \
-                       n->w0, bias(0), C, n->E1a, C, B, D, H, W)
+        const bool full0 = W % C0_TX == 0 && H % C0_R == 0 && D % C0_ZC == 0;
+#define CONV0(NJ, SRC)                                                                         \
+    do {                                                                                       \
+        if (full0)                                                                             \
+            hipLaunchKernelGGL((k_conv0<NJ, decltype(SRC), true>), dim3(nb0), dim3(256), 0,     \
+                               n->stream, SRC, n->w0, bias(0), C, n->E1a, C, B, D, H, W);      \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_conv0<NJ, decltype(SRC)>), dim3(nb0), dim3(256), 0, n->stream, \
+                               SRC, n->w0, bias(0), C, n->E1a, C, B, D, H, W);                 \
+    } while (0)
         const InPacked packed{n->xin};
         if (io && io->f64) {
             const InFields<double> src{(const double *)io->s, (const double *)io->l,
