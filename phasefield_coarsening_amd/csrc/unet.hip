@@ -1330,6 +1330,11 @@ constexpr int ZP_CCH = ZP_CSITES * 8;
 constexpr int ZQ_CPAD = 72;
 constexpr int ZQ_FPT = (ZP_FCH + 511) / 512, ZQ_CPT = (ZP_CCH + 511) / 512;
 
+// FULL (the tiles cover the grid): the output and staging stores are
+// issued unconditionally, so that the compiler can count them: vmcnt counts
+// stores too, and a store it cannot count makes the skip waves' wait for the
+// next planes' staging loads a wait for their output stores to complete.
+template <bool FULL>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_zpoly8(
     const bf16 *__restrict__ X, int ldx, int xoff, const bf16 *__restrict__ Wt,
     const bf16 *__restrict__ Wp, const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy,
@@ -1382,11 +1387,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int i = 0; i < ZQ_FPT; i++)
             pf[i] = *(const bf16x8 *)(pl + fo[i]);
     };
+    // (FULL: the chunks past the end rewrite the last one with its own value)
     auto store_fine = [&](int slot) {
 #pragma unroll
         for (int i = 0; i < ZQ_FPT; i++) {
-            const int c = tid + 512 * i;
-            if (c < ZP_FCH)
+            const int c = FULL ? min(tid + 512 * i, ZP_FCH - 1) : tid + 512 * i;
+            if (FULL || c < ZP_FCH)
                 *(bf16x8 *)(fsh + ((size_t)slot * ZP_FSITES + (c >> 2)) * ZP_FPAD + 8 * (c & 3)) = pf[i];
         }
     };
@@ -1399,8 +1405,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     auto store_coarse = [&](int slot) {
 #pragma unroll
         for (int i = 0; i < ZQ_CPT; i++) {
-            const int c = tid + 512 * i;
-            if (c < ZP_CCH)
+            const int c = FULL ? min(tid + 512 * i, ZP_CCH - 1) : tid + 512 * i;
+            if (FULL || c < ZP_CCH)
                 *(bf16x8 *)(csh + ((size_t)slot * ZP_CSITES + (c >> 3)) * ZQ_CPAD + 8 * (c & 7)) = pc[i];
         }
     };
@@ -1553,11 +1559,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         };
         auto store_out = [&](int zz) {
             const int x = x0 + 2 * xl + px;
-            if (x < Wd) {
+            if (FULL || x < Wd) {
 #pragma unroll
                 for (int rr = 0; rr < ZP_TY; rr++) {
                     const int y = y0 + rr;
-                    if (y >= H)
+                    if (!FULL && y >= H)
                         break;
                     const size_t o = ((((size_t)b * D + zz) * H + y) * Wd + x) * ldy + 16 * nt + 4 * q;
                     *(bf16x4 *)(Y + o) = leaky4(acc[rr]);
@@ -1569,15 +1575,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int i = 0; i < 21; i++)
             settle(ws[i]);
         settle(bias0);
-        for (int z = z0; z < zend; z++) {
-            const int pz = z & 1;
+        // one plane; FIRST (the chunk's first plane: no output of the plane
+        // before it) and ODD are static, so that the output stores are
+        // unconditional in the steady state and the compiler counts them in
+        // its waits for the staging loads (z0 and the chunk length are even)
+        auto plane = [&](auto FIRST, auto ODD, int z) {
             __syncthreads();  // every read of the previous plane is done, its partial written
-            if (z > z0) {
+            if (!decltype(FIRST)::value) {
                 add_partial();
                 store_out(z - 1);
             }
             store_fine((z - z0 + 2) % 3);
-            if (pz)
+            if (decltype(ODD)::value)
                 store_coarse(((z >> 1) + 1 - Z0 + 1) & 1);
             __syncthreads();
             load_fine(z + 2);
@@ -1619,6 +1628,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                     __builtin_amdgcn_sched_barrier(0);
                 }
             }
+        };
+        plane(IC<1>{}, IC<0>{}, z0);
+        plane(IC<0>{}, IC<1>{}, z0 + 1);
+        for (int z = z0 + 2; z < zend; z += 2) {
+            plane(IC<0>{}, IC<0>{}, z);
+            plane(IC<0>{}, IC<1>{}, z + 1);
         }
         __syncthreads();
         add_partial();
@@ -2856,8 +2871,12 @@ static void conv(ppf_unet *n, int mode, const bf16 *X, int ldx, int xoff, int ci
         // chunks of 16 planes (even)
         const int zc = D < 16 ? D : 16, ntx = (W + ZP_TX - 1) / ZP_TX, nty = (H + ZP_TY - 1) / ZP_TY;
         const unsigned nblk = (unsigned)((size_t)B * ((D + zc - 1) / zc) * nty * ntx);
-        hipLaunchKernelGGL(k_conv_zpoly8, dim3(nblk), dim3(512), 0, n->stream, X, ldx, xoff, Wt, Wp,
-                           bias, Yb, ldy, B, D, H, W, Xc, ldc, zc, ntx, nty);
+        if (W % ZP_TX == 0 && H % ZP_TY == 0)
+            hipLaunchKernelGGL(k_conv_zpoly8<true>, dim3(nblk), dim3(512), 0, n->stream, X, ldx, xoff, Wt,
+                               Wp, bias, Yb, ldy, B, D, H, W, Xc, ldc, zc, ntx, nty);
+        else
+            hipLaunchKernelGGL(k_conv_zpoly8<false>, dim3(nblk), dim3(512), 0, n->stream, X, ldx, xoff, Wt,
+                               Wp, bias, Yb, ldy, B, D, H, W, Xc, ldc, zc, ntx, nty);
         return;
     }
     if ((n->conv_kind == 6 || n->conv_kind == 7) && Xc && Wp && mode == 0 && coutw == cout &&
