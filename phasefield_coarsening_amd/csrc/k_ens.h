@@ -159,7 +159,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(typename P::T) == 4 ? 5 : 2))) void k_
     using T = typename P::T;
     constexpr bool DB = sizeof(T) == 4;
     constexpr int NB = DB ? 2 : 1;
-    __shared__ T sh[NB * ENS2_LDS];
+    __shared__ __align__(16) T sh[NB * ENS2_LDS];
     __shared__ int next;
     for (;;) {
         if (threadIdx.x == 0)

@@ -8,6 +8,8 @@
 // state (pass C, phasefield.c:393-401), which is the same thing.
 #pragma once
 
+#include <type_traits>
+
 #include "ppf_physics.h"
 #include "ppf_reduce.h"
 
@@ -59,6 +61,66 @@ template <typename T>
 struct Col {
     T s, l, c, h, mu, M;
 };
+
+// The LDS planes of the rotating two-step march hold the four fields of a
+// site (s, l, mu, M) side by side in 16-byte vectors: one per site in FP32,
+// two in FP64 ((s, l) and (mu, M), the second set NS sites after the
+// first).  A neighbour's four fields then come in one or two wide reads at
+// an immediate offset from the thread's own site, from one address register
+// per buffer, where separate field planes (a few KB apart, beyond the reach
+// of the paired reads' offsets) need an address register per field.
+template <typename T>
+struct LdsVec;
+template <>
+struct LdsVec<float> {
+    using V = float4;
+    static constexpr int NG = 1;
+};
+template <>
+struct LdsVec<double> {
+    using V = double2;
+    static constexpr int NG = 2;
+};
+
+template <int NS, typename T, typename V>
+__device__ __forceinline__ void lds_put4(V *b, int i, T s, T l, T mu, T M)
+{
+    if constexpr (sizeof(T) == 4) {
+        b[i] = make_float4(s, l, mu, M);
+    } else {
+        b[i] = make_double2(s, l);
+        b[NS + i] = make_double2(mu, M);
+    }
+}
+
+template <int NS, typename T, typename V>
+__device__ __forceinline__ void lds_get4(const V *b, int i, T &s, T &l, T &mu, T &M)
+{
+    if constexpr (sizeof(T) == 4) {
+        const V v = b[i];
+        s = v.x;
+        l = v.y;
+        mu = v.z;
+        M = v.w;
+    } else {
+        const V a = b[i], c = b[NS + i];
+        s = a.x;
+        l = a.y;
+        mu = c.x;
+        M = c.y;
+    }
+}
+
+// The four in-plane neighbours of site i (row width RW) into slots 2..5 of
+// the update's operand arrays: +y, -y, +x, -x.
+template <int NS, int RW, typename T, typename V>
+__device__ __forceinline__ void lds_nbrs(const V *b, int i, T sn[6], T ln[6], T mun[6], T Mn[6])
+{
+    lds_get4<NS>(b, i + RW, sn[2], ln[2], mun[2], Mn[2]);
+    lds_get4<NS>(b, i - RW, sn[3], ln[3], mun[3], Mn[3]);
+    lds_get4<NS>(b, i + 1, sn[4], ln[4], mun[4], Mn[4]);
+    lds_get4<NS>(b, i - 1, sn[5], ln[5], mun[5], Mn[5]);
+}
 
 // Marks lane offsets as redefined at the top of each plane of a z march (an
 // empty asm, no instruction): the compiler then forms each plane's
@@ -418,8 +480,15 @@ __global__ __launch_bounds__(TX *TY) void k_tiled(
 // ROT: the march without early loads rotates three register sets per
 // column instead of copying (see below); callers with a tighter register
 // budget (the no-flux FP32 ensemble kernel) switch it off.
+// FULL (ROT, periodic): every tile site lies in the grid (nx % TX == 0 and
+// ny % TY == 0), so the output stores are unconditional and are deferred:
+// a plane's step t+2 values stay in registers and are stored after the
+// next plane's loads have been issued.  Loads and stores share one in-order
+// counter of outstanding vector memory operations, so a store issued before
+// a plane's loads would have to complete before those loads could be used;
+// issued after them, it completes while the plane is computed.
 template <class P, int TX, int TY, bool PF = false, bool DB = false, int BC = 0,
-          bool ROT = P::kRotate>
+          bool ROT = P::kRotate, bool FULL = false>
 __device__ __forceinline__ void tile_march2(
     const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
     const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
@@ -547,15 +616,32 @@ __device__ __forceinline__ void tile_march2(
         Col<T> ta{s_p, l_p, 0, 0, mu_p, M_p}, tb{s_c, l_c, c_c, h_c, mu_c, M_c}, tn{};
         Col<T> qa{qs_p, ql_p, 0, 0, qmu_p, qM_p}, qb{qs_c, ql_c, qc_c, qh_c, qmu_c, qM_c}, qn{};
         Col<T> ua{}, ub{}, un{};
-        auto plane_step = [&](const int zs, const Col<T> &tp, const Col<T> &tc, Col<T> &tnx,
-                              const Col<T> &qp, const Col<T> &qc, Col<T> &qnx, const Col<T> &up,
-                              Col<T> &uc, Col<T> &unx) {
+        // LDS as 16-byte site vectors (LdsVec): A1 buffer b at v1 + b*NG*S1,
+        // A2 buffer b at v2 + b*NG*S2; site indices of this thread's tile,
+        // ring and outer sites
+        using V = typename LdsVec<T>::V;
+        constexpr int NG = LdsVec<T>::NG;
+        constexpr int W1 = TX + 4, W2 = TX + 2, S1 = (TY + 4) * W1, S2 = (TY + 2) * W2;
+        static_assert(sizeof(V) * NG == 4 * sizeof(T), "four fields per site");
+        V *const v1 = reinterpret_cast<V *>(&a1[0][0][0]);
+        V *const v2 = reinterpret_cast<V *>(&a2[0][0][0]);
+        const int i1 = (ty + 2) * W1 + tx + 2, ir1 = (ry + 1) * W1 + rx + 1, io1 = oy * W1 + ox;
+        const int i2 = (ty + 1) * W2 + tx + 1, ir2 = ry * W2 + rx;
+        // FULL: step t+2 values of plane zs-1, stored by the next plane
+        constexpr bool DEF = FULL && BC == 0;
+        T ps = 0, pl = 0, pc = 0;
+        // PEND (an integral constant): a deferred store of plane zs-2 is
+        // pending, which also means zs >= z0 + 2
+        auto plane_step = [&](auto PEND, const int zs, const Col<T> &tp, const Col<T> &tc,
+                              Col<T> &tnx, const Col<T> &qp, const Col<T> &qc, Col<T> &qnx,
+                              const Col<T> &up, Col<T> &uc, Col<T> &unx) {
+            constexpr bool pend = decltype(PEND)::value;
             PPF_FRESH("+v"(col), "+v"(ocol), "+v"(rcol), "+v"(o2col));
             const int zc = ghost_any<BC>(zs, nz), zn = ghost_any<BC>(zs + 1, nz);
             const int bsel = DB ? ((zs - z0 + 1) & 1) : 0;
-            T(*__restrict__ A1)[TY + 4][TX + 4] = a1 + 4 * bsel;
-            T(*__restrict__ A2)[TY + 2][TX + 2] = a2 + 4 * bsel;
-            const T(*__restrict__ A2p)[TY + 2][TX + 2] = a2 + 4 * (DB ? bsel ^ 1 : 0);
+            V *const A1 = v1 + bsel * (NG * S1);
+            V *const A2 = v2 + bsel * (NG * S2);
+            const V *const A2p = v2 + (DB ? bsel ^ 1 : 0) * (NG * S2);
             tnx.s = ld_plane(PLANE(s_in, zn), col);
             tnx.l = ld_plane(PLANE(l_in, zn), col);
             tnx.c = ld_plane(PLANE(c_in, zn), col);
@@ -570,24 +656,19 @@ __device__ __forceinline__ void tile_march2(
                 ol = ld_plane(PLANE(l_in, zc), o2col);
                 oc = ld_plane(PLANE(c_in, zc), o2col);
             }
-            // A1: step t of plane zs
-            A1[0][ty + 2][tx + 2] = tc.s;
-            A1[1][ty + 2][tx + 2] = tc.l;
-            A1[2][ty + 2][tx + 2] = tc.mu;
-            A1[3][ty + 2][tx + 2] = tc.M;
-            if (ring) {
-                A1[0][ry + 1][rx + 1] = qc.s;
-                A1[1][ry + 1][rx + 1] = qc.l;
-                A1[2][ry + 1][rx + 1] = qc.mu;
-                A1[3][ry + 1][rx + 1] = qc.M;
+            if constexpr (pend) {
+                __builtin_nontemporal_store(ps, at_plane(PLANE(s_out, zs - 2), ocol));
+                __builtin_nontemporal_store(pl, at_plane(PLANE(l_out, zs - 2), ocol));
+                __builtin_nontemporal_store(pc, at_plane(PLANE(c_out, zs - 2), ocol));
             }
+            // A1: step t of plane zs
+            lds_put4<S1>(A1, i1, tc.s, tc.l, tc.mu, tc.M);
+            if (ring)
+                lds_put4<S1>(A1, ir1, qc.s, qc.l, qc.mu, qc.M);
             if (outer) {
                 T oh, omu, oM;
                 P::point(os, ol, oc, k, oh, omu, oM);
-                A1[0][oy][ox] = os;
-                A1[1][oy][ox] = ol;
-                A1[2][oy][ox] = omu;
-                A1[3][oy][ox] = oM;
+                lds_put4<S1>(A1, io1, os, ol, omu, oM);
             }
             P::point(tnx.s, tnx.l, tnx.c, k, tnx.h, tnx.mu, tnx.M);
             if (ring)
@@ -601,11 +682,9 @@ __device__ __forceinline__ void tile_march2(
             if constexpr (BC == 1)
                 unx = uc;
             if (!zghost) {
-                const int ax = tx + 2, ay = ty + 2;
-                const T sn[6] = {tnx.s, tp.s, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
-                const T ln[6] = {tnx.l, tp.l, A1[1][ay + 1][ax], A1[1][ay - 1][ax], A1[1][ay][ax + 1], A1[1][ay][ax - 1]};
-                const T mun[6] = {tnx.mu, tp.mu, A1[2][ay + 1][ax], A1[2][ay - 1][ax], A1[2][ay][ax + 1], A1[2][ay][ax - 1]};
-                const T Mn[6] = {tnx.M, tp.M, A1[3][ay + 1][ax], A1[3][ay - 1][ax], A1[3][ay][ax + 1], A1[3][ay][ax - 1]};
+                T sn[6] = {tnx.s, tp.s}, ln[6] = {tnx.l, tp.l}, mun[6] = {tnx.mu, tp.mu},
+                  Mn[6] = {tnx.M, tp.M};
+                lds_nbrs<S1, W1>(A1, i1, sn, ln, mun, Mn);
                 double ds, dl, dc;
                 P::update(tc.s, tc.l, tc.c, tc.h, tc.mu, tc.M, sn, ln, mun, Mn, k, unx.s, unx.l,
                           unx.c, ds, dl, dc);
@@ -614,11 +693,9 @@ __device__ __forceinline__ void tile_march2(
             // ... and at the ring site
             T qs1 = 0, ql1 = 0, qmu1 = 0, qM1 = 0;
             if (ring && ring_in && !zghost) {
-                const int ax = rx + 1, ay = ry + 1;
-                const T sn[6] = {qnx.s, qp.s, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
-                const T ln[6] = {qnx.l, qp.l, A1[1][ay + 1][ax], A1[1][ay - 1][ax], A1[1][ay][ax + 1], A1[1][ay][ax - 1]};
-                const T mun[6] = {qnx.mu, qp.mu, A1[2][ay + 1][ax], A1[2][ay - 1][ax], A1[2][ay][ax + 1], A1[2][ay][ax - 1]};
-                const T Mn[6] = {qnx.M, qp.M, A1[3][ay + 1][ax], A1[3][ay - 1][ax], A1[3][ay][ax + 1], A1[3][ay][ax - 1]};
+                T sn[6] = {qnx.s, qp.s}, ln[6] = {qnx.l, qp.l}, mun[6] = {qnx.mu, qp.mu},
+                  Mn[6] = {qnx.M, qp.M};
+                lds_nbrs<S1, W1>(A1, ir1, sn, ln, mun, Mn);
                 T qc1, qh1;
                 double ds, dl, dc;
                 P::update(qc.s, qc.l, qc.c, qc.h, qc.mu, qc.M, sn, ln, mun, Mn, k, qs1, ql1, qc1, ds,
@@ -626,52 +703,44 @@ __device__ __forceinline__ void tile_march2(
                 P::point(qs1, ql1, qc1, k, qh1, qmu1, qM1);
             }
             // A2: step t+1 of plane zs (plain stores, see tile_march2's loop)
-#define PPF_PUT_A2(BY, BX)          \
-    do {                            \
-        A2[0][BY][BX] = unx.s;      \
-        A2[1][BY][BX] = unx.l;      \
-        A2[2][BY][BX] = unx.mu;     \
-        A2[3][BY][BX] = unx.M;      \
-    } while (0)
             auto store_a2 = [&]() {
                 if (BC == 0 || active)
-                    PPF_PUT_A2(ty + 1, tx + 1);
+                    lds_put4<S2>(A2, i2, unx.s, unx.l, unx.mu, unx.M);
                 if constexpr (BC == 1) {
                     if (active) {
                         if (x0 + tx == 0)
-                            PPF_PUT_A2(ty + 1, tx);
+                            lds_put4<S2>(A2, i2 - 1, unx.s, unx.l, unx.mu, unx.M);
                         if (x0 + tx == nx - 1)
-                            PPF_PUT_A2(ty + 1, tx + 2);
+                            lds_put4<S2>(A2, i2 + 1, unx.s, unx.l, unx.mu, unx.M);
                         if (y0 + ty == 0)
-                            PPF_PUT_A2(ty, tx + 1);
+                            lds_put4<S2>(A2, i2 - W2, unx.s, unx.l, unx.mu, unx.M);
                         if (y0 + ty == ny - 1)
-                            PPF_PUT_A2(ty + 2, tx + 1);
+                            lds_put4<S2>(A2, i2 + W2, unx.s, unx.l, unx.mu, unx.M);
                     }
                 }
-                if (ring && ring_in) {
-                    A2[0][ry][rx] = qs1;
-                    A2[1][ry][rx] = ql1;
-                    A2[2][ry][rx] = qmu1;
-                    A2[3][ry][rx] = qM1;
-                }
+                if (ring && ring_in)
+                    lds_put4<S2>(A2, ir2, qs1, ql1, qmu1, qM1);
             };
-#undef PPF_PUT_A2
             if (DB)
                 store_a2();
             // stage 2: step t+2 of plane zs-1 at the tile site
-            if (zs >= z0 + 1 && active) {
-                const int bx = tx + 1, by = ty + 1;
-                const T sn[6] = {unx.s, up.s, A2p[0][by + 1][bx], A2p[0][by - 1][bx], A2p[0][by][bx + 1], A2p[0][by][bx - 1]};
-                const T ln[6] = {unx.l, up.l, A2p[1][by + 1][bx], A2p[1][by - 1][bx], A2p[1][by][bx + 1], A2p[1][by][bx - 1]};
-                const T mun[6] = {unx.mu, up.mu, A2p[2][by + 1][bx], A2p[2][by - 1][bx], A2p[2][by][bx + 1], A2p[2][by][bx - 1]};
-                const T Mn[6] = {unx.M, up.M, A2p[3][by + 1][bx], A2p[3][by - 1][bx], A2p[3][by][bx + 1], A2p[3][by][bx - 1]};
+            if ((pend || zs >= z0 + 1) && (DEF || active)) {
+                T sn[6] = {unx.s, up.s}, ln[6] = {unx.l, up.l}, mun[6] = {unx.mu, up.mu},
+                  Mn[6] = {unx.M, up.M};
+                lds_nbrs<S2, W2>(A2p, i2, sn, ln, mun, Mn);
                 T so, lo, co;
                 double ds, dl, dc;
                 P::update(uc.s, uc.l, uc.c, uc.h, uc.mu, uc.M, sn, ln, mun, Mn, k, so, lo, co, ds, dl,
                           dc);
-                __builtin_nontemporal_store(so, at_plane(PLANE(s_out, zs - 1), ocol));
-                __builtin_nontemporal_store(lo, at_plane(PLANE(l_out, zs - 1), ocol));
-                __builtin_nontemporal_store(co, at_plane(PLANE(c_out, zs - 1), ocol));
+                if constexpr (DEF) {
+                    ps = so;
+                    pl = lo;
+                    pc = co;
+                } else {
+                    __builtin_nontemporal_store(so, at_plane(PLANE(s_out, zs - 1), ocol));
+                    __builtin_nontemporal_store(lo, at_plane(PLANE(l_out, zs - 1), ocol));
+                    __builtin_nontemporal_store(co, at_plane(PLANE(c_out, zs - 1), ocol));
+                }
             }
             if (!DB) {
                 __syncthreads();
@@ -686,23 +755,38 @@ __device__ __forceinline__ void tile_march2(
                 uc.M = unx.M;
             }
         };
+        using NoPend = std::integral_constant<bool, false>;
+        using Pend = std::integral_constant<bool, DEF>;
         int zs = z0 - 1;
+        if constexpr (DEF) {
+            // planes z0-1 .. z0+1 (zend >= z0 + 1): nothing pending yet
+            plane_step(NoPend{}, zs, ta, tb, tn, qa, qb, qn, ua, ub, un);
+            plane_step(NoPend{}, zs + 1, tb, tn, ta, qb, qn, qa, ub, un, ua);
+            plane_step(NoPend{}, zs + 2, tn, ta, tb, qn, qa, qb, un, ua, ub);
+            zs += 3;
+        }
         for (; zs + 2 <= zend; zs += 3) {
-            plane_step(zs, ta, tb, tn, qa, qb, qn, ua, ub, un);
-            plane_step(zs + 1, tb, tn, ta, qb, qn, qa, ub, un, ua);
-            plane_step(zs + 2, tn, ta, tb, qn, qa, qb, un, ua, ub);
+            plane_step(Pend{}, zs, ta, tb, tn, qa, qb, qn, ua, ub, un);
+            plane_step(Pend{}, zs + 1, tb, tn, ta, qb, qn, qa, ub, un, ua);
+            plane_step(Pend{}, zs + 2, tn, ta, tb, qn, qa, qb, un, ua, ub);
         }
         // the last one or two planes, rotating by copies.  (Policies whose
         // registers would not hold the third set at three waves per SIMD,
         // the bitwise FP64 one, take the loop below instead.)
         for (; zs <= zend; ++zs) {
-            plane_step(zs, ta, tb, tn, qa, qb, qn, ua, ub, un);
+            plane_step(Pend{}, zs, ta, tb, tn, qa, qb, qn, ua, ub, un);
             ta = tb;
             tb = tn;
             qa = qb;
             qb = qn;
             ua = ub;
             ub = un;
+        }
+        if constexpr (DEF) {
+            // plane zend-1, stepped by the last plane_step
+            __builtin_nontemporal_store(ps, at_plane(PLANE(s_out, zend - 1), ocol));
+            __builtin_nontemporal_store(pl, at_plane(PLANE(l_out, zend - 1), ocol));
+            __builtin_nontemporal_store(pc, at_plane(PLANE(c_out, zend - 1), ocol));
         }
     } else {
     for (int zs = z0 - 1; zs <= zend; ++zs) {
@@ -1197,9 +1281,10 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_tiled2s(
 
 // WPE: the waves per SIMD the register allocation aims for (FP64 needs a
 // cap: left alone it takes ~185 VGPRs, two waves per SIMD).  PF: prefetch
-// one plane ahead, DB: double-buffered LDS, one barrier per plane
-// (tile_march2).
-template <class P, int TX, int TY, int WPE, bool PF = false, bool DB = false, int BC = 0>
+// one plane ahead, DB: double-buffered LDS, one barrier per plane, FULL:
+// tiles that divide the plane, deferred output stores (tile_march2).
+template <class P, int TX, int TY, int WPE, bool PF = false, bool DB = false, int BC = 0,
+          bool FULL = false>
 __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_tiled2(
     const typename P::T *__restrict__ s_in, const typename P::T *__restrict__ l_in,
     const typename P::T *__restrict__ c_in, typename P::T *__restrict__ s_out,
@@ -1207,8 +1292,8 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE
     int tz, int ntx, int nty, int order, typename P::K k)
 {
     using T = typename P::T;
-    __shared__ T a1[(DB ? 8 : 4)][TY + 4][TX + 4];
-    __shared__ T a2[(DB ? 8 : 4)][TY + 2][TX + 2];
+    __shared__ __align__(16) T a1[(DB ? 8 : 4)][TY + 4][TX + 4];
+    __shared__ __align__(16) T a2[(DB ? 8 : 4)][TY + 2][TX + 2];
     const int ntile = ntx * nty;
     const int nblk = gridDim.x;
     int b = blockIdx.x;
@@ -1218,6 +1303,7 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE
     const int zc = (order & 1) ? b % ((nblk + ntile - 1) / ntile) : b / ntile;
     const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY, z0 = zc * tz;
     const int zend = min(z0 + tz, nz);
-    tile_march2<P, TX, TY, PF, DB, BC>(s_in, l_in, c_in, s_out, l_out, c_out, nx, ny, nz, x0, y0,
+    tile_march2<P, TX, TY, PF, DB, BC, P::kRotate, FULL>(s_in, l_in, c_in, s_out, l_out, c_out, nx,
+                                                         ny, nz, x0, y0,
                                        z0, zend, k, a1, a2);
 }

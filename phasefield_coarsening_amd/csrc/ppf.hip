@@ -246,7 +246,7 @@ struct FusedPick {
 };
 
 template <class P>
-static FusedPick<P> pick_fused(TileCfg t, int order, int bc = PPF_BC_PERIODIC)
+static FusedPick<P> pick_fused(TileCfg t, int order, int bc = PPF_BC_PERIODIC, bool full = false)
 {
     // k_tiled2: FP64 needs ~176 VGPRs: two waves per SIMD; at two waves per
     // SIMD the loads must be issued a plane ahead (PF) to be hidden.  Order
@@ -257,6 +257,8 @@ static FusedPick<P> pick_fused(TileCfg t, int order, int bc = PPF_BC_PERIODIC)
     // buffers would not fit.
     // Order bit 3: k_tiled2s, the ring sites in waves of their own after the
     // tile's (FP64 114 VGPRs: four waves per SIMD; FP32 55: eight).
+    // full: the tile divides the plane; the rotating march (the fast FP64
+    // and the FP32 forms, no early loads) then defers its output stores.
     constexpr bool F64 = sizeof(typename P::T) == 8;
     constexpr int W = F64 ? 2 : 5, WS = F64 ? 4 : 8;
     const int nt = t.tx * t.ty;
@@ -305,14 +307,28 @@ static FusedPick<P> pick_fused(TileCfg t, int order, int bc = PPF_BC_PERIODIC)
     } else if (order & 16) {
         return {nullptr, 0};
     }
-    if (t.tx == 64 && t.ty == 8)
+    constexpr bool ROT = P::kRotate;
+    const bool fr = ROT && full;
+    if (t.tx == 64 && t.ty == 8) {
+        if (!F64 && fr)
+            return {k_tiled2<P, 64, 8, W, false, true, 0, ROT>, nt};
         return {k_tiled2<P, 64, 8, W, F64, true>, nt};
-    if (t.tx == 32 && t.ty == 8)
+    }
+    if (t.tx == 32 && t.ty == 8) {
+        if (fr && (w3 || !F64))
+            return {k_tiled2<P, 32, 8, F64 ? 3 : W, false, true, 0, ROT>, nt};
         return {w3 ? k_tiled2<P, 32, 8, 3, false, true> : k_tiled2<P, 32, 8, W, F64, true>, nt};
-    if (t.tx == 16 && t.ty == 16)
+    }
+    if (t.tx == 16 && t.ty == 16) {
+        if (fr && (w3 || !F64))
+            return {k_tiled2<P, 16, 16, F64 ? 3 : W, false, true, 0, ROT>, nt};
         return {w3 ? k_tiled2<P, 16, 16, 3, false, true> : k_tiled2<P, 16, 16, W, F64, true>, nt};
-    if (t.tx == 32 && t.ty == 16)
+    }
+    if (t.tx == 32 && t.ty == 16) {
+        if (!F64 && fr)
+            return {k_tiled2<P, 32, 16, W, false, true, 0, ROT>, nt};
         return {k_tiled2<P, 32, 16, W, F64, true>, nt};
+    }
     if (t.tx == 64 && t.ty == 16)
         return {k_tiled2<P, 64, 16, W == 2 ? 1 : W, F64, !F64>, nt};
     return {nullptr, 0};
@@ -340,7 +356,9 @@ static int launch_fused(ppf_ctx *ctx, const typename P::K &k)
     const int ntx = (ctx->d.nx + ctx->tile2.tx - 1) / ctx->tile2.tx;
     const int nty = (ctx->d.ny + ctx->tile2.ty - 1) / ctx->tile2.ty;
     const int nzc = (ctx->d.nz + ctx->tz2 - 1) / ctx->tz2;
-    const FusedPick<P> pk = pick_fused<P>(ctx->tile2, ctx->order2, ctx->d.bc);
+    const FusedPick<P> pk =
+        pick_fused<P>(ctx->tile2, ctx->order2, ctx->d.bc,
+                      ctx->d.nx % ctx->tile2.tx == 0 && ctx->d.ny % ctx->tile2.ty == 0);
     if (!pk.f)
         return set_err(ctx, PPF_EARG, "no two-step kernel for tile %dx%d order %d", ctx->tile2.tx,
                        ctx->tile2.ty, ctx->order2);
