@@ -180,9 +180,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(typename P::T) == 4 ? 5 : 2))) void k_
         T *lo = (T *)d->f[1 - parity][1];
         T *co = (T *)d->f[1 - parity][2];
         const int dnx = d->nx, dny = d->ny, dnz = d->nz;
-        // register rotation in the march: periodic members only (the no-flux
-        // FP32 kernel would spill with it)
-        constexpr bool ROT = P::kRotate && BC == 0;
+        // no register rotation in the march: with both tile shapes in one
+        // kernel the FP32 form would spill with it at five waves per SIMD
+        constexpr bool ROT = false;
         if (w.shape == ENS2_32x8)
             tile_march2<P, 32, 8, false, DB, BC, ROT>(
                 si, li, ci, so, lo, co, dnx, dny, dnz, w.x0, w.y0, w.z0, w.zend, k,

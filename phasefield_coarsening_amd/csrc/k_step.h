@@ -54,6 +54,22 @@ __device__ __forceinline__ __attribute__((address_space(1))) T *at_plane(
     using C = __attribute__((address_space(1))) char;
     return (__attribute__((address_space(1))) T *)((C *)base + i * (unsigned)sizeof(T));
 }
+// The same with the lane's byte offset b = i*sizeof(T) already formed (a
+// march that keeps b across planes saves the shift per access group).
+template <typename T>
+__device__ __forceinline__ T ld_plane_b(const __attribute__((address_space(1))) T *base,
+                                        unsigned b)
+{
+    using C = const __attribute__((address_space(1))) char;
+    return *(const __attribute__((address_space(1))) T *)((C *)base + b);
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *at_plane_b(
+    __attribute__((address_space(1))) T *base, unsigned b)
+{
+    using C = __attribute__((address_space(1))) char;
+    return (__attribute__((address_space(1))) T *)((C *)base + b);
+}
 
 // The step-t (or t+1) state of one site column at one plane, as the z march
 // keeps it in registers: the raw fields and pass A's per-site values.
@@ -627,6 +643,9 @@ __device__ __forceinline__ void tile_march2(
         V *const v2 = reinterpret_cast<V *>(&a2[0][0][0]);
         const int i1 = (ty + 2) * W1 + tx + 2, ir1 = (ry + 1) * W1 + rx + 1, io1 = oy * W1 + ox;
         const int i2 = (ty + 1) * W2 + tx + 1, ir2 = ry * W2 + rx;
+        // the lane offsets as byte offsets
+        unsigned bcol = col * (unsigned)sizeof(T), bocol = ocol * (unsigned)sizeof(T),
+                 brcol = rcol * (unsigned)sizeof(T), bo2col = o2col * (unsigned)sizeof(T);
         // FULL: step t+2 values of plane zs-1, stored by the next plane
         constexpr bool DEF = FULL && BC == 0;
         T ps = 0, pl = 0, pc = 0;
@@ -636,30 +655,32 @@ __device__ __forceinline__ void tile_march2(
                               Col<T> &tnx, const Col<T> &qp, const Col<T> &qc, Col<T> &qnx,
                               const Col<T> &up, Col<T> &uc, Col<T> &unx) {
             constexpr bool pend = decltype(PEND)::value;
-            PPF_FRESH("+v"(col), "+v"(ocol), "+v"(rcol), "+v"(o2col));
+            PPF_FRESH("+v"(bcol), "+v"(bocol), "+v"(brcol), "+v"(bo2col));
             const int zc = ghost_any<BC>(zs, nz), zn = ghost_any<BC>(zs + 1, nz);
             const int bsel = DB ? ((zs - z0 + 1) & 1) : 0;
             V *const A1 = v1 + bsel * (NG * S1);
             V *const A2 = v2 + bsel * (NG * S2);
             const V *const A2p = v2 + (DB ? bsel ^ 1 : 0) * (NG * S2);
-            tnx.s = ld_plane(PLANE(s_in, zn), col);
-            tnx.l = ld_plane(PLANE(l_in, zn), col);
-            tnx.c = ld_plane(PLANE(c_in, zn), col);
+            tnx.s = ld_plane_b(PLANE(s_in, zn), bcol);
+            tnx.l = ld_plane_b(PLANE(l_in, zn), bcol);
+            tnx.c = ld_plane_b(PLANE(c_in, zn), bcol);
             if (ring) {
-                qnx.s = ld_plane(PLANE(s_in, zn), rcol);
-                qnx.l = ld_plane(PLANE(l_in, zn), rcol);
-                qnx.c = ld_plane(PLANE(c_in, zn), rcol);
+                qnx.s = ld_plane_b(PLANE(s_in, zn), brcol);
+                qnx.l = ld_plane_b(PLANE(l_in, zn), brcol);
+                qnx.c = ld_plane_b(PLANE(c_in, zn), brcol);
             }
-            T os = 0, ol = 0, oc = 0;
+            // (read only by the outer threads, which load them)
+            T os = __builtin_nondeterministic_value(os), ol = __builtin_nondeterministic_value(ol),
+              oc = __builtin_nondeterministic_value(oc);
             if (outer) {
-                os = ld_plane(PLANE(s_in, zc), o2col);
-                ol = ld_plane(PLANE(l_in, zc), o2col);
-                oc = ld_plane(PLANE(c_in, zc), o2col);
+                os = ld_plane_b(PLANE(s_in, zc), bo2col);
+                ol = ld_plane_b(PLANE(l_in, zc), bo2col);
+                oc = ld_plane_b(PLANE(c_in, zc), bo2col);
             }
             if constexpr (pend) {
-                __builtin_nontemporal_store(ps, at_plane(PLANE(s_out, zs - 2), ocol));
-                __builtin_nontemporal_store(pl, at_plane(PLANE(l_out, zs - 2), ocol));
-                __builtin_nontemporal_store(pc, at_plane(PLANE(c_out, zs - 2), ocol));
+                __builtin_nontemporal_store(ps, at_plane_b(PLANE(s_out, zs - 2), bocol));
+                __builtin_nontemporal_store(pl, at_plane_b(PLANE(l_out, zs - 2), bocol));
+                __builtin_nontemporal_store(pc, at_plane_b(PLANE(c_out, zs - 2), bocol));
             }
             // A1: step t of plane zs
             lds_put4<S1>(A1, i1, tc.s, tc.l, tc.mu, tc.M);
@@ -737,9 +758,9 @@ __device__ __forceinline__ void tile_march2(
                     pl = lo;
                     pc = co;
                 } else {
-                    __builtin_nontemporal_store(so, at_plane(PLANE(s_out, zs - 1), ocol));
-                    __builtin_nontemporal_store(lo, at_plane(PLANE(l_out, zs - 1), ocol));
-                    __builtin_nontemporal_store(co, at_plane(PLANE(c_out, zs - 1), ocol));
+                    __builtin_nontemporal_store(so, at_plane_b(PLANE(s_out, zs - 1), bocol));
+                    __builtin_nontemporal_store(lo, at_plane_b(PLANE(l_out, zs - 1), bocol));
+                    __builtin_nontemporal_store(co, at_plane_b(PLANE(c_out, zs - 1), bocol));
                 }
             }
             if (!DB) {
@@ -784,19 +805,27 @@ __device__ __forceinline__ void tile_march2(
         }
         if constexpr (DEF) {
             // plane zend-1, stepped by the last plane_step
-            __builtin_nontemporal_store(ps, at_plane(PLANE(s_out, zend - 1), ocol));
-            __builtin_nontemporal_store(pl, at_plane(PLANE(l_out, zend - 1), ocol));
-            __builtin_nontemporal_store(pc, at_plane(PLANE(c_out, zend - 1), ocol));
+            __builtin_nontemporal_store(ps, at_plane_b(PLANE(s_out, zend - 1), bocol));
+            __builtin_nontemporal_store(pl, at_plane_b(PLANE(l_out, zend - 1), bocol));
+            __builtin_nontemporal_store(pc, at_plane_b(PLANE(c_out, zend - 1), bocol));
         }
     } else {
+    // LDS as 16-byte site vectors, as in the rotating march above
+    using V = typename LdsVec<T>::V;
+    constexpr int NG = LdsVec<T>::NG;
+    constexpr int W1 = TX + 4, W2 = TX + 2, S1 = (TY + 4) * W1, S2 = (TY + 2) * W2;
+    V *const v1 = reinterpret_cast<V *>(&a1[0][0][0]);
+    V *const v2 = reinterpret_cast<V *>(&a2[0][0][0]);
+    const int i1 = (ty + 2) * W1 + tx + 2, ir1 = (ry + 1) * W1 + rx + 1, io1 = oy * W1 + ox;
+    const int i2 = (ty + 1) * W2 + tx + 1, ir2 = ry * W2 + rx;
     for (int zs = z0 - 1; zs <= zend; ++zs) {
         PPF_FRESH("+v"(col), "+v"(ocol), "+v"(rcol), "+v"(o2col));
         const int zc = ghost_any<BC>(zs, nz), zn = ghost_any<BC>(zs + 1, nz);
         // this plane's A1 and A2 buffers, and the A2 buffer of plane zs-1
         const int bsel = DB ? ((zs - z0 + 1) & 1) : 0;
-        T(*__restrict__ A1)[TY + 4][TX + 4] = a1 + 4 * bsel;
-        T(*__restrict__ A2)[TY + 2][TX + 2] = a2 + 4 * bsel;
-        const T(*__restrict__ A2p)[TY + 2][TX + 2] = a2 + 4 * (DB ? bsel ^ 1 : 0);
+        V *const A1 = v1 + bsel * (NG * S1);
+        V *const A2 = v2 + bsel * (NG * S2);
+        const V *const A2p = v2 + (DB ? bsel ^ 1 : 0) * (NG * S2);
         if (!PF) {
             rs_n = ld_plane(PLANE(s_in, zn), col);
             rl_n = ld_plane(PLANE(l_in, zn), col);
@@ -832,23 +861,13 @@ __device__ __forceinline__ void tile_march2(
             }
         }
         // A1: step t of plane zs
-        A1[0][ty + 2][tx + 2] = s_c;
-        A1[1][ty + 2][tx + 2] = l_c;
-        A1[2][ty + 2][tx + 2] = mu_c;
-        A1[3][ty + 2][tx + 2] = M_c;
-        if (ring) {
-            A1[0][ry + 1][rx + 1] = qs_c;
-            A1[1][ry + 1][rx + 1] = ql_c;
-            A1[2][ry + 1][rx + 1] = qmu_c;
-            A1[3][ry + 1][rx + 1] = qM_c;
-        }
+        lds_put4<S1>(A1, i1, s_c, l_c, mu_c, M_c);
+        if (ring)
+            lds_put4<S1>(A1, ir1, qs_c, ql_c, qmu_c, qM_c);
         if (outer) {
             T oh, omu, oM;
             P::point(os, ol, oc, k, oh, omu, oM);
-            A1[0][oy][ox] = os;
-            A1[1][oy][ox] = ol;
-            A1[2][oy][ox] = omu;
-            A1[3][oy][ox] = oM;
+            lds_put4<S1>(A1, io1, os, ol, omu, oM);
         }
         T h_n, mu_n, M_n, qh_n = 0, qmu_n = 0, qM_n = 0;
         P::point(rs_n, rl_n, rc_n, k, h_n, mu_n, M_n);
@@ -870,11 +889,8 @@ __device__ __forceinline__ void tile_march2(
             M1 = uM_c;
         }
         if (!zghost) {
-            const int ax = tx + 2, ay = ty + 2;
-            const T sn[6] = {rs_n, s_p, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
-            const T ln[6] = {rl_n, l_p, A1[1][ay + 1][ax], A1[1][ay - 1][ax], A1[1][ay][ax + 1], A1[1][ay][ax - 1]};
-            const T mun[6] = {mu_n, mu_p, A1[2][ay + 1][ax], A1[2][ay - 1][ax], A1[2][ay][ax + 1], A1[2][ay][ax - 1]};
-            const T Mn[6] = {M_n, M_p, A1[3][ay + 1][ax], A1[3][ay - 1][ax], A1[3][ay][ax + 1], A1[3][ay][ax - 1]};
+            T sn[6] = {rs_n, s_p}, ln[6] = {rl_n, l_p}, mun[6] = {mu_n, mu_p}, Mn[6] = {M_n, M_p};
+            lds_nbrs<S1, W1>(A1, i1, sn, ln, mun, Mn);
             double ds, dl, dc;
             P::update(s_c, l_c, c_c, h_c, mu_c, M_c, sn, ln, mun, Mn, k, s1, l1, c1, ds, dl, dc);
             P::point(s1, l1, c1, k, h1, mu1, M1);
@@ -882,11 +898,9 @@ __device__ __forceinline__ void tile_march2(
         // ... and at the ring site
         T qs1 = 0, ql1 = 0, qmu1 = 0, qM1 = 0;
         if (ring && ring_in && !zghost) {
-            const int ax = rx + 1, ay = ry + 1;
-            const T sn[6] = {qs_n, qs_p, A1[0][ay + 1][ax], A1[0][ay - 1][ax], A1[0][ay][ax + 1], A1[0][ay][ax - 1]};
-            const T ln[6] = {ql_n, ql_p, A1[1][ay + 1][ax], A1[1][ay - 1][ax], A1[1][ay][ax + 1], A1[1][ay][ax - 1]};
-            const T mun[6] = {qmu_n, qmu_p, A1[2][ay + 1][ax], A1[2][ay - 1][ax], A1[2][ay][ax + 1], A1[2][ay][ax - 1]};
-            const T Mn[6] = {qM_n, qM_p, A1[3][ay + 1][ax], A1[3][ay - 1][ax], A1[3][ay][ax + 1], A1[3][ay][ax - 1]};
+            T sn[6] = {qs_n, qs_p}, ln[6] = {ql_n, ql_p}, mun[6] = {qmu_n, qmu_p},
+              Mn[6] = {qM_n, qM_p};
+            lds_nbrs<S1, W1>(A1, ir1, sn, ln, mun, Mn);
             T qc1, qh1;
             double ds, dl, dc;
             P::update(qs_c, ql_c, qc_c, qh_c, qmu_c, qM_c, sn, ln, mun, Mn, k, qs1, ql1, qc1, ds, dl, dc);
@@ -894,49 +908,32 @@ __device__ __forceinline__ void tile_march2(
         }
         // A2: step t+1 of plane zs (a buffer of its own with DB; otherwise
         // after stage 2 and a barrier, below)
-        // (plain stores: routing them through a helper lambda changed the
-        // periodic kernel's register allocation and made it spill)
-#define PPF_PUT_A2(BY, BX)          \
-    do {                            \
-        A2[0][BY][BX] = s1;         \
-        A2[1][BY][BX] = l1;         \
-        A2[2][BY][BX] = mu1;        \
-        A2[3][BY][BX] = M1;         \
-    } while (0)
         auto store_a2 = [&]() {
             if (BC == 0 || active)
-                PPF_PUT_A2(ty + 1, tx + 1);
+                lds_put4<S2>(A2, i2, s1, l1, mu1, M1);
             if constexpr (BC == 1) {
                 // the ghost slots this site mirrors
                 if (active) {
                     if (x0 + tx == 0)
-                        PPF_PUT_A2(ty + 1, tx);
+                        lds_put4<S2>(A2, i2 - 1, s1, l1, mu1, M1);
                     if (x0 + tx == nx - 1)
-                        PPF_PUT_A2(ty + 1, tx + 2);
+                        lds_put4<S2>(A2, i2 + 1, s1, l1, mu1, M1);
                     if (y0 + ty == 0)
-                        PPF_PUT_A2(ty, tx + 1);
+                        lds_put4<S2>(A2, i2 - W2, s1, l1, mu1, M1);
                     if (y0 + ty == ny - 1)
-                        PPF_PUT_A2(ty + 2, tx + 1);
+                        lds_put4<S2>(A2, i2 + W2, s1, l1, mu1, M1);
                 }
             }
-            if (ring && ring_in) {
-                A2[0][ry][rx] = qs1;
-                A2[1][ry][rx] = ql1;
-                A2[2][ry][rx] = qmu1;
-                A2[3][ry][rx] = qM1;
-            }
+            if (ring && ring_in)
+                lds_put4<S2>(A2, ir2, qs1, ql1, qmu1, qM1);
         };
-#undef PPF_PUT_A2
         if (DB)
             store_a2();
         // stage 2: step t+2 of plane zs-1 at the tile site (A2p holds step
         // t+1 of plane zs-1)
         if (zs >= z0 + 1 && active) {
-            const int bx = tx + 1, by = ty + 1;
-            const T sn[6] = {s1, us_p, A2p[0][by + 1][bx], A2p[0][by - 1][bx], A2p[0][by][bx + 1], A2p[0][by][bx - 1]};
-            const T ln[6] = {l1, ul_p, A2p[1][by + 1][bx], A2p[1][by - 1][bx], A2p[1][by][bx + 1], A2p[1][by][bx - 1]};
-            const T mun[6] = {mu1, umu_p, A2p[2][by + 1][bx], A2p[2][by - 1][bx], A2p[2][by][bx + 1], A2p[2][by][bx - 1]};
-            const T Mn[6] = {M1, uM_p, A2p[3][by + 1][bx], A2p[3][by - 1][bx], A2p[3][by][bx + 1], A2p[3][by][bx - 1]};
+            T sn[6] = {s1, us_p}, ln[6] = {l1, ul_p}, mun[6] = {mu1, umu_p}, Mn[6] = {M1, uM_p};
+            lds_nbrs<S2, W2>(A2p, i2, sn, ln, mun, Mn);
             T so, lo, co;
             double ds, dl, dc;
             P::update(us_c, ul_c, uc_c, uh_c, umu_c, uM_c, sn, ln, mun, Mn, k, so, lo, co, ds, dl, dc);
