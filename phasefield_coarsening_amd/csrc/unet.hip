@@ -747,6 +747,159 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     }
 }
 
+// ----------------------------------------------------------------------------
+// k_conv_poly's upsampled-channel part alone (its UP_ONLY mode: bias + the
+// polyphase sum into FP32 partial sums Yf [site][ldy]) for 64 output
+// channels, with the weights read once per workgroup.  The four waves of a
+// workgroup own four consecutive 16-row tiles of the same fine plane z and
+// x parity px, so they multiply with the same weights: per step (32-channel
+// chunk kc, coarse offsets oz, ox) the 16 fragments of (py, oy, j), 16 KB,
+// are loaded from global memory by the whole workgroup into one of two LDS
+// buffers while the previous step's MFMAs run, and every wave reads them
+// from there (k_conv_poly has each wave stream them from the cache, which
+// bounds it by the cache bandwidth).  The coarse rows each wave multiplies
+// are loaded as in k_conv_poly, one step ahead.  Same MFMAs in the same
+// order per accumulator as k_conv_poly<16, 4, WPE, true>, so the partial
+// sums are equal bit for bit.  NW waves of R rows (NW R = 64: 4 x 16 rows
+// at one wave per SIMD, 8 x 8 at two).  W % 32 == 0, H % 64 == 0, D even.
+// ----------------------------------------------------------------------------
+template <int R, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 4, NW / 4))) void k_conv_polyup(
+    const bf16 *__restrict__ Wp, const float *__restrict__ bias, int ldy, int B, int D, int H,
+    int Wd, const bf16 *__restrict__ Xc, int ldc, int cup, float *__restrict__ Yf)
+{
+    static_assert(R * NW == 64 && R % 2 == 0, "64 rows per workgroup");
+    constexpr int NJ = 4, RC = R / 2 + 2, CoutW = 64;
+    // per thread: WB 16-byte pieces of one 4 KB weight block
+    constexpr int TPB = 16 * NW, WB = 16 / NW;
+    __shared__ __attribute__((aligned(16))) bf16 wsh[2][4][CoutW * 32];  // 32 KB
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nxt2 = Wd / 16, nytg = H / (NW * R);
+    unsigned g = blockIdx.x;
+    const int ytg = (int)(g % nytg);
+    g /= nytg;
+    const int xt2 = (int)(g % nxt2);
+    g /= nxt2;
+    const int z = (int)(g % D);
+    const int b = (int)(g / D);
+    const int px = xt2 & 1, xt = xt2 >> 1, pz = z & 1;
+    const int xl = lane & 15, q = lane >> 4;
+    const int y0 = (NW * ytg + wave) * R;
+    const int Dc = D / 2, Hc = H / 2, Wc = Wd / 2;
+    // coarse planes (z >> 1) + pz - 1 + oz, rows y0/2 - 1 + cr, x (16 xt +
+    // xl) + px - 1 + ox
+    const int rowlen_c = Wc * ldc;
+    int xoc[2];
+#pragma unroll
+    for (int ox = 0; ox < 2; ox++)
+        xoc[ox] = (wrapm(16 * xt + xl + px - 1 + ox, Wc) * ldc + 8 * q) * 2;
+    __amdgpu_buffer_rsrc_t plc[2];
+#pragma unroll
+    for (int oz = 0; oz < 2; oz++)
+        plc[oz] = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(Xc + ((size_t)b * Dc + wrapm((z >> 1) + pz - 1 + oz, Dc)) * Hc *
+                              (size_t)rowlen_c),
+            (short)0, Hc * rowlen_c * 2, 0x00020000);
+    int roc[RC];
+#pragma unroll
+    for (int cr = 0; cr < RC; cr++)
+        roc[cr] = wrapm(y0 / 2 - 1 + cr, Hc) * rowlen_c;
+    // weights Wp[par][grp][kc/32][64][32] (par = 4 pz + 2 py + px, grp = 4 oz
+    // + 2 oy + ox): step s = (kc, oz, ox) needs the four 4 KB blocks (py, oy);
+    // thread t copies 16 WB bytes of block t / TPB
+    const __amdgpu_buffer_rsrc_t wpr = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)Wp, (short)0, 64 * CoutW * cup * 2, 0x00020000);
+    const int tb = threadIdx.x / TPB, tpy = tb >> 1, toy = tb & 1, tof = (threadIdx.x % TPB) * 8 * WB;
+    const int nsteps = cup / 32 * 4;
+    auto wsrc = [&](int s) {
+        const int kc = s >> 2, oz = (s >> 1) & 1, ox = s & 1;
+        return ((((4 * pz + 2 * tpy + px) * 8 + 4 * oz + 2 * toy + ox) * (cup / 32) + kc) * CoutW * 32 +
+                tof) * 2;
+    };
+    bf16x8 wr[WB];
+    auto wload = [&](int s) {
+        const int off = wsrc(s);
+#pragma unroll
+        for (int i = 0; i < WB; i++)
+            wr[i] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(wpr, off + 16 * i, 0, 0));
+    };
+    auto wstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < WB; i++)
+            *(bf16x8 *)(&wsh[buf][tb][tof + 8 * i]) = wr[i];
+    };
+    auto cload = [&](bf16x8(&c)[RC], int s) {
+        const int kc = (s >> 2) * 32, oz = (s >> 1) & 1, ox = s & 1;
+#pragma unroll
+        for (int cr = 0; cr < RC; cr++)
+            c[cr] = __builtin_bit_cast(
+                bf16x8, __builtin_amdgcn_raw_buffer_load_b128(plc[oz], xoc[ox], (roc[cr] + kc) * 2, 0));
+    };
+    f32x4 acc[R][NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const f32x4 bj = bias4(bias, 16 * j + 4 * q);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            acc[r][j] = bj;
+    }
+    // one step: fragments from LDS buffer buf, then the MFMAs in
+    // k_conv_poly's order (oy outermost)
+    auto mm = [&](const bf16x8(&c)[RC], int buf) {
+#pragma unroll
+        for (int oy = 0; oy < 2; oy++) {
+            bf16x8 w[2][NJ];
+#pragma unroll
+            for (int py = 0; py < 2; py++)
+#pragma unroll
+                for (int j = 0; j < NJ; j++)
+                    w[py][j] = *(const bf16x8 *)(&wsh[buf][2 * py + oy][(16 * j + xl) * 32 + 8 * q]);
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int j = 0; j < NJ; j++)
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        w[r & 1][j], c[(r >> 1) + (r & 1) + oy], acc[r][j], 0, 0, 0);
+        }
+    };
+    bf16x8 c0[RC], c1[RC];
+    wload(0);
+    cload(c0, 0);
+    wstore(0);
+    __syncthreads();
+    // steps in pairs (nsteps is a multiple of 4): even steps multiply c0
+    // from buffer 0, odd ones c1 from buffer 1; each step first issues the
+    // next step's loads
+    // (past the last step the loads re-read the last step, unused)
+    for (int s = 0; s < nsteps; s += 2) {
+        wload(s + 1);
+        cload(c1, s + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        mm(c0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        wstore(1);
+        __syncthreads();
+        const int sn = min(s + 2, nsteps - 1);
+        wload(sn);
+        cload(c0, sn);
+        __builtin_amdgcn_sched_barrier(0);
+        mm(c1, 1);
+        __builtin_amdgcn_sched_barrier(0);
+        wstore(0);
+        __syncthreads();
+    }
+    // lane holds channels 16j+4q .. +3 of site (32 xt + 2 xl + px, y0+r)
+    const int x = 32 * xt + 2 * xl + px;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const size_t vox = (((size_t)b * D + z) * H + y0 + r) * Wd + x;
+#pragma unroll
+        for (int j = 0; j < NJ; j++)
+            *(f32x4 *)(Yf + vox * ldy + 16 * j + 4 * q) = acc[r][j];
+    }
+}
+
 // LDS layout of the staged planes.  A fragment read (ds_read_b128) has lane
 // (l % 16, l / 16) read 16 bytes at site l % 16 of a row, channel quad
 // l / 16.  Measured on gfx950 (bank-conflict counter over site strides of
@@ -3161,10 +3314,18 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
         if (rc)
             return rc;
         const int nxt2 = 2 * ((W2 + 31) / 32), nyt = (H2 + 15) / 16;
-        const dim3 gu((unsigned)(((size_t)B * D2 * nyt * nxt2 + 3) / 4));
-        hipLaunchKernelGGL((k_conv_poly<16, 4, 1, true>), gu, dim3(256), 0, n->stream, n->U2, ld2, off2,
-                           6 * C, n->wc + L[6].wb_off, 2 * C, wp6, bias(6), 2 * C, (bf16 *)nullptr, 2 * C,
-                           B, D2, H2, W2, 1, nxt2, nyt, n->E3b, 4 * C, 4 * C, n->part);
+        if (W2 % 32 == 0 && H2 % 64 == 0) {
+            // weights shared by the workgroup's four row tiles through LDS
+            const dim3 gu((unsigned)((size_t)B * D2 * nxt2 * (H2 / 64)));
+            hipLaunchKernelGGL((k_conv_polyup<8, 8>), gu, dim3(512), 0, n->stream, wp6, bias(6), 2 * C, B,
+                               D2, H2, W2, n->E3b, 4 * C, 4 * C, n->part);
+        } else {
+            const dim3 gu((unsigned)(((size_t)B * D2 * nyt * nxt2 + 3) / 4));
+            hipLaunchKernelGGL((k_conv_poly<16, 4, 1, true>), gu, dim3(256), 0, n->stream, n->U2, ld2,
+                               off2, 6 * C, n->wc + L[6].wb_off, 2 * C, wp6, bias(6), 2 * C,
+                               (bf16 *)nullptr, 2 * C, B, D2, H2, W2, 1, nxt2, nyt, n->E3b, 4 * C, 4 * C,
+                               n->part);
+        }
         const unsigned nb = (unsigned)((size_t)B * (D2 / zs6) * sy6 * sx6);
         hipLaunchKernelGGL((k_conv_zs64<2, 1>), dim3(nb), dim3(512), 0, n->stream, n->U2, ld2, off2,
                            wb(6), bias(6), n->D2a, 2 * C, 0, B, D2, H2, W2, zs6, sx6, sy6, 2 * C, 0, 6 * C,

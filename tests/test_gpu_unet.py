@@ -155,8 +155,8 @@ def test_forward_matches_reference(kind, shape):
     assert np.abs(got - unet_ref.predict(p, x, bf16=False)).mean() < 2e-2 * scale
 
 
-@pytest.mark.parametrize("n", [32, 64])
-def test_forward_32cubed_matches_reference(n):
+@pytest.mark.parametrize("shape", [(1, 32, 32, 32), (1, 64, 64, 64), (1, 16, 128, 64)])
+def test_forward_32cubed_matches_reference(shape):
     """At 32^3 and 64^3 every layer of kinds 2, 6 and 7 runs a z-marching
     kernel where one exists (input-plane-stationary ones at levels 1 and 2,
     the pooling and the head fused into the level-1 layers).  Kind 6 also
@@ -164,7 +164,9 @@ def test_forward_32cubed_matches_reference(n):
     waves (two partial sums), L6 as a polyphase pass over the upsampled
     channels into FP32 partial sums followed by the skip channels, and (at
     64^3, level 3 of 16^3) L4 and L5 by the same kernel per half of the
-    output channels, L5 in two passes through FP32 partial sums.  Kind 7
+    output channels, L5 in two passes through FP32 partial sums.  At 16 x
+    128 x 64 (level 2: 8 x 64 x 32) kind 6's polyphase pass of L6 is the
+    one whose workgroups share the weights through LDS.  Kind 7
     keeps one running sum per output: a different FP32 association flips a
     BF16 rounding now and then, and the deeper layers spread each flip over
     their neighbourhoods, so kinds 6 and 7 differ slightly almost everywhere
@@ -173,7 +175,7 @@ def test_forward_32cubed_matches_reference(n):
     the larger mean error of kinds 2 and 7: a wrong product group would move
     the outputs by a sizeable fraction of the scale."""
     p = _params(1)
-    x, _ = _data(2, (1, n, n, n))
+    x, _ = _data(2, shape)
     want = unet_ref.predict(p, x, bf16=True)
     scale = np.abs(want - x).max()
     err = {}
@@ -183,8 +185,8 @@ def test_forward_32cubed_matches_reference(n):
             net.set_conv_kernel(kind)
             d = np.abs(net.forward(x) - want)
             err[kind] = d.mean()
-            print("kind %d %d^3: max |d| %.3e, mean |d| %.3e, head scale %.3e"
-                  % (kind, n, d.max(), d.mean(), scale))
+            print("kind %d %s: max |d| %.3e, mean |d| %.3e, head scale %.3e"
+                  % (kind, shape, d.max(), d.mean(), scale))
             assert d.max() < 2e-2 * scale and d.mean() < 3e-3 * scale
     assert err[6] <= 1.25 * max(err[2], err[7])
 
