@@ -517,12 +517,13 @@ static void choose_tiles(ppf_ctx *ctx)
     // 256^3, level at 512^3.
     // The fast FP64 form: 32x8 tiles (145 against 142 GLUPS for 16x16 at
     // 512^3, 130-134 against 119 at 256^3, level at 1024^3).  FP32 on grids
-    // of 512^3 sites and more: 64x8 tiles without ring waves (order 2;
-    // 269 against 258 GLUPS for the ring-wave kernel at 512^3, 259-261
-    // against 250 at 1024^3); below that the ring waves stay (266 against
-    // 233-252 at 256^3).
+    // of 512^3 sites and more, and from 256^3 on where 64x8 tiles divide
+    // the plane (the deferred-store form): 64x8 tiles without ring waves
+    // (order 2; 305 against 258 GLUPS for the ring-wave kernel at 512^3, 299
+    // against 263-273 at 256^3); otherwise the ring waves stay.
     ctx->fuse = 2;
-    const bool big = ctx->n >= ((size_t)1 << 27);
+    const bool big = ctx->n >= ((size_t)1 << 27) ||
+                     (ctx->n >= ((size_t)1 << 24) && ctx->d.nx % 64 == 0 && ctx->d.ny % 8 == 0);
     ctx->tile2 = f32 ? (big ? TileCfg{64, 8} : TileCfg{32, 16})
                      : ctx->kernel == PPF_KERNEL_FAST ? TileCfg{32, 8} : TileCfg{16, 16};
     ctx->order2 = f32 ? (big ? 2 : 10) : 6;
