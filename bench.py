@@ -216,7 +216,8 @@ def unet_line(grid=256, reps=3):
             "apply_ms": round(best_a, 3), "apply_TFLOPs": round(flop / best_a / 1e9, 1)}
 
 
-def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium", no_op=True):
+def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium", no_op=True,
+                   first_jump=False):
     """Learned jumps against exact stepping at grid^3 (500 spheres,
     R ~ U[6, 12]) to t_end: wall times of the surrogate (K = 200 jumps, 20
     FP32 relax steps, judge, exact re-run on rejection), of plain FP64
@@ -229,7 +230,7 @@ def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium
     cluster, with the weights trained there.  no_op: also run the same
     controller with jumps that change nothing (the network's head zeroed) and
     report its errors: what the trained jumps gain in accuracy at the same
-    cost."""
+    cost.  first_jump: the controller tries a jump on the first cycle too."""
     import numpy as np
 
     from phasefield_coarsening_amd import surrogate as SG
@@ -263,7 +264,7 @@ def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium
         with S.Solver(shape, precision=S.F32) as sv, UNet(seed=1) as net:
             net.set_params(params)
             sv.upload(*s0)
-            ctl = SG.Controller(sv, net, k=k, relax=relax)
+            ctl = SG.Controller(sv, net, k=k, relax=relax, first_jump=first_jump)
             summ = ctl.advance(t_end)
             ps, _ = sv.particles(0.5, max_sizes=1)
             ss = sv.reduce()
@@ -280,7 +281,8 @@ def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium
            "speedup_vs_exact_fp64": round(t_exact / summ["seconds"], 3),
            "speedup_vs_exact_fp32": round(t_exact32 / summ["seconds"], 3),
            "accepted": summ["accepted"], "rejected": summ["rejected"],
-           "skipped": summ["skipped"], "R_err_rel": r_err, "frac_err_rel": f_err}
+           "skipped": summ["skipped"], "R_err_rel": r_err, "frac_err_rel": f_err,
+           "first_jump": first_jump}
     if no_op:
         wz = w.copy()
         nh = sum(int(np.prod(shp)) for name, shp in layer_shapes(32) if name.startswith("head"))
@@ -291,7 +293,7 @@ def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium
 
 
 def ensemble_line(S, members=256, steps=1000, seed=4):
-    """Batched ensemble (configs[3] shape): members with edges in {48, 64, 96,
+    """Batched ensemble of 256 uneven periodic domains: members with edges in {48, 64, 96,
     128}, solid fraction U[0.1, 0.4], radii U[3, 10], FP32, autotuned, one
     persistent launch per step."""
     from phasefield_coarsening_amd import surrogate as SG
@@ -452,6 +454,7 @@ def main():
         out["surrogate"] = surrogate_line(S)
         out["surrogate_long"] = surrogate_line(S, t_end=19800, no_op=False)
         out["surrogate_solidifying"] = surrogate_line(S, c_init="config", no_op=False)
+        out["surrogate_first_jump"] = surrogate_line(S, no_op=False, first_jump=True)
     if "copy" in side:
         cc = side["copy"]
         out["roofline"]["copy_ceiling_GBs"] = round(cc, 1)

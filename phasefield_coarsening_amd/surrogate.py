@@ -169,10 +169,16 @@ class Controller:
       holds back the one at step 660, which it would reject);
     - back-off: after a rejection the next ``wait`` cycles run exactly
       (``wait`` = 1, doubling per consecutive rejection up to ``max_wait``).
+    ``first_jump``: try a jump on the first cycle too, which has no rate to
+    gate on, leaving it to the judge (on the 256^3 coarsening workload to
+    t_end 2200: 3.31x instead of 3.09x plain FP64, radius error 0.75%
+    instead of 0.36%; in the solidifying regime the jump is rejected and
+    costs 3%).
     """
 
     def __init__(self, solver, net, k=200, relax=20, tol_radius=0.02, tol_fraction=0.02,
-                 threshold=0.5, judge="area", max_wait=8, max_cycle_change=0.25, timing=False):
+                 threshold=0.5, judge="area", max_wait=8, max_cycle_change=0.25, timing=False,
+                 first_jump=False):
         if judge not in ("area", "particles"):
             raise ValueError("judge must be 'area' or 'particles'")
         self.sv = solver
@@ -186,6 +192,7 @@ class Controller:
         self.max_wait = max_wait
         self.max_change = max_cycle_change
         self.timing = timing
+        self.first_jump = first_jump
         self.ckpt = S.Solver(solver.shape, bc=solver.bc, precision=solver.precision)
         self.log = []
         self.phase_seconds = {"checkpoint": 0.0, "jump": 0.0, "judge": 0.0, "relax": 0.0,
@@ -225,9 +232,12 @@ class Controller:
         while nsteps - done >= cyc:
             r0, f0, _ = last if last is not None else self._timed("judge", self._measure)
             last = None
-            fast = prev is None or r0 <= 0 or prev[0] <= 0 or \
-                abs(r0 - prev[0]) > self.max_change * prev[0] or \
-                abs(f0 - prev[1]) > self.max_change * max(prev[1], 1e-12)
+            if prev is None:
+                fast = not self.first_jump or r0 <= 0
+            else:
+                fast = r0 <= 0 or prev[0] <= 0 or \
+                    abs(r0 - prev[0]) > self.max_change * prev[0] or \
+                    abs(f0 - prev[1]) > self.max_change * max(prev[1], 1e-12)
             prev = (r0, f0)
             if skip > 0 or fast:
                 skip = max(skip - 1, 0)
