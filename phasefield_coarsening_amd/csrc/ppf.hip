@@ -310,23 +310,31 @@ static FusedPick<P> pick_fused(TileCfg t, int order, int bc = PPF_BC_PERIODIC, b
     constexpr bool ROT = P::kRotate;
     const bool fr = ROT && full;
     if (t.tx == 64 && t.ty == 8) {
-        if (!F64 && fr)
-            return {k_tiled2<P, 64, 8, W, false, true, 0, ROT>, nt};
+        if constexpr (!F64 && ROT) {
+            if (fr)
+                return {k_tiled2<P, 64, 8, W, false, true, 0, true>, nt};
+        }
         return {k_tiled2<P, 64, 8, W, F64, true>, nt};
     }
     if (t.tx == 32 && t.ty == 8) {
-        if (fr && (w3 || !F64))
-            return {k_tiled2<P, 32, 8, F64 ? 3 : W, false, true, 0, ROT>, nt};
+        if constexpr (ROT) {
+            if (fr && (w3 || !F64))
+                return {k_tiled2<P, 32, 8, F64 ? 3 : W, false, true, 0, true>, nt};
+        }
         return {w3 ? k_tiled2<P, 32, 8, 3, false, true> : k_tiled2<P, 32, 8, W, F64, true>, nt};
     }
     if (t.tx == 16 && t.ty == 16) {
-        if (fr && (w3 || !F64))
-            return {k_tiled2<P, 16, 16, F64 ? 3 : W, false, true, 0, ROT>, nt};
+        if constexpr (ROT) {
+            if (fr && (w3 || !F64))
+                return {k_tiled2<P, 16, 16, F64 ? 3 : W, false, true, 0, true>, nt};
+        }
         return {w3 ? k_tiled2<P, 16, 16, 3, false, true> : k_tiled2<P, 16, 16, W, F64, true>, nt};
     }
     if (t.tx == 32 && t.ty == 16) {
-        if (!F64 && fr)
-            return {k_tiled2<P, 32, 16, W, false, true, 0, ROT>, nt};
+        if constexpr (!F64 && ROT) {
+            if (fr)
+                return {k_tiled2<P, 32, 16, W, false, true, 0, true>, nt};
+        }
         return {k_tiled2<P, 32, 16, W, F64, true>, nt};
     }
     if (t.tx == 64 && t.ty == 16)
