@@ -1941,15 +1941,19 @@ constexpr int ZS64_R = 4;  // rows per wave (8 spill)
 //   1 (partial input): from FP32 partial sums P [site][pld] (columns co0
 //     ..): both K halves sum from zero and the combine adds (P + first half)
 //     + second half, P read a plane ahead; bf16(leaky) into Y;
-//   2 (partial output): from the bias, the plain FP32 sum into P.
+//   2 (partial output): from the bias, the plain FP32 sum into P;
+//   3: as 0, and the 2x2x2 average pooling of the outputs into Q [B][D/2]
+//     [H/2][W/2][64] (k_pool_fwd's sums: the even lane of an x pair adds the
+//     pair's two rows on the even plane, keeps the sum, adds the odd
+//     plane's and stores; z chunks start at even planes).
 template <int CK, int MODE = 0>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_zs64(
     const bf16 *__restrict__ X, int ldx, int xoff, const bf16 *__restrict__ Wt,
     const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy, int yoff, int B, int D, int H,
     int Wd, int zc, int ntx, int nty, int wcout, int co0, int wld, int woff, float *__restrict__ P,
-    int pld)
+    int pld, bf16 *__restrict__ Q = nullptr)
 {
-    constexpr bool PIN = MODE == 1, POUT = MODE == 2;
+    constexpr bool PIN = MODE == 1, POUT = MODE == 2, POOL = MODE == 3;
     static_assert(MODE == 0 || CK == 2, "partial sums with the K split");
     constexpr int R = ZS64_R, TX = 16, HX = TX + 2, TY = CK == 1 ? 2 * R : R, HY = TY + 2;
     constexpr int CIN = 32 * CK, PAD = CK == 1 ? LDS_PAD32 : LDS_PAD64, PARTS = CIN / 8;
@@ -2029,20 +2033,56 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             }
         }
     };
+    // (POOL: the running pooling sums of this wave's row pair)
+    float pp[4];
     auto combine = [&](int zz, int k) {
         if constexpr (CK == 2) {
             const f32x4 *const h0 = red + (((k & 1) * 8 + jb) * R) * 64 + lane;
             const f32x4 *const h1 = red + (((k & 1) * 8 + 4 + jb) * R) * 64 + lane;
+            bf16x4 o4[R / 2];
 #pragma unroll
             for (int i = 0; i < R / 2; i++) {
                 const int rr = (R / 2) * g + i;
-                if (PIN)
+                if (PIN) {
                     *(bf16x4 *)(Y + out_at(zz, rr)) = leaky4((pin[i] + h0[rr * 64]) + h1[rr * 64]);
-                else if (POUT)
+                } else if (POUT) {
                     *(f32x4 *)(P + ((((size_t)b * D + zz) * H + y0 + rr) * Wd + x0 + xl) * pld + co0 + 16 * jb +
                                4 * q) = h0[rr * 64] + h1[rr * 64];
-                else
-                    *(bf16x4 *)(Y + out_at(zz, rr)) = leaky4(h0[rr * 64] + h1[rr * 64]);
+                } else {
+                    o4[i] = leaky4(h0[rr * 64] + h1[rr * 64]);
+                    *(bf16x4 *)(Y + out_at(zz, rr)) = o4[i];
+                }
+            }
+            if constexpr (POOL) {
+                static_assert(R / 2 == 2, "one row pair per wave");
+                bf16x4 nb[2];  // the x + 1 neighbour's values (lane xl ^ 1)
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    const uint2 u = __builtin_bit_cast(uint2, o4[i]);
+                    uint2 v;
+                    v.x = __builtin_amdgcn_mov_dpp(u.x, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+                    v.y = __builtin_amdgcn_mov_dpp(u.y, 0xB1, 0xF, 0xF, false);
+                    nb[i] = __builtin_bit_cast(bf16x4, v);
+                }
+                const bool odd = zz & 1;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    float t = odd ? pp[e] : 0.f;
+                    t += (float)o4[0][e];
+                    t += (float)nb[0][e];
+                    t += (float)o4[1][e];
+                    t += (float)nb[1][e];
+                    pp[e] = t;
+                }
+                if (odd && !(xl & 1)) {
+                    bf16x4 r4;
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                        r4[e] = (bf16)(pp[e] * 0.125f);
+                    const size_t pv = (((size_t)b * (D >> 1) + (zz >> 1)) * (H >> 1) + ((y0 + 2 * g) >> 1)) *
+                                          (Wd >> 1) + ((x0 + xl) >> 1);
+                    *(bf16x4 *)(Q + pv * 64 + 16 * jb + 4 * q) = r4;
+                }
             }
         }
     };
@@ -3267,10 +3307,27 @@ static int forward(ppf_unet *n, int B, int D, int H, int W, bool need_x0,
     // level 2
     conv(n, 0, n->P1, C, 0, C, wb(2), 2 * C, bias(2), 2 * C, n->E2a, nullptr, 2 * C, 0, B, D2, H2,
          W2);
-    conv(n, 0, n->E2a, 2 * C, 0, 2 * C, wb(3), 2 * C, bias(3), 2 * C, n->U2, nullptr, ld2, off2, B,
-         D2, H2, W2);
-    hipLaunchKernelGGL(k_pool_fwd, dim3(row_blocks(B * D3 * H3, W3 * 2 * C / 8)), dim3(256), 0,
-                       n->stream, n->U2, ld2, off2, 2 * C, n->P2, B, D2, H2, W2);
+    // L3 and the level-2 pooling: one kernel where L3 runs the
+    // input-plane-stationary kernel with the K split (kind 6) on z chunks
+    // that start at even planes
+    {
+        const int sy = H2 / ZS64_R, sx = W2 / 16;
+        const int zs = (n->conv_kind == 6 && C == 32 && W2 % 16 == 0 && H2 % ZS64_R == 0 &&
+                        (size_t)H2 * W2 * 2 * C < (1u << 31))
+                           ? zs_chunk(D2, (size_t)B * sx * sy)
+                           : 0;
+        if (zs && zs % 2 == 0 && ld2 % 4 == 0 && off2 % 4 == 0) {
+            const unsigned nb = (unsigned)((size_t)B * (D2 / zs) * sy * sx);
+            hipLaunchKernelGGL((k_conv_zs64<2, 3>), dim3(nb), dim3(512), 0, n->stream, n->E2a, 2 * C, 0,
+                               wb(3), bias(3), n->U2, ld2, off2, B, D2, H2, W2, zs, sx, sy, 64, 0, 64, 0,
+                               (float *)nullptr, 0, n->P2);
+        } else {
+            conv(n, 0, n->E2a, 2 * C, 0, 2 * C, wb(3), 2 * C, bias(3), 2 * C, n->U2, nullptr, ld2, off2,
+                 B, D2, H2, W2);
+            hipLaunchKernelGGL(k_pool_fwd, dim3(row_blocks(B * D3 * H3, W3 * 2 * C / 8)), dim3(256), 0,
+                               n->stream, n->U2, ld2, off2, 2 * C, n->P2, B, D2, H2, W2);
+        }
+    }
     // level 3
     // the level-3 layers of the base-32 network (kind 6): the 64-channel
     // input-plane-stationary kernel per half of the 128 output channels;
