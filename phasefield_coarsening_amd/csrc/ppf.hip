@@ -308,31 +308,30 @@ static FusedPick<P> pick_fused(TileCfg t, int order, int bc = PPF_BC_PERIODIC, b
         return {nullptr, 0};
     }
     constexpr bool ROT = P::kRotate;
-    const bool fr = ROT && full;
     if (t.tx == 64 && t.ty == 8) {
         if constexpr (!F64 && ROT) {
-            if (fr)
+            if (full)
                 return {k_tiled2<P, 64, 8, W, false, true, 0, true>, nt};
         }
         return {k_tiled2<P, 64, 8, W, F64, true>, nt};
     }
     if (t.tx == 32 && t.ty == 8) {
         if constexpr (ROT) {
-            if (fr && (w3 || !F64))
+            if (full && (w3 || !F64))
                 return {k_tiled2<P, 32, 8, F64 ? 3 : W, false, true, 0, true>, nt};
         }
         return {w3 ? k_tiled2<P, 32, 8, 3, false, true> : k_tiled2<P, 32, 8, W, F64, true>, nt};
     }
     if (t.tx == 16 && t.ty == 16) {
         if constexpr (ROT) {
-            if (fr && (w3 || !F64))
+            if (full && (w3 || !F64))
                 return {k_tiled2<P, 16, 16, F64 ? 3 : W, false, true, 0, true>, nt};
         }
         return {w3 ? k_tiled2<P, 16, 16, 3, false, true> : k_tiled2<P, 16, 16, W, F64, true>, nt};
     }
     if (t.tx == 32 && t.ty == 16) {
         if constexpr (!F64 && ROT) {
-            if (fr)
+            if (full)
                 return {k_tiled2<P, 32, 16, W, false, true, 0, true>, nt};
         }
         return {k_tiled2<P, 32, 16, W, F64, true>, nt};
