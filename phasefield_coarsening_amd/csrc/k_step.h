@@ -649,6 +649,11 @@ __device__ __forceinline__ void tile_march2(
         // FULL: step t+2 values of plane zs-1, stored by the next plane
         constexpr bool DEF = FULL && BC == 0;
         T ps = 0, pl = 0, pc = 0;
+        // OEARLY (FULL): the outer site's loads are issued a plane ahead
+        // (eo*), and the point values of the z+1 columns are computed after
+        // the plane's barrier
+        constexpr bool OEARLY = DEF;
+        T eos = 0, eol = 0, eoc = 0;
         // PEND (an integral constant): a deferred store of plane zs-2 is
         // pending, which also means zs >= z0 + 2
         auto plane_step = [&](auto PEND, const int zs, const Col<T> &tp, const Col<T> &tc,
@@ -669,10 +674,15 @@ __device__ __forceinline__ void tile_march2(
                 qnx.l = ld_plane_b(PLANE(l_in, zn), brcol);
                 qnx.c = ld_plane_b(PLANE(c_in, zn), brcol);
             }
-            // (read only by the outer threads, which load them)
+            // (read only by the outer threads, which load them; OEARLY: loaded
+            // by the previous plane)
             T os = __builtin_nondeterministic_value(os), ol = __builtin_nondeterministic_value(ol),
               oc = __builtin_nondeterministic_value(oc);
-            if (outer) {
+            if constexpr (OEARLY) {
+                os = eos;
+                ol = eol;
+                oc = eoc;
+            } else if (outer) {
                 os = ld_plane_b(PLANE(s_in, zc), bo2col);
                 ol = ld_plane_b(PLANE(l_in, zc), bo2col);
                 oc = ld_plane_b(PLANE(c_in, zc), bo2col);
@@ -691,10 +701,25 @@ __device__ __forceinline__ void tile_march2(
                 P::point(os, ol, oc, k, oh, omu, oM);
                 lds_put4<S1>(A1, io1, os, ol, omu, oM);
             }
-            P::point(tnx.s, tnx.l, tnx.c, k, tnx.h, tnx.mu, tnx.M);
-            if (ring)
-                P::point(qnx.s, qnx.l, qnx.c, k, qnx.h, qnx.mu, qnx.M);
-            __syncthreads();
+            if constexpr (OEARLY) {
+                // the next plane's outer site, and the z+1 columns' point
+                // values after the barrier: no wave waits for this plane's
+                // loads before it
+                if (outer) {
+                    eos = ld_plane_b(PLANE(s_in, zn), bo2col);
+                    eol = ld_plane_b(PLANE(l_in, zn), bo2col);
+                    eoc = ld_plane_b(PLANE(c_in, zn), bo2col);
+                }
+                __syncthreads();
+                P::point(tnx.s, tnx.l, tnx.c, k, tnx.h, tnx.mu, tnx.M);
+                if (ring)
+                    P::point(qnx.s, qnx.l, qnx.c, k, qnx.h, qnx.mu, qnx.M);
+            } else {
+                P::point(tnx.s, tnx.l, tnx.c, k, tnx.h, tnx.mu, tnx.M);
+                if (ring)
+                    P::point(qnx.s, qnx.l, qnx.c, k, qnx.h, qnx.mu, qnx.M);
+                __syncthreads();
+            }
 
             // stage 1: step t+1 of plane zs at the tile site (mirror
             // boundaries: a ghost plane is not stepped, it keeps the step
@@ -779,6 +804,14 @@ __device__ __forceinline__ void tile_march2(
         using NoPend = std::integral_constant<bool, false>;
         using Pend = std::integral_constant<bool, DEF>;
         int zs = z0 - 1;
+        if constexpr (OEARLY) {
+            if (outer) {  // the first plane's outer site
+                const int zo = ghost_any<BC>(zs, nz);
+                eos = ld_plane_b(PLANE(s_in, zo), bo2col);
+                eol = ld_plane_b(PLANE(l_in, zo), bo2col);
+                eoc = ld_plane_b(PLANE(c_in, zo), bo2col);
+            }
+        }
         if constexpr (DEF) {
             // planes z0-1 .. z0+1 (zend >= z0 + 1): nothing pending yet
             plane_step(NoPend{}, zs, ta, tb, tn, qa, qb, qn, ua, ub, un);
