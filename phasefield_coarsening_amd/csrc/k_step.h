@@ -851,6 +851,15 @@ __device__ __forceinline__ void tile_march2(
     V *const v2 = reinterpret_cast<V *>(&a2[0][0][0]);
     const int i1 = (ty + 2) * W1 + tx + 2, ir1 = (ry + 1) * W1 + rx + 1, io1 = oy * W1 + ox;
     const int i2 = (ty + 1) * W2 + tx + 1, ir2 = ry * W2 + rx;
+    // without PF: the outer site's loads are issued a plane ahead, and the
+    // z+1 columns' point values are computed after the plane's barrier (as
+    // in the rotating march: no wave waits for the plane's loads before it)
+    if (!PF && outer) {
+        const int zo = ghost_any<BC>(z0 - 1, nz);
+        os = ld_plane(PLANE(s_in, zo), o2col);
+        ol = ld_plane(PLANE(l_in, zo), o2col);
+        oc = ld_plane(PLANE(c_in, zo), o2col);
+    }
     for (int zs = z0 - 1; zs <= zend; ++zs) {
         PPF_FRESH("+v"(col), "+v"(ocol), "+v"(rcol), "+v"(o2col));
         const int zc = ghost_any<BC>(zs, nz), zn = ghost_any<BC>(zs + 1, nz);
@@ -867,11 +876,6 @@ __device__ __forceinline__ void tile_march2(
                 qs_n = ld_plane(PLANE(s_in, zn), rcol);
                 ql_n = ld_plane(PLANE(l_in, zn), rcol);
                 qc_n = ld_plane(PLANE(c_in, zn), rcol);
-            }
-            if (outer) {
-                os = ld_plane(PLANE(s_in, zc), o2col);
-                ol = ld_plane(PLANE(l_in, zc), o2col);
-                oc = ld_plane(PLANE(c_in, zc), o2col);
             }
         }
         // PF: the next iteration's values (plane zs+2 of the columns, zs+1
@@ -901,12 +905,24 @@ __device__ __forceinline__ void tile_march2(
             T oh, omu, oM;
             P::point(os, ol, oc, k, oh, omu, oM);
             lds_put4<S1>(A1, io1, os, ol, omu, oM);
+            if (!PF) {
+                os = ld_plane(PLANE(s_in, zn), o2col);
+                ol = ld_plane(PLANE(l_in, zn), o2col);
+                oc = ld_plane(PLANE(c_in, zn), o2col);
+            }
         }
         T h_n, mu_n, M_n, qh_n = 0, qmu_n = 0, qM_n = 0;
-        P::point(rs_n, rl_n, rc_n, k, h_n, mu_n, M_n);
-        if (ring)
-            P::point(qs_n, ql_n, qc_n, k, qh_n, qmu_n, qM_n);
+        if (PF) {
+            P::point(rs_n, rl_n, rc_n, k, h_n, mu_n, M_n);
+            if (ring)
+                P::point(qs_n, ql_n, qc_n, k, qh_n, qmu_n, qM_n);
+        }
         __syncthreads();
+        if (!PF) {
+            P::point(rs_n, rl_n, rc_n, k, h_n, mu_n, M_n);
+            if (ring)
+                P::point(qs_n, ql_n, qc_n, k, qh_n, qmu_n, qM_n);
+        }
 
         // stage 1: step t+1 of plane zs at the tile site.  Mirror
         // boundaries: a ghost plane (zs = -1 or nz) is not stepped; plane
