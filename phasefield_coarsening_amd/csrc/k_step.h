@@ -649,10 +649,10 @@ __device__ __forceinline__ void tile_march2(
         // FULL: step t+2 values of plane zs-1, stored by the next plane
         constexpr bool DEF = FULL && BC == 0;
         T ps = 0, pl = 0, pc = 0;
-        // OEARLY (FULL): the outer site's loads are issued a plane ahead
-        // (eo*), and the point values of the z+1 columns are computed after
-        // the plane's barrier
-        constexpr bool OEARLY = DEF;
+        // OEARLY: the outer site's loads are issued a plane ahead (eo*), and
+        // the point values of the z+1 columns are computed after the plane's
+        // barrier
+        constexpr bool OEARLY = true;
         T eos = 0, eol = 0, eoc = 0;
         // PEND (an integral constant): a deferred store of plane zs-2 is
         // pending, which also means zs >= z0 + 2
