@@ -679,9 +679,18 @@ __device__ __forceinline__ void tile_march2(
             T os = __builtin_nondeterministic_value(os), ol = __builtin_nondeterministic_value(ol),
               oc = __builtin_nondeterministic_value(oc);
             if constexpr (OEARLY) {
+                // this plane's outer site (loaded by the previous plane), and
+                // the next plane's loads issued before the pending stores, so
+                // that the wait for this plane's loads leaves the stores in
+                // flight
                 os = eos;
                 ol = eol;
                 oc = eoc;
+                if (outer) {
+                    eos = ld_plane_b(PLANE(s_in, zn), bo2col);
+                    eol = ld_plane_b(PLANE(l_in, zn), bo2col);
+                    eoc = ld_plane_b(PLANE(c_in, zn), bo2col);
+                }
             } else if (outer) {
                 os = ld_plane_b(PLANE(s_in, zc), bo2col);
                 ol = ld_plane_b(PLANE(l_in, zc), bo2col);
@@ -702,14 +711,8 @@ __device__ __forceinline__ void tile_march2(
                 lds_put4<S1>(A1, io1, os, ol, omu, oM);
             }
             if constexpr (OEARLY) {
-                // the next plane's outer site, and the z+1 columns' point
-                // values after the barrier: no wave waits for this plane's
-                // loads before it
-                if (outer) {
-                    eos = ld_plane_b(PLANE(s_in, zn), bo2col);
-                    eol = ld_plane_b(PLANE(l_in, zn), bo2col);
-                    eoc = ld_plane_b(PLANE(c_in, zn), bo2col);
-                }
+                // the z+1 columns' point values after the barrier: no wave
+                // waits for this plane's loads before it
                 __syncthreads();
                 P::point(tnx.s, tnx.l, tnx.c, k, tnx.h, tnx.mu, tnx.M);
                 if (ring)
