@@ -225,8 +225,9 @@ def unet_line(grid=256, reps=3, apply_reps=10):
 def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium", no_op=True,
                    first_jump=False):
     """Learned jumps against exact stepping at grid^3 (500 spheres,
-    R ~ U[6, 12]) to t_end: wall times of the surrogate (K = 200 jumps, 20
-    FP32 relax steps, judge, exact re-run on rejection), of plain FP64
+    R ~ U[6, 12]) to t_end: wall times of the surrogate (jumps of k steps
+    with the weights trained for k, 20 FP32 relax steps, judge, exact re-run
+    on rejection), of plain FP64
     stepping (the fast FP64 kernel) and of plain FP32 stepping, and the
     errors of the mean particle radius (connected components, over all
     particles) and of the solid fraction at t_end against FP64.  c_init
@@ -241,7 +242,7 @@ def surrogate_line(S, grid=256, t_end=2200, k=200, relax=20, c_init="equilibrium
 
     from phasefield_coarsening_amd import surrogate as SG
     from phasefield_coarsening_amd.unet import UNet, layer_shapes
-    wname = "unet_k200_b32_eq_f16.npy" if c_init == "equilibrium" else "unet_k200_b32_f16.npy"
+    wname = "unet_k%d_b32%s_f16.npy" % (k, "_eq" if c_init == "equilibrium" else "")
     w = np.load(os.path.join(HERE, "phasefield_coarsening_amd", "weights", wname)).astype(np.float32)
     shape = (grid, grid, grid)
     cen, rad = S.spheres(5, 500, shape, 6.0, 12.0)
@@ -461,6 +462,10 @@ def main():
         out["surrogate_long"] = surrogate_line(S, t_end=19800, no_op=False)
         out["surrogate_solidifying"] = surrogate_line(S, c_init="config", no_op=False)
         out["surrogate_first_jump"] = surrogate_line(S, no_op=False, first_jump=True)
+        out["surrogate_k400"] = surrogate_line(S, t_end=4400, k=400, no_op=False)
+        out["surrogate_k400_long"] = surrogate_line(S, t_end=19800, k=400, no_op=False)
+        out["surrogate_k400_first_jump"] = surrogate_line(S, t_end=4400, k=400, no_op=False,
+                                                          first_jump=True)
     if "copy" in side:
         cc = side["copy"]
         out["roofline"]["copy_ceiling_GBs"] = round(cc, 1)
