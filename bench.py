@@ -462,11 +462,11 @@ def main():
         out["surrogate_long"] = surrogate_line(S, t_end=19800, no_op=False)
         out["surrogate_solidifying"] = surrogate_line(S, c_init="config", no_op=False)
         out["surrogate_first_jump"] = surrogate_line(S, no_op=False, first_jump=True)
-        out["surrogate_k400"] = surrogate_line(S, t_end=4400, k=400, no_op=False)
+        out["surrogate_k400"] = surrogate_line(S, t_end=4400, k=400)
         out["surrogate_k400_long"] = surrogate_line(S, t_end=19800, k=400, no_op=False)
         out["surrogate_k400_first_jump"] = surrogate_line(S, t_end=4400, k=400, no_op=False,
                                                           first_jump=True)
-        out["surrogate_k800_long"] = surrogate_line(S, t_end=19800, k=800, no_op=False)
+        out["surrogate_k800_long"] = surrogate_line(S, t_end=19800, k=800)
     if "copy" in side:
         cc = side["copy"]
         out["roofline"]["copy_ceiling_GBs"] = round(cc, 1)
