@@ -467,6 +467,8 @@ def main():
         out["surrogate_k400_first_jump"] = surrogate_line(S, t_end=4400, k=400, no_op=False,
                                                           first_jump=True)
         out["surrogate_k800_long"] = surrogate_line(S, t_end=19800, k=800)
+        out["surrogate_k400_solidifying"] = surrogate_line(S, t_end=4400, k=400, c_init="config",
+                                                           no_op=False)
     if "copy" in side:
         cc = side["copy"]
         out["roofline"]["copy_ceiling_GBs"] = round(cc, 1)

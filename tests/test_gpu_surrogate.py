@@ -263,3 +263,33 @@ def test_coarsening_regime_800_step_jumps_long_horizon():
     assert abs(f_sur - f_ref) / f_ref < 0.02
     assert abs(ps["n_particles"] - pr["n_particles"]) <= 0.03 * pr["n_particles"]
     assert abs(ss["sum_c"] - target) / target < 1e-8
+
+
+def test_solidifying_regime_400_step_jumps():
+    """The configured composition (the box solidifies into one cluster) with
+    400-step jumps (weights trained on that regime's 20000-step
+    trajectories, pairs 400 steps apart) to t_end = 4400: the rate gate runs
+    the solidification transient exactly, at least five jumps are accepted
+    and none rejected, and the solid fraction lands within 2% of exact FP64
+    (observed 0.10%), with the solute on its FP64 target."""
+    shape = (256, 256, 256)
+    cen, rad = S.spheres(5, 500, shape, 6.0, 12.0)
+    with S.Solver(shape, kernel=S.KERNEL_FAST) as ref:
+        ref.init_spheres(cen, rad)
+        s0 = ref.download()
+        ref.step(4400)
+        st = ref.reduce()
+    path = os.path.join(_lib.PKG_DIR, "weights", "unet_k400_b32_f16.npy")
+    with S.Solver(shape, precision=S.F32) as sv, _net(path) as net:
+        sv.upload(*s0)
+        del s0
+        target = sv.mass_target()[0]
+        ctl = SG.Controller(sv, net, k=400)
+        summ = ctl.advance(4400)
+        ss = sv.reduce()
+        ctl.close()
+    print(summ)
+    assert summ["accepted"] >= 5 and summ["rejected"] == 0
+    f_ref, f_sur = st["sum_h"] / st["n_sites"], ss["sum_h"] / ss["n_sites"]
+    assert abs(f_sur - f_ref) / f_ref < 0.02
+    assert abs(ss["sum_c"] - target) / target < 1e-8
